@@ -1,0 +1,327 @@
+"""fastreg_amd — MI355X (gfx950) fastreg registration solver.
+
+The product is the native library ``libfastreg_hip.so`` (HIP kernels + the C-ABI declared in
+``include/fastreg_c.h``) and the ``fastreg_app`` CLI.  This module is a thin ctypes binding of
+that C-ABI for Python callers (tests, bench.py): it holds no solver logic, and it raises if the
+native library has not been built — there is no CPU fallback.
+
+Device memory is handled through the library's own plumbing (``fr_dev_malloc`` and friends), so
+no other GPU runtime is needed in the process.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+ROOT_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "libfastreg_hip.so")
+HEADER_PATH = os.path.join(ROOT_DIR, "include", "fastreg_c.h")
+
+FR_OK = 0
+FR_ERR_INVALID = 1
+FR_ERR_HIP = 2
+FR_ERR_ALLOC = 3
+FR_ERR_IO = 4
+FR_ERR_PARSE = 5
+FR_ERR_STATE = 6
+
+
+class FastRegError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fastreg error {code}: {msg}")
+        self.code = code
+
+
+class FrParams(C.Structure):
+    _fields_ = [("vol_dims", C.c_int * 3), ("vol_size", C.c_float * 3), ("no_labels", C.c_int),
+                ("max_iter", C.c_int), ("max_update_norm", C.c_float), ("rho_0", C.c_float), ("tau", C.c_float),
+                ("alpha", C.c_float), ("w_reg", C.c_float), ("p", C.c_float), ("C", C.c_float),
+                ("reset_iter", C.c_int), ("verbosity", C.c_int)]
+
+
+class FrStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("stop_reason", C.c_int), ("last_max_update", C.c_float),
+                ("last_max_voxel", C.c_longlong), ("last_energy", C.c_double), ("min_det_true", C.c_double),
+                ("min_det_ref", C.c_double), ("dice_all", C.c_double), ("solver_ms", C.c_float),
+                ("total_ms", C.c_float)]
+
+
+_lib = None
+
+
+def header_functions():
+    """Names of the functions declared in include/fastreg_c.h."""
+    txt = open(HEADER_PATH).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fr_[a-z0-9_]+)\s*\(", txt)))
+
+
+def lib():
+    """Load libfastreg_hip.so (raises if it is missing: build with `python -m fastreg_amd.build`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"native library missing: {LIB_PATH}; build it with `python -m fastreg_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, i, f, d, ll, sz = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_longlong, C.c_size_t
+    P = C.POINTER
+
+    def sig(name, *args, res=C.c_int):
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = list(args)
+
+    sig("fr_last_error", res=C.c_char_p)
+    sig("fr_params_default", P(FrParams))
+    sig("fr_params_load_ini", C.c_char_p, P(FrParams))
+    sig("fr_create", P(FrParams), P(vp))
+    sig("fr_destroy", vp)
+    sig("fr_get_params", vp, P(FrParams))
+    sig("fr_set_stream", vp, vp)
+    sig("fr_register", vp, vp, vp, vp, vp)
+    sig("fr_get_deformation", vp, vp)
+    sig("fr_get_inverse_deformation", vp, vp)
+    sig("fr_get_stats", vp, P(FrStats))
+    sig("fr_get_dice", vp, i, vp, vp, vp)
+    sig("fr_get_history", vp, i, vp, vp)
+    sig("fr_get_log", vp, C.c_char_p, sz)
+    sig("fr_solver_init", vp, vp, vp, vp, vp)
+    sig("fr_solver_step", vp, f, P(f), P(ll))
+    sig("fr_solver_run", vp)
+    sig("fr_solver_finalize", vp, vp, vp, vp, vp)
+    sig("fr_solver_set_state", vp, vp, vp, vp, vp)
+    sig("fr_solver_get_state", vp, vp, vp, vp, vp)
+    sig("fr_solver_energy", vp, P(d), P(d))
+    sig("fr_warp", i, i, i, vp, vp, vp, vp, vp, vp)
+    sig("fr_time_kernels", vp, i, P(f), P(f))
+    sig("fr_dev_malloc", P(vp), sz)
+    sig("fr_dev_free", vp)
+    sig("fr_memcpy_htod", vp, vp, sz)
+    sig("fr_memcpy_dtoh", vp, vp, sz)
+    sig("fr_memcpy_dtod", vp, vp, sz)
+    sig("fr_dev_memset", vp, i, sz)
+    sig("fr_dev_sync")
+    sig("fr_device_info", C.c_char_p, sz)
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != FR_OK:
+        raise FastRegError(rc, lib().fr_last_error().decode(errors="replace"))
+    return rc
+
+
+# ---- parameters -------------------------------------------------------------------------------
+
+def default_params(**kw) -> FrParams:
+    p = FrParams()
+    check(lib().fr_params_default(C.byref(p)))
+    for k, v in kw.items():
+        if k == "vol_dims":
+            p.vol_dims[:] = list(v)
+        elif k == "vol_size":
+            p.vol_size[:] = list(v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def load_ini(path, base: FrParams | None = None) -> FrParams:
+    p = base if base is not None else default_params()
+    check(lib().fr_params_load_ini(os.fsencode(path), C.byref(p)))
+    return p
+
+
+# ---- device buffers ---------------------------------------------------------------------------
+
+class DeviceArray:
+    """A device allocation holding a numpy-typed array (via the library's fr_dev_* plumbing)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        ptr = C.c_void_p()
+        check(lib().fr_dev_malloc(C.byref(ptr), self.nbytes))
+        self.ptr = ptr
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        check(lib().fr_memcpy_htod(self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes))
+
+    def get(self):
+        out = np.empty(self.shape, self.dtype)
+        check(lib().fr_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().fr_dev_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_info() -> str:
+    buf = C.create_string_buffer(512)
+    check(lib().fr_device_info(buf, 512))
+    return buf.value.decode()
+
+
+def warp(img, lab, psi4, dims):
+    """apply(): trilinear image and floor-corner labels at psi (float4 per voxel), on the GPU."""
+    X, Y, Z = dims
+    n = X * Y * Z
+    d_img = DeviceArray.from_host(np.asarray(img, np.float32).reshape(n))
+    d_lab = DeviceArray.from_host(np.asarray(lab, np.int16).reshape(n))
+    d_psi = DeviceArray.from_host(np.asarray(psi4, np.float32).reshape(n, 4))
+    o_img, o_lab = DeviceArray(n, np.float32), DeviceArray(n, np.int16)
+    check(lib().fr_warp(X, Y, Z, d_img.ptr, d_lab.ptr, d_psi.ptr, o_img.ptr, o_lab.ptr, None))
+    return o_img.get(), o_lab.get()
+
+
+# ---- solver context ---------------------------------------------------------------------------
+
+@dataclass
+class Stats:
+    iterations: int
+    stop_reason: int
+    last_max_update: float
+    last_max_voxel: int
+    last_energy: float
+    min_det_true: float
+    min_det_ref: float
+    dice_all: float
+    solver_ms: float
+    total_ms: float
+
+
+class Solver:
+    """One fr_ctx: the FastReg pipeline for one volume size and parameter set."""
+
+    def __init__(self, params: FrParams):
+        self.params = params
+        self.dims = tuple(params.vol_dims)
+        self.n = self.dims[0] * self.dims[1] * self.dims[2]
+        h = C.c_void_p()
+        check(lib().fr_create(C.byref(params), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().fr_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # pipeline
+    def register(self, tgt, tgt_lab, src, src_lab):
+        """FastReg::operator() on host arrays; returns (img1, lab1, img2, lab2) = (T o psi_inv,
+        LT o psi_inv, S o psi, LS o psi)."""
+        n = self.n
+        bufs = [DeviceArray.from_host(np.asarray(tgt, np.float32).reshape(n)),
+                DeviceArray.from_host(np.asarray(tgt_lab, np.int16).reshape(n)),
+                DeviceArray.from_host(np.asarray(src, np.float32).reshape(n)),
+                DeviceArray.from_host(np.asarray(src_lab, np.int16).reshape(n))]
+        check(lib().fr_register(self.h, *[b.ptr for b in bufs]))
+        return tuple(b.get() for b in bufs)
+
+    # solver pieces
+    def init(self, tgt, tgt_lab, src, src_lab):
+        n = self.n
+        self._inputs = [DeviceArray.from_host(np.asarray(tgt, np.float32).reshape(n)),
+                        DeviceArray.from_host(np.asarray(tgt_lab, np.int16).reshape(n)),
+                        DeviceArray.from_host(np.asarray(src, np.float32).reshape(n)),
+                        DeviceArray.from_host(np.asarray(src_lab, np.int16).reshape(n))]
+        check(lib().fr_solver_init(self.h, *[b.ptr for b in self._inputs]))
+
+    def step(self, t):
+        m, am = C.c_float(), C.c_longlong()
+        check(lib().fr_solver_step(self.h, t, C.byref(m), C.byref(am)))
+        return m.value, am.value
+
+    def run(self):
+        check(lib().fr_solver_run(self.h))
+
+    def finalize(self, want_outputs=True):
+        n = self.n
+        if want_outputs:
+            outs = [DeviceArray(n, np.float32), DeviceArray(n, np.int16), DeviceArray(n, np.float32),
+                    DeviceArray(n, np.int16)]
+            check(lib().fr_solver_finalize(self.h, *[b.ptr for b in outs]))
+            return tuple(b.get() for b in outs)
+        check(lib().fr_solver_finalize(self.h, None, None, None, None))
+        return None
+
+    def set_state(self, v=None, rho=None, psi=None, psi_inv=None):
+        n = self.n
+        arrs = []
+        for a, shape in ((v, (n, 4)), (rho, (n,)), (psi, (n, 4)), (psi_inv, (n, 4))):
+            arrs.append(None if a is None else DeviceArray.from_host(np.asarray(a, np.float32).reshape(shape)))
+        check(lib().fr_solver_set_state(self.h, *[None if a is None else a.ptr for a in arrs]))
+
+    def get_state(self):
+        n = self.n
+        v, rho, psi, pinv = DeviceArray((n, 4), np.float32), DeviceArray(n, np.float32), \
+            DeviceArray((n, 4), np.float32), DeviceArray((n, 4), np.float32)
+        check(lib().fr_solver_get_state(self.h, v.ptr, rho.ptr, psi.ptr, pinv.ptr))
+        return {"v": v.get(), "rho": rho.get(), "psi": psi.get(), "psi_inv": pinv.get()}
+
+    def energy(self):
+        a, b = C.c_double(), C.c_double()
+        check(lib().fr_solver_energy(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def stats(self) -> Stats:
+        s = FrStats()
+        check(lib().fr_get_stats(self.h, C.byref(s)))
+        return Stats(**{k: getattr(s, k) for k, _ in FrStats._fields_})
+
+    def history(self, n=None):
+        n = self.params.max_iter if n is None else n
+        m = np.zeros(max(n, 1), np.float32)
+        e = np.zeros(max(n, 1), np.float64)
+        check(lib().fr_get_history(self.h, n, m.ctypes.data_as(C.c_void_p), e.ctypes.data_as(C.c_void_p)))
+        return m[:n], e[:n]
+
+    def dice(self, n=None):
+        n = self.params.no_labels if n is None else n
+        dice = np.zeros(n, np.float64)
+        ov = np.zeros(n, np.int64)
+        vx = np.zeros(n, np.int64)
+        check(lib().fr_get_dice(self.h, n, dice.ctypes.data_as(C.c_void_p), ov.ctypes.data_as(C.c_void_p),
+                                vx.ctypes.data_as(C.c_void_p)))
+        return dice, ov, vx
+
+    def log(self, size=1 << 20):
+        buf = C.create_string_buffer(size)
+        check(lib().fr_get_log(self.h, buf, size))
+        return buf.value.decode()
+
+    def time_kernels(self, reps=20):
+        a, b = C.c_float(), C.c_float()
+        check(lib().fr_time_kernels(self.h, reps, C.byref(a), C.byref(b)))
+        return a.value, b.value

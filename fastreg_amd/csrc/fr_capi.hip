@@ -1,0 +1,759 @@
+// C-ABI of the fastreg solver (include/fastreg_c.h): contexts, the iteration loop, the
+// reference's messages, params.ini parsing and device-memory plumbing.
+
+#include "../../include/fastreg_c.h"
+#include "fr_internal.hpp"
+
+#include <cerrno>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+using frk::Dims;
+
+// ---------------------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------------------
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail(FR_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                         \
+    } while (0)
+
+extern "C" const char *fr_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------------------------
+
+struct fr_ctx {
+    fr_params prm{};
+    Dims d{};
+    int n = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    // state
+    float *T = nullptr, *S = nullptr;
+    int16_t *LT = nullptr, *LS = nullptr;
+    float *rho[2] = {nullptr, nullptr};
+    float *v[2] = {nullptr, nullptr};
+    float *pinv[2] = {nullptr, nullptr};
+    float *psi = nullptr;
+    int cur = 0;
+    bool initialised = false;
+    // loop control and history
+    frk::Ctl *ctl = nullptr;
+    unsigned long long *maxpack = nullptr;  // max_iter slots
+    double *energy = nullptr;               // 3 * max_iter: energy, e_data, e_reg
+    double *partials = nullptr;
+    int nparts = 0;
+    // finalisation
+    unsigned *dice_counts = nullptr;  // overlap[no_labels], voxels[no_labels], bad
+    float *det_parts = nullptr;
+    int ndet = 0;
+    int16_t *lab_tmp = nullptr;
+    // host copies of the last run
+    std::vector<float> h_max;
+    std::vector<long long> h_argmax;
+    std::vector<double> h_energy, h_edata, h_ereg;
+    std::vector<long long> h_overlap, h_voxels;
+    long long h_bad = 0;
+    int run_start_buf = 1;
+    fr_stats stats{};
+    std::string log;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool have_run = false, have_final = false;
+};
+
+static int collect_run(fr_ctx *c);
+
+static int check_params(const fr_params *p) {
+    for (int k = 0; k < 3; ++k)
+        if (p->vol_dims[k] < 3) return fail(FR_ERR_INVALID, "VOL_DIMS must be >= 3 along every axis (got %d %d %d)",
+                                            p->vol_dims[0], p->vol_dims[1], p->vol_dims[2]);
+    long long n = (long long)p->vol_dims[0] * p->vol_dims[1] * p->vol_dims[2];
+    if (3 * n >= (1LL << 31)) return fail(FR_ERR_INVALID, "volume of %lld voxels is too large (3*N must fit in int32)", n);
+    if (p->max_iter < 0) return fail(FR_ERR_INVALID, "MAX_ITER must be >= 0");
+    if (p->no_labels < 1 || p->no_labels > 32768) return fail(FR_ERR_INVALID, "NO_LABELS must be in [1, 32768]");
+    if (p->verbosity < 0 || p->verbosity > 2) return fail(FR_ERR_INVALID, "verbosity must be 0, 1 or 2");
+    return FR_OK;
+}
+
+extern "C" int fr_params_default(fr_params *p) {
+    if (!p) return fail(FR_ERR_INVALID, "null params");
+    memset(p, 0, sizeof *p);
+    p->vol_dims[0] = 193;
+    p->vol_dims[1] = 229;
+    p->vol_dims[2] = 193;
+    p->vol_size[0] = p->vol_size[1] = p->vol_size[2] = 1.f;
+    p->no_labels = 6;
+    p->max_iter = 2048;
+    p->max_update_norm = -1e-10f;
+    p->rho_0 = 1.f;
+    p->tau = 0.2f;
+    p->alpha = 0.004f;
+    p->w_reg = 0.2f;
+    p->p = 2.f;
+    p->C = 0.25f;
+    p->reset_iter = -1;
+    p->verbosity = 0;
+    return FR_OK;
+}
+
+static void free_ctx(fr_ctx *c) {
+    if (!c) return;
+    void *bufs[] = {c->T, c->S, c->LT, c->LS, c->rho[0], c->rho[1], c->v[0], c->v[1], c->pinv[0], c->pinv[1], c->psi,
+                    c->ctl, c->maxpack, c->energy, c->partials, c->dice_counts, c->det_parts, c->lab_tmp};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+template <typename T>
+static int dalloc(T **p, size_t count) {
+    *p = nullptr;
+    hipError_t e = hipMalloc((void **)p, (count ? count : 1) * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(FR_ERR_ALLOC, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    return FR_OK;
+}
+
+extern "C" int fr_create(const fr_params *p, fr_ctx **out) {
+    if (!p || !out) return fail(FR_ERR_INVALID, "null argument");
+    *out = nullptr;
+    int rc = check_params(p);
+    if (rc) return rc;
+    fr_ctx *c = new fr_ctx();
+    c->prm = *p;
+    c->d = Dims{p->vol_dims[0], p->vol_dims[1], p->vol_dims[2]};
+    c->n = p->vol_dims[0] * p->vol_dims[1] * p->vol_dims[2];
+    size_t n = (size_t)c->n;
+    int mi = p->max_iter > 0 ? p->max_iter : 1;
+    c->nparts = frk::energy_partials_count(c->d);
+    c->ndet = frk::min_det_partials_count(c->d);
+    if ((rc = dalloc(&c->T, n)) || (rc = dalloc(&c->S, n)) || (rc = dalloc(&c->LT, n)) || (rc = dalloc(&c->LS, n)) ||
+        (rc = dalloc(&c->rho[0], n)) || (rc = dalloc(&c->rho[1], n)) || (rc = dalloc(&c->v[0], 3 * n)) ||
+        (rc = dalloc(&c->v[1], 3 * n)) || (rc = dalloc(&c->pinv[0], 3 * n)) || (rc = dalloc(&c->pinv[1], 3 * n)) ||
+        (rc = dalloc(&c->psi, 3 * n)) || (rc = dalloc(&c->ctl, 1)) || (rc = dalloc(&c->maxpack, (size_t)mi)) ||
+        (rc = dalloc(&c->energy, 3 * (size_t)mi)) || (rc = dalloc(&c->partials, 2 * (size_t)c->nparts)) ||
+        (rc = dalloc(&c->dice_counts, 2 * (size_t)p->no_labels + 1)) || (rc = dalloc(&c->det_parts, 2 * (size_t)c->ndet)) ||
+        (rc = dalloc(&c->lab_tmp, n))) {
+        free_ctx(c);
+        return rc;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        return fail(FR_ERR_HIP, "stream/event creation: %s", hipGetErrorString(e));
+    }
+    c->stream = c->own_stream;
+    *out = c;
+    return FR_OK;
+}
+
+extern "C" int fr_destroy(fr_ctx *c) {
+    if (!c) return FR_OK;
+    (void)hipStreamSynchronize(c->stream);
+    free_ctx(c);
+    return FR_OK;
+}
+
+extern "C" int fr_get_params(const fr_ctx *c, fr_params *p) {
+    if (!c || !p) return fail(FR_ERR_INVALID, "null argument");
+    *p = c->prm;
+    return FR_OK;
+}
+
+extern "C" int fr_set_stream(fr_ctx *c, void *s) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->stream = s ? (hipStream_t)s : c->own_stream;
+    return FR_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// solver
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int fr_solver_init(fr_ctx *c, const float *tgt, const int16_t *tgt_lab, const float *src,
+                              const int16_t *src_lab) {
+    if (!c || !tgt || !tgt_lab || !src || !src_lab) return fail(FR_ERR_INVALID, "null argument");
+    HIP_TRY(frk::launch_init(c->d, tgt, tgt_lab, src, src_lab, c->T, c->LT, c->S, c->LS, c->rho[0], c->rho[1], c->v[0],
+                             c->v[1], c->pinv[0], c->pinv[1], c->psi, c->prm.rho_0, c->stream));
+    c->cur = 0;
+    c->initialised = true;
+    c->have_run = c->have_final = false;
+    return FR_OK;
+}
+
+static void iter_args(fr_ctx *c, frk::IterArgs &a) {
+    a.d = c->d;
+    a.nbx = (c->d.X + frk::TILE_X - 1) / frk::TILE_X;
+    a.nby = (c->d.Y + frk::TILE_Y - 1) / frk::TILE_Y;
+    a.nblocks = a.nbx * a.nby * c->d.Z;
+    a.rho_o = c->rho[c->cur];
+    a.rho_n = c->rho[c->cur ^ 1];
+    a.v_o = c->v[c->cur];
+    a.v_n = c->v[c->cur ^ 1];
+    a.pinv_o = c->pinv[c->cur];
+    a.pinv_n = c->pinv[c->cur ^ 1];
+    a.psi = c->psi;
+    a.S = c->S;
+    a.T = c->T;
+    a.alpha = c->prm.alpha;
+    a.tau = c->prm.tau;
+    a.w_reg = c->prm.w_reg;
+    a.thr = c->prm.max_update_norm;
+    a.ctl = c->ctl;
+    a.maxpack = c->maxpack;
+    a.energy = c->energy;
+    a.check_energy = 0;
+    a.iter = 1;
+}
+
+// per-iteration scalars of update_velocity_kernel (solver.cu:198-199): (p+1)/t and
+// C * p^2 * t^(p-2), in float as the reference computes them
+static void vel_scalars(const fr_params &p, float t, float *m1t, float *m2) {
+    float m1 = p.p + 1.f;
+    *m1t = m1 / t;
+    *m2 = p.C * powf(p.p, 2.f) * powf(t, p.p - 2.f);
+}
+
+static bool verbose_now(const fr_params &p, int iter) {
+    return (p.verbosity == 1 && (iter == 1 || iter % 100 == 0 || iter == p.max_iter)) || p.verbosity == 2;
+}
+
+// one iteration's launches: [energy], pass A, pass B
+static int launch_iteration(fr_ctx *c, int iter, float t, bool with_energy) {
+    frk::IterArgs a;
+    iter_args(c, a);
+    a.iter = iter;
+    vel_scalars(c->prm, t, &a.m1t, &a.m2);
+    if (with_energy) {
+        HIP_TRY(frk::launch_energy(c->d, c->S, c->T, c->psi, c->partials, c->nparts, c->stream));
+        double *e = c->energy;
+        int mi = c->prm.max_iter > 0 ? c->prm.max_iter : 1;
+        HIP_TRY(frk::launch_energy_finalize(c->partials, c->nparts, (double)c->prm.w_reg, e + mi + (iter - 1),
+                                            e + 2 * mi + (iter - 1), e + (iter - 1), c->stream));
+        a.check_energy = 1;
+    }
+    HIP_TRY(frk::launch_pass_a(a, c->stream));
+    HIP_TRY(frk::launch_pass_b(a, c->stream));
+    c->cur ^= 1;
+    return FR_OK;
+}
+
+static int reset_control(fr_ctx *c) {
+    int mi = c->prm.max_iter > 0 ? c->prm.max_iter : 1;
+    HIP_TRY(hipMemsetAsync(c->ctl, 0, sizeof(frk::Ctl), c->stream));
+    HIP_TRY(hipMemsetAsync(c->maxpack, 0, sizeof(unsigned long long) * mi, c->stream));
+    HIP_TRY(hipMemsetAsync(c->energy, 0, sizeof(double) * 3 * mi, c->stream));
+    return FR_OK;
+}
+
+static void unpack_max(unsigned long long mp, float *m, long long *idx) {
+    float nr = 0.f;
+    unsigned bits = (unsigned)(mp >> 32);
+    memcpy(&nr, &bits, 4);
+    *m = nr;
+    *idx = (nr > 0.f) ? (long long)(0xFFFFFFFFu - (unsigned)(mp & 0xFFFFFFFFu)) : 0;
+}
+
+extern "C" int fr_solver_step(fr_ctx *c, float t, float *max_norm, long long *max_voxel) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (!c->initialised) return fail(FR_ERR_STATE, "fr_solver_step before fr_solver_init");
+    int rc;
+    if (c->have_run && (rc = collect_run(c))) return rc;
+    if ((rc = reset_control(c))) return rc;
+    if (rc) return rc;
+    if ((rc = launch_iteration(c, 1, t, false))) return rc;
+    unsigned long long mp = 0;
+    HIP_TRY(hipMemcpyAsync(&mp, c->maxpack, sizeof mp, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float m;
+    long long idx;
+    unpack_max(mp, &m, &idx);
+    if (max_norm) *max_norm = m;
+    if (max_voxel) *max_voxel = idx;
+    return FR_OK;
+}
+
+static void logf_(std::string &s, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    s += buf;
+}
+
+extern "C" int fr_solver_run(fr_ctx *c) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (!c->initialised) return fail(FR_ERR_STATE, "fr_solver_run before fr_solver_init");
+    int rc = reset_control(c);
+    if (rc) return rc;
+    const fr_params &p = c->prm;
+    c->run_start_buf = c->cur;  // ping-pong index at the start of the run
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    float alpha = p.alpha;
+    float t = alpha;
+    for (int iter = 1; iter <= p.max_iter; ++iter) {
+        if ((rc = launch_iteration(c, iter, t, verbose_now(p, iter)))) return rc;
+        // the t schedule of solver.cu:428-432
+        t += alpha;
+        if (p.reset_iter > 0 && (iter + 1) % p.reset_iter == 0) t = p.alpha;
+    }
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    c->have_run = true;
+    c->have_final = false;
+    c->stats = fr_stats{};
+    return FR_OK;
+}
+
+// collect the run's history and replay the reference's messages (solver.cu:330-426)
+static int collect_run(fr_ctx *c) {
+    const fr_params &p = c->prm;
+    int mi = p.max_iter > 0 ? p.max_iter : 1;
+    frk::Ctl ctl{};
+    std::vector<unsigned long long> mp(mi);
+    std::vector<double> e(3 * (size_t)mi);
+    HIP_TRY(hipMemcpyAsync(&ctl, c->ctl, sizeof ctl, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(mp.data(), c->maxpack, sizeof(unsigned long long) * mi, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(e.data(), c->energy, sizeof(double) * 3 * mi, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+
+    int done = p.max_iter, reason = 0;
+    if (ctl.stopped) {
+        done = ctl.stop_iter;
+        reason = ctl.stop_reason;
+    } else if (p.max_iter > 0) {
+        float m;
+        long long idx;
+        unpack_max(mp[p.max_iter - 1], &m, &idx);
+        if (m <= p.max_update_norm) reason = 1;
+    }
+    // the ping-pong index after `done` completed iterations
+    c->cur = c->run_start_buf ^ (done & 1);
+
+    c->h_max.assign(done, 0.f);
+    c->h_argmax.assign(done, 0);
+    for (int k = 0; k < done; ++k) unpack_max(mp[k], &c->h_max[k], &c->h_argmax[k]);
+    int ne = (reason == 2) ? done + 1 : done;
+    c->h_energy.assign(ne, std::numeric_limits<double>::infinity());
+    c->h_edata.assign(ne, 0.0);
+    c->h_ereg.assign(ne, 0.0);
+    for (int k = 0; k < ne && k < mi; ++k)
+        if (verbose_now(p, k + 1)) {
+            c->h_energy[k] = e[k];
+            c->h_edata[k] = e[mi + k];
+            c->h_ereg[k] = e[2 * mi + k];
+        }
+
+    std::string &L = c->log;
+    L.clear();
+    double last_e = std::numeric_limits<double>::infinity();
+    for (int k = 1; k <= ne && k <= p.max_iter; ++k) {
+        if (k == 1 || k % 100 == 0) logf_(L, "iter. no. %d\n", k);
+        bool vn = verbose_now(p, k);
+        if (vn) {
+            last_e = c->h_energy[k - 1];
+            logf_(L, "data energy + w_reg * reg energy = %g + %g * %g = %g\n", c->h_edata[k - 1], (double)p.w_reg,
+                  c->h_ereg[k - 1], last_e);
+        }
+        if (reason == 2 && k == ne) break;
+        if (vn) {
+            long long am = c->h_argmax[k - 1];
+            long long XY = (long long)c->d.X * c->d.Y;
+            long long zz = am / XY, yy = (am - zz * XY) / c->d.X, xx = am - c->d.X * (yy + c->d.Y * zz);
+            logf_(L, "max. update norm %g at voxel (%lld, %lld, %lld)\n", (double)c->h_max[k - 1], xx, yy, zz);
+        }
+        if (reason == 1 && k == done) {
+            logf_(L, "SOLVER CONVERGED AFTER %d ITERATIONS\n", k);
+            break;
+        }
+        if (k == p.max_iter) {
+            logf_(L, "SOLVER REACHED MAX. NO. OF ITERATIONS WITHOUT CONVERGING\n");
+            break;
+        }
+    }
+    fr_stats &st = c->stats;
+    st.iterations = done;
+    st.stop_reason = reason;
+    st.last_max_update = done > 0 ? c->h_max[done - 1] : 0.f;
+    st.last_max_voxel = done > 0 ? c->h_argmax[done - 1] : 0;
+    st.last_energy = last_e;
+    st.solver_ms = ms;
+    st.min_det_true = st.min_det_ref = std::numeric_limits<double>::quiet_NaN();
+    st.dice_all = std::numeric_limits<double>::quiet_NaN();
+    c->have_run = false;
+    return FR_OK;
+}
+
+extern "C" int fr_solver_finalize(fr_ctx *c, float *img1, int16_t *lab1, float *img2, int16_t *lab2) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (!c->initialised) return fail(FR_ERR_STATE, "fr_solver_finalize before fr_solver_init");
+    int rc;
+    if (c->have_run && (rc = collect_run(c))) return rc;
+    const fr_params &p = c->prm;
+    // S o psi labels drive Dice; write them to the caller's buffer or a scratch one
+    int16_t *ls = lab2 ? lab2 : c->lab_tmp;
+    HIP_TRY(frk::launch_warp(c->d, c->S, c->LS, c->psi, 3, img2, ls, c->stream));
+    if (img1 || lab1) HIP_TRY(frk::launch_warp(c->d, c->T, c->LT, c->pinv[c->cur], 3, img1, lab1, c->stream));
+    HIP_TRY(hipMemsetAsync(c->dice_counts, 0, sizeof(unsigned) * (2 * (size_t)p.no_labels + 1), c->stream));
+    unsigned *ov = c->dice_counts, *vx = c->dice_counts + p.no_labels, *bad = c->dice_counts + 2 * p.no_labels;
+    HIP_TRY(frk::launch_dice(c->d, ls, c->LT, p.no_labels, ov, vx, bad, c->stream));
+    HIP_TRY(frk::launch_min_det(c->d, c->psi, c->det_parts, c->det_parts + c->ndet, c->stream));
+    std::vector<unsigned> cnt(2 * (size_t)p.no_labels + 1);
+    std::vector<float> dp(2 * (size_t)c->ndet);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), c->dice_counts, sizeof(unsigned) * cnt.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(dp.data(), c->det_parts, sizeof(float) * dp.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->h_overlap.assign(p.no_labels, 0);
+    c->h_voxels.assign(p.no_labels, 0);
+    for (int l = 0; l < p.no_labels; ++l) {
+        c->h_overlap[l] = cnt[l];
+        c->h_voxels[l] = cnt[p.no_labels + l];
+    }
+    c->h_bad = cnt[2 * p.no_labels];
+    float mt = INFINITY, mr = INFINITY;
+    for (int j = 0; j < c->ndet; ++j) {
+        mt = fminf(mt, dp[j]);
+        mr = fminf(mr, dp[c->ndet + j]);
+    }
+    c->stats.min_det_true = mt;
+    c->stats.min_det_ref = mr;
+    // Dice messages in the reference's float arithmetic (solver.cu:37-47)
+    std::string &L = c->log;
+    logf_(L, "POST-REGISTRATION\nDICE COEFFICIENT\n");
+    float num = 0.f, den = 0.f;
+    long long onum = 0, oden = 0;
+    for (int l = 1; l < p.no_labels; ++l) {
+        num += (float)c->h_overlap[l];
+        den += (float)c->h_voxels[l];
+        onum += c->h_overlap[l];
+        oden += c->h_voxels[l];
+        logf_(L, "label %d: %g\n", l, (double)(2.f * (float)c->h_overlap[l] / (float)c->h_voxels[l]));
+    }
+    logf_(L, "all labels: %g\n", (double)(2.f * num / den));
+    c->stats.dice_all = oden > 0 ? 2.0 * (double)onum / (double)oden : std::numeric_limits<double>::quiet_NaN();
+    c->have_final = true;
+    return FR_OK;
+}
+
+extern "C" int fr_register(fr_ctx *c, float *img1, int16_t *lab1, float *img2, int16_t *lab2) {
+    if (!c || !img1 || !lab1 || !img2 || !lab2) return fail(FR_ERR_INVALID, "null argument");
+    auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    if ((rc = fr_solver_init(c, img1, lab1, img2, lab2))) return rc;
+    if ((rc = fr_solver_run(c))) return rc;
+    if ((rc = fr_solver_finalize(c, img1, lab1, img2, lab2))) return rc;
+    auto t1 = std::chrono::steady_clock::now();
+    c->stats.total_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+    return FR_OK;
+}
+
+extern "C" int fr_get_stats(const fr_ctx *c0, fr_stats *st) {
+    fr_ctx *c = const_cast<fr_ctx *>(c0);
+    if (!c || !st) return fail(FR_ERR_INVALID, "null argument");
+    int rc;
+    if (c->have_run && (rc = collect_run(c))) return rc;
+    *st = c->stats;
+    return FR_OK;
+}
+
+extern "C" int fr_get_history(const fr_ctx *c0, int n, float *max_norm, double *energy) {
+    fr_ctx *c = const_cast<fr_ctx *>(c0);
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    int rc;
+    if (c->have_run && (rc = collect_run(c))) return rc;
+    for (int k = 0; k < n; ++k) {
+        if (max_norm) max_norm[k] = k < (int)c->h_max.size() ? c->h_max[k] : 0.f;
+        if (energy) energy[k] = k < (int)c->h_energy.size() ? c->h_energy[k] : std::numeric_limits<double>::infinity();
+    }
+    return FR_OK;
+}
+
+extern "C" int fr_get_log(const fr_ctx *c0, char *buf, size_t len) {
+    fr_ctx *c = const_cast<fr_ctx *>(c0);
+    if (!c || !buf || !len) return fail(FR_ERR_INVALID, "null argument");
+    int rc;
+    if (c->have_run && (rc = collect_run(c))) return rc;
+    size_t m = c->log.size() < len - 1 ? c->log.size() : len - 1;
+    memcpy(buf, c->log.data(), m);
+    buf[m] = 0;
+    return FR_OK;
+}
+
+extern "C" int fr_get_dice(const fr_ctx *c, int n, double *dice, long long *overlap, long long *voxels) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (!c->have_final) return fail(FR_ERR_STATE, "no finalised registration");
+    for (int l = 0; l < n; ++l) {
+        bool in = l < (int)c->h_overlap.size();
+        long long o = in ? c->h_overlap[l] : 0, v = in ? c->h_voxels[l] : 0;
+        if (overlap) overlap[l] = o;
+        if (voxels) voxels[l] = v;
+        if (dice) dice[l] = v ? 2.0 * (double)o / (double)v : std::numeric_limits<double>::quiet_NaN();
+    }
+    return FR_OK;
+}
+
+extern "C" int fr_solver_set_state(fr_ctx *c, const float *v4, const float *rho, const float *psi4,
+                                   const float *pinv4) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (!c->initialised) return fail(FR_ERR_STATE, "fr_solver_set_state before fr_solver_init");
+    int rc;
+    if (c->have_run && (rc = collect_run(c))) return rc;
+    size_t n = (size_t)c->n;
+    if (v4) HIP_TRY(frk::launch_pack4to3(v4, c->v[0], c->v[1], c->n, c->stream));
+    if (rho) {
+        HIP_TRY(hipMemcpyAsync(c->rho[0], rho, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->rho[1], rho, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (psi4) HIP_TRY(frk::launch_pack4to3(psi4, c->psi, nullptr, c->n, c->stream));
+    if (pinv4) HIP_TRY(frk::launch_pack4to3(pinv4, c->pinv[0], c->pinv[1], c->n, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FR_OK;
+}
+
+extern "C" int fr_solver_get_state(fr_ctx *c, float *v4, float *rho, float *psi4, float *pinv4) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (!c->initialised) return fail(FR_ERR_STATE, "fr_solver_get_state before fr_solver_init");
+    int rc;
+    if (c->have_run && (rc = collect_run(c))) return rc;
+    if (v4) HIP_TRY(frk::launch_unpack3to4(c->v[c->cur], v4, c->n, c->stream));
+    if (rho) HIP_TRY(hipMemcpyAsync(rho, c->rho[c->cur], (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    if (psi4) HIP_TRY(frk::launch_unpack3to4(c->psi, psi4, c->n, c->stream));
+    if (pinv4) HIP_TRY(frk::launch_unpack3to4(c->pinv[c->cur], pinv4, c->n, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FR_OK;
+}
+
+extern "C" int fr_get_deformation(fr_ctx *c, float *d_psi4) {
+    if (!c || !d_psi4) return fail(FR_ERR_INVALID, "null argument");
+    return fr_solver_get_state(c, nullptr, nullptr, d_psi4, nullptr);
+}
+
+extern "C" int fr_get_inverse_deformation(fr_ctx *c, float *d_pinv4) {
+    if (!c || !d_pinv4) return fail(FR_ERR_INVALID, "null argument");
+    return fr_solver_get_state(c, nullptr, nullptr, nullptr, d_pinv4);
+}
+
+extern "C" int fr_solver_energy(fr_ctx *c, double *e_data, double *e_reg) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (!c->initialised) return fail(FR_ERR_STATE, "fr_solver_energy before fr_solver_init");
+    int rc;
+    if (c->have_run && (rc = collect_run(c))) return rc;
+    double *tmp = nullptr;
+    if ((rc = dalloc(&tmp, 3))) return rc;
+    hipError_t e = frk::launch_energy(c->d, c->S, c->T, c->psi, c->partials, c->nparts, c->stream);
+    if (e == hipSuccess)
+        e = frk::launch_energy_finalize(c->partials, c->nparts, (double)c->prm.w_reg, tmp + 1, tmp + 2, tmp, c->stream);
+    double h[3] = {0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, tmp, sizeof h, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(FR_ERR_HIP, "energy: %s", hipGetErrorString(e));
+    if (e_data) *e_data = h[1];
+    if (e_reg) *e_reg = h[2];
+    return FR_OK;
+}
+
+extern "C" int fr_warp(int X, int Y, int Z, const float *img, const int16_t *lab, const float *psi4, float *out_img,
+                       int16_t *out_lab, void *stream) {
+    if (X < 1 || Y < 1 || Z < 1 || !psi4 || (!out_img && !out_lab) || (out_img && !img) || (out_lab && !lab))
+        return fail(FR_ERR_INVALID, "bad fr_warp arguments");
+    if (3LL * X * Y * Z >= (1LL << 31)) return fail(FR_ERR_INVALID, "volume too large");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(frk::launch_warp(Dims{X, Y, Z}, img, lab, psi4, 4, out_img, out_lab, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return FR_OK;
+}
+
+extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_a, float *ms_b) {
+    if (!c || reps < 1) return fail(FR_ERR_INVALID, "bad arguments");
+    if (!c->initialised) return fail(FR_ERR_STATE, "fr_time_kernels before fr_solver_init");
+    int rc;
+    if (c->have_run && (rc = collect_run(c))) return rc;
+    if ((rc = reset_control(c))) return rc;
+    float *psi_save = nullptr;
+    if ((rc = dalloc(&psi_save, 3 * (size_t)c->n))) return rc;
+    frk::IterArgs a;
+    iter_args(c, a);
+    a.iter = 1;
+    vel_scalars(c->prm, c->prm.alpha, &a.m1t, &a.m2);
+    hipError_t e = hipMemcpyAsync(psi_save, c->psi, 3 * (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+    float ta = 0.f, tb = 0.f;
+    // pass A reads the current buffers and writes the other ones: repeating it is idempotent
+    if (e == hipSuccess) e = frk::launch_pass_a(a, c->stream);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = frk::launch_pass_a(a, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ta, c->ev0, c->ev1);
+    // pass B updates psi in place; psi is restored afterwards
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = frk::launch_pass_b(a, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&tb, c->ev0, c->ev1);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->psi, psi_save, 3 * (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(psi_save);
+    if (e != hipSuccess) return fail(FR_ERR_HIP, "fr_time_kernels: %s", hipGetErrorString(e));
+    if (ms_a) *ms_a = ta / reps;
+    if (ms_b) *ms_b = tb / reps;
+    return FR_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// params.ini (src/apps/demo.cpp:96-149)
+// ---------------------------------------------------------------------------------------------
+
+static std::string trim(const std::string &s) {
+    size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+    return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+static bool parse_int(const std::string &s, int *out) {
+    char *end = nullptr;
+    errno = 0;
+    long v = strtol(s.c_str(), &end, 10);
+    if (s.empty() || *end || errno || v < INT32_MIN || v > INT32_MAX) return false;
+    *out = (int)v;
+    return true;
+}
+
+static bool parse_float(const std::string &s, float *out) {
+    char *end = nullptr;
+    errno = 0;
+    float v = strtof(s.c_str(), &end);
+    if (s.empty() || *end) return false;
+    *out = v;
+    return true;
+}
+
+extern "C" int fr_params_load_ini(const char *path, fr_params *p) {
+    if (!path || !p) return fail(FR_ERR_INVALID, "null argument");
+    FILE *f = fopen(path, "r");
+    if (!f) return fail(FR_ERR_IO, "cannot open %s", path);
+    char line[4096];
+    int lineno = 0;
+    fr_params q = *p;
+    while (fgets(line, sizeof line, f)) {
+        ++lineno;
+        std::string s = trim(line);
+        if (s.empty() || s[0] == '#' || s[0] == ';') continue;
+        size_t eq = s.find('=');
+        if (eq == std::string::npos) {
+            fclose(f);
+            return fail(FR_ERR_PARSE, "%s:%d: expected KEY=VALUE", path, lineno);
+        }
+        std::string key = trim(s.substr(0, eq)), val = trim(s.substr(eq + 1));
+        size_t hash = val.find('#');
+        if (hash != std::string::npos) val = trim(val.substr(0, hash));
+        bool ok = true;
+        if (key == "VOL_DIMS_X") ok = parse_int(val, &q.vol_dims[0]);
+        else if (key == "VOL_DIMS_Y") ok = parse_int(val, &q.vol_dims[1]);
+        else if (key == "VOL_DIMS_Z") ok = parse_int(val, &q.vol_dims[2]);
+        else if (key == "VOL_SIZE_X") ok = parse_float(val, &q.vol_size[0]);
+        else if (key == "VOL_SIZE_Y") ok = parse_float(val, &q.vol_size[1]);
+        else if (key == "VOL_SIZE_Z") ok = parse_float(val, &q.vol_size[2]);
+        else if (key == "NO_LABELS") ok = parse_int(val, &q.no_labels);
+        else if (key == "MAX_ITER") ok = parse_int(val, &q.max_iter);
+        else if (key == "MAX_UPDATE_NORM") ok = parse_float(val, &q.max_update_norm);
+        else if (key == "RHO_0") ok = parse_float(val, &q.rho_0);
+        else if (key == "RESET_ITER") ok = parse_int(val, &q.reset_iter);
+        else if (key == "TAU") ok = parse_float(val, &q.tau);
+        else if (key == "ALPHA") ok = parse_float(val, &q.alpha);
+        else if (key == "W_REG") ok = parse_float(val, &q.w_reg);
+        else if (key == "p") ok = parse_float(val, &q.p);
+        else if (key == "C") ok = parse_float(val, &q.C);
+        else {
+            fclose(f);
+            return fail(FR_ERR_PARSE, "%s:%d: unrecognised option '%s'", path, lineno, key.c_str());
+        }
+        if (!ok) {
+            fclose(f);
+            return fail(FR_ERR_PARSE, "%s:%d: invalid value '%s' for %s", path, lineno, val.c_str(), key.c_str());
+        }
+    }
+    fclose(f);
+    *p = q;
+    return FR_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// device memory plumbing
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int fr_dev_malloc(void **d, size_t bytes) {
+    if (!d) return fail(FR_ERR_INVALID, "null argument");
+    hipError_t e = hipMalloc(d, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(FR_ERR_ALLOC, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return FR_OK;
+}
+extern "C" int fr_dev_free(void *d) {
+    HIP_TRY(hipFree(d));
+    return FR_OK;
+}
+extern "C" int fr_memcpy_htod(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return FR_OK;
+}
+extern "C" int fr_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return FR_OK;
+}
+extern "C" int fr_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return FR_OK;
+}
+extern "C" int fr_dev_memset(void *d, int value, size_t bytes) {
+    HIP_TRY(hipMemset(d, value, bytes));
+    return FR_OK;
+}
+extern "C" int fr_dev_sync(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return FR_OK;
+}
+extern "C" int fr_device_info(char *buf, size_t len) {
+    if (!buf || !len) return fail(FR_ERR_INVALID, "null argument");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t pr;
+    HIP_TRY(hipGetDeviceProperties(&pr, dev));
+    snprintf(buf, len, "%s (%s), %d CUs, %.1f GiB, %.0f MHz", pr.name, pr.gcnArchName, pr.multiProcessorCount,
+             pr.totalGlobalMem / 1073741824.0, pr.clockRate / 1000.0);
+    return FR_OK;
+}

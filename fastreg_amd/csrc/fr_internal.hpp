@@ -1,0 +1,67 @@
+// Internal declarations shared by the HIP kernels (fr_kernels.hip) and the C-ABI (fr_capi.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace frk {
+
+// Grid dimensions; index = x + X*(y + Y*z).  3*X*Y*Z must fit in int (checked at create).
+struct Dims {
+    int X, Y, Z;
+};
+
+// Device-side control words of one run of the iteration loop.
+struct Ctl {
+    int stopped;      // set by the first launch that sees a stop condition
+    int stop_iter;    // iterations completed when stopped
+    int stop_reason;  // 1 = converged, 2 = NaN energy
+    int pad;
+};
+
+// Tile of the iteration kernels: 64 x 4 voxels of one z-slice per 256-thread block.
+constexpr int TILE_X = 64;
+constexpr int TILE_Y = 4;
+constexpr int BLOCK = TILE_X * TILE_Y;
+
+struct IterArgs {
+    Dims d;
+    int nbx, nby, nblocks;       // tile grid
+    const float *rho_o;          // density, read
+    float *rho_n;                // density, written
+    const float *v_o;            // velocity xyz-packed, read
+    float *v_n;                  // velocity xyz-packed, written
+    const float *pinv_o;         // psi_inv xyz-packed, read
+    float *pinv_n;               // psi_inv xyz-packed, written
+    float *psi;                  // psi xyz-packed (pass B updates in place)
+    const float *S, *T;          // source / target intensity
+    float alpha, m1t, m2, tau, w_reg, thr;
+    int iter;                    // 1-based iteration number
+    int check_energy;            // stop on NaN energy of this iteration
+    Ctl *ctl;
+    unsigned long long *maxpack; // per-iteration packed (norm bits, ~index) maxima
+    const double *energy;        // per-iteration energy
+};
+
+// launchers (all asynchronous on `s`)
+hipError_t launch_init(Dims d, const float *tgt, const int16_t *tgt_lab, const float *src, const int16_t *src_lab,
+                       float *T, int16_t *LT, float *S, int16_t *LS, float *rho0, float *rho1, float *v0, float *v1,
+                       float *pinv0, float *pinv1, float *psi, float rho_0, hipStream_t s);
+hipError_t launch_pass_a(const IterArgs &a, hipStream_t s);
+hipError_t launch_pass_b(const IterArgs &a, hipStream_t s);
+hipError_t launch_energy(Dims d, const float *S, const float *T, const float *psi, double *partials, int nparts,
+                         hipStream_t s);
+hipError_t launch_energy_finalize(const double *partials, int nparts, double w_reg, double *e_data, double *e_reg,
+                                  double *energy, hipStream_t s);
+int energy_partials_count(Dims d);
+// warp: def has `stride` floats per voxel (3 = packed xyz, 4 = float4)
+hipError_t launch_warp(Dims d, const float *img, const int16_t *lab, const float *def, int stride, float *out_img,
+                       int16_t *out_lab, hipStream_t s);
+hipError_t launch_dice(Dims d, const int16_t *la, const int16_t *lb, int no_labels, unsigned *overlap, unsigned *voxels,
+                       unsigned *bad, hipStream_t s);
+int min_det_partials_count(Dims d);
+hipError_t launch_min_det(Dims d, const float *psi, float *part_true, float *part_ref, hipStream_t s);
+hipError_t launch_pack4to3(const float *src4, float *dst3a, float *dst3b, int n, hipStream_t s);
+hipError_t launch_unpack3to4(const float *src3, float *dst4, int n, hipStream_t s);
+
+}  // namespace frk

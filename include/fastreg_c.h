@@ -1,0 +1,179 @@
+/*
+ * fastreg_c.h — C-ABI of the MI355X (gfx950) fastreg registration solver.
+ *
+ * This is the drop-in boundary for fastreg's optimisation loop.  The reference exposes it as C++
+ * classes (include/fastreg.hpp:12-38, include/solver.hpp:64-113); here every entry point is
+ * plain C (pointers and sizes, no HIP or C++ types), and the C++ classes in fastreg.hpp /
+ * solver.hpp of this repository are thin wrappers over it.  Device pointers are HIP device
+ * allocations of the caller (or from fr_dev_malloc); volumes are x-fastest,
+ * index = x + X*(y + Y*z).
+ *
+ * Every function returns 0 (FR_OK) on success or an FR_ERR_* code; fr_last_error() gives the
+ * message of the calling thread's last failure.  Nothing here calls exit() (the reference's
+ * cudaSafeCall does, include/safe_call.hpp:14-26).
+ */
+#ifndef FASTREG_C_H
+#define FASTREG_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FR_OK 0
+#define FR_ERR_INVALID 1 /* bad argument, dims or parameter */
+#define FR_ERR_HIP 2     /* a HIP runtime call failed */
+#define FR_ERR_ALLOC 3   /* device or host allocation failed */
+#define FR_ERR_IO 4      /* file could not be read or written */
+#define FR_ERR_PARSE 5   /* params.ini syntax error or unknown key */
+#define FR_ERR_STATE 6   /* call out of order (e.g. step before init) */
+
+/* Solver parameters.  Field meanings and defaults follow the reference's Params
+ * (include/params.hpp:7-26) and params.ini keys (src/apps/demo.cpp:96-149, params.ini:1-25). */
+typedef struct fr_params {
+    int vol_dims[3];       /* VOL_DIMS_X, VOL_DIMS_Y, VOL_DIMS_Z */
+    float vol_size[3];     /* VOL_SIZE_X/Y/Z (read and kept; the solver uses unit voxel spacing,
+                              as the reference does) */
+    int no_labels;         /* NO_LABELS: label values 0..no_labels-1 are counted in Dice */
+    int max_iter;          /* MAX_ITER */
+    float max_update_norm; /* MAX_UPDATE_NORM: stop when the max update norm is <= this */
+    float rho_0;           /* RHO_0 */
+    float tau;             /* TAU */
+    float alpha;           /* ALPHA */
+    float w_reg;           /* W_REG */
+    float p;               /* p */
+    float C;               /* C */
+    int reset_iter;        /* RESET_ITER (default -1: never reset t) */
+    int verbosity;         /* 0, 1 (--verbose) or 2 (--vverbose) */
+} fr_params;
+
+/* Statistics of the last fr_register / fr_solver_run. */
+typedef struct fr_stats {
+    int iterations;            /* iterations whose PDE updates ran */
+    int stop_reason;           /* 0 = reached MAX_ITER, 1 = converged, 2 = NaN energy */
+    float last_max_update;     /* max update norm of the last iteration that ran */
+    long long last_max_voxel;  /* its linear voxel index */
+    double last_energy;        /* last evaluated energy (verbosity > 0), else +inf */
+    double min_det_true;       /* min over the interior of det(D psi) (central differences) */
+    double min_det_ref;        /* the same with the reference's det() formula */
+    double dice_all;           /* Dice over labels 1..no_labels-1, the reference's convention */
+    float solver_ms;           /* HIP-event time of the iteration loop */
+    float total_ms;            /* host wall time of fr_register */
+} fr_stats;
+
+typedef struct fr_ctx fr_ctx;
+
+/* message of the calling thread's last failed call ("" if none) */
+const char *fr_last_error(void);
+
+/* ---- parameters -------------------------------------------------------------------------- */
+
+/* The reference's params.ini defaults (params.ini:1-25) plus reset_iter = -1, verbosity = 0
+ * (include/params.hpp:11-12). */
+int fr_params_default(fr_params *p);
+
+/* Parse KEY=VALUE lines ('#' comments, blank lines) with the reference's key set
+ * (src/apps/demo.cpp:96-149).  Unknown keys and malformed values fail with FR_ERR_PARSE, as the
+ * reference's Boost parser does.  Keys absent from the file keep the values already in *p. */
+int fr_params_load_ini(const char *path, fr_params *p);
+
+/* ---- pipeline: the FastReg class (include/fastreg.hpp:12-38, src/fastreg/fastreg.cpp) ------ */
+
+/* Allocate all solver state on the current HIP device for p->vol_dims.  FastReg::FastReg
+ * (src/fastreg/fastreg.cpp:24-46) + Solver::Solver (src/fastreg/solver.cpp:7-65). */
+int fr_create(const fr_params *p, fr_ctx **out);
+int fr_destroy(fr_ctx *ctx);
+int fr_get_params(const fr_ctx *ctx, fr_params *p);
+
+/* Run work on `hip_stream` (a hipStream_t) instead of the context's own stream; NULL restores
+ * the context's own stream. */
+int fr_set_stream(fr_ctx *ctx, void *hip_stream);
+
+/* FastReg::operator() (src/fastreg/fastreg.cpp:58-75): register the source (img2, lab2) to the
+ * target (img1, lab1), all device pointers of no_voxels elements, and overwrite them in place:
+ * img1/lab1 <- target o psi_inv, img2/lab2 <- source o psi.  Every call starts from a fresh
+ * state (psi = psi_inv = id, v = 0, rho = rho_0).  Blocks until the outputs are written. */
+int fr_register(fr_ctx *ctx, float *d_img1, int16_t *d_lab1, float *d_img2, int16_t *d_lab2);
+
+/* FastReg::getDeformationField (include/fastreg.hpp:22): psi as float4 (x, y, z, 0) per voxel,
+ * absolute voxel coordinates, into device memory of 4*no_voxels floats. */
+int fr_get_deformation(fr_ctx *ctx, float *d_psi_xyzw);
+/* psi_inv in the same layout */
+int fr_get_inverse_deformation(fr_ctx *ctx, float *d_psi_inv_xyzw);
+
+int fr_get_stats(const fr_ctx *ctx, fr_stats *st);
+
+/* Dice of the last registration (src/fastreg/cuda/solver.cu:9-50): per-label 2*overlap/voxels
+ * for labels 0..n-1 (NaN when a label is absent) and the raw counts; any pointer may be NULL. */
+int fr_get_dice(const fr_ctx *ctx, int n, double *dice, long long *overlap, long long *voxels);
+
+/* Per-iteration history of the last run: max update norm and energy (+inf where the energy
+ * was not evaluated), n entries from iteration 1. */
+int fr_get_history(const fr_ctx *ctx, int n, float *max_norm, double *energy);
+
+/* The reference's stdout messages for the last run (src/fastreg/cuda/solver.cu:330-447,
+ * src/fastreg/reductor.cpp:56-59), NUL-terminated, truncated to len. */
+int fr_get_log(const fr_ctx *ctx, char *buf, size_t len);
+
+/* ---- solver: Solver::estimate_psi (src/fastreg/cuda/solver.cu:295-444) in pieces ---------- */
+
+/* Fresh state from the four device inputs (copied; the inputs are not modified):
+ * FastReg::operator()'s initialise calls (src/fastreg/fastreg.cpp:59-63) plus estimate_psi's
+ * pre-loop (solver.cu:308,317). */
+int fr_solver_init(fr_ctx *ctx, const float *d_tgt, const int16_t *d_tgt_lab, const float *d_src,
+                   const int16_t *d_src_lab);
+
+/* One iteration body (solver.cu:335-407) with the given t; waits and returns the iteration's max
+ * update norm and its voxel (either pointer may be NULL). */
+int fr_solver_step(fr_ctx *ctx, float t, float *max_norm, long long *max_voxel);
+
+/* The iteration loop (solver.cu:327-433) for MAX_ITER iterations from the current state with the
+ * reference's t schedule, stopping rules and messages; asynchronous on the context stream
+ * (fr_get_stats / fr_get_history / fr_get_log synchronise). */
+int fr_solver_run(fr_ctx *ctx);
+
+/* Final warps and figures (solver.cu:435-443, fastreg.cpp:71-72): img1/lab1 <- target o psi_inv,
+ * img2/lab2 <- source o psi, Dice, min det J.  Any output pointer may be NULL. */
+int fr_solver_finalize(fr_ctx *ctx, float *d_img1, int16_t *d_lab1, float *d_img2, int16_t *d_lab2);
+
+/* State in the reference's layouts: v, psi, psi_inv float4 (x, y, z, 0) per voxel, rho float.
+ * NULL skips a field.  set_state also sets the boundary voxels from the given arrays. */
+int fr_solver_set_state(fr_ctx *ctx, const float *d_v4, const float *d_rho, const float *d_psi4,
+                        const float *d_psi_inv4);
+int fr_solver_get_state(fr_ctx *ctx, float *d_v4, float *d_rho, float *d_psi4, float *d_psi_inv4);
+
+/* Energy of the current state (src/fastreg/reductor.cpp:45-62 with solver.cu:357-363):
+ * e_data = 0.5 sum (T - S o psi)^2, e_reg = 0.5 sum ||D(psi - id)||_F^2, summed in FP64. */
+int fr_solver_energy(fr_ctx *ctx, double *e_data, double *e_reg);
+
+/* ---- kernels ----------------------------------------------------------------------------- */
+
+/* apply (src/fastreg/cuda/vector_fields.cu:81-107): out_img = trilinear img at psi(p), out_lab =
+ * lab at the floor corner of clamp(psi(p)), all voxels; psi float4 per voxel.  Either label
+ * pointer may be NULL.  hip_stream may be NULL (default stream). */
+int fr_warp(int X, int Y, int Z, const float *d_img, const int16_t *d_lab, const float *d_psi4, float *d_out_img,
+            int16_t *d_out_lab, void *hip_stream);
+
+/* Average device time per launch of the solver's two iteration kernels, from HIP events around
+ * `reps` launches each on the context stream (the state is restored afterwards). */
+int fr_time_kernels(fr_ctx *ctx, int reps, float *ms_pass_a, float *ms_pass_b);
+
+/* ---- device memory plumbing (for callers without a HIP binding of their own) -------------- */
+
+int fr_dev_malloc(void **d_ptr, size_t bytes);
+int fr_dev_free(void *d_ptr);
+int fr_memcpy_htod(void *d_dst, const void *h_src, size_t bytes);
+int fr_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes);
+int fr_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes);
+int fr_dev_memset(void *d_ptr, int value, size_t bytes);
+int fr_dev_sync(void);
+/* name, arch, CU count and memory of the current device, NUL-terminated */
+int fr_device_info(char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTREG_C_H */

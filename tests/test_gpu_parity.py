@@ -1,0 +1,214 @@
+"""GPU parity tests: the HIP solver (through the C-ABI) against the oracle.
+
+Bit-exact is the bar throughout: the kernels and the oracle perform the same IEEE operations
+in the same order (no contraction, fma only where the reference has one, correctly rounded
+division and sqrt), so every float field, label and max-norm must be identical.  Energies are
+FP64 sums in different orders and are compared with rtol 1e-12.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import fastreg_amd as fr
+from fastreg_amd import phantoms
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _pair_dims(pair):
+    return pair.dims
+
+
+def _params(dims, **kw):
+    base = dict(vol_dims=dims, no_labels=9, max_iter=10, verbosity=0)
+    base.update(kw)
+    return fr.default_params(**base)
+
+
+def test_device_is_gfx950():
+    info = fr.device_info()
+    say("device:", info)
+    assert "gfx950" in info
+
+
+@pytest.mark.parametrize("dims,seed", [((23, 19, 17), 0), ((64, 5, 9), 1), ((7, 33, 12), 2)])
+def test_warp_bit_exact(oracle_mod, dims, seed):
+    say("warp", dims, seed)
+    O = oracle_mod
+    X, Y, Z = dims
+    n = X * Y * Z
+    rng = np.random.default_rng(seed)
+    img = rng.random(n).astype(np.float32)
+    lab = rng.integers(0, 50, n).astype(np.int16)
+    zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    psi = np.zeros((n, 4), np.float32)
+    psi[:, 0] = xx.ravel() + rng.normal(0, 3, n)
+    psi[:, 1] = yy.ravel() + rng.normal(0, 3, n)
+    psi[:, 2] = zz.ravel() + rng.normal(0, 3, n)
+    psi[::7, :3] = np.round(psi[::7, :3])          # exact integers hit the weight-0 neighbour
+    psi[::11, 0] = X - 1                           # clamp edge
+    psi[::13, 1] = -5.0                            # below the grid
+    g_img, g_lab = fr.warp(img, lab, psi, dims)
+    phi2 = np.stack([img, lab.astype(np.float32)], 1)
+    o = O.apply(dims, phi2, psi)
+    assert np.array_equal(g_img, o[:, 0])
+    assert np.array_equal(g_lab, o[:, 1].astype(np.int16))
+
+
+def _both(dims, O, seed, src_pair=None, **kw):
+    rng = np.random.default_rng(seed)
+    X, Y, Z = dims
+    if src_pair is None:
+        pair = phantoms.blobs(shape=(Z, Y, X), seed=seed)
+    else:
+        pair = src_pair
+    p = _params(dims, no_labels=pair.no_labels, **kw)
+    g = fr.Solver(p)
+    g.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+    o = O.Oracle(dims, "f32")
+    o.init(pair.target, pair.target_labels, pair.source, pair.source_labels, p.rho_0)
+    return g, o, pair, rng
+
+
+def _assert_state_equal(gs, os_, what=""):
+    for k in ("v", "rho", "psi", "psi_inv"):
+        a, b = gs[k], os_[k]
+        if not np.array_equal(a, b):
+            bad = np.flatnonzero((a != b).reshape(len(a), -1).any(-1) if a.ndim > 1 else (a != b))
+            raise AssertionError(f"{what} field {k}: {len(bad)} voxels differ, first {bad[:5]}: "
+                                 f"gpu {a[bad[:3]]} oracle {b[bad[:3]]}")
+
+
+@pytest.mark.parametrize("dims,seed", [((24, 20, 18), 3), ((67, 9, 11), 4), ((12, 31, 8), 5)])
+def test_step_teacher_forced_bit_exact(oracle_mod, dims, seed):
+    """From an identical random (smooth, large-amplitude) state, one iteration on the GPU equals
+    one oracle iteration in every field and in the max update norm."""
+    say("step", dims, seed)
+    O = oracle_mod
+    g, o, pair, rng = _both(dims, O, seed)
+    st = phantoms.random_state(dims, rng, amp=1.5)
+    g.set_state(**st)
+    o.set_state(**st)
+    for k, t in enumerate([0.004, 0.008, 0.5]):
+        mg, ag = g.step(t)
+        mo, ao = o.step(t)
+        _assert_state_equal(g.get_state(), o.get_state(), f"step {k}")
+        assert mg == mo and ag == ao, (mg, mo, ag, ao)
+
+
+def test_run_bit_exact_with_log(oracle_mod):
+    """A 25-iteration verbose run: state, max-norm history, log lines and Dice equal the oracle."""
+    O = oracle_mod
+    dims = (32, 28, 24)
+    say("run", dims)
+    g, o, pair, rng = _both(dims, O, 7, max_iter=25, verbosity=2, alpha=0.4)
+    g.run()
+    prm = O.SolverParams(max_iter=25, verbosity=2, no_labels=pair.no_labels, alpha=0.4)
+    r = o.run(prm)
+    st = g.stats()
+    assert st.iterations == r["iterations"] == 25
+    mh, eh = g.history()
+    assert np.array_equal(mh, r["max_hist"])
+    np.testing.assert_allclose(eh, r["energy_hist"][:25], rtol=1e-12)
+    _assert_state_equal(g.get_state(), o.get_state(), "run")
+    outs = g.finalize()
+    oo = o.outputs()
+    for a, b, nm in zip(outs, oo, ("img1", "lab1", "img2", "lab2")):
+        assert np.array_equal(a, b), nm
+    dice, ov, vx = g.dice()
+    oov, ovx, _ = o.dice_counts(pair.no_labels)
+    assert np.array_equal(ov, oov) and np.array_equal(vx, ovx)
+    mt, mr = o.min_det()
+    st = g.stats()
+    assert st.min_det_true == mt and st.min_det_ref == mr
+
+
+def test_register_matches_oracle_c1_shape(oracle_mod):
+    """fr_register on the 64^3 blob phantom (40 iterations): in-place outputs, Dice and
+    min det J equal the oracle's."""
+    O = oracle_mod
+    pair = phantoms.blobs()
+    dims = pair.dims
+    say("register", dims)
+    p = _params(dims, no_labels=pair.no_labels, max_iter=40)
+    g = fr.Solver(p)
+    outs = g.register(pair.target, pair.target_labels, pair.source, pair.source_labels)
+    o = O.Oracle(dims, "f32")
+    o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+    o.run(O.SolverParams(max_iter=40, no_labels=pair.no_labels))
+    for a, b, nm in zip(outs, o.outputs(), ("img1", "lab1", "img2", "lab2")):
+        assert np.array_equal(a.reshape(-1), b), nm
+    _, ov, vx = g.dice()
+    oov, ovx, _ = o.dice_counts(pair.no_labels)
+    assert np.array_equal(ov, oov) and np.array_equal(vx, ovx)
+    st = g.stats()
+    assert (st.min_det_true, st.min_det_ref) == o.min_det()
+    assert st.iterations == 40 and st.stop_reason == 0
+
+
+def test_energy_matches_oracle(oracle_mod):
+    O = oracle_mod
+    dims = (30, 26, 22)
+    g, o, pair, rng = _both(dims, O, 9)
+    st = phantoms.random_state(dims, rng, amp=2.0)
+    g.set_state(**st)
+    o.set_state(**st)
+    ge = g.energy()
+    oe = o.energy()
+    np.testing.assert_allclose(ge, oe, rtol=1e-12)
+
+
+def test_stop_rules(oracle_mod):
+    O = oracle_mod
+    dims = (20, 18, 16)
+    # converged after the first iteration (MAX_UPDATE_NORM huge)
+    g, o, pair, _ = _both(dims, O, 11, max_iter=50, max_update_norm=1e30)
+    g.run()
+    st = g.stats()
+    r = o.run(O.SolverParams(max_iter=50, max_update_norm=1e30, no_labels=pair.no_labels))
+    assert st.iterations == r["iterations"] == 1 and st.stop_reason == r["stop_reason"] == 1
+    assert "SOLVER CONVERGED AFTER 1 ITERATIONS" in g.log()
+    _assert_state_equal(g.get_state(), o.get_state(), "converged")
+    # RESET_ITER t schedule
+    g, o, pair, _ = _both(dims, O, 12, max_iter=9, reset_iter=3, alpha=0.3)
+    g.run()
+    o.run(O.SolverParams(max_iter=9, reset_iter=3, alpha=0.3, no_labels=pair.no_labels))
+    _assert_state_equal(g.get_state(), o.get_state(), "reset_iter")
+    # NaN energy stops before the first update (verbosity 1)
+    X, Y, Z = dims
+    tgt = pair.target.copy()
+    tgt.reshape(-1)[5] = np.nan
+    p = _params(dims, no_labels=pair.no_labels, max_iter=5, verbosity=1)
+    g = fr.Solver(p)
+    g.init(tgt, pair.target_labels, pair.source, pair.source_labels)
+    g.run()
+    st = g.stats()
+    assert st.iterations == 0 and st.stop_reason == 2
+    o = O.Oracle(dims, "f32")
+    o.init(tgt, pair.target_labels, pair.source, pair.source_labels)
+    r = o.run(O.SolverParams(max_iter=5, verbosity=1, no_labels=pair.no_labels))
+    assert r["iterations"] == 0 and r["stop_reason"] == 2
+
+
+def test_full_size_brain_three_iterations(oracle_mod):
+    """At the full 182x218x182 size: three GPU iterations equal three oracle iterations."""
+    O = oracle_mod
+    pair = phantoms.brain()
+    dims = pair.dims
+    say("brain", dims)
+    p = _params(dims, no_labels=pair.no_labels, max_iter=3, alpha=0.5)
+    g = fr.Solver(p)
+    g.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+    g.run()
+    o = O.Oracle(dims, "f32")
+    o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+    r = o.run(O.SolverParams(max_iter=3, alpha=0.5, no_labels=pair.no_labels))
+    mh, _ = g.history()
+    assert np.array_equal(mh, r["max_hist"])
+    _assert_state_equal(g.get_state(), o.get_state(), "brain")

@@ -1,0 +1,551 @@
+"""Known-answer tests that pin the oracle (CPU only).
+
+The reference ships no tests, golden vectors or fixtures, and cannot be built here (CUDA with
+Boost/OpenCV/ITK), so each oracle function is pinned by closed forms, invariants, special
+cases that reduce to a library routine (scipy.ndimage.map_coordinates, numpy.gradient), and
+brute force on tiny inputs, chosen so that a dropped term, a wrong sign or index, or a
+transposed operand fails at least one of them.
+"""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+from fastreg_amd import phantoms
+
+
+def grid(dims):
+    X, Y, Z = dims
+    zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    return xx.ravel().astype(np.float64), yy.ravel().astype(np.float64), zz.ravel().astype(np.float64)
+
+
+def identity4(dims, dtype=np.float32):
+    x, y, z = grid(dims)
+    out = np.zeros((x.size, 4), dtype)
+    out[:, 0], out[:, 1], out[:, 2] = x, y, z
+    return out
+
+
+def interior_mask(dims, lo=1):
+    X, Y, Z = dims
+    x, y, z = grid(dims)
+    return (x >= lo) & (x <= X - 1 - lo) & (y >= lo) & (y <= Y - 1 - lo) & (z >= lo) & (z <= Z - 1 - lo)
+
+
+# ---------------------------------------------------------------------------------------------
+# scalar helpers
+# ---------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_lerp_endpoints(oracle_mod, prec):
+    O = oracle_mod
+    assert O.lerp(3.25, -7.5, 0.0, prec) == -7.5
+    assert O.lerp(3.25, -7.5, 1.0, prec) == 3.25
+    assert O.lerp(2.0, 4.0, 0.5, prec) == 3.0       # t*v0 + (1-t)*v1: weight t belongs to v0
+
+
+def test_det_ref_special_cases(oracle_mod):
+    O = oracle_mod
+    I = np.eye(4)
+    I[3, 3] = 0
+    assert O.det_ref(I) == 1.0
+    D = np.diag([2.0, 3.0, 5.0, 0.0])
+    assert O.det_ref(D) == 30.0 and O.det_true(D) == 30.0
+    # symmetric off-diagonal pair in rows/cols 1-2: true det 0.75; the reference's det() drops
+    # the -M00*M12*M21 term (cuda_utils.hpp:389-396) and gives 1
+    M = I.copy()
+    M[1, 2] = M[2, 1] = 0.5
+    assert O.det_true(M) == 0.75
+    assert O.det_ref(M) == 1.0
+    # with M20 = 2 as well: det_ref = 1 - (M02 M11 + M20)(M12 M21 M00) = 1 - 2*0.25 = 0.5,
+    # true det = 1*(1 - 0.25) - 0 + 0 = 0.75
+    M[2, 0] = 2.0
+    assert O.det_ref(M) == 0.5
+    assert O.det_true(M) == 0.75
+    # det_ref is not invariant under transposition: with M11 = 2, M12 = 0.5, M21 = 0.25, M20 = 2
+    # det_ref(M) = 2 - (0*2 + 2)(0.5*0.25) = 1.75 but det_ref(M^T) = 2 - (2*2 + 0)(0.25*0.5) = 1.5
+    N = I.copy()
+    N[1, 1], N[1, 2], N[2, 1], N[2, 0] = 2.0, 0.5, 0.25, 2.0
+    assert O.det_ref(N) == 1.75 and O.det_ref(N.T.copy()) == 1.5
+
+
+def test_norm_rounds_toward_zero(oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(0)
+    for _ in range(2000):
+        v = np.zeros(4, np.float32)
+        v[:3] = rng.normal(size=3).astype(np.float32) * np.float32(10.0 ** rng.integers(-3, 3))
+        s = np.float32(O.norm(v, "f32"))
+        q = np.float32(np.float32(v[0] * v[0] + v[1] * v[1]) + v[2] * v[2])   # the float32 sum of squares
+        # s is the largest float32 with s^2 <= q (exact in float64)
+        assert float(s) ** 2 <= float(q)
+        assert float(np.nextafter(s, np.float32(np.inf))) ** 2 > float(q)
+    assert O.norm(np.array([3, 4, 0, 0], np.float32)) == 5.0
+
+
+# ---------------------------------------------------------------------------------------------
+# warp (apply) and the interpolation helpers
+# ---------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_apply_identity_is_exact_copy(oracle_mod, prec):
+    O = oracle_mod
+    dims = (9, 7, 5)
+    n = 9 * 7 * 5
+    rng = np.random.default_rng(1)
+    phi = np.stack([rng.random(n), rng.integers(0, 9, n)], 1)
+    out = O.apply(dims, phi, identity4(dims), prec)
+    assert np.array_equal(out, phi.astype(out.dtype))
+
+
+def test_apply_integer_shift_is_clamped_shift(oracle_mod):
+    O = oracle_mod
+    dims = (10, 8, 6)
+    X, Y, Z = dims
+    n = X * Y * Z
+    rng = np.random.default_rng(2)
+    img = rng.random(n).astype(np.float32)
+    lab = rng.integers(0, 20, n).astype(np.float32)
+    sh = np.array([2, -3, 1])
+    psi = identity4(dims)
+    psi[:, :3] += sh
+    out = O.apply(dims, np.stack([img, lab], 1), psi)
+    x, y, z = grid(dims)
+    src = (np.clip(x + sh[0], 0, X - 1) + X * (np.clip(y + sh[1], 0, Y - 1) + Y * np.clip(z + sh[2], 0, Z - 1))).astype(int)
+    assert np.array_equal(out[:, 0], img[src])
+    assert np.array_equal(out[:, 1], lab[src])
+
+
+def test_apply_matches_map_coordinates_and_floor_labels(oracle_mod):
+    O = oracle_mod
+    dims = (13, 11, 9)
+    X, Y, Z = dims
+    n = X * Y * Z
+    rng = np.random.default_rng(3)
+    img = rng.random((Z, Y, X))
+    lab = rng.integers(0, 30, (Z, Y, X)).astype(np.float64)
+    psi = identity4(dims, np.float64)
+    psi[:, :3] += rng.normal(0, 2.5, (n, 3))
+    out = O.apply(dims, np.stack([img.ravel(), lab.ravel()], 1), psi, "f64")
+    ref = ndimage.map_coordinates(img, [psi[:, 2], psi[:, 1], psi[:, 0]], order=1, mode="nearest")
+    np.testing.assert_allclose(out[:, 0], ref, rtol=0, atol=1e-12)
+    # labels: the value at the floor of the clamped position (not the nearest voxel)
+    cx = np.floor(np.clip(psi[:, 0], 0, X - 1)).astype(int)
+    cy = np.floor(np.clip(psi[:, 1], 0, Y - 1)).astype(int)
+    cz = np.floor(np.clip(psi[:, 2], 0, Z - 1)).astype(int)
+    assert np.array_equal(out[:, 1], lab[cz, cy, cx])
+    rounded = ndimage.map_coordinates(lab, [psi[:, 2], psi[:, 1], psi[:, 0]], order=0, mode="nearest")
+    assert not np.array_equal(out[:, 1], rounded)   # round-to-nearest would be a different warp
+
+
+def test_interpolate_laplacian_matches_map_coordinates(oracle_mod):
+    O = oracle_mod
+    dims = (12, 10, 8)
+    X, Y, Z = dims
+    n = X * Y * Z
+    rng = np.random.default_rng(4)
+    L = np.zeros((n, 4))
+    L[:, :3] = rng.normal(size=(n, 3))
+    assert np.array_equal(O.interpolate_laplacian(dims, L, identity4(dims, np.float64), "f64"), L)
+    psi = identity4(dims, np.float64)
+    psi[:, :3] += rng.normal(0, 2.0, (n, 3))
+    out = O.interpolate_laplacian(dims, L, psi, "f64")
+    for c in range(3):
+        ref = ndimage.map_coordinates(L[:, c].reshape(Z, Y, X), [psi[:, 2], psi[:, 1], psi[:, 0]], order=1,
+                                      mode="nearest")
+        np.testing.assert_allclose(out[:, c], ref, atol=1e-12)
+    assert np.all(out[:, 3] == 0)
+
+
+# ---------------------------------------------------------------------------------------------
+# derivatives
+# ---------------------------------------------------------------------------------------------
+
+def test_volume_gradient_ramp_and_numpy(oracle_mod):
+    O = oracle_mod
+    dims = (9, 8, 7)
+    x, y, z = grid(dims)
+    S = 3 * x - 2 * y + 5 * z
+    g = O.volume_gradient(dims, np.stack([S, np.zeros_like(S)], 1))
+    m = interior_mask(dims)
+    assert np.array_equal(g[m, :3], np.tile([3.0, -2.0, 5.0], (m.sum(), 1)).astype(np.float32))
+    assert np.all(g[~m] == 0) and np.all(g[:, 3] == 0)
+    X, Y, Z = dims
+    rng = np.random.default_rng(5)
+    R = rng.random((Z, Y, X))
+    g = O.volume_gradient(dims, np.stack([R.ravel(), np.zeros(R.size)], 1), "f64")
+    gz, gy, gx = np.gradient(R)
+    np.testing.assert_allclose(g[m, 0], gx.ravel()[m], atol=1e-14)
+    np.testing.assert_allclose(g[m, 1], gy.ravel()[m], atol=1e-14)
+    np.testing.assert_allclose(g[m, 2], gz.ravel()[m], atol=1e-14)
+
+
+def test_jacobian_linear_map_rows_are_components(oracle_mod):
+    O = oracle_mod
+    dims = (7, 6, 5)
+    x, y, z = grid(dims)
+    A = np.array([[1.0, 2.0, 0.5], [-1.0, 3.0, 0.25], [4.0, -0.5, 2.0]])   # not symmetric
+    psi = np.zeros((x.size, 4))
+    psi[:, :3] = (A @ np.stack([x, y, z])).T
+    J = O.jacobian(dims, psi, "f64")
+    m = interior_mask(dims)
+    assert np.array_equal(J[m][:, :3, :3], np.broadcast_to(A, (m.sum(), 3, 3)))
+    assert np.all(J[~m] == 0) and np.all(J[:, 3, :] == 0) and np.all(J[:, :, 3] == 0)
+
+
+def test_deformation_jacobian_affine_with_mirrored_faces(oracle_mod):
+    O = oracle_mod
+    dims = (6, 5, 7)
+    X, Y, Z = dims
+    x, y, z = grid(dims)
+    A = np.array([[1.5, 0.25, 0.0], [0.5, 1.0, -0.25], [0.0, 0.75, 2.0]])
+    psi = np.zeros((x.size, 4))
+    psi[:, :3] = (A @ np.stack([x, y, z])).T + np.array([0.5, -1.0, 2.0])
+    J = O.deformation_jacobian(dims, psi, "f64")
+    D = A - np.eye(3)
+    for i in range(x.size):
+        exp = D.copy()
+        # on a face the neighbour is mirrored, so the central difference along that axis is 0
+        if x[i] in (0, X - 1): exp[:, 0] = 0
+        if y[i] in (0, Y - 1): exp[:, 1] = 0
+        if z[i] in (0, Z - 1): exp[:, 2] = 0
+        assert np.array_equal(J[i, :3, :3], exp), i
+    assert np.all(O.deformation_jacobian(dims, identity4(dims)) == 0)
+
+
+def test_laplacian_closed_forms(oracle_mod):
+    O = oracle_mod
+    dims = (8, 7, 6)
+    x, y, z = grid(dims)
+    assert np.all(O.laplacian(dims, identity4(dims)) == 0)
+    psi = np.zeros((x.size, 4))
+    psi[:, 0] = x * x + 3 * y * y + 5 * z * z     # Laplacian 18
+    psi[:, 1] = x * y                              # Laplacian 0
+    psi[:, 2] = -2 * z * z + y                     # Laplacian -4
+    sentinel = np.full((x.size, 4), 7.0)
+    L = O.laplacian(dims, psi, "f64", out=sentinel)
+    m = interior_mask(dims)
+    assert np.array_equal(L[m, :3], np.tile([-18.0, 0.0, 4.0], (m.sum(), 1)))
+    assert np.all(L[~m] == 7.0)                    # boundary voxels are not written
+
+
+def test_laplacian_velocity_reference_indices(oracle_mod):
+    O = oracle_mod
+    dims = (9, 10, 8)
+    X, Y, Z = dims
+    x, y, z = grid(dims)
+    # all components positive and linear: backward one-sided second differences vanish
+    v = np.zeros((x.size, 4))
+    v[:, 0] = 1 + 0.5 * x + 0.25 * y
+    v[:, 1] = 2 + 0.125 * z
+    v[:, 2] = 1 + x + y + z
+    out = O.laplacian_velocity(dims, v, "f64", out=np.full((x.size, 4), 9.0))
+    rng_mask = (x >= 2) & (x <= X - 3) & (y >= 2) & (y <= Y - 2) & (z >= 2) & (z <= Z - 3)
+    assert np.all(out[rng_mask, :3] == 0)
+    assert np.all(out[~rng_mask] == 9.0)          # written exactly on x in [2,X-3], y in [2,Y-2], z in [2,Z-3]
+    # vy < 0, linear in y with slope -0.5: the reference's forward branch reads y+2, y-1, y
+    # (not y+2, y+1, y), so vyy = 4 * slope = -2 and Delta v_y = +2; a textbook stencil gives 0
+    v = np.zeros((x.size, 4))
+    v[:, 0] = 1.0
+    v[:, 1] = -2 - 0.5 * y
+    v[:, 2] = 1.0
+    out = O.laplacian_velocity(dims, v, "f64")
+    inner = rng_mask & (y <= Y - 3)
+    assert np.all(out[inner, 1] == 2.0)
+    assert np.all(out[inner, 0] == 0) and np.all(out[inner, 2] == 0)
+    # at y = Y-2 the (y+2) neighbour is the linear address of (x, 0, z+1), where v_y = -2:
+    # vyy = -2 - 2*(-2 - 0.5*(Y-3)) + (-2 - 0.5*(Y-2)) = 0.5*Y - 2.5 + ... evaluated directly:
+    edge = rng_mask & (y == Y - 2)
+    vyy = -2.0 - 2.0 * (-2 - 0.5 * (Y - 3)) + (-2 - 0.5 * (Y - 2))
+    assert np.all(out[edge, 1] == -vyy)
+
+
+def test_dv_v_closed_forms(oracle_mod):
+    O = oracle_mod
+    dims = (8, 7, 6)
+    x, y, z = grid(dims)
+    m = interior_mask(dims)
+    v = np.zeros((x.size, 4))
+    v[:, :3] = [0.5, -0.25, 2.0]
+    assert np.all(O.dv_v(dims, v, "f64")[m] == 0)          # constant field
+    # entropy flux on the diagonal: u = x > 0 -> 0.5*(u^2 - u(x+1)^2) = -(x + 0.5)
+    v = np.zeros((x.size, 4))
+    v[:, 0] = x + 1
+    r = O.dv_v(dims, v, "f64")
+    np.testing.assert_array_equal(r[m, 0], 0.5 * ((x[m] + 1) ** 2 - (x[m] + 2) ** 2))
+    assert np.all(r[m, 1:] == 0)
+    # u < 0: 0.5*(-u^2 + u(x+1)^2)
+    v[:, 0] = -(x + 1)
+    r = O.dv_v(dims, v, "f64")
+    np.testing.assert_array_equal(r[m, 0], 0.5 * (-(x[m] + 1) ** 2 + (x[m] + 2) ** 2))
+    # upwind off the diagonal: vx = 0.25 y, vy = +1 -> vy*(vx - vx(y-1)) = 0.25; vy = -1 -> -0.25
+    v = np.zeros((x.size, 4))
+    v[:, 0] = 0.25 * y
+    v[:, 1] = 1.0
+    assert np.all(O.dv_v(dims, v, "f64")[m, 0] == 0.25)
+    v[:, 1] = -1.0
+    assert np.all(O.dv_v(dims, v, "f64")[m, 0] == -0.25)
+
+
+# ---------------------------------------------------------------------------------------------
+# PDE pieces
+# ---------------------------------------------------------------------------------------------
+
+def test_divergence_constant_and_upwind(oracle_mod):
+    O = oracle_mod
+    dims = (9, 8, 7)
+    X, Y, Z = dims
+    n = X * Y * Z
+    m = interior_mask(dims)
+    v = np.zeros((n, 4))
+    v[:, :3] = [0.5, -0.25, 0.125]
+    assert np.all(O.divergence(dims, np.ones(n), v, "f64")[m] == 0)
+    # a single voxel moving in +x carries its mass to x+1 (upwind donor cell)
+    v = np.zeros((n, 4))
+    p = 4 + X * (4 + Y * 3)
+    v[p, 0] = 1.0
+    rho = np.ones(n)
+    rho[p] = 2.0
+    div = O.divergence(dims, rho, v, "f64")
+    assert div[p] == -2.0 and div[p + 1] == 2.0
+    assert np.count_nonzero(div) == 2
+    # compactly supported random flow conserves mass: the interior sum of div vanishes
+    rng = np.random.default_rng(6)
+    v = np.zeros((n, 4))
+    inner = interior_mask(dims, 2)
+    v[inner, :3] = rng.normal(size=(inner.sum(), 3))
+    div = O.divergence(dims, rng.random(n) + 0.5, v, "f64")
+    assert abs(div[m].sum()) < 1e-12
+
+
+def test_update_velocity_friction_and_potential(oracle_mod):
+    O = oracle_mod
+    dims = (6, 6, 6)
+    n = 216
+    m = interior_mask(dims)
+    z4 = np.zeros((n, 4))
+    v = np.zeros((n, 4))
+    v[:, :3] = [1.0, -2.0, 4.0]
+    # friction only: v - alpha*((p+1)/t)*v with alpha 0.25, t 0.5, p 2 -> factor -0.5
+    r = O.update_velocity(dims, v, z4, z4, np.ones(n), z4, 0.25, 0.5, 0.2, 2.0, 0.25, "f64")
+    assert np.all(r[m, :3] == [-0.5, 1.0, -2.0]) and np.all(r[~m] == v[~m])
+    # potential only: -alpha * C p^2 t^(p-2) / (|rho| + 1e-5) * grad U
+    nU = np.zeros((n, 4))
+    nU[:, :3] = [2.0, 0.0, -1.0]
+    rho = -np.full(n, 3.0)
+    r = O.update_velocity(dims, z4, z4, z4, rho, nU, 0.25, 0.5, 0.2, 3.0, 0.5, "f64")
+    q = 0.5 * 9.0 * 0.5 ** 1.0 / (3.0 + float(np.float32(1e-5)))   # the reference's 1e-5f
+    np.testing.assert_allclose(r[m, :3], np.broadcast_to(-0.25 * q * np.array([2.0, 0.0, -1.0]), (m.sum(), 3)),
+                               rtol=1e-15)
+    # diffusion enters with a minus sign inside: v -= alpha*(-tau*dv)
+    dl = np.zeros((n, 4))
+    dl[:, 0] = 1.0
+    r = O.update_velocity(dims, z4, z4, dl, np.ones(n), z4, 0.5, 1.0, 0.25, 2.0, 0.25, "f64")
+    assert np.all(r[m, 0] == 0.125)
+
+
+def test_potential_gradient_closed_form(oracle_mod):
+    O = oracle_mod
+    dims = (6, 5, 4)
+    n = 120
+    m = interior_mask(dims)
+    rng = np.random.default_rng(7)
+    S, T = rng.random(n), rng.random(n)
+    nab = np.zeros((n, 4))
+    nab[:, :3] = rng.normal(size=(n, 3))
+    Lo = np.zeros((n, 4))
+    Lo[:, :3] = rng.normal(size=(n, 3))
+    J = np.zeros((n, 4, 4))
+    J[:, 0, 0] = J[:, 1, 1] = J[:, 2, 2] = 2.0          # det_ref = 8
+    r = O.potential_gradient(dims, np.stack([S, S * 0], 1), np.stack([T, T * 0], 1), nab, J, Lo, 0.2, "f64")
+    exp = 8.0 * ((S - T)[:, None] * nab[:, :3] + 0.2 * Lo[:, :3])
+    np.testing.assert_allclose(r[m, :3], exp[m], rtol=1e-14)
+    assert np.all(r[~m] == 0)
+
+
+def test_update_psi_and_psi_inv(oracle_mod):
+    O = oracle_mod
+    dims = (7, 6, 5)
+    n = 210
+    m = interior_mask(dims)
+    v = np.zeros((n, 4))
+    v[:, :3] = [1.0, -0.5, 0.25]
+    psi, upd = O.update_psi(dims, identity4(dims, np.float64), v, 0.5, "f64")
+    assert np.array_equal(psi[m, :3], identity4(dims, np.float64)[m, :3] + [0.5, -0.25, 0.125])
+    assert np.array_equal(upd[m, :3], np.tile([0.5, -0.25, 0.125], (m.sum(), 1)))
+    assert np.all(upd[~m] == 0) and np.array_equal(psi[~m], identity4(dims, np.float64)[~m])
+    J = np.zeros((n, 4, 4))
+    J[:, 0, 1] = 1.0          # row x reads v.y
+    J[:, 1, 0] = 2.0
+    J[:, 2, 2] = 1.0
+    pinv = O.update_psi_inv(dims, identity4(dims, np.float64), J, v, 0.5, "f64")
+    assert np.array_equal(pinv[m, :3] - identity4(dims, np.float64)[m, :3],
+                          np.tile([0.25, -1.0, -0.125], (m.sum(), 1)))
+
+
+def test_reduce_max_brute_force(oracle_mod):
+    O = oracle_mod
+    dims = (10, 9, 8)
+    n = 720
+    rng = np.random.default_rng(8)
+    u = np.zeros((n, 4), np.float32)
+    u[:, :3] = rng.normal(size=(n, 3))
+    m, am = O.reduce_max(dims, u)
+    nr = np.sqrt((u[:, :3].astype(np.float64) ** 2).sum(1))
+    assert am == int(np.argmax(nr))
+    assert abs(m - nr.max()) <= 2e-7 * nr.max()
+    u[5] = u[am]                                     # a tie: the lowest index wins
+    assert O.reduce_max(dims, u)[1] == 5
+    assert O.reduce_max(dims, np.zeros((n, 4), np.float32)) == (0.0, 0)
+
+
+# ---------------------------------------------------------------------------------------------
+# the solver
+# ---------------------------------------------------------------------------------------------
+
+def _pair(shape=(18, 16, 14), seed=1):
+    return phantoms.blobs(shape=shape, seed=seed, n_blobs=6, max_v=1.5, steps=4)
+
+
+@pytest.mark.parametrize("prec,rtol", [("f64", 1e-12), ("f32", 2e-6)])
+def test_iteration_one_closed_form(oracle_mod, prec, rtol):
+    """From v = 0, psi = psi_inv = id: v1 = -alpha*C p^2/(rho0+1e-5)*(S-T)*grad S on the interior,
+    psi1 = id + alpha v1, psi_inv1 = id - alpha v1, max update = alpha*max|v1|."""
+    O = oracle_mod
+    pair = _pair()
+    dims = pair.dims
+    X, Y, Z = dims
+    o = O.Oracle(dims, prec)
+    o.init(pair.target, pair.target_labels, pair.source, pair.source_labels, 1.0)
+    alpha = 0.004
+    mx, _ = o.step(alpha, alpha=alpha)
+    st = o.get_state()
+    S = pair.source.astype(np.float64).reshape(Z, Y, X)
+    T = pair.target.astype(np.float64).reshape(Z, Y, X)
+    gz, gy, gx = np.gradient(S)
+    g = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], 1)
+    m = interior_mask(dims)
+    v1 = np.zeros((X * Y * Z, 3))
+    v1[m] = -alpha * (0.25 * 4.0 / (1.0 + 1e-5)) * ((S - T).ravel()[m, None] * g[m])
+    scale = np.abs(v1).max()
+    np.testing.assert_allclose(st["v"][:, :3], v1, rtol=0, atol=rtol * scale)
+    ident = identity4(dims, np.float64)[:, :3]
+    # psi is stored as absolute coordinates: allow the rounding of the coordinate itself
+    coord = (np.finfo(np.float32).eps if prec == "f32" else np.finfo(np.float64).eps) * max(dims)
+    np.testing.assert_allclose(st["psi"][:, :3], ident + alpha * v1, rtol=0, atol=rtol * alpha * scale + coord)
+    np.testing.assert_allclose(st["psi_inv"][:, :3], ident - alpha * v1, rtol=0, atol=rtol * alpha * scale + coord)
+    assert np.all(st["rho"] == 1.0)              # v0 = 0: no flux in the first density update
+    np.testing.assert_allclose(mx, alpha * np.sqrt((v1 ** 2).sum(1)).max(), rtol=10 * rtol)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_self_registration_is_a_fixed_point(oracle_mod, prec):
+    O = oracle_mod
+    pair = _pair(seed=2)
+    dims = pair.dims
+    o = O.Oracle(dims, prec)
+    o.init(pair.target, pair.target_labels, pair.target, pair.target_labels)
+    s0 = o.get_state()
+    r = o.run(O.SolverParams(max_iter=15, no_labels=pair.no_labels))
+    s1 = o.get_state()
+    for k in s0:
+        assert np.array_equal(s0[k], s1[k]), k
+    assert np.all(r["max_hist"] == 0)
+    ov, vx, bad = o.dice_counts(pair.no_labels)
+    present = vx[1:] > 0
+    assert bad == 0 and np.all(2 * ov[1:][present] == vx[1:][present])
+
+
+def test_fp32_and_fp64_steps_agree(oracle_mod):
+    """Teacher-forced: from the same state, one FP32 iteration and one FP64 iteration agree to
+    1e-5 relative L2 in every field (the per-iteration parity bar).  Over many iterations the two
+    drift apart by more (about 1e-3 after 40 default iterations): psi and psi_inv are absolute
+    voxel coordinates, and FP32 drops updates below half an ulp of the coordinate."""
+    O = oracle_mod
+    pair = _pair(seed=3)
+    dims = pair.dims
+    rng = np.random.default_rng(30)
+    st = phantoms.random_state(dims, rng, amp=1.5)
+    ident = identity4(dims, np.float64)
+    res = {}
+    for prec in ("f32", "f64"):
+        o = O.Oracle(dims, prec)
+        o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+        o.set_state(**{k: v.astype(o.dtype) for k, v in st.items()})
+        o.step(0.04, alpha=0.02)
+        res[prec] = o.get_state()
+    for k in ("v", "rho", "psi", "psi_inv"):
+        a = res["f32"][k].astype(np.float64)
+        b = res["f64"][k]
+        if k in ("psi", "psi_inv"):
+            a, b = a - ident, b - ident
+        if k == "rho":
+            a, b = a - 1.0, b - 1.0
+        err = np.linalg.norm(a - b) / np.linalg.norm(b)
+        assert err < 1e-5, (k, err)
+
+
+def test_t_schedule_with_reset_iter(oracle_mod):
+    """RESET_ITER=3: t runs alpha, 2a, a, 2a, 3a, a (solver.cu:428-432); stepping by hand with
+    those t values reproduces the loop bit for bit."""
+    O = oracle_mod
+    pair = _pair(seed=4)
+    dims = pair.dims
+    a = np.float32(0.25)
+    o = O.Oracle(dims, "f32")
+    o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+    o.run(O.SolverParams(max_iter=6, reset_iter=3, alpha=float(a), no_labels=pair.no_labels))
+    ts = [a, a + a, a, a + a, (a + a) + a, a]
+    h = O.Oracle(dims, "f32")
+    h.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+    for t in ts:
+        h.step(float(t), alpha=float(a))
+    s1, s2 = o.get_state(), h.get_state()
+    for k in s1:
+        assert np.array_equal(s1[k], s2[k]), k
+
+
+def test_stop_rules_and_messages(oracle_mod, capfd):
+    O = oracle_mod
+    pair = _pair(seed=5)
+    dims = pair.dims
+    o = O.Oracle(dims, "f32")
+    o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+    r = o.run(O.SolverParams(max_iter=20, max_update_norm=1e30, no_labels=pair.no_labels, verbosity=1), log=True)
+    out = capfd.readouterr().out
+    assert r["iterations"] == 1 and r["stop_reason"] == 1
+    assert out.splitlines()[0] == "iter. no. 1"
+    assert "SOLVER CONVERGED AFTER 1 ITERATIONS" in out
+    tgt = pair.target.copy()
+    tgt.reshape(-1)[3] = np.nan
+    o = O.Oracle(dims, "f32")
+    o.init(tgt, pair.target_labels, pair.source, pair.source_labels)
+    r = o.run(O.SolverParams(max_iter=20, no_labels=pair.no_labels, verbosity=1))
+    assert r["iterations"] == 0 and r["stop_reason"] == 2
+    o = O.Oracle(dims, "f32")
+    o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+    r = o.run(O.SolverParams(max_iter=3, no_labels=pair.no_labels), log=True)
+    out = capfd.readouterr().out
+    assert r["iterations"] == 3 and r["stop_reason"] == 0
+    assert out.strip().splitlines()[-1] == "SOLVER REACHED MAX. NO. OF ITERATIONS WITHOUT CONVERGING"
+
+
+def test_dice_skips_last_slice(oracle_mod):
+    O = oracle_mod
+    pair = _pair(seed=6)
+    dims = pair.dims
+    X, Y, Z = dims
+    lab = pair.target_labels.copy()
+    lab2 = lab.copy()
+    lab2[-1, :, :] = 1            # last z slice
+    lab2[:, -1, :] = 2            # last y row
+    lab2[:, :, -1] = 3            # last x column
+    o = O.Oracle(dims, "f32")
+    o.init(pair.target, lab, pair.target, lab2)
+    ov, vx, bad = o.dice_counts(pair.no_labels)
+    present = vx[1:] > 0
+    assert bad == 0 and np.all(2 * ov[1:][present] == vx[1:][present])
+    lab2[0, 0, 0] = (lab[0, 0, 0] + 1) % pair.no_labels   # a counted voxel breaks the equality
+    o.init(pair.target, lab, pair.target, lab2)
+    ov2, vx2, _ = o.dice_counts(pair.no_labels)
+    assert ov2.sum() == ov.sum() - 1
