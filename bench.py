@@ -31,8 +31,11 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--kernel-reps", type=int, default=50)
-    ap.add_argument("--cpu-iters", type=int, default=2, help="oracle iterations for the CPU baseline sample")
+    ap.add_argument("--cpu-iters", type=int, default=60, help="oracle iterations for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="before timing, run one device-to-device copy of this many MiB (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this solver runs one registration per GPU and this build measures one GPU; "
@@ -51,6 +54,15 @@ def main():
     n_int = (X - 2) * (Y - 2) * (Z - 2)
     print(f"[bench] phantom {dims} ready in {time.time() - t0:.1f}s; device {fr.device_info()}", file=sys.stderr,
           flush=True)
+
+    if args.calib_copy_mib > 0:
+        nb = args.calib_copy_mib << 20
+        src, dst = fr.DeviceArray(nb, np.uint8), fr.DeviceArray(nb, np.uint8)
+        fr.check(fr.lib().fr_dev_memset(src.ptr, 1, nb))
+        fr.check(fr.lib().fr_memcpy_dtod(dst.ptr, src.ptr, nb))
+        fr.check(fr.lib().fr_dev_sync())
+        src.free()
+        dst.free()
 
     # warm-up on its own context
     pw = fr.default_params(vol_dims=dims, no_labels=pair.no_labels, max_iter=max(args.warmup, 1))

@@ -61,10 +61,11 @@ struct fr_ctx {
     float *pinv[2] = {nullptr, nullptr};
     float *psi = nullptr;
     int cur = 0;
+    int zc = 8;  // z-slices per block of the iteration kernels
     bool initialised = false;
     // loop control and history
     frk::Ctl *ctl = nullptr;
-    unsigned long long *maxpack = nullptr;  // max_iter slots
+    unsigned long long *maxpack = nullptr;  // max_iter x NSLOT slots
     double *energy = nullptr;               // 3 * max_iter: energy, e_data, e_reg
     double *partials = nullptr;
     int nparts = 0;
@@ -160,7 +161,7 @@ extern "C" int fr_create(const fr_params *p, fr_ctx **out) {
     if ((rc = dalloc(&c->T, n)) || (rc = dalloc(&c->S, n)) || (rc = dalloc(&c->LT, n)) || (rc = dalloc(&c->LS, n)) ||
         (rc = dalloc(&c->rho[0], n)) || (rc = dalloc(&c->rho[1], n)) || (rc = dalloc(&c->v[0], 3 * n)) ||
         (rc = dalloc(&c->v[1], 3 * n)) || (rc = dalloc(&c->pinv[0], 3 * n)) || (rc = dalloc(&c->pinv[1], 3 * n)) ||
-        (rc = dalloc(&c->psi, 3 * n)) || (rc = dalloc(&c->ctl, 1)) || (rc = dalloc(&c->maxpack, (size_t)mi)) ||
+        (rc = dalloc(&c->psi, 3 * n)) || (rc = dalloc(&c->ctl, 1)) || (rc = dalloc(&c->maxpack, (size_t)mi * frk::NSLOT)) ||
         (rc = dalloc(&c->energy, 3 * (size_t)mi)) || (rc = dalloc(&c->partials, 2 * (size_t)c->nparts)) ||
         (rc = dalloc(&c->dice_counts, 2 * (size_t)p->no_labels + 1)) || (rc = dalloc(&c->det_parts, 2 * (size_t)c->ndet)) ||
         (rc = dalloc(&c->lab_tmp, n))) {
@@ -218,7 +219,9 @@ static void iter_args(fr_ctx *c, frk::IterArgs &a) {
     a.d = c->d;
     a.nbx = (c->d.X + frk::TILE_X - 1) / frk::TILE_X;
     a.nby = (c->d.Y + frk::TILE_Y - 1) / frk::TILE_Y;
-    a.nblocks = a.nbx * a.nby * c->d.Z;
+    a.zc = c->zc;
+    a.nbz = (c->d.Z - 2 + a.zc - 1) / a.zc;
+    a.nblocks = a.nbx * a.nby * a.nbz;
     a.rho_o = c->rho[c->cur];
     a.rho_n = c->rho[c->cur ^ 1];
     a.v_o = c->v[c->cur];
@@ -236,6 +239,7 @@ static void iter_args(fr_ctx *c, frk::IterArgs &a) {
     a.maxpack = c->maxpack;
     a.energy = c->energy;
     a.check_energy = 0;
+    a.check_conv = c->prm.max_update_norm >= 0.f;  // a norm is >= 0: a negative threshold never stops
     a.iter = 1;
 }
 
@@ -274,9 +278,15 @@ static int launch_iteration(fr_ctx *c, int iter, float t, bool with_energy) {
 static int reset_control(fr_ctx *c) {
     int mi = c->prm.max_iter > 0 ? c->prm.max_iter : 1;
     HIP_TRY(hipMemsetAsync(c->ctl, 0, sizeof(frk::Ctl), c->stream));
-    HIP_TRY(hipMemsetAsync(c->maxpack, 0, sizeof(unsigned long long) * mi, c->stream));
+    HIP_TRY(hipMemsetAsync(c->maxpack, 0, sizeof(unsigned long long) * mi * frk::NSLOT, c->stream));
     HIP_TRY(hipMemsetAsync(c->energy, 0, sizeof(double) * 3 * mi, c->stream));
     return FR_OK;
+}
+
+static unsigned long long slots_max(const unsigned long long *sl) {
+    unsigned long long m = 0;
+    for (int j = 0; j < frk::NSLOT; ++j) m = sl[j] > m ? sl[j] : m;
+    return m;
 }
 
 static void unpack_max(unsigned long long mp, float *m, long long *idx) {
@@ -295,9 +305,10 @@ extern "C" int fr_solver_step(fr_ctx *c, float t, float *max_norm, long long *ma
     if ((rc = reset_control(c))) return rc;
     if (rc) return rc;
     if ((rc = launch_iteration(c, 1, t, false))) return rc;
-    unsigned long long mp = 0;
-    HIP_TRY(hipMemcpyAsync(&mp, c->maxpack, sizeof mp, hipMemcpyDeviceToHost, c->stream));
+    unsigned long long sl[frk::NSLOT];
+    HIP_TRY(hipMemcpyAsync(sl, c->maxpack, sizeof sl, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    unsigned long long mp = slots_max(sl);
     float m;
     long long idx;
     unpack_max(mp, &m, &idx);
@@ -343,14 +354,16 @@ static int collect_run(fr_ctx *c) {
     const fr_params &p = c->prm;
     int mi = p.max_iter > 0 ? p.max_iter : 1;
     frk::Ctl ctl{};
-    std::vector<unsigned long long> mp(mi);
+    std::vector<unsigned long long> mps((size_t)mi * frk::NSLOT), mp(mi);
     std::vector<double> e(3 * (size_t)mi);
     HIP_TRY(hipMemcpyAsync(&ctl, c->ctl, sizeof ctl, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(mp.data(), c->maxpack, sizeof(unsigned long long) * mi, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(mps.data(), c->maxpack, sizeof(unsigned long long) * mps.size(), hipMemcpyDeviceToHost,
+                           c->stream));
     HIP_TRY(hipMemcpyAsync(e.data(), c->energy, sizeof(double) * 3 * mi, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    for (int k = 0; k < mi; ++k) mp[k] = slots_max(&mps[(size_t)k * frk::NSLOT]);
 
     int done = p.max_iter, reason = 0;
     if (ctl.stopped) {

@@ -19,14 +19,18 @@ struct Ctl {
     int pad;
 };
 
-// Tile of the iteration kernels: 64 x 4 voxels of one z-slice per 256-thread block.
+// Tile of the iteration kernels: 64 x 4 columns per 256-thread block, marching over a chunk of
+// z-slices.
 constexpr int TILE_X = 64;
 constexpr int TILE_Y = 4;
 constexpr int BLOCK = TILE_X * TILE_Y;
+// Pass B spreads its per-iteration 64-bit atomic maxima over NSLOT addresses.
+constexpr int NSLOT = 32;
 
 struct IterArgs {
     Dims d;
-    int nbx, nby, nblocks;       // tile grid
+    int nbx, nby, nbz, nblocks;  // tile grid: x tiles, y tiles, z chunks
+    int zc;                      // z-slices per chunk
     const float *rho_o;          // density, read
     float *rho_n;                // density, written
     const float *v_o;            // velocity xyz-packed, read
@@ -38,8 +42,9 @@ struct IterArgs {
     float alpha, m1t, m2, tau, w_reg, thr;
     int iter;                    // 1-based iteration number
     int check_energy;            // stop on NaN energy of this iteration
+    int check_conv;              // test the previous iteration's max norm against thr
     Ctl *ctl;
-    unsigned long long *maxpack; // per-iteration packed (norm bits, ~index) maxima
+    unsigned long long *maxpack; // per-iteration packed (norm bits, ~index) maxima, NSLOT each
     const double *energy;        // per-iteration energy
 };
 
