@@ -99,7 +99,7 @@ def lib():
     sig("fr_solver_get_state", vp, vp, vp, vp, vp)
     sig("fr_solver_energy", vp, P(d), P(d))
     sig("fr_warp", i, i, i, vp, vp, vp, vp, vp, vp)
-    sig("fr_time_kernels", vp, i, P(f), P(f))
+    sig("fr_time_kernels", vp, i, P(f), P(f), P(f))
     sig("fr_dev_malloc", P(vp), sz)
     sig("fr_dev_free", vp)
     sig("fr_memcpy_htod", vp, vp, sz)
@@ -322,6 +322,7 @@ class Solver:
         return buf.value.decode()
 
     def time_kernels(self, reps=20):
-        a, b = C.c_float(), C.c_float()
-        check(lib().fr_time_kernels(self.h, reps, C.byref(a), C.byref(b)))
-        return a.value, b.value
+        """Average device ms per launch of (pass L, pass A, pass B)."""
+        l, a, b = C.c_float(), C.c_float(), C.c_float()
+        check(lib().fr_time_kernels(self.h, reps, C.byref(l), C.byref(a), C.byref(b)))
+        return l.value, a.value, b.value

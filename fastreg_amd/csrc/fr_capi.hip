@@ -60,6 +60,7 @@ struct fr_ctx {
     float *v[2] = {nullptr, nullptr};
     float *pinv[2] = {nullptr, nullptr};
     float *psi = nullptr;
+    float *L = nullptr;  // -Laplacian(psi), boundary kept at 0
     int cur = 0;
     int zc = 8;  // z-slices per block of the iteration kernels
     bool initialised = false;
@@ -125,7 +126,7 @@ extern "C" int fr_params_default(fr_params *p) {
 static void free_ctx(fr_ctx *c) {
     if (!c) return;
     void *bufs[] = {c->T, c->S, c->LT, c->LS, c->rho[0], c->rho[1], c->v[0], c->v[1], c->pinv[0], c->pinv[1], c->psi,
-                    c->ctl, c->maxpack, c->energy, c->partials, c->dice_counts, c->det_parts, c->lab_tmp};
+                    c->L, c->ctl, c->maxpack, c->energy, c->partials, c->dice_counts, c->det_parts, c->lab_tmp};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -161,7 +162,7 @@ extern "C" int fr_create(const fr_params *p, fr_ctx **out) {
     if ((rc = dalloc(&c->T, n)) || (rc = dalloc(&c->S, n)) || (rc = dalloc(&c->LT, n)) || (rc = dalloc(&c->LS, n)) ||
         (rc = dalloc(&c->rho[0], n)) || (rc = dalloc(&c->rho[1], n)) || (rc = dalloc(&c->v[0], 3 * n)) ||
         (rc = dalloc(&c->v[1], 3 * n)) || (rc = dalloc(&c->pinv[0], 3 * n)) || (rc = dalloc(&c->pinv[1], 3 * n)) ||
-        (rc = dalloc(&c->psi, 3 * n)) || (rc = dalloc(&c->ctl, 1)) || (rc = dalloc(&c->maxpack, (size_t)mi * frk::NSLOT)) ||
+        (rc = dalloc(&c->psi, 3 * n)) || (rc = dalloc(&c->L, 3 * n)) || (rc = dalloc(&c->ctl, 1)) || (rc = dalloc(&c->maxpack, (size_t)mi * frk::NSLOT)) ||
         (rc = dalloc(&c->energy, 3 * (size_t)mi)) || (rc = dalloc(&c->partials, 2 * (size_t)c->nparts)) ||
         (rc = dalloc(&c->dice_counts, 2 * (size_t)p->no_labels + 1)) || (rc = dalloc(&c->det_parts, 2 * (size_t)c->ndet)) ||
         (rc = dalloc(&c->lab_tmp, n))) {
@@ -209,6 +210,7 @@ extern "C" int fr_solver_init(fr_ctx *c, const float *tgt, const int16_t *tgt_la
     if (!c || !tgt || !tgt_lab || !src || !src_lab) return fail(FR_ERR_INVALID, "null argument");
     HIP_TRY(frk::launch_init(c->d, tgt, tgt_lab, src, src_lab, c->T, c->LT, c->S, c->LS, c->rho[0], c->rho[1], c->v[0],
                              c->v[1], c->pinv[0], c->pinv[1], c->psi, c->prm.rho_0, c->stream));
+    HIP_TRY(hipMemsetAsync(c->L, 0, (size_t)c->n * 3 * sizeof(float), c->stream));
     c->cur = 0;
     c->initialised = true;
     c->have_run = c->have_final = false;
@@ -229,6 +231,7 @@ static void iter_args(fr_ctx *c, frk::IterArgs &a) {
     a.pinv_o = c->pinv[c->cur];
     a.pinv_n = c->pinv[c->cur ^ 1];
     a.psi = c->psi;
+    a.L = c->L;
     a.S = c->S;
     a.T = c->T;
     a.alpha = c->prm.alpha;
@@ -269,6 +272,7 @@ static int launch_iteration(fr_ctx *c, int iter, float t, bool with_energy) {
                                             e + 2 * mi + (iter - 1), e + (iter - 1), c->stream));
         a.check_energy = 1;
     }
+    HIP_TRY(frk::launch_pass_l(a, c->stream));
     HIP_TRY(frk::launch_pass_a(a, c->stream));
     HIP_TRY(frk::launch_pass_b(a, c->stream));
     c->cur ^= 1;
@@ -612,7 +616,7 @@ extern "C" int fr_warp(int X, int Y, int Z, const float *img, const int16_t *lab
     return FR_OK;
 }
 
-extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_a, float *ms_b) {
+extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_l, float *ms_a, float *ms_b) {
     if (!c || reps < 1) return fail(FR_ERR_INVALID, "bad arguments");
     if (!c->initialised) return fail(FR_ERR_STATE, "fr_time_kernels before fr_solver_init");
     int rc;
@@ -625,8 +629,15 @@ extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_a, float *ms_b) {
     a.iter = 1;
     vel_scalars(c->prm, c->prm.alpha, &a.m1t, &a.m2);
     hipError_t e = hipMemcpyAsync(psi_save, c->psi, 3 * (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
-    float ta = 0.f, tb = 0.f;
-    // pass A reads the current buffers and writes the other ones: repeating it is idempotent
+    float tl = 0.f, ta = 0.f, tb = 0.f;
+    // pass L and pass A read the current buffers and write L / the other buffers: repeating them
+    // is idempotent
+    if (e == hipSuccess) e = frk::launch_pass_l(a, c->stream);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = frk::launch_pass_l(a, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&tl, c->ev0, c->ev1);
     if (e == hipSuccess) e = frk::launch_pass_a(a, c->stream);  // warm-up
     if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
     for (int r = 0; r < reps && e == hipSuccess; ++r) e = frk::launch_pass_a(a, c->stream);
@@ -644,6 +655,7 @@ extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_a, float *ms_b) {
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(psi_save);
     if (e != hipSuccess) return fail(FR_ERR_HIP, "fr_time_kernels: %s", hipGetErrorString(e));
+    if (ms_l) *ms_l = tl / reps;
     if (ms_a) *ms_a = ta / reps;
     if (ms_b) *ms_b = tb / reps;
     return FR_OK;

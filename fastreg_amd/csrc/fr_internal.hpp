@@ -38,6 +38,7 @@ struct IterArgs {
     const float *pinv_o;         // psi_inv xyz-packed, read
     float *pinv_n;               // psi_inv xyz-packed, written
     float *psi;                  // psi xyz-packed (pass B updates in place)
+    float *L;                    // -Laplacian(psi) xyz-packed (pass L writes the interior, boundary 0)
     const float *S, *T;          // source / target intensity
     float alpha, m1t, m2, tau, w_reg, thr;
     int iter;                    // 1-based iteration number
@@ -52,6 +53,7 @@ struct IterArgs {
 hipError_t launch_init(Dims d, const float *tgt, const int16_t *tgt_lab, const float *src, const int16_t *src_lab,
                        float *T, int16_t *LT, float *S, int16_t *LS, float *rho0, float *rho1, float *v0, float *v1,
                        float *pinv0, float *pinv1, float *psi, float rho_0, hipStream_t s);
+hipError_t launch_pass_l(const IterArgs &a, hipStream_t s);
 hipError_t launch_pass_a(const IterArgs &a, hipStream_t s);
 hipError_t launch_pass_b(const IterArgs &a, hipStream_t s);
 hipError_t launch_energy(Dims d, const float *S, const float *T, const float *psi, double *partials, int nparts,

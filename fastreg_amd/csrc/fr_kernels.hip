@@ -1,17 +1,18 @@
 // HIP kernels of the fastreg registration solver for gfx950 (MI355X).
 //
-// One solver iteration is two launches:
-//   pass A  (per interior voxel, reads the old state, writes the new one into the other buffer of
-//           a ping-pong pair): upwind divergence and density update, J_inv = D psi_inv and the
-//           reference's det(), L = -Laplacian(psi) evaluated at the 8 corners of psi_inv(p) and
-//           interpolated there, T o psi_inv, grad S, the potential gradient, (Dv)v, the upwind
+// One solver iteration is three launches over 64 x 4-column tiles, each thread marching a chunk
+// of z-slices with its column's z-neighbours in registers:
+//   pass L  L = -Laplacian(psi) on the interior, into a buffer (boundary 0).
+//   pass A  (reads the old state, writes the new one into the other buffer of a ping-pong pair):
+//           upwind divergence and density update, J_inv = D psi_inv and the reference's det(),
+//           L and T interpolated at psi_inv(p), grad S, the potential gradient, (Dv)v, the upwind
 //           velocity Laplacian, the velocity update and the psi_inv update.
-//   pass B  (per interior voxel, after all of v_new exists): psi += alpha * v_new o psi, and the
-//           max update norm of the iteration, reduced in the wave, the block and one 64-bit
-//           atomic max per block into a per-iteration slot.
+//   pass B  (after all of v_new exists): psi += alpha * v_new o psi, and the max update norm of the
+//           iteration, reduced in the wave, the block and one 64-bit atomic max per block into
+//           one of NSLOT per-iteration slots.
 // The reference runs the same iteration as 13 launches on 5 streams with 12 float4 fields and
-// two 64-byte Jacobians in HBM (src/fastreg/cuda/solver.cu:295-444); here the only iterated
-// state is rho, v, psi, psi_inv (xyz-packed, 12 B per vector) plus the read-only S and T.
+// two 64-byte Jacobians in HBM (src/fastreg/cuda/solver.cu:295-444); here the iterated state is
+// rho, v, psi, psi_inv (xyz-packed, 12 B per vector), plus L and the read-only S and T.
 //
 // Arithmetic follows the reference kernel by kernel in the same operation order.  The library
 // is compiled with -ffp-contract=off, so a multiply-add rounds twice unless written fmaf():
@@ -99,22 +100,6 @@ __device__ __forceinline__ float det_true(float m00, float m01, float m02, float
     return m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
 }
 
-// -Laplacian(psi) at q (vector_fields.cu:275-300); 0 off the interior, where the reference's
-// L buffer is never written
-__device__ __forceinline__ F3 lap_at(Dims d, const float *__restrict__ psi, int qx, int qy, int qz) {
-    if (qx < 1 || qx > d.X - 2 || qy < 1 || qy > d.Y - 2 || qz < 1 || qz > d.Z - 2) return f3(0.f, 0.f, 0.f);
-    int q = lin(d, qx, qy, qz);
-    int sy = d.X, sz = d.X * d.Y;
-    F3 s = mul3(-6.f, ld3(psi, q));
-    s = add3(s, ld3(psi, q + 1));
-    s = add3(s, ld3(psi, q - 1));
-    s = add3(s, ld3(psi, q + sy));
-    s = add3(s, ld3(psi, q - sy));
-    s = add3(s, ld3(psi, q + sz));
-    s = add3(s, ld3(psi, q - sz));
-    return mul3(-1.f, s);
-}
-
 // XCD-aware tile order: the hardware hands consecutive workgroups to the 8 XCDs round-robin;
 // remap so that each XCD works on one contiguous run of tiles (a z-slab), keeping stencil and
 // gather neighbours in that XCD's L2.
@@ -173,160 +158,41 @@ __device__ __forceinline__ bool should_stop(const IterArgs &a, bool record) {
 }
 
 // --------------------------------------------------------------------------------------------
-// LDS planes of the iteration kernels
+// pass L and pass A
 // --------------------------------------------------------------------------------------------
 
-// plane regions around a TILE_X x TILE_Y tile: halo 2 (68 x 8 cells) and halo 1 (66 x 6)
-constexpr int H2W = TILE_X + 4, H2H = TILE_Y + 4, H2N = H2W * H2H;
-constexpr int H1W = TILE_X + 2, H1H = TILE_Y + 2, H1N = H1W * H1H;
-constexpr int H2C = (H2N + BLOCK - 1) / BLOCK;   // cells per thread, halo-2 planes (3)
-constexpr int H1C = (H1N + BLOCK - 1) / BLOCK;   // cells per thread, halo-1 planes (2)
-constexpr int H2R = (H2N + 63) / 64 * 64;   // halo-2 plane cells rounded up to whole waves (576)
-constexpr int H1R = (H1N + 63) / 64 * 64;   // halo-1 plane cells rounded up to whole waves (448)
+// Buffer resources (raw, stride 0): loads at byte offsets >= num_records return 0.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr int RSRC_FLAGS = 0x00020000;
 
-// The cells a thread stages for a plane region.  Cell c = threadIdx.x + j*BLOCK takes the voxel
-// at the linear address x + X*(y + Y*z) when x is inside its row and the address is inside the
-// volume, else 0.  For y = Y that address is (x, 0, z+1): the reference's velocity Laplacian
-// reads it at y = Y-2 (vector_fields.cu:320,338).
-template <int H, int NCELL>
-struct CellSet {
-    int base[NCELL];   // x + X*y of the cell, or INT_MIN when the cell is unused / off its row
-};
-
-template <int H, int NCELL>
-__device__ __forceinline__ CellSet<H, NCELL> cells(Dims d, int x0, int y0) {
-    constexpr int W = TILE_X + 2 * H, NC = W * (TILE_Y + 2 * H);
-    CellSet<H, NCELL> s;
-#pragma unroll
-    for (int j = 0; j < NCELL; ++j) {
-        const int c = threadIdx.x + j * BLOCK;
-        const int cy = c / W, cx = c - cy * W;
-        const int x = x0 - H + cx, y = y0 - H + cy;
-        s.base[j] = (c < NC && x >= 0 && x < d.X) ? x + d.X * y : INT_MIN;
-    }
-    return s;
+__device__ __forceinline__ Rsrc rsrc(const float *p, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, bytes, RSRC_FLAGS);
+}
+__device__ __forceinline__ float bld1(Rsrc r, unsigned voff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+}
+// Three dword loads, which the compiler merges into one buffer_load_dwordx3.  (This compiler lowers
+// the vector-returning __builtin_amdgcn_raw_buffer_load_b96 / _b128 to a single-dword load and
+// leaves the other components undefined, so they are not used.)
+__device__ __forceinline__ F3 bld3(Rsrc r, unsigned voff) {
+    return f3(bld1(r, voff), bld1(r, voff + 4u), bld1(r, voff + 8u));
 }
 
-template <int NCELL>
-struct Pre1 {
-    float v[NCELL];
-};
-template <int NCELL>
-struct Pre3 {
-    F3 v[NCELL];
-};
-
-template <int H, int NCELL>
-__device__ __forceinline__ Pre1<NCELL> fetch1(const CellSet<H, NCELL> &s, const float *__restrict__ g, int plane_off,
-                                              int N) {
-    Pre1<NCELL> r;
-#pragma unroll
-    for (int j = 0; j < NCELL; ++j) {
-        const int l = s.base[j] + plane_off;
-        r.v[j] = (s.base[j] != INT_MIN && l >= 0 && l < N) ? g[l] : 0.f;
-    }
-    return r;
+// Trilinear blend of a field at corner g with the +1 neighbours at byte offsets dx, dy, dz from
+// it.  The +1 neighbours are not clamped: when one leaves the grid its weight is exactly 0 and the
+// address either holds another voxel of the field (finite) or lies past the end of the buffer
+// (read as 0), so the blend equals the reference's (cuda_utils.hpp:60-136).
+__device__ __forceinline__ float tri_at1(Rsrc r, unsigned g, unsigned dx, unsigned dy, unsigned dz, const Corner &k) {
+    return tri(bld1(r, g + dx + dy + dz), bld1(r, g + dx + dy), bld1(r, g + dx + dz), bld1(r, g + dx),
+               bld1(r, g + dy + dz), bld1(r, g + dy), bld1(r, g + dz), bld1(r, g), k.a, k.b, k.c);
 }
-
-template <int H, int NCELL>
-__device__ __forceinline__ Pre3<NCELL> fetch3(const CellSet<H, NCELL> &s, const float *__restrict__ g, int plane_off,
-                                              int N) {
-    Pre3<NCELL> r;
-#pragma unroll
-    for (int j = 0; j < NCELL; ++j) {
-        const int l = s.base[j] + plane_off;
-        r.v[j] = (s.base[j] != INT_MIN && l >= 0 && l < N) ? ld3(g, l) : f3(0.f, 0.f, 0.f);
-    }
-    return r;
-}
-
-template <int H, int NCELL>
-__device__ __forceinline__ void put1(float *__restrict__ lds, const Pre1<NCELL> &p) {
-    constexpr int NC = (TILE_X + 2 * H) * (TILE_Y + 2 * H);
-#pragma unroll
-    for (int j = 0; j < NCELL; ++j) {
-        const int c = threadIdx.x + j * BLOCK;
-        if (c < NC) lds[c] = p.v[j];
-    }
-}
-
-template <int H, int NCELL>
-__device__ __forceinline__ void put3(float *__restrict__ lds, const Pre3<NCELL> &p) {
-    constexpr int NC = (TILE_X + 2 * H) * (TILE_Y + 2 * H);
-#pragma unroll
-    for (int j = 0; j < NCELL; ++j) {
-        const int c = threadIdx.x + j * BLOCK;
-        if (c < NC) {
-            lds[3 * c] = p.v[j].x;
-            lds[3 * c + 1] = p.v[j].y;
-            lds[3 * c + 2] = p.v[j].z;
-        }
-    }
-}
-
-__device__ __forceinline__ F3 lds3(const float *__restrict__ p, int c) { return f3(p[3 * c], p[3 * c + 1], p[3 * c + 2]); }
-
-// a cell of a component-major LDS plane (sub-planes PS floats apart)
-template <int PS>
-__device__ __forceinline__ F3 ldsc(const float *__restrict__ p, int c) { return f3(p[c], p[c + PS], p[c + 2 * PS]); }
-
-// L = -Laplacian(psi) on plane z over the halo-1 region, from three consecutive component-major
-// halo-2 planes of psi (z-1, z, z+1); 0 off the interior (vector_fields.cu:275-300)
-__device__ __forceinline__ void lap_plane(float *__restrict__ Lout, const float *__restrict__ Qm,
-                                          const float *__restrict__ Q0, const float *__restrict__ Qp, Dims d, int x0,
-                                          int y0, int z) {
-#pragma unroll
-    for (int j = 0; j < H1C; ++j) {
-        const int c = threadIdx.x + j * BLOCK;
-        if (c < H1N) {
-            const int cy = c / H1W, cx = c - cy * H1W;
-            const int x = x0 - 1 + cx, y = y0 - 1 + cy;
-            F3 r = f3(0.f, 0.f, 0.f);
-            if (x >= 1 && x <= d.X - 2 && y >= 1 && y <= d.Y - 2 && z >= 1 && z <= d.Z - 2) {
-                const int q = (cy + 1) * H2W + (cx + 1);
-                F3 s = mul3(-6.f, ldsc<H2R>(Q0, q));
-                s = add3(s, ldsc<H2R>(Q0, q + 1));
-                s = add3(s, ldsc<H2R>(Q0, q - 1));
-                s = add3(s, ldsc<H2R>(Q0, q + H2W));
-                s = add3(s, ldsc<H2R>(Q0, q - H2W));
-                s = add3(s, ldsc<H2R>(Qp, q));
-                s = add3(s, ldsc<H2R>(Qm, q));
-                r = mul3(-1.f, s);
-            }
-            Lout[3 * c] = r.x;
-            Lout[3 * c + 1] = r.y;
-            Lout[3 * c + 2] = r.z;
-        }
-    }
-}
-
-__device__ __forceinline__ int mod3(int z) { return z % 3; }
-
-// L o psi_inv and T o psi_inv by global gathers, for positions outside the LDS window
-__device__ __noinline__ F3 gather_lo(Dims d, const float *__restrict__ psi, Corner k) {
-    const F3 l111 = lap_at(d, psi, k.hx, k.hy, k.hz), l110 = lap_at(d, psi, k.hx, k.hy, k.gz);
-    const F3 l101 = lap_at(d, psi, k.hx, k.gy, k.hz), l100 = lap_at(d, psi, k.hx, k.gy, k.gz);
-    const F3 l011 = lap_at(d, psi, k.gx, k.hy, k.hz), l010 = lap_at(d, psi, k.gx, k.hy, k.gz);
-    const F3 l001 = lap_at(d, psi, k.gx, k.gy, k.hz), l000 = lap_at(d, psi, k.gx, k.gy, k.gz);
-    return f3(tri(l111.x, l110.x, l101.x, l100.x, l011.x, l010.x, l001.x, l000.x, k.a, k.b, k.c),
-              tri(l111.y, l110.y, l101.y, l100.y, l011.y, l010.y, l001.y, l000.y, k.a, k.b, k.c),
-              tri(l111.z, l110.z, l101.z, l100.z, l011.z, l010.z, l001.z, l000.z, k.a, k.b, k.c));
-}
-
-__device__ __noinline__ float gather_t(Dims d, const float *__restrict__ T, Corner k) {
-    return tri(T[lin(d, k.hx, k.hy, k.hz)], T[lin(d, k.hx, k.hy, k.gz)], T[lin(d, k.hx, k.gy, k.hz)],
-               T[lin(d, k.hx, k.gy, k.gz)], T[lin(d, k.gx, k.hy, k.hz)], T[lin(d, k.gx, k.hy, k.gz)],
-               T[lin(d, k.gx, k.gy, k.hz)], T[lin(d, k.gx, k.gy, k.gz)], k.a, k.b, k.c);
-}
-
-__device__ __forceinline__ F3 gather_v(Dims d, const float *__restrict__ v, Corner k) {
-    const F3 v111 = ld3(v, lin(d, k.hx, k.hy, k.hz)), v110 = ld3(v, lin(d, k.hx, k.hy, k.gz));
-    const F3 v101 = ld3(v, lin(d, k.hx, k.gy, k.hz)), v100 = ld3(v, lin(d, k.hx, k.gy, k.gz));
-    const F3 v011 = ld3(v, lin(d, k.gx, k.hy, k.hz)), v010 = ld3(v, lin(d, k.gx, k.hy, k.gz));
-    const F3 v001 = ld3(v, lin(d, k.gx, k.gy, k.hz)), v000 = ld3(v, lin(d, k.gx, k.gy, k.gz));
-    return f3(tri(v111.x, v110.x, v101.x, v100.x, v011.x, v010.x, v001.x, v000.x, k.a, k.b, k.c),
-              tri(v111.y, v110.y, v101.y, v100.y, v011.y, v010.y, v001.y, v000.y, k.a, k.b, k.c),
-              tri(v111.z, v110.z, v101.z, v100.z, v011.z, v010.z, v001.z, v000.z, k.a, k.b, k.c));
+__device__ __forceinline__ F3 tri_at3(Rsrc r, unsigned g, unsigned dx, unsigned dy, unsigned dz, const Corner &k) {
+    const F3 f111 = bld3(r, g + dx + dy + dz), f110 = bld3(r, g + dx + dy), f101 = bld3(r, g + dx + dz),
+             f100 = bld3(r, g + dx), f011 = bld3(r, g + dy + dz), f010 = bld3(r, g + dy), f001 = bld3(r, g + dz),
+             f000 = bld3(r, g);
+    return f3(tri(f111.x, f110.x, f101.x, f100.x, f011.x, f010.x, f001.x, f000.x, k.a, k.b, k.c),
+              tri(f111.y, f110.y, f101.y, f100.y, f011.y, f010.y, f001.y, f000.y, k.a, k.b, k.c),
+              tri(f111.z, f110.z, f101.z, f100.z, f011.z, f010.z, f001.z, f000.z, k.a, k.b, k.c));
 }
 
 struct TileCoord {
@@ -344,290 +210,161 @@ __device__ __forceinline__ TileCoord tile_of(const IterArgs &a, int zc) {
     return t;
 }
 
-__device__ __forceinline__ F3 col3(const float *__restrict__ g, Dims d, int i, int z, bool col) {
-    return (col && z >= 0 && z < d.Z) ? ld3(g, i + d.X * d.Y * z) : f3(0.f, 0.f, 0.f);
-}
-__device__ __forceinline__ float col1(const float *__restrict__ g, Dims d, int i, int z, bool col) {
-    return (col && z >= 0 && z < d.Z) ? g[i + d.X * d.Y * z] : 0.f;
-}
-
-// --------------------------------------------------------------------------------------------
-// pass A
-// --------------------------------------------------------------------------------------------
-
-// Buffer resources (raw, stride 0): loads at byte offsets >= num_records return 0, which the
-// staging below uses for every cell outside the volume.
-typedef __amdgpu_buffer_rsrc_t Rsrc;
-constexpr int RSRC_FLAGS = 0x00020000;
-constexpr unsigned OOB = 0x80000000u;   // byte offset beyond every field (all fields are < 2^31 B)
-
-__device__ __forceinline__ Rsrc rsrc(const float *p, int bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, bytes, RSRC_FLAGS);
-}
-__device__ __forceinline__ float bld1(Rsrc r, unsigned voff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
-}
-// Three dword loads, which the compiler merges into one buffer_load_dwordx3.  (This compiler lowers
-// the vector-returning __builtin_amdgcn_raw_buffer_load_b96 / _b128 to a single-dword load and
-// leaves the other components undefined, so they are not used.)
-__device__ __forceinline__ F3 bld3(Rsrc r, unsigned voff) {
-    return f3(bld1(r, voff), bld1(r, voff + 4u), bld1(r, voff + 8u));
-}
-
-typedef __attribute__((address_space(3))) void *LdsPtr;
-
-// Stage plane z of a field (C floats per voxel) over a halo-H region into LDS by direct
-// buffer-to-LDS dword loads: wave w writes cells [64w + 256j, +64) of the plane, one dword per lane
-// (component k of a vector field goes to the k-th sub-plane, PS floats apart: the LDS copy is
-// component-major).  `e[j]` is the cell's element index x + X*y (INT_MIN off the row / past the
-// region); the byte offset 4*(C*(e + XY*z) + k) is range-checked by the hardware, so
-// out-of-volume cells (including negative linear addresses) read as 0, and y = Y reads
-// (x, 0, z+1) as the reference does.
-template <int C, int H, int NCELL, int PS>
-__device__ __forceinline__ void stage(float *lds, Rsrc r, const int (&e)[NCELL], int XY, int z) {
-    constexpr int NC = (TILE_X + 2 * H) * (TILE_Y + 2 * H);
-    const int w = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < NCELL; ++j) {
-        const int first = 64 * w + BLOCK * j;   // wave-uniform
-        if (first < NC) {
-            const unsigned voff = e[j] == INT_MIN ? OOB : (unsigned)(4 * C * (e[j] + XY * z));
-#pragma unroll
-            for (int k = 0; k < C; ++k)
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (LdsPtr)(lds + k * PS + first), 4, voff + 4u * k, 0, 0, 0);
-        }
+// pass L: L = -Laplacian(psi) on the interior (vector_fields.cu:275-300), into a buffer whose
+// boundary voxels stay 0.  One thread per (x, y) column of the tile, marching the chunk's z range
+// with psi(z-1), psi(z), psi(z+1) of the column in registers.
+__global__ __launch_bounds__(BLOCK) void k_pass_l(IterArgs a) {
+    if (__builtin_nontemporal_load(&a.ctl->stopped)) return;
+    const Dims d = a.d;
+    const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
+    const TileCoord tc = tile_of(a, a.zc);
+    const int x = tc.x0 + (threadIdx.x & (TILE_X - 1)), y = tc.y0 + threadIdx.x / TILE_X;
+    if (x < 1 || x > d.X - 2 || y < 1 || y > d.Y - 2) return;
+    const Rsrc rq = rsrc(a.psi, 12 * N);
+    const unsigned sx = 12u, sy = 12u * d.X, sz = 12u * XY;
+    unsigned o = 12u * (x + d.X * y + XY * tc.zs);
+    F3 zm = bld3(rq, o - sz), c = bld3(rq, o);
+    for (int z = tc.zs; z < tc.ze; ++z, o += sz) {
+        const F3 zp = bld3(rq, o + sz);
+        F3 s = mul3(-6.f, c);
+        s = add3(s, bld3(rq, o + sx));
+        s = add3(s, bld3(rq, o - sx));
+        s = add3(s, bld3(rq, o + sy));
+        s = add3(s, bld3(rq, o - sy));
+        s = add3(s, zp);
+        s = add3(s, zm);
+        st3(a.L, o / 12u, mul3(-1.f, s));
+        zm = c;
+        c = zp;
     }
 }
 
-template <int H, int NCELL>
-__device__ __forceinline__ void cell_elems(int (&e)[NCELL], Dims d, int x0, int y0) {
-    constexpr int W = TILE_X + 2 * H, NC = W * (TILE_Y + 2 * H);
-#pragma unroll
-    for (int j = 0; j < NCELL; ++j) {
-        const int c = threadIdx.x + j * BLOCK;
-        const int cy = c / W, cx = c - cy * W;
-        const int x = x0 - H + cx, y = y0 - H + cy;
-        e[j] = (c < NC && x >= 0 && x < d.X) ? x + d.X * y : INT_MIN;
-    }
-}
-
-__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }   // vmcnt(0)
-
-
-// LDS of pass A.  The buffer-to-LDS loads take their LDS address from M0, which addresses the
-// first 64 KB only, so every staging target comes first and the computed L ring last.
-struct PassALds {
-    float Vp[2][H2R * 3];   // v_old, plane z (double-buffered), halo 2, component-major
-    float Pp[2][H1R * 3];   // psi_inv_old, plane z, halo 1, component-major
-    float Rp[2][H1R];       // rho_old, plane z, halo 1
-    float Sp[2][H1R];       // S, plane z, halo 1
-    float Qr[3][H2R * 3];   // psi_old, planes z, z+1, z+2 (ring), halo 2, component-major
-    float Tr[4][H1R];       // T, planes z-1 .. z+2 (ring), halo 1
-    float Lr[3][H1N * 3];   // -Laplacian(psi_old), planes z-1, z, z+1 (ring), halo 1, written by ds_write
-};
-static_assert(offsetof(PassALds, Lr) <= 65536, "buffer-to-LDS targets must lie in the first 64 KB");
-
+// pass A: steps 1-8 and 10 of one iteration per interior voxel.  One thread per (x, y) column of
+// the tile, marching the chunk's z range; the column's z-neighbours (v at z-2..z+2, rho, psi_inv
+// and S at z-1..z+1) slide through registers, x/y neighbours are read from the current plane
+// (L1-resident: the neighbouring lanes and waves read the same rows), L and T are gathered at
+// psi_inv(p).
 __global__ __launch_bounds__(BLOCK) void k_pass_a(IterArgs a) {
     if (should_stop(a, blockIdx.x == 0 && threadIdx.x == 0)) return;
     const Dims d = a.d;
     const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
     const TileCoord tc = tile_of(a, a.zc);
-    const int tx = threadIdx.x & (TILE_X - 1), ty = threadIdx.x / TILE_X;
-    const int x = tc.x0 + tx, y = tc.y0 + ty;
-    const bool col = x < d.X && y < d.Y;
-    const bool icol = x >= 1 && x <= d.X - 2 && y >= 1 && y <= d.Y - 2;
-    const int i0 = x + d.X * y;   // column base index
-
-    __shared__ PassALds sm;
-    auto &Vp = sm.Vp;
-    auto &Pp = sm.Pp;
-    auto &Rp = sm.Rp;
-    auto &Sp = sm.Sp;
-    auto &Qr = sm.Qr;
-    auto &Tr = sm.Tr;
-    auto &Lr = sm.Lr;
-
+    const int x = tc.x0 + (threadIdx.x & (TILE_X - 1)), y = tc.y0 + threadIdx.x / TILE_X;
+    if (x < 1 || x > d.X - 2 || y < 1 || y > d.Y - 2) return;
     const Rsrc rv = rsrc(a.v_o, 12 * N), rr = rsrc(a.rho_o, 4 * N), rp = rsrc(a.pinv_o, 12 * N);
-    const Rsrc rs = rsrc(a.S, 4 * N), rt = rsrc(a.T, 4 * N), rq = rsrc(a.psi, 12 * N);
-    const int zs = tc.zs, ze = tc.ze;
-    int e2[H2C], e1[H1C];
-    cell_elems<2, H2C>(e2, d, tc.x0, tc.y0);
-    cell_elems<1, H1C>(e1, d, tc.x0, tc.y0);
+    const Rsrc rs = rsrc(a.S, 4 * N), rt = rsrc(a.T, 4 * N), rl = rsrc(a.L, 12 * N);
+    const unsigned X12 = 12u * d.X, XY12 = 12u * XY, X4 = 4u * d.X, XY4 = 4u * XY;
+    const bool lap_xy = x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2;
 
-    // prologue: psi planes zs-2 .. zs+1 through the ring -> L planes zs-1, zs; T planes zs-1 .. zs+1;
-    // the other planes at zs
-    stage<3, 2, H2C, H2R>(Qr[(zs + 1) % 3], rq, e2, XY, zs - 2);
-    stage<3, 2, H2C, H2R>(Qr[(zs + 2) % 3], rq, e2, XY, zs - 1);
-    stage<3, 2, H2C, H2R>(Qr[zs % 3], rq, e2, XY, zs);
-    stage<1, 1, H1C, H1R>(Tr[(zs + 3) % 4], rt, e1, XY, zs - 1);
-    stage<1, 1, H1C, H1R>(Tr[zs % 4], rt, e1, XY, zs);
-    stage<1, 1, H1C, H1R>(Tr[(zs + 1) % 4], rt, e1, XY, zs + 1);
-    stage<3, 2, H2C, H2R>(Vp[zs & 1], rv, e2, XY, zs);
-    stage<3, 1, H1C, H1R>(Pp[zs & 1], rp, e1, XY, zs);
-    stage<1, 1, H1C, H1R>(Rp[zs & 1], rr, e1, XY, zs);
-    stage<1, 1, H1C, H1R>(Sp[zs & 1], rs, e1, XY, zs);
-    wait_vm();
-    __syncthreads();
-    lap_plane(Lr[(zs + 2) % 3], Qr[(zs + 1) % 3], Qr[(zs + 2) % 3], Qr[zs % 3], d, tc.x0, tc.y0, zs - 1);
-    __syncthreads();
-    stage<3, 2, H2C, H2R>(Qr[(zs + 1) % 3], rq, e2, XY, zs + 1);
-    wait_vm();
-    __syncthreads();
-    lap_plane(Lr[zs % 3], Qr[(zs + 2) % 3], Qr[zs % 3], Qr[(zs + 1) % 3], d, tc.x0, tc.y0, zs);
-    __syncthreads();   // the next stage overwrites the psi plane zs-1 that lap_plane just read
-    stage<3, 2, H2C, H2R>(Qr[(zs + 2) % 3], rq, e2, XY, zs + 2);
+    int i = x + d.X * y + XY * tc.zs;
+    // own-column z-windows
+    F3 v_m2 = bld3(rv, 12u * (i - 2 * XY)), v_m1 = bld3(rv, 12u * (i - XY)), v_0 = bld3(rv, 12u * i);
+    F3 v_p1 = bld3(rv, 12u * (i + XY)), v_p2 = bld3(rv, 12u * (i + 2 * XY));
+    float r_m1 = bld1(rr, 4u * (i - XY)), r_0 = bld1(rr, 4u * i), r_p1 = bld1(rr, 4u * (i + XY));
+    F3 p_m1 = bld3(rp, 12u * (i - XY)), p_0 = bld3(rp, 12u * i), p_p1 = bld3(rp, 12u * (i + XY));
+    float s_m1 = bld1(rs, 4u * (i - XY)), s_0 = bld1(rs, 4u * i), s_p1 = bld1(rs, 4u * (i + XY));
 
-    // own-column z-windows: v at z-2..z+2, rho / psi_inv / S at z-1..z+1
-    const unsigned cb = col ? 0u : OOB;
-    auto c3 = [&](Rsrc r, int z) -> F3 { return bld3(r, col ? (unsigned)(12 * (i0 + XY * z)) : OOB); };
-    auto c1 = [&](Rsrc r, int z) -> float { return bld1(r, col ? (unsigned)(4 * (i0 + XY * z)) : OOB); };
-    (void)cb;
-    F3 v_m2 = c3(rv, zs - 2), v_m1 = c3(rv, zs - 1), v_0 = c3(rv, zs), v_p1 = c3(rv, zs + 1), v_p2 = c3(rv, zs + 2);
-    float r_m1 = c1(rr, zs - 1), r_0 = c1(rr, zs), r_p1 = c1(rr, zs + 1);
-    F3 p_m1 = c3(rp, zs - 1), p_0 = c3(rp, zs), p_p1 = c3(rp, zs + 1);
-    float s_m1 = c1(rs, zs - 1), s_0 = c1(rs, zs), s_p1 = c1(rs, zs + 1);
-    wait_vm();
-    __syncthreads();
+    for (int z = tc.zs; z < tc.ze; ++z, i += XY) {
+        const unsigned o12 = 12u * i, o4 = 4u * i;
+        // 1. density (solver.cu:101-173)
+        const F3 vc = v_0;
+        const float rc = r_0;
+        const float t11 = (vc.x > 0.f) ? -vc.x * rc : -vc.x * bld1(rr, o4 + 4u);
+        const float vxm = bld1(rv, o12 - 12u);
+        const float t12 = (vxm > 0.f) ? vxm * bld1(rr, o4 - 4u) : vxm * rc;
+        const float t21 = (vc.y > 0.f) ? -vc.y * rc : -vc.y * bld1(rr, o4 + X4);
+        const float vym = bld1(rv, o12 - X12 + 4u);
+        const float t22 = (vym > 0.f) ? vym * bld1(rr, o4 - X4) : vym * rc;
+        const float t31 = (vc.z > 0.f) ? -vc.z * rc : -vc.z * r_p1;
+        const float vzm = v_m1.z;
+        const float t32 = (vzm > 0.f) ? vzm * r_m1 : vzm * rc;
+        const float div = ((((t11 + t12) + t21) + t22) + t31) + t32;
+        const float rho_n = fmaf(a.alpha, div, rc);
 
-    const int c2 = (ty + 2) * H2W + (tx + 2);   // own cell in halo-2 planes
-    const int c1o = (ty + 1) * H1W + (tx + 1);  // own cell in halo-1 planes
+        // 2. J_inv and det() (vector_fields.cu:503-535)
+        const F3 jx = mul3(0.5f, sub3(bld3(rp, o12 + 12u), bld3(rp, o12 - 12u)));
+        const F3 jy = mul3(0.5f, sub3(bld3(rp, o12 + X12), bld3(rp, o12 - X12)));
+        const F3 jz = mul3(0.5f, sub3(p_p1, p_m1));
+        const float det = det_ref(jx.x, jy.x, jz.x, jx.y, jy.y, jz.y, jx.z, jy.z, jz.z);
 
-    for (int z = zs; z < ze; ++z) {
-        const int cur = z & 1, nxt = cur ^ 1;
-        // L plane z+1 from psi planes z, z+1, z+2
-        lap_plane(Lr[(z + 1) % 3], Qr[z % 3], Qr[(z + 1) % 3], Qr[(z + 2) % 3], d, tc.x0, tc.y0, z + 1);
-        // stage the next step's planes while this step computes
-        if (z + 1 < ze) {
-            stage<3, 2, H2C, H2R>(Vp[nxt], rv, e2, XY, z + 1);
-            stage<3, 1, H1C, H1R>(Pp[nxt], rp, e1, XY, z + 1);
-            stage<1, 1, H1C, H1R>(Rp[nxt], rr, e1, XY, z + 1);
-            stage<1, 1, H1C, H1R>(Sp[nxt], rs, e1, XY, z + 1);
-            stage<1, 1, H1C, H1R>(Tr[(z + 2) % 4], rt, e1, XY, z + 2);
-        }
-        __syncthreads();
-        if (z + 1 < ze) stage<3, 2, H2C, H2R>(Qr[z % 3], rq, e2, XY, z + 3);
+        // 3-4. L o psi_inv and T o psi_inv (vector_fields.cu:231-247, 81-99)
+        const F3 pc = p_0;
+        const Corner k = corner(d, pc.x, pc.y, pc.z);
+        const unsigned g = (unsigned)(k.gx + d.X * (k.gy + d.Y * k.gz));
+        const F3 Lo = tri_at3(rl, 12u * g, 12u, X12, XY12, k);
+        const float Tv = tri_at1(rt, 4u * g, 4u, X4, XY4, k);
 
-        const float *V = Vp[cur], *P = Pp[cur], *R = Rp[cur], *Sv = Sp[cur];
-        if (icol) {
-            const int i = i0 + XY * z;
-            // 1. density (solver.cu:101-173)
-            const F3 vc = v_0;
-            const float rc = r_0;
-            const float t11 = (vc.x > 0.f) ? -vc.x * rc : -vc.x * R[c1o + 1];
-            const float vxm = V[c2 - 1];
-            const float t12 = (vxm > 0.f) ? vxm * R[c1o - 1] : vxm * rc;
-            const float t21 = (vc.y > 0.f) ? -vc.y * rc : -vc.y * R[c1o + H1W];
-            const float vym = V[H2R + c2 - H2W];
-            const float t22 = (vym > 0.f) ? vym * R[c1o - H1W] : vym * rc;
-            const float t31 = (vc.z > 0.f) ? -vc.z * rc : -vc.z * r_p1;
-            const float vzm = v_m1.z;
-            const float t32 = (vzm > 0.f) ? vzm * r_m1 : vzm * rc;
-            const float div = ((((t11 + t12) + t21) + t22) + t31) + t32;
-            const float rho_n = fmaf(a.alpha, div, rc);
+        // 5. grad S and the potential gradient (vector_fields.cu:155-192, solver.cu:56-80)
+        const F3 gS = f3((bld1(rs, o4 + 4u) - bld1(rs, o4 - 4u)) * 0.5f, (bld1(rs, o4 + X4) - bld1(rs, o4 - X4)) * 0.5f,
+                         (s_p1 - s_m1) * 0.5f);
+        const float diff = s_0 - Tv;
+        const F3 nU = mul3(det, add3(mul3(diff, gS), mul3(a.w_reg, Lo)));
 
-            // 2. J_inv and det() (vector_fields.cu:503-535)
-            const F3 jx = mul3(0.5f, sub3(ldsc<H1R>(P, c1o + 1), ldsc<H1R>(P, c1o - 1)));
-            const F3 jy = mul3(0.5f, sub3(ldsc<H1R>(P, c1o + H1W), ldsc<H1R>(P, c1o - H1W)));
-            const F3 jz = mul3(0.5f, sub3(p_p1, p_m1));
-            const float det = det_ref(jx.x, jy.x, jz.x, jx.y, jy.y, jz.y, jx.z, jy.z, jz.z);
-
-            // 3-4. L o psi_inv and T o psi_inv: from the LDS rings when psi_inv(p) lies within one
-            // voxel below p on every axis, else by global gathers
-            const F3 pc = p_0;
-            const Corner k = corner(d, pc.x, pc.y, pc.z);
-            F3 Lo;
-            float Tp;
-            const int ox = k.gx - x, oy = k.gy - y, oz = k.gz - z;
-            if (ox >= -1 && ox <= 0 && oy >= -1 && oy <= 0 && oz >= -1 && oz <= 0) {
-                const int q = (ty + 1 + oy) * H1W + (tx + 1 + ox);   // corner g in halo-1 planes
-                const float *L0 = Lr[k.gz % 3], *L1 = Lr[(k.gz + 1) % 3];
-                const float *T0 = Tr[k.gz % 4], *T1 = Tr[(k.gz + 1) % 4];
-                const F3 l111 = lds3(L1, q + H1W + 1), l110 = lds3(L0, q + H1W + 1);
-                const F3 l101 = lds3(L1, q + 1), l100 = lds3(L0, q + 1);
-                const F3 l011 = lds3(L1, q + H1W), l010 = lds3(L0, q + H1W);
-                const F3 l001 = lds3(L1, q), l000 = lds3(L0, q);
-                Lo = f3(tri(l111.x, l110.x, l101.x, l100.x, l011.x, l010.x, l001.x, l000.x, k.a, k.b, k.c),
-                        tri(l111.y, l110.y, l101.y, l100.y, l011.y, l010.y, l001.y, l000.y, k.a, k.b, k.c),
-                        tri(l111.z, l110.z, l101.z, l100.z, l011.z, l010.z, l001.z, l000.z, k.a, k.b, k.c));
-                Tp = tri(T1[q + H1W + 1], T0[q + H1W + 1], T1[q + 1], T0[q + 1], T1[q + H1W], T0[q + H1W], T1[q], T0[q],
-                         k.a, k.b, k.c);
-            } else {
-                Lo = gather_lo(d, a.psi, k);
-                Tp = gather_t(d, a.T, k);
-            }
-
-            // 5. grad S and the potential gradient (vector_fields.cu:155-192, solver.cu:56-80)
-            const F3 gS = f3((Sv[c1o + 1] - Sv[c1o - 1]) * 0.5f, (Sv[c1o + H1W] - Sv[c1o - H1W]) * 0.5f,
-                             (s_p1 - s_m1) * 0.5f);
-            const float diff = s_0 - Tp;
-            const F3 nU = mul3(det, add3(mul3(diff, gS), mul3(a.w_reg, Lo)));
-
-            // 6. (Dv)v (vector_fields.cu:352-404)
-            const F3 xp = ldsc<H2R>(V, c2 + 1), xm = ldsc<H2R>(V, c2 - 1);
-            const F3 yp = ldsc<H2R>(V, c2 + H2W), ym = ldsc<H2R>(V, c2 - H2W);
-            const F3 zp = v_p1, zm = v_m1;
+        // 6. (Dv)v (vector_fields.cu:352-404)
+        const F3 xp = bld3(rv, o12 + 12u), xm = bld3(rv, o12 - 12u);
+        const F3 yp = bld3(rv, o12 + X12), ym = bld3(rv, o12 - X12);
+        const F3 zp = v_p1, zm = v_m1;
 #define MX(q) fmaxf((q), 0.f)
 #define MN(q) fminf((q), 0.f)
-            const float ax1 = ((MX(vc.x) * MX(vc.x) - MN(vc.x) * MN(vc.x)) + MN(xp.x) * MN(xp.x)) - MX(xp.x) * MX(xp.x);
-            const float ax2 = vc.y * ((vc.y > 0.f) ? (vc.x - ym.x) : (yp.x - vc.x));
-            const float ax3 = vc.z * ((vc.z > 0.f) ? (vc.x - zm.x) : (zp.x - vc.x));
-            const float ay1 = vc.x * ((vc.x > 0.f) ? (vc.y - xm.y) : (xp.y - vc.y));
-            const float ay2 = ((MX(vc.y) * MX(vc.y) - MN(vc.y) * MN(vc.y)) + MN(yp.y) * MN(yp.y)) - MX(yp.y) * MX(yp.y);
-            const float ay3 = vc.z * ((vc.z > 0.f) ? (vc.y - zm.y) : (zp.y - vc.y));
-            const float az1 = vc.x * ((vc.x > 0.f) ? (vc.z - xm.z) : (xp.z - vc.z));
-            const float az2 = vc.y * ((vc.y > 0.f) ? (vc.z - ym.z) : (yp.z - vc.z));
-            const float az3 = ((MX(vc.z) * MX(vc.z) - MN(vc.z) * MN(vc.z)) + MN(zp.z) * MN(zp.z)) - MX(zp.z) * MX(zp.z);
+        const float ax1 = ((MX(vc.x) * MX(vc.x) - MN(vc.x) * MN(vc.x)) + MN(xp.x) * MN(xp.x)) - MX(xp.x) * MX(xp.x);
+        const float ax2 = vc.y * ((vc.y > 0.f) ? (vc.x - ym.x) : (yp.x - vc.x));
+        const float ax3 = vc.z * ((vc.z > 0.f) ? (vc.x - zm.x) : (zp.x - vc.x));
+        const float ay1 = vc.x * ((vc.x > 0.f) ? (vc.y - xm.y) : (xp.y - vc.y));
+        const float ay2 = ((MX(vc.y) * MX(vc.y) - MN(vc.y) * MN(vc.y)) + MN(yp.y) * MN(yp.y)) - MX(yp.y) * MX(yp.y);
+        const float ay3 = vc.z * ((vc.z > 0.f) ? (vc.y - zm.y) : (zp.y - vc.y));
+        const float az1 = vc.x * ((vc.x > 0.f) ? (vc.z - xm.z) : (xp.z - vc.z));
+        const float az2 = vc.y * ((vc.y > 0.f) ? (vc.z - ym.z) : (yp.z - vc.z));
+        const float az3 = ((MX(vc.z) * MX(vc.z) - MN(vc.z) * MN(vc.z)) + MN(zp.z) * MN(zp.z)) - MX(zp.z) * MX(zp.z);
 #undef MX
 #undef MN
-            const F3 dvv = f3((0.5f * ax1 + ax2) + ax3, (ay1 + 0.5f * ay2) + ay3, (az1 + az2) + 0.5f * az3);
+        const F3 dvv = f3((0.5f * ax1 + ax2) + ax3, (ay1 + 0.5f * ay2) + ay3, (az1 + az2) + 0.5f * az3);
 
-            // 7. upwind velocity Laplacian, the reference's indices and ranges (vector_fields.cu:307-346);
-            // both branches have the form (A - 2B) + C
-            F3 dlt = f3(0.f, 0.f, 0.f);
-            if (x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2 && z >= 2 && z <= d.Z - 3) {
-                const bool nx = vc.x < 0.f, ny = vc.y < 0.f, nz = vc.z < 0.f;
-                const F3 Ax = nx ? ldsc<H2R>(V, c2 + 2) : vc, Bx = nx ? xp : xm, Cx = nx ? vc : ldsc<H2R>(V, c2 - 2);
-                const F3 Ay = ny ? ldsc<H2R>(V, c2 + 2 * H2W) : vc, Cy = ny ? vc : ldsc<H2R>(V, c2 - 2 * H2W);
-                const F3 Az = nz ? v_p2 : vc, Cz = nz ? vc : v_m2;
-                const F3 vxx = add3(sub3(Ax, mul3(2.f, Bx)), Cx);
-                const F3 vyy = add3(sub3(Ay, mul3(2.f, ym)), Cy);
-                const F3 vzz = add3(sub3(Az, mul3(2.f, zm)), Cz);
-                dlt = mul3(-1.f, add3(add3(vxx, vyy), vzz));
-            }
-
-            // 8. velocity update with this iteration's rho (solver.cu:188-211)
-            const F3 fr = mul3(a.m1t, vc);
-            const F3 df = mul3(a.tau, dlt);
-            const float qv = a.m2 / (fabsf(rho_n) + 1e-5f);
-            const F3 po = mul3(qv, nU);
-            const F3 up = mul3(a.alpha, add3(sub3(add3(fr, dvv), df), po));
-            const F3 vn = f3(vc.x - up.x, vc.y - up.y, vc.z - up.z);
-
-            // 10. psi_inv -= alpha * (J_inv v_new) (solver.cu:263-279)
-            const F3 Jv = f3((jx.x * vn.x + jy.x * vn.y) + jz.x * vn.z, (jx.y * vn.x + jy.y * vn.y) + jz.y * vn.z,
-                             (jx.z * vn.x + jy.z * vn.y) + jz.z * vn.z);
-            const F3 ui = mul3(a.alpha, Jv);
-            a.rho_n[i] = rho_n;
-            st3(a.v_n, i, vn);
-            st3(a.pinv_n, i, f3(pc.x - ui.x, pc.y - ui.y, pc.z - ui.z));
+        // 7. upwind velocity Laplacian, the reference's indices and ranges (vector_fields.cu:307-346);
+        // both branches have the form (A - 2B) + C.  At y = Y-2 the (y+2) read is the linear
+        // address (x, 0, z+1), inside the buffer.
+        F3 dlt = f3(0.f, 0.f, 0.f);
+        if (lap_xy && z >= 2 && z <= d.Z - 3) {
+            const bool nx = vc.x < 0.f, ny = vc.y < 0.f, nz = vc.z < 0.f;
+            const F3 Ax = nx ? bld3(rv, o12 + 24u) : vc, Bx = nx ? xp : xm, Cx = nx ? vc : bld3(rv, o12 - 24u);
+            const F3 Ay = ny ? bld3(rv, o12 + 2u * X12) : vc, Cy = ny ? vc : bld3(rv, o12 - 2u * X12);
+            const F3 Az = nz ? v_p2 : vc, Cz = nz ? vc : v_m2;
+            const F3 vxx = add3(sub3(Ax, mul3(2.f, Bx)), Cx);
+            const F3 vyy = add3(sub3(Ay, mul3(2.f, ym)), Cy);
+            const F3 vzz = add3(sub3(Az, mul3(2.f, zm)), Cz);
+            dlt = mul3(-1.f, add3(add3(vxx, vyy), vzz));
         }
+
+        // 8. velocity update with this iteration's rho (solver.cu:188-211)
+        const F3 fr = mul3(a.m1t, vc);
+        const F3 df = mul3(a.tau, dlt);
+        const float qv = a.m2 / (fabsf(rho_n) + 1e-5f);
+        const F3 po = mul3(qv, nU);
+        const F3 up = mul3(a.alpha, add3(sub3(add3(fr, dvv), df), po));
+        const F3 vn = f3(vc.x - up.x, vc.y - up.y, vc.z - up.z);
+
+        // 10. psi_inv -= alpha * (J_inv v_new) (solver.cu:263-279)
+        const F3 Jv = f3((jx.x * vn.x + jy.x * vn.y) + jz.x * vn.z, (jx.y * vn.x + jy.y * vn.y) + jz.y * vn.z,
+                         (jx.z * vn.x + jy.z * vn.y) + jz.z * vn.z);
+        const F3 ui = mul3(a.alpha, Jv);
+        a.rho_n[i] = rho_n;
+        st3(a.v_n, i, vn);
+        st3(a.pinv_n, i, f3(pc.x - ui.x, pc.y - ui.y, pc.z - ui.z));
+
         // slide the own-column windows
         v_m2 = v_m1;
         v_m1 = v_0;
         v_0 = v_p1;
         v_p1 = v_p2;
-        v_p2 = c3(rv, z + 3);
+        v_p2 = bld3(rv, 12u * (i + 3 * XY));
         r_m1 = r_0;
         r_0 = r_p1;
-        r_p1 = c1(rr, z + 2);
+        r_p1 = bld1(rr, 4u * (i + 2 * XY));
         p_m1 = p_0;
         p_0 = p_p1;
-        p_p1 = c3(rp, z + 2);
+        p_p1 = bld3(rp, 12u * (i + 2 * XY));
         s_m1 = s_0;
         s_0 = s_p1;
-        s_p1 = c1(rs, z + 2);
-        wait_vm();
-        __syncthreads();
+        s_p1 = bld1(rs, 4u * (i + 2 * XY));
     }
 }
 
@@ -640,19 +377,21 @@ __global__ __launch_bounds__(BLOCK) void k_pass_a(IterArgs a) {
 __global__ __launch_bounds__(BLOCK) void k_pass_b(IterArgs a) {
     if (__builtin_nontemporal_load(&a.ctl->stopped)) return;
     const Dims d = a.d;
-    const int XY = d.X * d.Y;
+    const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
     const TileCoord tc = tile_of(a, a.zc);
     const int tx = threadIdx.x & (TILE_X - 1), ty = threadIdx.x / TILE_X;
     const int x = tc.x0 + tx, y = tc.y0 + ty;
     unsigned long long pack = 0;
     if (x >= 1 && x <= d.X - 2 && y >= 1 && y <= d.Y - 2) {
-        const float *__restrict__ v = a.v_n;
+        const Rsrc rv = rsrc(a.v_n, 12 * N);
+        const unsigned X12 = 12u * d.X, XY12 = 12u * XY;
         for (int z = tc.zs; z < tc.ze; ++z) {
             // psi += alpha * v_new o psi; updates = that increment (solver.cu:230-247)
             const int i = x + d.X * y + XY * z;
             const F3 p = ld3(a.psi, i);
             const Corner k = corner(d, p.x, p.y, p.z);
-            const F3 w = gather_v(d, v, k);
+            const unsigned g = (unsigned)(k.gx + d.X * (k.gy + d.Y * k.gz));
+            const F3 w = tri_at3(rv, 12u * g, 12u, X12, XY12, k);
             const F3 u = mul3(a.alpha, w);
             st3(a.psi, i, f3(p.x + u.x, p.y + u.y, p.z + u.z));
             // norm with a round-down sqrt (cuda_utils.hpp:251-253); NaN never wins (strict >,
@@ -900,6 +639,11 @@ hipError_t launch_init(Dims d, const float *tgt, const int16_t *tgt_lab, const f
     int n = d.X * d.Y * d.Z;
     hipLaunchKernelGGL(k_init, dim3(grid_for(n, 256)), dim3(256), 0, s, d, tgt, tgt_lab, src, src_lab, T, LT, S, LS,
                        rho0, rho1, v0, v1, pinv0, pinv1, psi, rho_0);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass_l(const IterArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_pass_l, dim3(a.nblocks), dim3(BLOCK), 0, s, a);
     return hipGetLastError();
 }
 

@@ -156,9 +156,11 @@ int fr_solver_energy(fr_ctx *ctx, double *e_data, double *e_reg);
 int fr_warp(int X, int Y, int Z, const float *d_img, const int16_t *d_lab, const float *d_psi4, float *d_out_img,
             int16_t *d_out_lab, void *hip_stream);
 
-/* Average device time per launch of the solver's two iteration kernels, from HIP events around
- * `reps` launches each on the context stream (the state is restored afterwards). */
-int fr_time_kernels(fr_ctx *ctx, int reps, float *ms_pass_a, float *ms_pass_b);
+/* Average device time per launch of the solver's three iteration kernels (pass L: -Laplacian of
+ * psi; pass A: density, velocity and psi_inv updates; pass B: psi update and max norm), from HIP
+ * events around `reps` launches each on the context stream (the state is restored afterwards).
+ * Any output pointer may be NULL. */
+int fr_time_kernels(fr_ctx *ctx, int reps, float *ms_pass_l, float *ms_pass_a, float *ms_pass_b);
 
 /* ---- device memory plumbing (for callers without a HIP binding of their own) -------------- */
 
