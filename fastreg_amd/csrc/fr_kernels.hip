@@ -241,9 +241,197 @@ __global__ __launch_bounds__(BLOCK) void k_pass_l(IterArgs a) {
 
 // pass A: steps 1-8 and 10 of one iteration per interior voxel.  One thread per (x, y) column of
 // the tile, marching the chunk's z range; the column's z-neighbours (v at z-2..z+2, rho, psi_inv
-// and S at z-1..z+1) slide through registers, x/y neighbours are read from the current plane
-// (L1-resident: the neighbouring lanes and waves read the same rows), L and T are gathered at
-// psi_inv(p).
+// and S at z-1..z+1) slide through registers.  The loads of plane z+1 (x/y neighbours, and the L
+// and T taps at psi_inv) are issued before plane z is computed, so that their latency overlaps
+// the arithmetic: software pipelining across the z-march.
+struct PlaneLoads {
+    float r_xp, r_xm, r_yp, r_ym;    // rho at x+-1, y+-1
+    F3 v_xp, v_xm, v_yp, v_ym;       // v at x+-1, y+-1
+    F3 v_x2, v_y2;                   // v at x+-2 and y+-2 on the side the upwind Laplacian reads
+    F3 p_xp, p_xm, p_yp, p_ym;       // psi_inv at x+-1, y+-1
+    float s_xp, s_xm, s_yp, s_ym;    // S at x+-1, y+-1
+    Corner k;                        // corner and weights of psi_inv(p)
+    F3 l111, l110, l101, l100, l011, l010, l001, l000;   // L taps
+    float t111, t110, t101, t100, t011, t010, t001, t000; // T taps
+};
+
+struct PassAView {
+    Dims d;
+    Rsrc rv, rr, rp, rs, rt, rl;
+    unsigned X12, XY12, X4, XY4;
+};
+
+// issue the loads of the voxel at (linear index i) whose own v and psi_inv are vc and pc
+__device__ __forceinline__ void fetch_plane(PlaneLoads &n, const PassAView &w, int i, F3 vc, F3 pc) {
+    const unsigned o12 = 12u * i, o4 = 4u * i;
+    n.r_xp = bld1(w.rr, o4 + 4u);
+    n.r_xm = bld1(w.rr, o4 - 4u);
+    n.r_yp = bld1(w.rr, o4 + w.X4);
+    n.r_ym = bld1(w.rr, o4 - w.X4);
+    n.v_xp = bld3(w.rv, o12 + 12u);
+    n.v_xm = bld3(w.rv, o12 - 12u);
+    n.v_yp = bld3(w.rv, o12 + w.X12);
+    n.v_ym = bld3(w.rv, o12 - w.X12);
+    n.v_x2 = bld3(w.rv, vc.x < 0.f ? o12 + 24u : o12 - 24u);
+    n.v_y2 = bld3(w.rv, vc.y < 0.f ? o12 + 2u * w.X12 : o12 - 2u * w.X12);
+    n.p_xp = bld3(w.rp, o12 + 12u);
+    n.p_xm = bld3(w.rp, o12 - 12u);
+    n.p_yp = bld3(w.rp, o12 + w.X12);
+    n.p_ym = bld3(w.rp, o12 - w.X12);
+    n.s_xp = bld1(w.rs, o4 + 4u);
+    n.s_xm = bld1(w.rs, o4 - 4u);
+    n.s_yp = bld1(w.rs, o4 + w.X4);
+    n.s_ym = bld1(w.rs, o4 - w.X4);
+    n.k = corner(w.d, pc.x, pc.y, pc.z);
+    const unsigned g = (unsigned)(n.k.gx + w.d.X * (n.k.gy + w.d.Y * n.k.gz));
+    const unsigned g12 = 12u * g, g4 = 4u * g;
+    n.l111 = bld3(w.rl, g12 + 12u + w.X12 + w.XY12);
+    n.l110 = bld3(w.rl, g12 + 12u + w.X12);
+    n.l101 = bld3(w.rl, g12 + 12u + w.XY12);
+    n.l100 = bld3(w.rl, g12 + 12u);
+    n.l011 = bld3(w.rl, g12 + w.X12 + w.XY12);
+    n.l010 = bld3(w.rl, g12 + w.X12);
+    n.l001 = bld3(w.rl, g12 + w.XY12);
+    n.l000 = bld3(w.rl, g12);
+    n.t111 = bld1(w.rt, g4 + 4u + w.X4 + w.XY4);
+    n.t110 = bld1(w.rt, g4 + 4u + w.X4);
+    n.t101 = bld1(w.rt, g4 + 4u + w.XY4);
+    n.t100 = bld1(w.rt, g4 + 4u);
+    n.t011 = bld1(w.rt, g4 + w.X4 + w.XY4);
+    n.t010 = bld1(w.rt, g4 + w.X4);
+    n.t001 = bld1(w.rt, g4 + w.XY4);
+    n.t000 = bld1(w.rt, g4);
+}
+
+// own-column z-windows; psi_inv runs one plane further ahead than it is used, so that the corner
+// of the plane being prefetched comes from a value loaded a whole step earlier
+struct ColumnWindow {
+    F3 v_m2, v_m1, v_0, v_p1, v_p2, p_m1, p_0, p_p1, p_p2;
+    float r_m1, r_0, r_p1, s_m1, s_0, s_p1;
+};
+
+__device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, const PlaneLoads &n,
+                                        const ColumnWindow &c, int i, bool lap) {
+    // 1. density (solver.cu:101-173)
+    const F3 vc = c.v_0;
+    const float rc = c.r_0;
+    const float t11 = (vc.x > 0.f) ? -vc.x * rc : -vc.x * n.r_xp;
+    const float vxm = n.v_xm.x;
+    const float t12 = (vxm > 0.f) ? vxm * n.r_xm : vxm * rc;
+    const float t21 = (vc.y > 0.f) ? -vc.y * rc : -vc.y * n.r_yp;
+    const float vym = n.v_ym.y;
+    const float t22 = (vym > 0.f) ? vym * n.r_ym : vym * rc;
+    const float t31 = (vc.z > 0.f) ? -vc.z * rc : -vc.z * c.r_p1;
+    const float vzm = c.v_m1.z;
+    const float t32 = (vzm > 0.f) ? vzm * c.r_m1 : vzm * rc;
+    const float div = ((((t11 + t12) + t21) + t22) + t31) + t32;
+    const float rho_n = fmaf(a.alpha, div, rc);
+
+    // 2. J_inv and det() (vector_fields.cu:503-535)
+    const F3 jx = mul3(0.5f, sub3(n.p_xp, n.p_xm));
+    const F3 jy = mul3(0.5f, sub3(n.p_yp, n.p_ym));
+    const F3 jz = mul3(0.5f, sub3(c.p_p1, c.p_m1));
+    const float det = det_ref(jx.x, jy.x, jz.x, jx.y, jy.y, jz.y, jx.z, jy.z, jz.z);
+
+    // 3-4. L o psi_inv and T o psi_inv (vector_fields.cu:231-247, 81-99)
+    const Corner &k = n.k;
+    const F3 Lo = f3(tri(n.l111.x, n.l110.x, n.l101.x, n.l100.x, n.l011.x, n.l010.x, n.l001.x, n.l000.x, k.a, k.b, k.c),
+                     tri(n.l111.y, n.l110.y, n.l101.y, n.l100.y, n.l011.y, n.l010.y, n.l001.y, n.l000.y, k.a, k.b, k.c),
+                     tri(n.l111.z, n.l110.z, n.l101.z, n.l100.z, n.l011.z, n.l010.z, n.l001.z, n.l000.z, k.a, k.b, k.c));
+    const float Tv = tri(n.t111, n.t110, n.t101, n.t100, n.t011, n.t010, n.t001, n.t000, k.a, k.b, k.c);
+
+    // 5. grad S and the potential gradient (vector_fields.cu:155-192, solver.cu:56-80)
+    const F3 gS = f3((n.s_xp - n.s_xm) * 0.5f, (n.s_yp - n.s_ym) * 0.5f, (c.s_p1 - c.s_m1) * 0.5f);
+    const float diff = c.s_0 - Tv;
+    const F3 nU = mul3(det, add3(mul3(diff, gS), mul3(a.w_reg, Lo)));
+
+    // 6. (Dv)v (vector_fields.cu:352-404)
+    const F3 xp = n.v_xp, xm = n.v_xm, yp = n.v_yp, ym = n.v_ym;
+    const F3 zp = c.v_p1, zm = c.v_m1;
+#define MX(q) fmaxf((q), 0.f)
+#define MN(q) fminf((q), 0.f)
+    const float ax1 = ((MX(vc.x) * MX(vc.x) - MN(vc.x) * MN(vc.x)) + MN(xp.x) * MN(xp.x)) - MX(xp.x) * MX(xp.x);
+    const float ax2 = vc.y * ((vc.y > 0.f) ? (vc.x - ym.x) : (yp.x - vc.x));
+    const float ax3 = vc.z * ((vc.z > 0.f) ? (vc.x - zm.x) : (zp.x - vc.x));
+    const float ay1 = vc.x * ((vc.x > 0.f) ? (vc.y - xm.y) : (xp.y - vc.y));
+    const float ay2 = ((MX(vc.y) * MX(vc.y) - MN(vc.y) * MN(vc.y)) + MN(yp.y) * MN(yp.y)) - MX(yp.y) * MX(yp.y);
+    const float ay3 = vc.z * ((vc.z > 0.f) ? (vc.y - zm.y) : (zp.y - vc.y));
+    const float az1 = vc.x * ((vc.x > 0.f) ? (vc.z - xm.z) : (xp.z - vc.z));
+    const float az2 = vc.y * ((vc.y > 0.f) ? (vc.z - ym.z) : (yp.z - vc.z));
+    const float az3 = ((MX(vc.z) * MX(vc.z) - MN(vc.z) * MN(vc.z)) + MN(zp.z) * MN(zp.z)) - MX(zp.z) * MX(zp.z);
+#undef MX
+#undef MN
+    const F3 dvv = f3((0.5f * ax1 + ax2) + ax3, (ay1 + 0.5f * ay2) + ay3, (az1 + az2) + 0.5f * az3);
+
+    // 7. upwind velocity Laplacian, the reference's indices and ranges (vector_fields.cu:307-346);
+    // both branches have the form (A - 2B) + C.  At y = Y-2 the (y+2) read is the linear address
+    // (x, 0, z+1), inside the buffer.
+    F3 dlt = f3(0.f, 0.f, 0.f);
+    if (lap) {
+        const bool nx = vc.x < 0.f, ny = vc.y < 0.f, nz = vc.z < 0.f;
+        const F3 Ax = nx ? n.v_x2 : vc, Bx = nx ? xp : xm, Cx = nx ? vc : n.v_x2;
+        const F3 Ay = ny ? n.v_y2 : vc, Cy = ny ? vc : n.v_y2;
+        const F3 Az = nz ? c.v_p2 : vc, Cz = nz ? vc : c.v_m2;
+        const F3 vxx = add3(sub3(Ax, mul3(2.f, Bx)), Cx);
+        const F3 vyy = add3(sub3(Ay, mul3(2.f, ym)), Cy);
+        const F3 vzz = add3(sub3(Az, mul3(2.f, zm)), Cz);
+        dlt = mul3(-1.f, add3(add3(vxx, vyy), vzz));
+    }
+
+    // 8. velocity update with this iteration's rho (solver.cu:188-211)
+    const F3 fr = mul3(a.m1t, vc);
+    const F3 df = mul3(a.tau, dlt);
+    const float qv = a.m2 / (fabsf(rho_n) + 1e-5f);
+    const F3 po = mul3(qv, nU);
+    const F3 up = mul3(a.alpha, add3(sub3(add3(fr, dvv), df), po));
+    const F3 vn = f3(vc.x - up.x, vc.y - up.y, vc.z - up.z);
+
+    // 10. psi_inv -= alpha * (J_inv v_new) (solver.cu:263-279)
+    const F3 pc = c.p_0;
+    const F3 Jv = f3((jx.x * vn.x + jy.x * vn.y) + jz.x * vn.z, (jx.y * vn.x + jy.y * vn.y) + jz.y * vn.z,
+                     (jx.z * vn.x + jy.z * vn.y) + jz.z * vn.z);
+    const F3 ui = mul3(a.alpha, Jv);
+    a.rho_n[i] = rho_n;
+    st3(a.v_n, i, vn);
+    st3(a.pinv_n, i, f3(pc.x - ui.x, pc.y - ui.y, pc.z - ui.z));
+}
+
+// the window entries entering at the next step, loaded at the start of a step (before its stores,
+// which the compiler cannot move loads across) so that the slide at its end does not wait
+struct WindowIn {
+    F3 v;      // v at z+3
+    F3 p;      // psi_inv at z+3
+    float r;   // rho at z+2
+    float s;   // S at z+2
+};
+
+__device__ __forceinline__ WindowIn window_in(const PassAView &w, int i, int XY) {
+    WindowIn n;
+    n.v = bld3(w.rv, 12u * (i + 3 * XY));
+    n.p = bld3(w.rp, 12u * (i + 3 * XY));
+    n.r = bld1(w.rr, 4u * (i + 2 * XY));
+    n.s = bld1(w.rs, 4u * (i + 2 * XY));
+    return n;
+}
+
+__device__ __forceinline__ void slide(ColumnWindow &c, const WindowIn &n) {
+    c.v_m2 = c.v_m1;
+    c.v_m1 = c.v_0;
+    c.v_0 = c.v_p1;
+    c.v_p1 = c.v_p2;
+    c.v_p2 = n.v;
+    c.r_m1 = c.r_0;
+    c.r_0 = c.r_p1;
+    c.r_p1 = n.r;
+    c.p_m1 = c.p_0;
+    c.p_0 = c.p_p1;
+    c.p_p1 = c.p_p2;
+    c.p_p2 = n.p;
+    c.s_m1 = c.s_0;
+    c.s_0 = c.s_p1;
+    c.s_p1 = n.s;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_pass_a(IterArgs a) {
     if (should_stop(a, blockIdx.x == 0 && threadIdx.x == 0)) return;
     const Dims d = a.d;
@@ -251,120 +439,51 @@ __global__ __launch_bounds__(BLOCK) void k_pass_a(IterArgs a) {
     const TileCoord tc = tile_of(a, a.zc);
     const int x = tc.x0 + (threadIdx.x & (TILE_X - 1)), y = tc.y0 + threadIdx.x / TILE_X;
     if (x < 1 || x > d.X - 2 || y < 1 || y > d.Y - 2) return;
-    const Rsrc rv = rsrc(a.v_o, 12 * N), rr = rsrc(a.rho_o, 4 * N), rp = rsrc(a.pinv_o, 12 * N);
-    const Rsrc rs = rsrc(a.S, 4 * N), rt = rsrc(a.T, 4 * N), rl = rsrc(a.L, 12 * N);
-    const unsigned X12 = 12u * d.X, XY12 = 12u * XY, X4 = 4u * d.X, XY4 = 4u * XY;
+    PassAView w;
+    w.d = d;
+    w.rv = rsrc(a.v_o, 12 * N);
+    w.rr = rsrc(a.rho_o, 4 * N);
+    w.rp = rsrc(a.pinv_o, 12 * N);
+    w.rs = rsrc(a.S, 4 * N);
+    w.rt = rsrc(a.T, 4 * N);
+    w.rl = rsrc(a.L, 12 * N);
+    w.X12 = 12u * d.X;
+    w.XY12 = 12u * XY;
+    w.X4 = 4u * d.X;
+    w.XY4 = 4u * XY;
     const bool lap_xy = x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2;
 
     int i = x + d.X * y + XY * tc.zs;
-    // own-column z-windows
-    F3 v_m2 = bld3(rv, 12u * (i - 2 * XY)), v_m1 = bld3(rv, 12u * (i - XY)), v_0 = bld3(rv, 12u * i);
-    F3 v_p1 = bld3(rv, 12u * (i + XY)), v_p2 = bld3(rv, 12u * (i + 2 * XY));
-    float r_m1 = bld1(rr, 4u * (i - XY)), r_0 = bld1(rr, 4u * i), r_p1 = bld1(rr, 4u * (i + XY));
-    F3 p_m1 = bld3(rp, 12u * (i - XY)), p_0 = bld3(rp, 12u * i), p_p1 = bld3(rp, 12u * (i + XY));
-    float s_m1 = bld1(rs, 4u * (i - XY)), s_0 = bld1(rs, 4u * i), s_p1 = bld1(rs, 4u * (i + XY));
+    ColumnWindow c;
+    c.v_m2 = bld3(w.rv, 12u * (i - 2 * XY));
+    c.v_m1 = bld3(w.rv, 12u * (i - XY));
+    c.v_0 = bld3(w.rv, 12u * i);
+    c.v_p1 = bld3(w.rv, 12u * (i + XY));
+    c.v_p2 = bld3(w.rv, 12u * (i + 2 * XY));
+    c.r_m1 = bld1(w.rr, 4u * (i - XY));
+    c.r_0 = bld1(w.rr, 4u * i);
+    c.r_p1 = bld1(w.rr, 4u * (i + XY));
+    c.p_m1 = bld3(w.rp, 12u * (i - XY));
+    c.p_0 = bld3(w.rp, 12u * i);
+    c.p_p1 = bld3(w.rp, 12u * (i + XY));
+    c.p_p2 = bld3(w.rp, 12u * (i + 2 * XY));
+    c.s_m1 = bld1(w.rs, 4u * (i - XY));
+    c.s_0 = bld1(w.rs, 4u * i);
+    c.s_p1 = bld1(w.rs, 4u * (i + XY));
 
-    for (int z = tc.zs; z < tc.ze; ++z, i += XY) {
-        const unsigned o12 = 12u * i, o4 = 4u * i;
-        // 1. density (solver.cu:101-173)
-        const F3 vc = v_0;
-        const float rc = r_0;
-        const float t11 = (vc.x > 0.f) ? -vc.x * rc : -vc.x * bld1(rr, o4 + 4u);
-        const float vxm = bld1(rv, o12 - 12u);
-        const float t12 = (vxm > 0.f) ? vxm * bld1(rr, o4 - 4u) : vxm * rc;
-        const float t21 = (vc.y > 0.f) ? -vc.y * rc : -vc.y * bld1(rr, o4 + X4);
-        const float vym = bld1(rv, o12 - X12 + 4u);
-        const float t22 = (vym > 0.f) ? vym * bld1(rr, o4 - X4) : vym * rc;
-        const float t31 = (vc.z > 0.f) ? -vc.z * rc : -vc.z * r_p1;
-        const float vzm = v_m1.z;
-        const float t32 = (vzm > 0.f) ? vzm * r_m1 : vzm * rc;
-        const float div = ((((t11 + t12) + t21) + t22) + t31) + t32;
-        const float rho_n = fmaf(a.alpha, div, rc);
-
-        // 2. J_inv and det() (vector_fields.cu:503-535)
-        const F3 jx = mul3(0.5f, sub3(bld3(rp, o12 + 12u), bld3(rp, o12 - 12u)));
-        const F3 jy = mul3(0.5f, sub3(bld3(rp, o12 + X12), bld3(rp, o12 - X12)));
-        const F3 jz = mul3(0.5f, sub3(p_p1, p_m1));
-        const float det = det_ref(jx.x, jy.x, jz.x, jx.y, jy.y, jz.y, jx.z, jy.z, jz.z);
-
-        // 3-4. L o psi_inv and T o psi_inv (vector_fields.cu:231-247, 81-99)
-        const F3 pc = p_0;
-        const Corner k = corner(d, pc.x, pc.y, pc.z);
-        const unsigned g = (unsigned)(k.gx + d.X * (k.gy + d.Y * k.gz));
-        const F3 Lo = tri_at3(rl, 12u * g, 12u, X12, XY12, k);
-        const float Tv = tri_at1(rt, 4u * g, 4u, X4, XY4, k);
-
-        // 5. grad S and the potential gradient (vector_fields.cu:155-192, solver.cu:56-80)
-        const F3 gS = f3((bld1(rs, o4 + 4u) - bld1(rs, o4 - 4u)) * 0.5f, (bld1(rs, o4 + X4) - bld1(rs, o4 - X4)) * 0.5f,
-                         (s_p1 - s_m1) * 0.5f);
-        const float diff = s_0 - Tv;
-        const F3 nU = mul3(det, add3(mul3(diff, gS), mul3(a.w_reg, Lo)));
-
-        // 6. (Dv)v (vector_fields.cu:352-404)
-        const F3 xp = bld3(rv, o12 + 12u), xm = bld3(rv, o12 - 12u);
-        const F3 yp = bld3(rv, o12 + X12), ym = bld3(rv, o12 - X12);
-        const F3 zp = v_p1, zm = v_m1;
-#define MX(q) fmaxf((q), 0.f)
-#define MN(q) fminf((q), 0.f)
-        const float ax1 = ((MX(vc.x) * MX(vc.x) - MN(vc.x) * MN(vc.x)) + MN(xp.x) * MN(xp.x)) - MX(xp.x) * MX(xp.x);
-        const float ax2 = vc.y * ((vc.y > 0.f) ? (vc.x - ym.x) : (yp.x - vc.x));
-        const float ax3 = vc.z * ((vc.z > 0.f) ? (vc.x - zm.x) : (zp.x - vc.x));
-        const float ay1 = vc.x * ((vc.x > 0.f) ? (vc.y - xm.y) : (xp.y - vc.y));
-        const float ay2 = ((MX(vc.y) * MX(vc.y) - MN(vc.y) * MN(vc.y)) + MN(yp.y) * MN(yp.y)) - MX(yp.y) * MX(yp.y);
-        const float ay3 = vc.z * ((vc.z > 0.f) ? (vc.y - zm.y) : (zp.y - vc.y));
-        const float az1 = vc.x * ((vc.x > 0.f) ? (vc.z - xm.z) : (xp.z - vc.z));
-        const float az2 = vc.y * ((vc.y > 0.f) ? (vc.z - ym.z) : (yp.z - vc.z));
-        const float az3 = ((MX(vc.z) * MX(vc.z) - MN(vc.z) * MN(vc.z)) + MN(zp.z) * MN(zp.z)) - MX(zp.z) * MX(zp.z);
-#undef MX
-#undef MN
-        const F3 dvv = f3((0.5f * ax1 + ax2) + ax3, (ay1 + 0.5f * ay2) + ay3, (az1 + az2) + 0.5f * az3);
-
-        // 7. upwind velocity Laplacian, the reference's indices and ranges (vector_fields.cu:307-346);
-        // both branches have the form (A - 2B) + C.  At y = Y-2 the (y+2) read is the linear
-        // address (x, 0, z+1), inside the buffer.
-        F3 dlt = f3(0.f, 0.f, 0.f);
-        if (lap_xy && z >= 2 && z <= d.Z - 3) {
-            const bool nx = vc.x < 0.f, ny = vc.y < 0.f, nz = vc.z < 0.f;
-            const F3 Ax = nx ? bld3(rv, o12 + 24u) : vc, Bx = nx ? xp : xm, Cx = nx ? vc : bld3(rv, o12 - 24u);
-            const F3 Ay = ny ? bld3(rv, o12 + 2u * X12) : vc, Cy = ny ? vc : bld3(rv, o12 - 2u * X12);
-            const F3 Az = nz ? v_p2 : vc, Cz = nz ? vc : v_m2;
-            const F3 vxx = add3(sub3(Ax, mul3(2.f, Bx)), Cx);
-            const F3 vyy = add3(sub3(Ay, mul3(2.f, ym)), Cy);
-            const F3 vzz = add3(sub3(Az, mul3(2.f, zm)), Cz);
-            dlt = mul3(-1.f, add3(add3(vxx, vyy), vzz));
-        }
-
-        // 8. velocity update with this iteration's rho (solver.cu:188-211)
-        const F3 fr = mul3(a.m1t, vc);
-        const F3 df = mul3(a.tau, dlt);
-        const float qv = a.m2 / (fabsf(rho_n) + 1e-5f);
-        const F3 po = mul3(qv, nU);
-        const F3 up = mul3(a.alpha, add3(sub3(add3(fr, dvv), df), po));
-        const F3 vn = f3(vc.x - up.x, vc.y - up.y, vc.z - up.z);
-
-        // 10. psi_inv -= alpha * (J_inv v_new) (solver.cu:263-279)
-        const F3 Jv = f3((jx.x * vn.x + jy.x * vn.y) + jz.x * vn.z, (jx.y * vn.x + jy.y * vn.y) + jz.y * vn.z,
-                         (jx.z * vn.x + jy.z * vn.y) + jz.z * vn.z);
-        const F3 ui = mul3(a.alpha, Jv);
-        a.rho_n[i] = rho_n;
-        st3(a.v_n, i, vn);
-        st3(a.pinv_n, i, f3(pc.x - ui.x, pc.y - ui.y, pc.z - ui.z));
-
-        // slide the own-column windows
-        v_m2 = v_m1;
-        v_m1 = v_0;
-        v_0 = v_p1;
-        v_p1 = v_p2;
-        v_p2 = bld3(rv, 12u * (i + 3 * XY));
-        r_m1 = r_0;
-        r_0 = r_p1;
-        r_p1 = bld1(rr, 4u * (i + 2 * XY));
-        p_m1 = p_0;
-        p_0 = p_p1;
-        p_p1 = bld3(rp, 12u * (i + 2 * XY));
-        s_m1 = s_0;
-        s_0 = s_p1;
-        s_p1 = bld1(rs, 4u * (i + 2 * XY));
+    PlaneLoads n0, n1;
+    fetch_plane(n0, w, i, c.v_0, c.p_0);
+    // two steps per trip, alternating the staged plane loads between n0 and n1
+    for (int z = tc.zs; z < tc.ze; z += 2, i += 2 * XY) {
+        const WindowIn in0 = window_in(w, i, XY);
+        if (z + 1 < tc.ze) fetch_plane(n1, w, i + XY, c.v_p1, c.p_p1);
+        voxel_a(a, w, n0, c, i, lap_xy && z >= 2 && z <= d.Z - 3);
+        slide(c, in0);
+        if (z + 1 >= tc.ze) break;
+        const WindowIn in1 = window_in(w, i + XY, XY);
+        if (z + 2 < tc.ze) fetch_plane(n0, w, i + 2 * XY, c.v_p1, c.p_p1);
+        voxel_a(a, w, n1, c, i + XY, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3);
+        slide(c, in1);
     }
 }
 
