@@ -33,6 +33,8 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--cpu-iters", type=int, default=60, help="oracle iterations for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--zc", default="0,0,0",
+                    help="interior z-slices per block of pass L, pass A, pass B (0 = automatic)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="before timing, run one device-to-device copy of this many MiB (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -42,6 +44,7 @@ def main():
               f"--gpus {args.gpus} is not supported (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    zc = tuple(int(v) for v in args.zc.split(","))
     import numpy as np
     import fastreg_amd as fr
     from fastreg_amd import phantoms
@@ -67,6 +70,7 @@ def main():
     # warm-up on its own context
     pw = fr.default_params(vol_dims=dims, no_labels=pair.no_labels, max_iter=max(args.warmup, 1))
     w = fr.Solver(pw)
+    w.set_chunking(*zc)
     w.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
     w.run()
     w.stats()
@@ -74,6 +78,8 @@ def main():
 
     p = fr.default_params(vol_dims=dims, no_labels=pair.no_labels, max_iter=args.steps)
     s = fr.Solver(p)
+    s.set_chunking(*zc)
+    chunking = s.chunking()
     s.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
     fr.check(fr.lib().fr_dev_sync())
     tw0 = time.perf_counter()
@@ -86,7 +92,8 @@ def main():
     print(f"[bench] {args.steps} iterations: {ms:.2f} ms (events), {wall_ms:.2f} ms (wall)", file=sys.stderr,
           flush=True)
 
-    # per-kernel durations (HIP events around back-to-back launches on the context stream)
+    # per-kernel durations: HIP events between the launches of iterations run in sequence on the
+    # context stream (the caches as the iteration loop leaves them)
     ms_l, ms_a, ms_b = s.time_kernels(args.kernel_reps)
     # algorithmic bytes per interior voxel: pass L reads psi 12 and writes L 12; pass A reads
     # rho 4, v 12, psi_inv 12, S 4, L 12 and T 4 (gathered at psi_inv) and writes rho 4, v 12,
@@ -138,7 +145,8 @@ def main():
                 "velocity exponential)",
         "config": {"workload": "plain solver, 182x218x182 fp32, params.ini weights", "dims": list(dims),
                    "iterations": args.steps, "voxels": n, "dice_all": round(st2.dice_all, 6),
-                   "min_det_true": st2.min_det_true, "wall_ms": round(wall_ms, 2)},
+                   "min_det_true": st2.min_det_true, "wall_ms": round(wall_ms, 2),
+                   "z_chunk": {"pass_l": chunking[0], "pass_a": chunking[1], "pass_b": chunking[2]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

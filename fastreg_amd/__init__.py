@@ -84,6 +84,8 @@ def lib():
     sig("fr_destroy", vp)
     sig("fr_get_params", vp, P(FrParams))
     sig("fr_set_stream", vp, vp)
+    sig("fr_set_chunking", vp, i, i, i)
+    sig("fr_get_chunking", vp, P(i), P(i), P(i))
     sig("fr_register", vp, vp, vp, vp, vp)
     sig("fr_get_deformation", vp, vp)
     sig("fr_get_inverse_deformation", vp, vp)
@@ -321,8 +323,19 @@ class Solver:
         check(lib().fr_get_log(self.h, buf, size))
         return buf.value.decode()
 
+    def set_chunking(self, zc_l=0, zc_a=0, zc_b=0):
+        """Interior z-slices per block of pass L, pass A, pass B (0 = the automatic choice)."""
+        check(lib().fr_set_chunking(self.h, zc_l, zc_a, zc_b))
+
+    def chunking(self):
+        """(zc_l, zc_a, zc_b) in use."""
+        v = [C.c_int() for _ in range(3)]
+        check(lib().fr_get_chunking(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
     def time_kernels(self, reps=20):
-        """Average device ms per launch of (pass L, pass A, pass B)."""
+        """Average device ms per launch of (pass L, pass A, pass B), timed inside `reps`
+        iterations run in sequence."""
         l, a, b = C.c_float(), C.c_float(), C.c_float()
         check(lib().fr_time_kernels(self.h, reps, C.byref(l), C.byref(a), C.byref(b)))
         return l.value, a.value, b.value

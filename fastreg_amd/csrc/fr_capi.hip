@@ -62,7 +62,8 @@ struct fr_ctx {
     float *psi = nullptr;
     float *L = nullptr;  // -Laplacian(psi), boundary kept at 0
     int cur = 0;
-    int zc = 8;  // z-slices per block of the iteration kernels
+    int zc[3] = {8, 8, 8};       // interior z-slices per block of pass L, pass A, pass B
+    int zc_auto[3] = {8, 8, 8};  // the choice made at create
     bool initialised = false;
     // loop control and history
     frk::Ctl *ctl = nullptr;
@@ -95,7 +96,12 @@ static int check_params(const fr_params *p) {
         if (p->vol_dims[k] < 3) return fail(FR_ERR_INVALID, "VOL_DIMS must be >= 3 along every axis (got %d %d %d)",
                                             p->vol_dims[0], p->vol_dims[1], p->vol_dims[2]);
     long long n = (long long)p->vol_dims[0] * p->vol_dims[1] * p->vol_dims[2];
-    if (3 * n >= (1LL << 31)) return fail(FR_ERR_INVALID, "volume of %lld voxels is too large (3*N must fit in int32)", n);
+    if ((long long)p->vol_dims[0] * p->vol_dims[1] >= (1LL << 24))
+        return fail(FR_ERR_INVALID, "an x-y slice of %lld voxels is too large (X*Y must be < 2^24)",
+                    (long long)p->vol_dims[0] * p->vol_dims[1]);
+    // byte offsets into the packed 3-vector fields (12 B per voxel) are 32-bit buffer offsets
+    if (12 * n >= (1LL << 31))
+        return fail(FR_ERR_INVALID, "volume of %lld voxels is too large (12*N bytes must be < 2^31)", n);
     if (p->max_iter < 0) return fail(FR_ERR_INVALID, "MAX_ITER must be >= 0");
     if (p->no_labels < 1 || p->no_labels > 32768) return fail(FR_ERR_INVALID, "NO_LABELS must be in [1, 32768]");
     if (p->verbosity < 0 || p->verbosity > 2) return fail(FR_ERR_INVALID, "verbosity must be 0, 1 or 2");
@@ -146,6 +152,31 @@ static int dalloc(T **p, size_t count) {
     return FR_OK;
 }
 
+// z-slices per block of the iteration kernels.  Pass A holds its column windows and a second
+// plane of staged loads in registers (two blocks per CU), so its chunks are sized for the whole
+// grid to be resident at once: every block then marches one long column chunk, the halo planes
+// a chunk re-reads (z-2..z+2 of v) are amortised over it and no second wave of blocks runs
+// behind a partly idle first one.  Pass L and pass B are light streaming kernels with many
+// blocks per CU; they use short chunks.
+static int auto_chunking(fr_ctx *c) {
+    int dev = 0, cus = 0, occ = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(frk::pass_occupancy(1, &occ));
+    const long long tiles = (long long)((c->d.X - 2 + frk::XS_A - 1) / frk::XS_A) *
+                            ((c->d.Y - 2 + frk::TILE_Y - 1) / frk::TILE_Y);
+    const long long slots = (long long)(cus > 0 ? cus : 1) * (occ > 0 ? occ : 1);
+    const int interior = c->d.Z - 2;
+    long long chunks = slots / tiles;
+    if (chunks < 1) chunks = 1;
+    if (chunks > interior) chunks = interior;
+    c->zc_auto[0] = 8;
+    c->zc_auto[1] = (int)((interior + chunks - 1) / chunks);
+    c->zc_auto[2] = 8;
+    for (int k = 0; k < 3; ++k) c->zc[k] = c->zc_auto[k];
+    return FR_OK;
+}
+
 extern "C" int fr_create(const fr_params *p, fr_ctx **out) {
     if (!p || !out) return fail(FR_ERR_INVALID, "null argument");
     *out = nullptr;
@@ -177,6 +208,10 @@ extern "C" int fr_create(const fr_params *p, fr_ctx **out) {
         return fail(FR_ERR_HIP, "stream/event creation: %s", hipGetErrorString(e));
     }
     c->stream = c->own_stream;
+    if ((rc = auto_chunking(c))) {
+        free_ctx(c);
+        return rc;
+    }
     *out = c;
     return FR_OK;
 }
@@ -201,6 +236,23 @@ extern "C" int fr_set_stream(fr_ctx *c, void *s) {
     return FR_OK;
 }
 
+extern "C" int fr_set_chunking(fr_ctx *c, int zc_l, int zc_a, int zc_b) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    const int in[3] = {zc_l, zc_a, zc_b};
+    for (int k = 0; k < 3; ++k)
+        if (in[k] < 0) return fail(FR_ERR_INVALID, "z-chunk sizes must be >= 0 (0 = automatic)");
+    for (int k = 0; k < 3; ++k) c->zc[k] = in[k] ? in[k] : c->zc_auto[k];
+    return FR_OK;
+}
+
+extern "C" int fr_get_chunking(const fr_ctx *c, int *zc_l, int *zc_a, int *zc_b) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (zc_l) *zc_l = c->zc[0];
+    if (zc_a) *zc_a = c->zc[1];
+    if (zc_b) *zc_b = c->zc[2];
+    return FR_OK;
+}
+
 // ---------------------------------------------------------------------------------------------
 // solver
 // ---------------------------------------------------------------------------------------------
@@ -219,11 +271,7 @@ extern "C" int fr_solver_init(fr_ctx *c, const float *tgt, const int16_t *tgt_la
 
 static void iter_args(fr_ctx *c, frk::IterArgs &a) {
     a.d = c->d;
-    a.nbx = (c->d.X + frk::TILE_X - 1) / frk::TILE_X;
-    a.nby = (c->d.Y + frk::TILE_Y - 1) / frk::TILE_Y;
-    a.zc = c->zc;
-    a.nbz = (c->d.Z - 2 + a.zc - 1) / a.zc;
-    a.nblocks = a.nbx * a.nby * a.nbz;
+    a.nbx = a.nby = a.nbz = a.nblocks = a.zc = a.xs = 0;  // set per launch
     a.rho_o = c->rho[c->cur];
     a.rho_n = c->rho[c->cur ^ 1];
     a.v_o = c->v[c->cur];
@@ -272,9 +320,9 @@ static int launch_iteration(fr_ctx *c, int iter, float t, bool with_energy) {
                                             e + 2 * mi + (iter - 1), e + (iter - 1), c->stream));
         a.check_energy = 1;
     }
-    HIP_TRY(frk::launch_pass_l(a, c->stream));
-    HIP_TRY(frk::launch_pass_a(a, c->stream));
-    HIP_TRY(frk::launch_pass_b(a, c->stream));
+    HIP_TRY(frk::launch_pass_l(a, c->zc[0], c->stream));
+    HIP_TRY(frk::launch_pass_a(a, c->zc[1], c->stream));
+    HIP_TRY(frk::launch_pass_b(a, c->zc[2], c->stream));
     c->cur ^= 1;
     return FR_OK;
 }
@@ -609,7 +657,7 @@ extern "C" int fr_warp(int X, int Y, int Z, const float *img, const int16_t *lab
                        int16_t *out_lab, void *stream) {
     if (X < 1 || Y < 1 || Z < 1 || !psi4 || (!out_img && !out_lab) || (out_img && !img) || (out_lab && !lab))
         return fail(FR_ERR_INVALID, "bad fr_warp arguments");
-    if (3LL * X * Y * Z >= (1LL << 31)) return fail(FR_ERR_INVALID, "volume too large");
+    if (4LL * X * Y * Z >= (1LL << 31)) return fail(FR_ERR_INVALID, "volume too large (4*N must be < 2^31)");
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(frk::launch_warp(Dims{X, Y, Z}, img, lab, psi4, 4, out_img, out_lab, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -617,7 +665,7 @@ extern "C" int fr_warp(int X, int Y, int Z, const float *img, const int16_t *lab
 }
 
 extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_l, float *ms_a, float *ms_b) {
-    if (!c || reps < 1) return fail(FR_ERR_INVALID, "bad arguments");
+    if (!c || reps < 1 || reps > 10000) return fail(FR_ERR_INVALID, "bad arguments");
     if (!c->initialised) return fail(FR_ERR_STATE, "fr_time_kernels before fr_solver_init");
     int rc;
     if (c->have_run && (rc = collect_run(c))) return rc;
@@ -628,36 +676,45 @@ extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_l, float *ms_a, fl
     iter_args(c, a);
     a.iter = 1;
     vel_scalars(c->prm, c->prm.alpha, &a.m1t, &a.m2);
-    hipError_t e = hipMemcpyAsync(psi_save, c->psi, 3 * (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
-    float tl = 0.f, ta = 0.f, tb = 0.f;
-    // pass L and pass A read the current buffers and write L / the other buffers: repeating them
-    // is idempotent
-    if (e == hipSuccess) e = frk::launch_pass_l(a, c->stream);  // warm-up
-    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
-    for (int r = 0; r < reps && e == hipSuccess; ++r) e = frk::launch_pass_l(a, c->stream);
-    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&tl, c->ev0, c->ev1);
-    if (e == hipSuccess) e = frk::launch_pass_a(a, c->stream);  // warm-up
-    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
-    for (int r = 0; r < reps && e == hipSuccess; ++r) e = frk::launch_pass_a(a, c->stream);
-    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ta, c->ev0, c->ev1);
-    // pass B updates psi in place; psi is restored afterwards
-    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
-    for (int r = 0; r < reps && e == hipSuccess; ++r) e = frk::launch_pass_b(a, c->stream);
-    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&tb, c->ev0, c->ev1);
+    // reps iterations in sequence (L, A, B, L, A, B, ...) with an event between consecutive
+    // launches, so each kernel runs with the caches as the iteration loop leaves them.  The
+    // iteration is not advanced: pass A re-reads the same old buffers, pass B accumulates into psi,
+    // which is restored afterwards.
+    std::vector<hipEvent_t> ev(3 * (size_t)reps + 1, nullptr);
+    hipError_t e = hipSuccess;
+    for (size_t k = 0; k < ev.size() && e == hipSuccess; ++k) e = hipEventCreate(&ev[k]);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(psi_save, c->psi, 3 * (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = frk::launch_pass_l(a, c->zc[0], c->stream);  // warm-up iteration
+    if (e == hipSuccess) e = frk::launch_pass_a(a, c->zc[1], c->stream);
+    if (e == hipSuccess) e = frk::launch_pass_b(a, c->zc[2], c->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        e = hipEventRecord(ev[3 * r], c->stream);
+        if (e == hipSuccess) e = frk::launch_pass_l(a, c->zc[0], c->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 * r + 1], c->stream);
+        if (e == hipSuccess) e = frk::launch_pass_a(a, c->zc[1], c->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 * r + 2], c->stream);
+        if (e == hipSuccess) e = frk::launch_pass_b(a, c->zc[2], c->stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev[3 * (size_t)reps], c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[3 * (size_t)reps]);
+    double sum[3] = {0, 0, 0};
+    for (int r = 0; r < reps && e == hipSuccess; ++r)
+        for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+            float ms = 0.f;
+            e = hipEventElapsedTime(&ms, ev[3 * r + k], ev[3 * r + k + 1]);
+            sum[k] += ms;
+        }
     if (e == hipSuccess)
         e = hipMemcpyAsync(c->psi, psi_save, 3 * (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
     (void)hipFree(psi_save);
     if (e != hipSuccess) return fail(FR_ERR_HIP, "fr_time_kernels: %s", hipGetErrorString(e));
-    if (ms_l) *ms_l = tl / reps;
-    if (ms_a) *ms_a = ta / reps;
-    if (ms_b) *ms_b = tb / reps;
+    if (ms_l) *ms_l = (float)(sum[0] / reps);
+    if (ms_a) *ms_a = (float)(sum[1] / reps);
+    if (ms_b) *ms_b = (float)(sum[2] / reps);
     return FR_OK;
 }
 

@@ -19,18 +19,24 @@ struct Ctl {
     int pad;
 };
 
-// Tile of the iteration kernels: 64 x 4 columns per 256-thread block, marching over a chunk of
-// z-slices.
-constexpr int TILE_X = 64;
+// Tile of the iteration kernels: one 64-lane wave per x-row, TILE_Y rows per 256-thread block,
+// marching over a chunk of z-slices.  A wave whose kernel takes x-neighbours from the adjacent
+// lanes (pass L: x+-1, pass A: x+-2) computes the XS_* interior lanes of its row and keeps the
+// edge lanes as a halo; consecutive tiles are XS_* columns apart.
+constexpr int WAVE = 64;
 constexpr int TILE_Y = 4;
-constexpr int BLOCK = TILE_X * TILE_Y;
+constexpr int BLOCK = WAVE * TILE_Y;
+constexpr int XS_L = WAVE - 2;
+constexpr int XS_A = WAVE - 4;
+constexpr int XS_B = WAVE;
 // Pass B spreads its per-iteration 64-bit atomic maxima over NSLOT addresses.
 constexpr int NSLOT = 32;
 
 struct IterArgs {
     Dims d;
-    int nbx, nby, nbz, nblocks;  // tile grid: x tiles, y tiles, z chunks
-    int zc;                      // z-slices per chunk
+    int nbx, nby, nbz, nblocks;  // tile grid: x tiles, y tiles, z chunks (set by the launcher)
+    int zc;                      // interior z-slices per chunk (set by the launcher)
+    int xs;                      // x-columns computed per wave (set by the launcher)
     const float *rho_o;          // density, read
     float *rho_n;                // density, written
     const float *v_o;            // velocity xyz-packed, read
@@ -53,9 +59,13 @@ struct IterArgs {
 hipError_t launch_init(Dims d, const float *tgt, const int16_t *tgt_lab, const float *src, const int16_t *src_lab,
                        float *T, int16_t *LT, float *S, int16_t *LS, float *rho0, float *rho1, float *v0, float *v1,
                        float *pinv0, float *pinv1, float *psi, float rho_0, hipStream_t s);
-hipError_t launch_pass_l(const IterArgs &a, hipStream_t s);
-hipError_t launch_pass_a(const IterArgs &a, hipStream_t s);
-hipError_t launch_pass_b(const IterArgs &a, hipStream_t s);
+// The iteration kernels over blocks of TILE_X x TILE_Y columns by `zc` interior z-slices (the
+// launch sets a's tile-grid fields).
+hipError_t launch_pass_l(const IterArgs &a, int zc, hipStream_t s);
+hipError_t launch_pass_a(const IterArgs &a, int zc, hipStream_t s);
+hipError_t launch_pass_b(const IterArgs &a, int zc, hipStream_t s);
+// resident blocks per CU of pass L (0), pass A (1) or pass B (2)
+hipError_t pass_occupancy(int which, int *blocks_per_cu);
 hipError_t launch_energy(Dims d, const float *S, const float *T, const float *psi, double *partials, int nparts,
                          hipStream_t s);
 hipError_t launch_energy_finalize(const double *partials, int nparts, double w_reg, double *e_data, double *e_reg,

@@ -79,6 +79,12 @@ __device__ __forceinline__ Corner corner(Dims d, float px, float py, float pz) {
 
 __device__ __forceinline__ int lin(Dims d, int x, int y, int z) { return x + d.X * (y + d.Y * z); }
 
+// linear index of a corner (coordinates in the grid) with 24-bit multiplies: X and X*Y are below
+// 2^24 (checked when a context is created), and the products' low 32 bits are the index
+__device__ __forceinline__ unsigned corner_index(Dims d, const Corner &k) {
+    return (unsigned)k.gx + __umul24((unsigned)d.X, (unsigned)k.gy) + __umul24((unsigned)(d.X * d.Y), (unsigned)k.gz);
+}
+
 // sqrt rounded toward zero, from the correctly rounded sqrt and the exact FMA residual
 __device__ __forceinline__ float sqrt_rd(float a) {
     float s = sqrtf(a);
@@ -178,6 +184,18 @@ __device__ __forceinline__ F3 bld3(Rsrc r, unsigned voff) {
     return f3(bld1(r, voff), bld1(r, voff + 4u), bld1(r, voff + 8u));
 }
 
+// Buffer stores; a store at an offset past num_records is dropped.  Byte offsets stay below 2^31
+// (the library rejects volumes with 12*N >= 2^31), so NO_STORE is past every buffer.
+constexpr unsigned NO_STORE = 0x80000000u;
+__device__ __forceinline__ void bst1(Rsrc r, unsigned voff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, 0, 0);
+}
+__device__ __forceinline__ void bst3(Rsrc r, unsigned voff, F3 v) {
+    bst1(r, voff, v.x);
+    bst1(r, voff + 4u, v.y);
+    bst1(r, voff + 8u, v.z);
+}
+
 // Trilinear blend of a field at corner g with the +1 neighbours at byte offsets dx, dy, dz from
 // it.  The +1 neighbours are not clamped: when one leaves the grid its weight is exactly 0 and the
 // address either holds another voxel of the field (finite) or lies past the end of the buffer
@@ -195,61 +213,78 @@ __device__ __forceinline__ F3 tri_at3(Rsrc r, unsigned g, unsigned dx, unsigned 
               tri(f111.z, f110.z, f101.z, f100.z, f011.z, f010.z, f001.z, f000.z, k.a, k.b, k.c));
 }
 
+// Values of the adjacent lanes of the wave (wave-wide DPP shifts; the lanes of a wave are
+// consecutive x): lane_next gives lane l the value of lane l+1 (x+1), lane_prev that of lane l-1
+// (x-1).  The end lanes receive 0 and are never computed.
+__device__ __forceinline__ float lane_next(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float lane_prev(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ F3 lane_next3(F3 v) { return f3(lane_next(v.x), lane_next(v.y), lane_next(v.z)); }
+__device__ __forceinline__ F3 lane_prev3(F3 v) { return f3(lane_prev(v.x), lane_prev(v.y), lane_prev(v.z)); }
+
+// Thread -> voxel column.  A block is TILE_Y waves, one per row y; a wave computes a.xs
+// consecutive interior columns in its middle lanes and holds (WAVE - a.xs) / 2 halo lanes on each
+// side, which load their column but store nothing.  `own` marks a lane that stores.
 struct TileCoord {
-    int x0, y0, zs, ze;
+    int x, y, zs, ze;
+    bool own;
 };
 
-__device__ __forceinline__ TileCoord tile_of(const IterArgs &a, int zc) {
+__device__ __forceinline__ TileCoord tile_of(const IterArgs &a) {
     const int b = xcd_remap(blockIdx.x, a.nblocks);
     const int bx = b % a.nbx, r = b / a.nbx, by = r % a.nby, bz = r / a.nby;
+    const int lane = threadIdx.x & (WAVE - 1), off = (WAVE - a.xs) >> 1;
     TileCoord t;
-    t.x0 = bx * TILE_X;
-    t.y0 = by * TILE_Y;
-    t.zs = 1 + bz * zc;
-    t.ze = min(t.zs + zc, a.d.Z - 1);
+    t.x = 1 + bx * a.xs + lane - off;
+    t.y = 1 + by * TILE_Y + threadIdx.x / WAVE;
+    t.own = lane >= off && lane < off + a.xs && t.x <= a.d.X - 2;
+    t.zs = 1 + bz * a.zc;
+    t.ze = min(t.zs + a.zc, a.d.Z - 1);
     return t;
 }
 
 // pass L: L = -Laplacian(psi) on the interior (vector_fields.cu:275-300), into a buffer whose
-// boundary voxels stay 0.  One thread per (x, y) column of the tile, marching the chunk's z range
-// with psi(z-1), psi(z), psi(z+1) of the column in registers.
+// boundary voxels stay 0.  One thread per (x, y) column, marching the chunk's z range with
+// psi(z-1), psi(z), psi(z+1) of the column in registers and psi(x+-1) from the adjacent lanes.
 __global__ __launch_bounds__(BLOCK) void k_pass_l(IterArgs a) {
     if (__builtin_nontemporal_load(&a.ctl->stopped)) return;
     const Dims d = a.d;
     const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
-    const TileCoord tc = tile_of(a, a.zc);
-    const int x = tc.x0 + (threadIdx.x & (TILE_X - 1)), y = tc.y0 + threadIdx.x / TILE_X;
-    if (x < 1 || x > d.X - 2 || y < 1 || y > d.Y - 2) return;
-    const Rsrc rq = rsrc(a.psi, 12 * N);
-    const unsigned sx = 12u, sy = 12u * d.X, sz = 12u * XY;
-    unsigned o = 12u * (x + d.X * y + XY * tc.zs);
+    const TileCoord tc = tile_of(a);
+    if (tc.y > d.Y - 2) return;  // uniform over the wave
+    const Rsrc rq = rsrc(a.psi, 12 * N), wl = rsrc(a.L, 12 * N);
+    const unsigned sy = 12u * d.X, sz = 12u * XY;
+    unsigned o = 12u * (tc.x + d.X * tc.y + XY * tc.zs);
     F3 zm = bld3(rq, o - sz), c = bld3(rq, o);
     for (int z = tc.zs; z < tc.ze; ++z, o += sz) {
-        const F3 zp = bld3(rq, o + sz);
+        const F3 zp = bld3(rq, o + sz), yp = bld3(rq, o + sy), ym = bld3(rq, o - sy);
         F3 s = mul3(-6.f, c);
-        s = add3(s, bld3(rq, o + sx));
-        s = add3(s, bld3(rq, o - sx));
-        s = add3(s, bld3(rq, o + sy));
-        s = add3(s, bld3(rq, o - sy));
+        s = add3(s, lane_next3(c));
+        s = add3(s, lane_prev3(c));
+        s = add3(s, yp);
+        s = add3(s, ym);
         s = add3(s, zp);
         s = add3(s, zm);
-        st3(a.L, o / 12u, mul3(-1.f, s));
+        bst3(wl, tc.own ? o : NO_STORE, mul3(-1.f, s));
         zm = c;
         c = zp;
     }
 }
 
-// pass A: steps 1-8 and 10 of one iteration per interior voxel.  One thread per (x, y) column of
-// the tile, marching the chunk's z range; the column's z-neighbours (v at z-2..z+2, rho, psi_inv
-// and S at z-1..z+1) slide through registers.  The loads of plane z+1 (x/y neighbours, and the L
-// and T taps at psi_inv) are issued before plane z is computed, so that their latency overlaps
-// the arithmetic: software pipelining across the z-march.
+// pass A: steps 1-8 and 10 of one iteration per interior voxel.  One thread per (x, y) column,
+// marching the chunk's z range; the column's z-neighbours (v at z-2..z+2, rho, psi_inv and S at
+// z-1..z+1) slide through registers and the x-neighbours come from the adjacent lanes.  The loads
+// of plane z+1 (y-neighbours, and the L and T taps at psi_inv) are issued before plane z is
+// computed, so that their latency overlaps the arithmetic: software pipelining across the z-march.
 struct PlaneLoads {
-    float r_xp, r_xm, r_yp, r_ym;    // rho at x+-1, y+-1
-    F3 v_xp, v_xm, v_yp, v_ym;       // v at x+-1, y+-1
-    F3 v_x2, v_y2;                   // v at x+-2 and y+-2 on the side the upwind Laplacian reads
-    F3 p_xp, p_xm, p_yp, p_ym;       // psi_inv at x+-1, y+-1
-    float s_xp, s_xm, s_yp, s_ym;    // S at x+-1, y+-1
+    float r_yp, r_ym;                // rho at y+-1
+    F3 v_yp, v_ym;                   // v at y+-1
+    F3 v_y2;                         // v at y+-2 on the side the upwind Laplacian reads
+    F3 p_yp, p_ym;                   // psi_inv at y+-1
+    float s_yp, s_ym;                // S at y+-1
     Corner k;                        // corner and weights of psi_inv(p)
     F3 l111, l110, l101, l100, l011, l010, l001, l000;   // L taps
     float t111, t110, t101, t100, t011, t010, t001, t000; // T taps
@@ -257,33 +292,25 @@ struct PlaneLoads {
 
 struct PassAView {
     Dims d;
-    Rsrc rv, rr, rp, rs, rt, rl;
+    Rsrc rv, rr, rp, rs, rt, rl;  // loads: v, rho, psi_inv (old), S, T, L
+    Rsrc wv, wr, wp;              // stores: v, rho, psi_inv (new)
     unsigned X12, XY12, X4, XY4;
 };
 
 // issue the loads of the voxel at (linear index i) whose own v and psi_inv are vc and pc
 __device__ __forceinline__ void fetch_plane(PlaneLoads &n, const PassAView &w, int i, F3 vc, F3 pc) {
     const unsigned o12 = 12u * i, o4 = 4u * i;
-    n.r_xp = bld1(w.rr, o4 + 4u);
-    n.r_xm = bld1(w.rr, o4 - 4u);
     n.r_yp = bld1(w.rr, o4 + w.X4);
     n.r_ym = bld1(w.rr, o4 - w.X4);
-    n.v_xp = bld3(w.rv, o12 + 12u);
-    n.v_xm = bld3(w.rv, o12 - 12u);
     n.v_yp = bld3(w.rv, o12 + w.X12);
     n.v_ym = bld3(w.rv, o12 - w.X12);
-    n.v_x2 = bld3(w.rv, vc.x < 0.f ? o12 + 24u : o12 - 24u);
     n.v_y2 = bld3(w.rv, vc.y < 0.f ? o12 + 2u * w.X12 : o12 - 2u * w.X12);
-    n.p_xp = bld3(w.rp, o12 + 12u);
-    n.p_xm = bld3(w.rp, o12 - 12u);
     n.p_yp = bld3(w.rp, o12 + w.X12);
     n.p_ym = bld3(w.rp, o12 - w.X12);
-    n.s_xp = bld1(w.rs, o4 + 4u);
-    n.s_xm = bld1(w.rs, o4 - 4u);
     n.s_yp = bld1(w.rs, o4 + w.X4);
     n.s_ym = bld1(w.rs, o4 - w.X4);
     n.k = corner(w.d, pc.x, pc.y, pc.z);
-    const unsigned g = (unsigned)(n.k.gx + w.d.X * (n.k.gy + w.d.Y * n.k.gz));
+    const unsigned g = corner_index(w.d, n.k);
     const unsigned g12 = 12u * g, g4 = 4u * g;
     n.l111 = bld3(w.rl, g12 + 12u + w.X12 + w.XY12);
     n.l110 = bld3(w.rl, g12 + 12u + w.X12);
@@ -310,14 +337,23 @@ struct ColumnWindow {
     float r_m1, r_0, r_p1, s_m1, s_0, s_p1;
 };
 
+// every lane of the wave runs this (the x-neighbours are exchanged between lanes); `store` is
+// false on halo lanes and on the voxel past the chunk
 __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, const PlaneLoads &n,
-                                        const ColumnWindow &c, int i, bool lap) {
+                                        const ColumnWindow &c, int i, bool lap, bool store) {
+    // x-neighbours of rho, v, psi_inv, S from the adjacent lanes
+    const float r_xp = lane_next(c.r_0), r_xm = lane_prev(c.r_0);
+    const F3 v_xp = lane_next3(c.v_0), v_xm = lane_prev3(c.v_0);
+    const F3 v_xpp = lane_next3(v_xp), v_xmm = lane_prev3(v_xm);
+    const F3 p_xp = lane_next3(c.p_0), p_xm = lane_prev3(c.p_0);
+    const float s_xp = lane_next(c.s_0), s_xm = lane_prev(c.s_0);
+
     // 1. density (solver.cu:101-173)
     const F3 vc = c.v_0;
     const float rc = c.r_0;
-    const float t11 = (vc.x > 0.f) ? -vc.x * rc : -vc.x * n.r_xp;
-    const float vxm = n.v_xm.x;
-    const float t12 = (vxm > 0.f) ? vxm * n.r_xm : vxm * rc;
+    const float t11 = (vc.x > 0.f) ? -vc.x * rc : -vc.x * r_xp;
+    const float vxm = v_xm.x;
+    const float t12 = (vxm > 0.f) ? vxm * r_xm : vxm * rc;
     const float t21 = (vc.y > 0.f) ? -vc.y * rc : -vc.y * n.r_yp;
     const float vym = n.v_ym.y;
     const float t22 = (vym > 0.f) ? vym * n.r_ym : vym * rc;
@@ -328,7 +364,7 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const float rho_n = fmaf(a.alpha, div, rc);
 
     // 2. J_inv and det() (vector_fields.cu:503-535)
-    const F3 jx = mul3(0.5f, sub3(n.p_xp, n.p_xm));
+    const F3 jx = mul3(0.5f, sub3(p_xp, p_xm));
     const F3 jy = mul3(0.5f, sub3(n.p_yp, n.p_ym));
     const F3 jz = mul3(0.5f, sub3(c.p_p1, c.p_m1));
     const float det = det_ref(jx.x, jy.x, jz.x, jx.y, jy.y, jz.y, jx.z, jy.z, jz.z);
@@ -341,12 +377,12 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const float Tv = tri(n.t111, n.t110, n.t101, n.t100, n.t011, n.t010, n.t001, n.t000, k.a, k.b, k.c);
 
     // 5. grad S and the potential gradient (vector_fields.cu:155-192, solver.cu:56-80)
-    const F3 gS = f3((n.s_xp - n.s_xm) * 0.5f, (n.s_yp - n.s_ym) * 0.5f, (c.s_p1 - c.s_m1) * 0.5f);
+    const F3 gS = f3((s_xp - s_xm) * 0.5f, (n.s_yp - n.s_ym) * 0.5f, (c.s_p1 - c.s_m1) * 0.5f);
     const float diff = c.s_0 - Tv;
     const F3 nU = mul3(det, add3(mul3(diff, gS), mul3(a.w_reg, Lo)));
 
     // 6. (Dv)v (vector_fields.cu:352-404)
-    const F3 xp = n.v_xp, xm = n.v_xm, yp = n.v_yp, ym = n.v_ym;
+    const F3 xp = v_xp, xm = v_xm, yp = n.v_yp, ym = n.v_ym;
     const F3 zp = c.v_p1, zm = c.v_m1;
 #define MX(q) fmaxf((q), 0.f)
 #define MN(q) fminf((q), 0.f)
@@ -366,17 +402,17 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     // 7. upwind velocity Laplacian, the reference's indices and ranges (vector_fields.cu:307-346);
     // both branches have the form (A - 2B) + C.  At y = Y-2 the (y+2) read is the linear address
     // (x, 0, z+1), inside the buffer.
-    F3 dlt = f3(0.f, 0.f, 0.f);
-    if (lap) {
-        const bool nx = vc.x < 0.f, ny = vc.y < 0.f, nz = vc.z < 0.f;
-        const F3 Ax = nx ? n.v_x2 : vc, Bx = nx ? xp : xm, Cx = nx ? vc : n.v_x2;
-        const F3 Ay = ny ? n.v_y2 : vc, Cy = ny ? vc : n.v_y2;
-        const F3 Az = nz ? c.v_p2 : vc, Cz = nz ? vc : c.v_m2;
-        const F3 vxx = add3(sub3(Ax, mul3(2.f, Bx)), Cx);
-        const F3 vyy = add3(sub3(Ay, mul3(2.f, ym)), Cy);
-        const F3 vzz = add3(sub3(Az, mul3(2.f, zm)), Cz);
-        dlt = mul3(-1.f, add3(add3(vxx, vyy), vzz));
-    }
+    // evaluated on every lane and selected, so that the z-march has no branch
+    const bool nx = vc.x < 0.f, ny = vc.y < 0.f, nz = vc.z < 0.f;
+    const F3 v_x2 = nx ? v_xpp : v_xmm;
+    const F3 Ax = nx ? v_x2 : vc, Bx = nx ? xp : xm, Cx = nx ? vc : v_x2;
+    const F3 Ay = ny ? n.v_y2 : vc, Cy = ny ? vc : n.v_y2;
+    const F3 Az = nz ? c.v_p2 : vc, Cz = nz ? vc : c.v_m2;
+    const F3 vxx = add3(sub3(Ax, mul3(2.f, Bx)), Cx);
+    const F3 vyy = add3(sub3(Ay, mul3(2.f, ym)), Cy);
+    const F3 vzz = add3(sub3(Az, mul3(2.f, zm)), Cz);
+    const F3 dl = mul3(-1.f, add3(add3(vxx, vyy), vzz));
+    const F3 dlt = lap ? dl : f3(0.f, 0.f, 0.f);
 
     // 8. velocity update with this iteration's rho (solver.cu:188-211)
     const F3 fr = mul3(a.m1t, vc);
@@ -391,9 +427,12 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const F3 Jv = f3((jx.x * vn.x + jy.x * vn.y) + jz.x * vn.z, (jx.y * vn.x + jy.y * vn.y) + jz.y * vn.z,
                      (jx.z * vn.x + jy.z * vn.y) + jz.z * vn.z);
     const F3 ui = mul3(a.alpha, Jv);
-    a.rho_n[i] = rho_n;
-    st3(a.v_n, i, vn);
-    st3(a.pinv_n, i, f3(pc.x - ui.x, pc.y - ui.y, pc.z - ui.z));
+    // a voxel outside the chunk (the odd step of the last trip) stores at an offset past every
+    // buffer's range, which the buffer unit drops
+    const unsigned o4 = store ? 4u * i : NO_STORE, o12 = store ? 12u * i : NO_STORE;
+    bst1(w.wr, o4, rho_n);
+    bst3(w.wv, o12, vn);
+    bst3(w.wp, o12, f3(pc.x - ui.x, pc.y - ui.y, pc.z - ui.z));
 }
 
 // the window entries entering at the next step, loaded at the start of a step (before its stores,
@@ -436,9 +475,9 @@ __global__ __launch_bounds__(BLOCK) void k_pass_a(IterArgs a) {
     if (should_stop(a, blockIdx.x == 0 && threadIdx.x == 0)) return;
     const Dims d = a.d;
     const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
-    const TileCoord tc = tile_of(a, a.zc);
-    const int x = tc.x0 + (threadIdx.x & (TILE_X - 1)), y = tc.y0 + threadIdx.x / TILE_X;
-    if (x < 1 || x > d.X - 2 || y < 1 || y > d.Y - 2) return;
+    const TileCoord tc = tile_of(a);
+    const int x = tc.x, y = tc.y;
+    if (y > d.Y - 2) return;  // uniform over the wave
     PassAView w;
     w.d = d;
     w.rv = rsrc(a.v_o, 12 * N);
@@ -447,6 +486,9 @@ __global__ __launch_bounds__(BLOCK) void k_pass_a(IterArgs a) {
     w.rs = rsrc(a.S, 4 * N);
     w.rt = rsrc(a.T, 4 * N);
     w.rl = rsrc(a.L, 12 * N);
+    w.wv = rsrc(a.v_n, 12 * N);
+    w.wr = rsrc(a.rho_n, 4 * N);
+    w.wp = rsrc(a.pinv_n, 12 * N);
     w.X12 = 12u * d.X;
     w.XY12 = 12u * XY;
     w.X4 = 4u * d.X;
@@ -473,17 +515,28 @@ __global__ __launch_bounds__(BLOCK) void k_pass_a(IterArgs a) {
 
     PlaneLoads n0, n1;
     fetch_plane(n0, w, i, c.v_0, c.p_0);
-    // two steps per trip, alternating the staged plane loads between n0 and n1
-    for (int z = tc.zs; z < tc.ze; z += 2, i += 2 * XY) {
+    // drain the prologue's loads, so that the loop's memory-counter waits are set by the loop's
+    // own steady state and not by the prologue's last loads
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    // Two steps per trip, alternating the staged plane loads between n0 and n1.  The trip has no
+    // branch: the loads run ahead of the chunk unconditionally (range-checked buffer loads; the
+    // values are never stored) and the odd step of the last trip drops its stores, so the memory
+    // counter waits in the loop are the exact distances of one step of pipelining.
+    const int nz = tc.ze - tc.zs;
+    for (int s = 0, z = tc.zs; s < nz; s += 2, z += 2, i += 2 * XY) {
+        // (the scheduling barriers keep each half-step's loads ahead of its arithmetic)
         const WindowIn in0 = window_in(w, i, XY);
-        if (z + 1 < tc.ze) fetch_plane(n1, w, i + XY, c.v_p1, c.p_p1);
-        voxel_a(a, w, n0, c, i, lap_xy && z >= 2 && z <= d.Z - 3);
+        fetch_plane(n1, w, i + XY, c.v_p1, c.p_p1);
+        __builtin_amdgcn_sched_barrier(0);
+        voxel_a(a, w, n0, c, i, lap_xy && z >= 2 && z <= d.Z - 3, tc.own);
         slide(c, in0);
-        if (z + 1 >= tc.ze) break;
+        __builtin_amdgcn_sched_barrier(0);
         const WindowIn in1 = window_in(w, i + XY, XY);
-        if (z + 2 < tc.ze) fetch_plane(n0, w, i + 2 * XY, c.v_p1, c.p_p1);
-        voxel_a(a, w, n1, c, i + XY, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3);
+        fetch_plane(n0, w, i + 2 * XY, c.v_p1, c.p_p1);
+        __builtin_amdgcn_sched_barrier(0);
+        voxel_a(a, w, n1, c, i + XY, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, tc.own && s + 1 < nz);
         slide(c, in1);
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -497,11 +550,10 @@ __global__ __launch_bounds__(BLOCK) void k_pass_b(IterArgs a) {
     if (__builtin_nontemporal_load(&a.ctl->stopped)) return;
     const Dims d = a.d;
     const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
-    const TileCoord tc = tile_of(a, a.zc);
-    const int tx = threadIdx.x & (TILE_X - 1), ty = threadIdx.x / TILE_X;
-    const int x = tc.x0 + tx, y = tc.y0 + ty;
+    const TileCoord tc = tile_of(a);
+    const int x = tc.x, y = tc.y;
     unsigned long long pack = 0;
-    if (x >= 1 && x <= d.X - 2 && y >= 1 && y <= d.Y - 2) {
+    if (tc.own && y <= d.Y - 2) {
         const Rsrc rv = rsrc(a.v_n, 12 * N);
         const unsigned X12 = 12u * d.X, XY12 = 12u * XY;
         for (int z = tc.zs; z < tc.ze; ++z) {
@@ -509,7 +561,7 @@ __global__ __launch_bounds__(BLOCK) void k_pass_b(IterArgs a) {
             const int i = x + d.X * y + XY * z;
             const F3 p = ld3(a.psi, i);
             const Corner k = corner(d, p.x, p.y, p.z);
-            const unsigned g = (unsigned)(k.gx + d.X * (k.gy + d.Y * k.gz));
+            const unsigned g = corner_index(d, k);
             const F3 w = tri_at3(rv, 12u * g, 12u, X12, XY12, k);
             const F3 u = mul3(a.alpha, w);
             st3(a.psi, i, f3(p.x + u.x, p.y + u.y, p.z + u.z));
@@ -761,19 +813,38 @@ hipError_t launch_init(Dims d, const float *tgt, const int16_t *tgt_lab, const f
     return hipGetLastError();
 }
 
-hipError_t launch_pass_l(const IterArgs &a, hipStream_t s) {
+// the tile grid of one launch over the interior: xs x TILE_Y columns by zc slices per block
+static IterArgs chunked(IterArgs a, int zc, int xs) {
+    a.zc = zc < 1 ? 1 : zc;
+    a.xs = xs;
+    a.nbx = (a.d.X - 2 + xs - 1) / xs;
+    a.nby = (a.d.Y - 2 + TILE_Y - 1) / TILE_Y;
+    a.nbz = (a.d.Z - 2 + a.zc - 1) / a.zc;
+    a.nblocks = a.nbx * a.nby * a.nbz;
+    return a;
+}
+
+hipError_t launch_pass_l(const IterArgs &a0, int zc, hipStream_t s) {
+    const IterArgs a = chunked(a0, zc, XS_L);
     hipLaunchKernelGGL(k_pass_l, dim3(a.nblocks), dim3(BLOCK), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_pass_a(const IterArgs &a, hipStream_t s) {
+hipError_t launch_pass_a(const IterArgs &a0, int zc, hipStream_t s) {
+    const IterArgs a = chunked(a0, zc, XS_A);
     hipLaunchKernelGGL(k_pass_a, dim3(a.nblocks), dim3(BLOCK), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_pass_b(const IterArgs &a, hipStream_t s) {
+hipError_t launch_pass_b(const IterArgs &a0, int zc, hipStream_t s) {
+    const IterArgs a = chunked(a0, zc, XS_B);
     hipLaunchKernelGGL(k_pass_b, dim3(a.nblocks), dim3(BLOCK), 0, s, a);
     return hipGetLastError();
+}
+
+hipError_t pass_occupancy(int which, int *blocks_per_cu) {
+    const void *k = which == 0 ? (const void *)k_pass_l : which == 1 ? (const void *)k_pass_a : (const void *)k_pass_b;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, BLOCK, 0);
 }
 
 int energy_partials_count(Dims d) { return grid_for(d.X * d.Y * d.Z, RED_BLOCK, 4096); }

@@ -91,6 +91,13 @@ int fr_get_params(const fr_ctx *ctx, fr_params *p);
  * the context's own stream. */
 int fr_set_stream(fr_ctx *ctx, void *hip_stream);
 
+/* Interior z-slices per thread block of the three iteration kernels (pass L, pass A, pass B); 0
+ * restores the choice fr_create made for the device and volume.  Results do not depend on it (every
+ * voxel is computed the same way whatever the chunking).  No counterpart in the reference: it is a
+ * launch-shape setting of this implementation, for tuning. */
+int fr_set_chunking(fr_ctx *ctx, int zc_l, int zc_a, int zc_b);
+int fr_get_chunking(const fr_ctx *ctx, int *zc_l, int *zc_a, int *zc_b);
+
 /* FastReg::operator() (src/fastreg/fastreg.cpp:58-75): register the source (img2, lab2) to the
  * target (img1, lab1), all device pointers of no_voxels elements, and overwrite them in place:
  * img1/lab1 <- target o psi_inv, img2/lab2 <- source o psi.  Every call starts from a fresh
@@ -158,8 +165,9 @@ int fr_warp(int X, int Y, int Z, const float *d_img, const int16_t *d_lab, const
 
 /* Average device time per launch of the solver's three iteration kernels (pass L: -Laplacian of
  * psi; pass A: density, velocity and psi_inv updates; pass B: psi update and max norm), from HIP
- * events around `reps` launches each on the context stream (the state is restored afterwards).
- * Any output pointer may be NULL. */
+ * events between consecutive launches of `reps` iterations run in sequence on the context stream,
+ * as the iteration loop runs them (the state is restored afterwards).  Any output pointer may be
+ * NULL.  No counterpart in the reference: it is the per-kernel measurement of this solver. */
 int fr_time_kernels(fr_ctx *ctx, int reps, float *ms_pass_l, float *ms_pass_a, float *ms_pass_b);
 
 /* ---- device memory plumbing (for callers without a HIP binding of their own) -------------- */

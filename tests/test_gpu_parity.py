@@ -212,3 +212,25 @@ def test_full_size_brain_three_iterations(oracle_mod):
     mh, _ = g.history()
     assert np.array_equal(mh, r["max_hist"])
     _assert_state_equal(g.get_state(), o.get_state(), "brain")
+
+
+@pytest.mark.parametrize("zc", [(1, 1, 1), (3, 5, 2), (0, 0, 0), (40, 7, 40), (8, 60, 16)])
+def test_chunking_does_not_change_results(oracle_mod, zc):
+    """Every z-chunking of the three iteration kernels gives the oracle's iteration bit for bit
+    (chunk edges fall on different slices for each choice, including a single chunk and chunks
+    of one slice)."""
+    O = oracle_mod
+    dims = (70, 21, 40)
+    say("chunking", zc)
+    g, o, pair, rng = _both(dims, O, 13)
+    g.set_chunking(*zc)
+    got = g.chunking()
+    assert all(got[k] == zc[k] for k in range(3) if zc[k]), got
+    st = phantoms.random_state(dims, rng, amp=1.5)
+    g.set_state(**st)
+    o.set_state(**st)
+    for t in (0.01, 0.3):
+        mg, ag = g.step(t)
+        mo, ao = o.step(t)
+        assert mg == mo and ag == ao, (mg, mo, ag, ao)
+    _assert_state_equal(g.get_state(), o.get_state(), f"zc {zc}")
