@@ -23,6 +23,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+PROFILE = os.path.join(ROOT, "profiles", "solver_iteration_182x218x182.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of each pass from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (its `traffic_bytes_per_launch:` line), or None."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return {k: int(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].split())}
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -31,7 +44,8 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--kernel-reps", type=int, default=50)
-    ap.add_argument("--cpu-iters", type=int, default=60, help="oracle iterations for the CPU baseline sample")
+    ap.add_argument("--cpu-iters", type=int, default=120,
+                    help="oracle iterations for the CPU baseline sample (about 10-20 s on 16 host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--zc", default="0,0,0",
                     help="interior z-slices per block of pass L, pass A, pass B (0 = automatic)")
@@ -105,8 +119,11 @@ def main():
     dom = max(kern, key=lambda k: kern[k][0])
     dms, dbytes = kern[dom]
     achieved = dbytes / (dms / 1e3) / 1e9
+    prof = profiled_traffic()
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": prof.get(dom) if prof else None,
+                "traffic_source": os.path.relpath(PROFILE, ROOT) if prof else None,
                 "kernel_ms": {"pass_l": round(ms_l, 4), "pass_a": round(ms_a, 4), "pass_b": round(ms_b, 4)},
                 "alg_bytes_per_launch": {"pass_l": bytes_l, "pass_a": bytes_a, "pass_b": bytes_b},
                 "iteration_frac_88B": round(value * 88 / (HBM_PEAK_GBS * 1e9), 4)}
