@@ -164,7 +164,7 @@ static int auto_chunking(fr_ctx *c) {
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     HIP_TRY(frk::pass_occupancy(1, &occ));
     const long long tiles = (long long)((c->d.X - 2 + frk::XS_A - 1) / frk::XS_A) *
-                            ((c->d.Y - 2 + frk::TILE_Y - 1) / frk::TILE_Y);
+                            ((c->d.Y - 2 + frk::TY_A - 1) / frk::TY_A);
     const long long slots = (long long)(cus > 0 ? cus : 1) * (occ > 0 ? occ : 1);
     const int interior = c->d.Z - 2;
     long long chunks = slots / tiles;
@@ -271,7 +271,7 @@ extern "C" int fr_solver_init(fr_ctx *c, const float *tgt, const int16_t *tgt_la
 
 static void iter_args(fr_ctx *c, frk::IterArgs &a) {
     a.d = c->d;
-    a.nbx = a.nby = a.nbz = a.nblocks = a.zc = a.xs = 0;  // set per launch
+    a.nbx = a.nby = a.nbz = a.nblocks = a.zc = a.xs = a.ty = 0;  // set per launch
     a.rho_o = c->rho[c->cur];
     a.rho_n = c->rho[c->cur ^ 1];
     a.v_o = c->v[c->cur];

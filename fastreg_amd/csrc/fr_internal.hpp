@@ -19,13 +19,14 @@ struct Ctl {
     int pad;
 };
 
-// Tile of the iteration kernels: one 64-lane wave per x-row, TILE_Y rows per 256-thread block,
-// marching over a chunk of z-slices.  A wave whose kernel takes x-neighbours from the adjacent
-// lanes (pass L: x+-1, pass A: x+-2) computes the XS_* interior lanes of its row and keeps the
-// edge lanes as a halo; consecutive tiles are XS_* columns apart.
+// Tile of the iteration kernels: one 64-lane wave per x-row and TY_* rows per block, marching
+// over a chunk of z-slices.  Pass A runs 8 rows per block (one block of 8 waves per CU), so that
+// the rows a wave loads as y-neighbours and the region it gathers from are loaded into the CU's
+// L1 once for all eight waves.  A wave whose kernel takes x-neighbours from the adjacent lanes
+// (pass L: x+-1, pass A: x+-2) computes the XS_* interior lanes of its row and keeps the edge
+// lanes as a halo; consecutive tiles are XS_* columns apart.
 constexpr int WAVE = 64;
-constexpr int TILE_Y = 4;
-constexpr int BLOCK = WAVE * TILE_Y;
+constexpr int TY_L = 4, TY_A = 8, TY_B = 4;
 constexpr int XS_L = WAVE - 2;
 constexpr int XS_A = WAVE - 4;
 constexpr int XS_B = WAVE;
@@ -37,6 +38,7 @@ struct IterArgs {
     int nbx, nby, nbz, nblocks;  // tile grid: x tiles, y tiles, z chunks (set by the launcher)
     int zc;                      // interior z-slices per chunk (set by the launcher)
     int xs;                      // x-columns computed per wave (set by the launcher)
+    int ty;                      // rows (waves) per block (set by the launcher)
     const float *rho_o;          // density, read
     float *rho_n;                // density, written
     const float *v_o;            // velocity xyz-packed, read
@@ -59,7 +61,7 @@ struct IterArgs {
 hipError_t launch_init(Dims d, const float *tgt, const int16_t *tgt_lab, const float *src, const int16_t *src_lab,
                        float *T, int16_t *LT, float *S, int16_t *LS, float *rho0, float *rho1, float *v0, float *v1,
                        float *pinv0, float *pinv1, float *psi, float rho_0, hipStream_t s);
-// The iteration kernels over blocks of TILE_X x TILE_Y columns by `zc` interior z-slices (the
+// The iteration kernels over blocks of XS_* x TY_* columns by `zc` interior z-slices (the
 // launch sets a's tile-grid fields).
 hipError_t launch_pass_l(const IterArgs &a, int zc, hipStream_t s);
 hipError_t launch_pass_a(const IterArgs &a, int zc, hipStream_t s);

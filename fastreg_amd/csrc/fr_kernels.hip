@@ -225,7 +225,7 @@ __device__ __forceinline__ float lane_prev(float v) {
 __device__ __forceinline__ F3 lane_next3(F3 v) { return f3(lane_next(v.x), lane_next(v.y), lane_next(v.z)); }
 __device__ __forceinline__ F3 lane_prev3(F3 v) { return f3(lane_prev(v.x), lane_prev(v.y), lane_prev(v.z)); }
 
-// Thread -> voxel column.  A block is TILE_Y waves, one per row y; a wave computes a.xs
+// Thread -> voxel column.  A block is a.ty waves, one per row y; a wave computes a.xs
 // consecutive interior columns in its middle lanes and holds (WAVE - a.xs) / 2 halo lanes on each
 // side, which load their column but store nothing.  `own` marks a lane that stores.
 struct TileCoord {
@@ -239,7 +239,7 @@ __device__ __forceinline__ TileCoord tile_of(const IterArgs &a) {
     const int lane = threadIdx.x & (WAVE - 1), off = (WAVE - a.xs) >> 1;
     TileCoord t;
     t.x = 1 + bx * a.xs + lane - off;
-    t.y = 1 + by * TILE_Y + threadIdx.x / WAVE;
+    t.y = 1 + by * a.ty + threadIdx.x / WAVE;
     t.own = lane >= off && lane < off + a.xs && t.x <= a.d.X - 2;
     t.zs = 1 + bz * a.zc;
     t.ze = min(t.zs + a.zc, a.d.Z - 1);
@@ -249,7 +249,7 @@ __device__ __forceinline__ TileCoord tile_of(const IterArgs &a) {
 // pass L: L = -Laplacian(psi) on the interior (vector_fields.cu:275-300), into a buffer whose
 // boundary voxels stay 0.  One thread per (x, y) column, marching the chunk's z range with
 // psi(z-1), psi(z), psi(z+1) of the column in registers and psi(x+-1) from the adjacent lanes.
-__global__ __launch_bounds__(BLOCK) void k_pass_l(IterArgs a) {
+__global__ __launch_bounds__(WAVE * TY_L) void k_pass_l(IterArgs a) {
     if (__builtin_nontemporal_load(&a.ctl->stopped)) return;
     const Dims d = a.d;
     const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
@@ -471,13 +471,31 @@ __device__ __forceinline__ void slide(ColumnWindow &c, const WindowIn &n) {
     c.s_p1 = n.s;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_pass_a(IterArgs a) {
-    if (should_stop(a, blockIdx.x == 0 && threadIdx.x == 0)) return;
+// Scheduling groups of one half-step: a few loads first (the window entries, whose moves close the
+// step), then one load after every VALU_PER_LOAD VALU instructions.
+#define VALU_PER_LOAD 11
+#define SG_LOAD() __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0)
+#define SG_VALU() __builtin_amdgcn_sched_group_barrier(0x0002, VALU_PER_LOAD, 0)
+#define SG_PAIR() SG_LOAD(); SG_VALU()
+#define SG_PAIR4() SG_PAIR(); SG_PAIR(); SG_PAIR(); SG_PAIR()
+#define INTERLEAVE_LOADS() SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); \
+    SG_PAIR4(); SG_PAIR4(); SG_PAIR(); SG_PAIR()
+
+__global__ __launch_bounds__(WAVE * TY_A) void k_pass_a(IterArgs a) {
+    // The block's waves step through z together (a bare s_barrier per half-step, no memory
+    // fence), so that the rows one wave loads are still in L1 when the adjacent rows' waves read
+    // them as y-neighbours, and adjacent rows gather from overlapping regions at the same time.
+    // Every wave therefore runs the whole march: the stop test is taken once per block, and rows
+    // past the interior compute without storing.
+    __shared__ int stop_block;
+    if (threadIdx.x == 0) stop_block = should_stop(a, blockIdx.x == 0);
+    __syncthreads();
+    if (stop_block) return;  // uniform over the block
     const Dims d = a.d;
     const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
     const TileCoord tc = tile_of(a);
     const int x = tc.x, y = tc.y;
-    if (y > d.Y - 2) return;  // uniform over the wave
+    const bool own = tc.own && y <= d.Y - 2;
     PassAView w;
     w.d = d;
     w.rv = rsrc(a.v_o, 12 * N);
@@ -524,19 +542,22 @@ __global__ __launch_bounds__(BLOCK) void k_pass_a(IterArgs a) {
     // counter waits in the loop are the exact distances of one step of pipelining.
     const int nz = tc.ze - tc.zs;
     for (int s = 0, z = tc.zs; s < nz; s += 2, z += 2, i += 2 * XY) {
-        // (the scheduling barriers keep each half-step's loads ahead of its arithmetic)
+        // (each half-step interleaves the next plane's loads with the current plane's arithmetic,
+        // so that a wave keeps computing while its loads queue for the memory pipeline)
         const WindowIn in0 = window_in(w, i, XY);
         fetch_plane(n1, w, i + XY, c.v_p1, c.p_p1);
-        __builtin_amdgcn_sched_barrier(0);
-        voxel_a(a, w, n0, c, i, lap_xy && z >= 2 && z <= d.Z - 3, tc.own);
+        voxel_a(a, w, n0, c, i, lap_xy && z >= 2 && z <= d.Z - 3, own);
         slide(c, in0);
+        INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_barrier();
         const WindowIn in1 = window_in(w, i + XY, XY);
         fetch_plane(n0, w, i + 2 * XY, c.v_p1, c.p_p1);
-        __builtin_amdgcn_sched_barrier(0);
-        voxel_a(a, w, n1, c, i + XY, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, tc.own && s + 1 < nz);
+        voxel_a(a, w, n1, c, i + XY, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, own && s + 1 < nz);
         slide(c, in1);
+        INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_barrier();
     }
 }
 
@@ -546,7 +567,7 @@ __global__ __launch_bounds__(BLOCK) void k_pass_a(IterArgs a) {
 
 // One thread per (x, y) column of a 64 x 4 tile, marching `zc` z-slices: psi += alpha * v_new o
 // psi (gathered through L1/L2) and the running max of the update norm.
-__global__ __launch_bounds__(BLOCK) void k_pass_b(IterArgs a) {
+__global__ __launch_bounds__(WAVE * TY_B) void k_pass_b(IterArgs a) {
     if (__builtin_nontemporal_load(&a.ctl->stopped)) return;
     const Dims d = a.d;
     const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
@@ -576,12 +597,12 @@ __global__ __launch_bounds__(BLOCK) void k_pass_b(IterArgs a) {
     // max (norm, lowest index) over the block, then one atomic per block into one of the
     // iteration's NSLOT slots
     pack = wave_max_u64(pack);
-    __shared__ unsigned long long red[BLOCK / 64];
+    __shared__ unsigned long long red[TY_B];
     const int tid = threadIdx.x;
     if ((tid & 63) == 0) red[tid >> 6] = pack;
     __syncthreads();
     if (tid == 0) {
-        for (int w = 1; w < BLOCK / 64; ++w) pack = red[w] > pack ? red[w] : pack;
+        for (int w = 1; w < TY_B; ++w) pack = red[w] > pack ? red[w] : pack;
         unsigned long long *slot = &a.maxpack[(size_t)(a.iter - 1) * NSLOT + (blockIdx.x % NSLOT)];
         if (pack > __builtin_nontemporal_load(slot)) atomicMax(slot, pack);
     }
@@ -813,38 +834,40 @@ hipError_t launch_init(Dims d, const float *tgt, const int16_t *tgt_lab, const f
     return hipGetLastError();
 }
 
-// the tile grid of one launch over the interior: xs x TILE_Y columns by zc slices per block
-static IterArgs chunked(IterArgs a, int zc, int xs) {
+// the tile grid of one launch over the interior: xs x ty columns by zc slices per block
+static IterArgs chunked(IterArgs a, int zc, int xs, int ty) {
     a.zc = zc < 1 ? 1 : zc;
     a.xs = xs;
+    a.ty = ty;
     a.nbx = (a.d.X - 2 + xs - 1) / xs;
-    a.nby = (a.d.Y - 2 + TILE_Y - 1) / TILE_Y;
+    a.nby = (a.d.Y - 2 + ty - 1) / ty;
     a.nbz = (a.d.Z - 2 + a.zc - 1) / a.zc;
     a.nblocks = a.nbx * a.nby * a.nbz;
     return a;
 }
 
 hipError_t launch_pass_l(const IterArgs &a0, int zc, hipStream_t s) {
-    const IterArgs a = chunked(a0, zc, XS_L);
-    hipLaunchKernelGGL(k_pass_l, dim3(a.nblocks), dim3(BLOCK), 0, s, a);
+    const IterArgs a = chunked(a0, zc, XS_L, TY_L);
+    hipLaunchKernelGGL(k_pass_l, dim3(a.nblocks), dim3(WAVE * TY_L), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_pass_a(const IterArgs &a0, int zc, hipStream_t s) {
-    const IterArgs a = chunked(a0, zc, XS_A);
-    hipLaunchKernelGGL(k_pass_a, dim3(a.nblocks), dim3(BLOCK), 0, s, a);
+    const IterArgs a = chunked(a0, zc, XS_A, TY_A);
+    hipLaunchKernelGGL(k_pass_a, dim3(a.nblocks), dim3(WAVE * TY_A), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_pass_b(const IterArgs &a0, int zc, hipStream_t s) {
-    const IterArgs a = chunked(a0, zc, XS_B);
-    hipLaunchKernelGGL(k_pass_b, dim3(a.nblocks), dim3(BLOCK), 0, s, a);
+    const IterArgs a = chunked(a0, zc, XS_B, TY_B);
+    hipLaunchKernelGGL(k_pass_b, dim3(a.nblocks), dim3(WAVE * TY_B), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t pass_occupancy(int which, int *blocks_per_cu) {
     const void *k = which == 0 ? (const void *)k_pass_l : which == 1 ? (const void *)k_pass_a : (const void *)k_pass_b;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, BLOCK, 0);
+    const int ty = which == 0 ? TY_L : which == 1 ? TY_A : TY_B;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, WAVE * ty, 0);
 }
 
 int energy_partials_count(Dims d) { return grid_for(d.X * d.Y * d.Z, RED_BLOCK, 4096); }
