@@ -30,8 +30,9 @@ constexpr int TY_L = 4, TY_A = 8, TY_B = 4;
 constexpr int XS_L = WAVE - 2;
 constexpr int XS_A = WAVE - 4;
 constexpr int XS_B = WAVE;
-// Pass B spreads its per-iteration 64-bit atomic maxima over NSLOT addresses.
-constexpr int NSLOT = 32;
+// Pass B spreads its per-iteration 64-bit atomic maxima over NSLOT addresses (a multiple of the
+// wave size: one wave reads them back).
+constexpr int NSLOT = 256;
 
 struct IterArgs {
     Dims d;

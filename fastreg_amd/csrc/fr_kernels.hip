@@ -114,7 +114,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return (x < rem) ? x * (per + 1) + s : rem * (per + 1) + (x - rem) * per + s;
 }
 
-// max over the NSLOT slots of one iteration, evaluated by every wave of the block
+// max over the NSLOT slots of one iteration
 __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
     unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
     lo = __shfl_xor(lo, m, 64);
@@ -130,13 +130,19 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
+// (all 64 lanes of the calling wave must be active)
 __device__ __forceinline__ unsigned long long slots_max(const unsigned long long *slots) {
     const int lane = threadIdx.x & 63;
-    unsigned long long v = lane < NSLOT ? __builtin_nontemporal_load(&slots[lane]) : 0ull;
+    unsigned long long v = 0ull;
+    for (int j = lane; j < NSLOT; j += 64) {
+        const unsigned long long s = __builtin_nontemporal_load(&slots[j]);
+        v = s > v ? s : v;
+    }
     return wave_max_u64(v);
 }
 
-// stop rules checked at the start of each launch of iteration a.iter (uniform per block):
+// stop rules checked at the start of each launch of iteration a.iter, by a whole wave (uniform
+// over it):
 // converged at iter-1 (solver.cu:418-421) or NaN energy at iter (solver.cu:366-368)
 __device__ __forceinline__ bool should_stop(const IterArgs &a, bool record) {
     if (__builtin_nontemporal_load(&a.ctl->stopped)) return true;
@@ -488,7 +494,10 @@ __global__ __launch_bounds__(WAVE * TY_A) void k_pass_a(IterArgs a) {
     // Every wave therefore runs the whole march: the stop test is taken once per block, and rows
     // past the interior compute without storing.
     __shared__ int stop_block;
-    if (threadIdx.x == 0) stop_block = should_stop(a, blockIdx.x == 0);
+    if (threadIdx.x < 64) {  // the whole first wave: the slot maximum is a wave reduction
+        const bool st = should_stop(a, blockIdx.x == 0 && threadIdx.x == 0);
+        if (threadIdx.x == 0) stop_block = st;
+    }
     __syncthreads();
     if (stop_block) return;  // uniform over the block
     const Dims d = a.d;

@@ -234,3 +234,23 @@ def test_chunking_does_not_change_results(oracle_mod, zc):
         mo, ao = o.step(t)
         assert mg == mo and ag == ao, (mg, mo, ag, ao)
     _assert_state_equal(g.get_state(), o.get_state(), f"zc {zc}")
+
+
+def test_convergence_at_an_exact_threshold(oracle_mod):
+    """MAX_UPDATE_NORM set to the oracle's max norm of iteration 5: the GPU stops at the same
+    iteration with the same message.  The device-side test must read the maximum over all of the
+    iteration's slots (a partial maximum would stop earlier)."""
+    O = oracle_mod
+    dims = (70, 60, 50)
+    say("threshold", dims)
+    g0, o0, pair, _ = _both(dims, O, 21, max_iter=12, alpha=0.3)
+    mh = o0.run(O.SolverParams(max_iter=12, alpha=0.3, no_labels=pair.no_labels))["max_hist"]
+    thr = float(mh[4])
+    g, o, pair, _ = _both(dims, O, 21, max_iter=12, alpha=0.3, max_update_norm=thr)
+    r = o.run(O.SolverParams(max_iter=12, alpha=0.3, no_labels=pair.no_labels, max_update_norm=thr))
+    g.run()
+    st = g.stats()
+    assert r["stop_reason"] == 1 and r["iterations"] <= 5
+    assert st.iterations == r["iterations"] and st.stop_reason == r["stop_reason"], (st.iterations, r)
+    assert f"SOLVER CONVERGED AFTER {r['iterations']} ITERATIONS" in g.log()
+    _assert_state_equal(g.get_state(), o.get_state(), "threshold")
