@@ -85,6 +85,11 @@ def lib():
     sig("fr_get_params", vp, P(FrParams))
     sig("fr_set_stream", vp, vp)
     sig("fr_set_chunking", vp, i, i, i)
+    sig("fr_batch_create", P(vp), i, P(vp))
+    sig("fr_batch_destroy", vp)
+    sig("fr_batch_relayout", vp)
+    sig("fr_batch_run", vp)
+    sig("fr_batch_time", vp, P(f))
     sig("fr_get_chunking", vp, P(i), P(i), P(i))
     sig("fr_register", vp, vp, vp, vp, vp)
     sig("fr_get_deformation", vp, vp)
@@ -260,6 +265,10 @@ class Solver:
                         DeviceArray.from_host(np.asarray(src_lab, np.int16).reshape(n))]
         check(lib().fr_solver_init(self.h, *[b.ptr for b in self._inputs]))
 
+    def reinit(self):
+        """fr_solver_init again from the inputs of the last init (already on the device)."""
+        check(lib().fr_solver_init(self.h, *[b.ptr for b in self._inputs]))
+
     def step(self, t):
         m, am = C.c_float(), C.c_longlong()
         check(lib().fr_solver_step(self.h, t, C.byref(m), C.byref(am)))
@@ -339,3 +348,37 @@ class Solver:
         l, a, b = C.c_float(), C.c_float(), C.c_float()
         check(lib().fr_time_kernels(self.h, reps, C.byref(l), C.byref(a), C.byref(b)))
         return l.value, a.value, b.value
+
+
+class Batch:
+    """fr_batch: the iteration loops of several Solvers (volumes of any sizes, the same iteration
+    schedule) run together, one launch per pass per iteration, replayed from a hipGraph."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        h = C.c_void_p()
+        check(lib().fr_batch_create(arr, len(self.solvers), C.byref(h)))
+        self.h = h
+
+    def relayout(self):
+        check(lib().fr_batch_relayout(self.h))
+
+    def run(self):
+        check(lib().fr_batch_run(self.h))
+
+    def time_ms(self):
+        ms = C.c_float()
+        check(lib().fr_batch_time(self.h, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().fr_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

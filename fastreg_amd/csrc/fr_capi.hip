@@ -664,6 +664,204 @@ extern "C" int fr_warp(int X, int Y, int Z, const float *img, const int16_t *lab
     return FR_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// batches: many volumes advanced together, one launch per pass per iteration, run as a hipGraph
+// ---------------------------------------------------------------------------------------------
+
+struct fr_batch {
+    std::vector<fr_ctx *> ctx;
+    frk::IterArgs *d_base = nullptr;  // 3 per volume: pass L, A, B arguments at buffer parity 0
+    int *d_starts = nullptr;          // 3 x (n + 1): first block of each volume in each pass's grid
+    int total[3] = {0, 0, 0};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, done = nullptr;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};  // the whole run, per start parity
+    hipGraph_t graph[2] = {nullptr, nullptr};
+};
+
+static void free_batch(fr_batch *b) {
+    for (int k = 0; k < 2; ++k) {
+        if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+        if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+    }
+    if (b->d_base) (void)hipFree(b->d_base);
+    if (b->d_starts) (void)hipFree(b->d_starts);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// the launch-time arguments of every volume and the block ranges of each pass
+static int batch_layout(fr_batch *b) {
+    const int n = (int)b->ctx.size();
+    std::vector<frk::IterArgs> base(3 * (size_t)n);
+    std::vector<int> starts(3 * (size_t)(n + 1));
+    for (int pass = 0; pass < 3; ++pass) {
+        long long at = 0;
+        for (int q = 0; q < n; ++q) {
+            fr_ctx *c = b->ctx[q];
+            frk::IterArgs a;
+            const int save = c->cur;
+            c->cur = 0;
+            iter_args(c, a);
+            c->cur = save;
+            a = frk::pass_args(a, pass, c->zc[pass]);
+            base[3 * (size_t)q + pass] = a;
+            starts[(size_t)pass * (n + 1) + q] = (int)at;
+            at += (a.nblocks + 7) / 8 * 8;  // each volume's run starts at a multiple of 8 blocks
+            if (at >= INT_MAX / 2) return fail(FR_ERR_INVALID, "batch grid too large");
+        }
+        starts[(size_t)pass * (n + 1) + n] = (int)at;
+        b->total[pass] = (int)at;
+    }
+    HIP_TRY(hipMemcpy(b->d_base, base.data(), sizeof(frk::IterArgs) * base.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_starts, starts.data(), sizeof(int) * starts.size(), hipMemcpyHostToDevice));
+    for (int k = 0; k < 2; ++k) {  // captured runs refer to the old layout
+        if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+        if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+        b->exec[k] = nullptr;
+        b->graph[k] = nullptr;
+    }
+    return FR_OK;
+}
+
+extern "C" int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
+    if (!ctxs || n < 1 || !out) return fail(FR_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    for (int q = 0; q < n; ++q) {
+        if (!ctxs[q]) return fail(FR_ERR_INVALID, "null context %d", q);
+        for (int r = 0; r < q; ++r)
+            if (ctxs[r] == ctxs[q]) return fail(FR_ERR_INVALID, "context %d appears twice", q);
+        const fr_params &p = ctxs[q]->prm, &p0 = ctxs[0]->prm;
+        if (p.max_iter != p0.max_iter || p.alpha != p0.alpha || p.reset_iter != p0.reset_iter || p.p != p0.p ||
+            p.C != p0.C || p.verbosity != p0.verbosity)
+            return fail(FR_ERR_INVALID, "context %d: MAX_ITER, ALPHA, RESET_ITER, p, C and verbosity must be the same "
+                                        "for every volume of a batch", q);
+    }
+    fr_batch *b = new fr_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    int rc;
+    if ((rc = dalloc(&b->d_base, 3 * (size_t)n)) || (rc = dalloc(&b->d_starts, 3 * (size_t)(n + 1)))) {
+        free_batch(b);
+        return rc;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        free_batch(b);
+        return fail(FR_ERR_HIP, "batch stream/events: %s", hipGetErrorString(e));
+    }
+    if ((rc = batch_layout(b))) {
+        free_batch(b);
+        return rc;
+    }
+    *out = b;
+    return FR_OK;
+}
+
+extern "C" int fr_batch_destroy(fr_batch *b) {
+    if (!b) return FR_OK;
+    (void)hipStreamSynchronize(b->stream);
+    free_batch(b);
+    return FR_OK;
+}
+
+// Re-read every volume's z-chunking (after fr_set_chunking on its context).
+extern "C" int fr_batch_relayout(fr_batch *b) {
+    if (!b) return fail(FR_ERR_INVALID, "null batch");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return batch_layout(b);
+}
+
+// the launches of the whole run from buffer parity `par`, in the order fr_solver_run issues them
+static int batch_launches(fr_batch *b, int par) {
+    const fr_params &p = b->ctx[0]->prm;
+    const int n = (int)b->ctx.size();
+    float t = p.alpha;
+    for (int iter = 1; iter <= p.max_iter; ++iter) {
+        frk::BatchIter it;
+        it.iter = iter;
+        it.cur = par ^ ((iter - 1) & 1);
+        it.check_energy = verbose_now(p, iter) ? 1 : 0;
+        vel_scalars(p, t, &it.m1t, &it.m2);
+        if (it.check_energy) {
+            const int mi = p.max_iter > 0 ? p.max_iter : 1;
+            for (int q = 0; q < n; ++q) {
+                fr_ctx *c = b->ctx[q];
+                double *e = c->energy;
+                HIP_TRY(frk::launch_energy(c->d, c->S, c->T, c->psi, c->partials, c->nparts, b->stream));
+                HIP_TRY(frk::launch_energy_finalize(c->partials, c->nparts, (double)c->prm.w_reg, e + mi + (iter - 1),
+                                                    e + 2 * mi + (iter - 1), e + (iter - 1), b->stream));
+            }
+        }
+        for (int pass = 0; pass < 3; ++pass)
+            HIP_TRY(frk::launch_pass_batch(pass, b->d_base, b->d_starts + (size_t)pass * (n + 1), n, b->total[pass], it,
+                                           b->stream));
+        t += p.alpha;
+        if (p.reset_iter > 0 && (iter + 1) % p.reset_iter == 0) t = p.alpha;
+    }
+    return FR_OK;
+}
+
+extern "C" int fr_batch_run(fr_batch *b) {
+    if (!b) return fail(FR_ERR_INVALID, "null batch");
+    const int n = (int)b->ctx.size();
+    int rc;
+    const int par = b->ctx[0]->cur;
+    for (int q = 0; q < n; ++q) {
+        fr_ctx *c = b->ctx[q];
+        if (!c->initialised) return fail(FR_ERR_STATE, "batch volume %d: fr_batch_run before fr_solver_init", q);
+        if (c->have_run && (rc = collect_run(c))) return rc;  // settles c->cur of an earlier run
+        if (c->cur != par)
+            return fail(FR_ERR_STATE, "batch volumes are at different buffer parities (volume %d); re-initialise them", q);
+    }
+    for (int q = 0; q < n; ++q) {  // control words cleared eagerly (no memset nodes in the graph)
+        if ((rc = reset_control(b->ctx[q]))) return rc;
+        HIP_TRY(hipStreamSynchronize(b->ctx[q]->stream));
+    }
+    if (!b->exec[par]) {
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        rc = batch_launches(b, par);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(FR_ERR_HIP, "batch capture: %s", hipGetErrorString(e));
+        b->graph[par] = g;
+        HIP_TRY(hipGraphInstantiate(&b->exec[par], g, nullptr, nullptr, 0));
+    }
+    for (int q = 0; q < n; ++q) HIP_TRY(hipEventRecord(b->ctx[q]->ev0, b->stream));
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    HIP_TRY(hipGraphLaunch(b->exec[par], b->stream));
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    for (int q = 0; q < n; ++q) HIP_TRY(hipEventRecord(b->ctx[q]->ev1, b->stream));
+    HIP_TRY(hipEventRecord(b->done, b->stream));
+    const fr_params &p = b->ctx[0]->prm;
+    for (int q = 0; q < n; ++q) {
+        fr_ctx *c = b->ctx[q];
+        HIP_TRY(hipStreamWaitEvent(c->stream, b->done, 0));  // later calls on the context see the run
+        c->run_start_buf = par;
+        c->cur = par ^ (p.max_iter & 1);
+        c->have_run = true;
+        c->have_final = false;
+        c->stats = fr_stats{};
+    }
+    return FR_OK;
+}
+
+extern "C" int fr_batch_time(fr_batch *b, float *ms) {
+    if (!b || !ms) return fail(FR_ERR_INVALID, "bad arguments");
+    HIP_TRY(hipEventSynchronize(b->ev1));
+    HIP_TRY(hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return FR_OK;
+}
+
 extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_l, float *ms_a, float *ms_b) {
     if (!c || reps < 1 || reps > 10000) return fail(FR_ERR_INVALID, "bad arguments");
     if (!c->initialised) return fail(FR_ERR_STATE, "fr_time_kernels before fr_solver_init");

@@ -69,6 +69,23 @@ hipError_t launch_pass_a(const IterArgs &a, int zc, hipStream_t s);
 hipError_t launch_pass_b(const IterArgs &a, int zc, hipStream_t s);
 // resident blocks per CU of pass L (0), pass A (1) or pass B (2)
 hipError_t pass_occupancy(int which, int *blocks_per_cu);
+
+// The arguments of one pass (0 = L, 1 = A, 2 = B) of a volume with its tile grid set for zc.
+IterArgs pass_args(const IterArgs &a, int pass, int zc);
+
+// Values of one iteration shared by every volume of a batch.
+struct BatchIter {
+    int iter;          // 1-based iteration number
+    int cur;           // buffer parity: 0 = the launch-time arguments' old/new buffers, 1 = exchanged
+    int check_energy;  // stop on NaN energy of this iteration
+    float m1t, m2;
+};
+
+// One pass over a batch of volumes: d_base holds 3 arguments per volume (pass L, A, B; parity 0,
+// grid set), d_starts the first block of each volume in this pass's grid (npairs + 1 entries,
+// multiples of 8), nblocks the grid size.
+hipError_t launch_pass_batch(int pass, const IterArgs *d_base, const int *d_starts, int npairs, int nblocks,
+                             const BatchIter &it, hipStream_t s);
 hipError_t launch_energy(Dims d, const float *S, const float *T, const float *psi, double *partials, int nparts,
                          hipStream_t s);
 hipError_t launch_energy_finalize(const double *partials, int nparts, double w_reg, double *e_data, double *e_reg,

@@ -239,8 +239,11 @@ struct TileCoord {
     bool own;
 };
 
-__device__ __forceinline__ TileCoord tile_of(const IterArgs &a) {
-    const int b = xcd_remap(blockIdx.x, a.nblocks);
+// blk: the block's index within the launch grid of this volume (blockIdx.x for a single volume;
+// in a batch, the index within the volume's run of blocks, which starts at a multiple of 8 so that
+// the XCD of a block is still blk % 8)
+__device__ __forceinline__ TileCoord tile_of(const IterArgs &a, int blk) {
+    const int b = xcd_remap(blk, a.nblocks);
     const int bx = b % a.nbx, r = b / a.nbx, by = r % a.nby, bz = r / a.nby;
     const int lane = threadIdx.x & (WAVE - 1), off = (WAVE - a.xs) >> 1;
     TileCoord t;
@@ -255,11 +258,11 @@ __device__ __forceinline__ TileCoord tile_of(const IterArgs &a) {
 // pass L: L = -Laplacian(psi) on the interior (vector_fields.cu:275-300), into a buffer whose
 // boundary voxels stay 0.  One thread per (x, y) column, marching the chunk's z range with
 // psi(z-1), psi(z), psi(z+1) of the column in registers and psi(x+-1) from the adjacent lanes.
-__global__ __launch_bounds__(WAVE * TY_L) void k_pass_l(IterArgs a) {
+__device__ __forceinline__ void pass_l(const IterArgs &a, int blk) {
     if (__builtin_nontemporal_load(&a.ctl->stopped)) return;
     const Dims d = a.d;
     const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
-    const TileCoord tc = tile_of(a);
+    const TileCoord tc = tile_of(a, blk);
     if (tc.y > d.Y - 2) return;  // uniform over the wave
     const Rsrc rq = rsrc(a.psi, 12 * N), wl = rsrc(a.L, 12 * N);
     const unsigned sy = 12u * d.X, sz = 12u * XY;
@@ -487,7 +490,7 @@ __device__ __forceinline__ void slide(ColumnWindow &c, const WindowIn &n) {
 #define INTERLEAVE_LOADS() SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); \
     SG_PAIR4(); SG_PAIR4(); SG_PAIR(); SG_PAIR()
 
-__global__ __launch_bounds__(WAVE * TY_A) void k_pass_a(IterArgs a) {
+__device__ __forceinline__ void pass_a(const IterArgs &a, int blk) {
     // The block's waves step through z together (a bare s_barrier per half-step, no memory
     // fence), so that the rows one wave loads are still in L1 when the adjacent rows' waves read
     // them as y-neighbours, and adjacent rows gather from overlapping regions at the same time.
@@ -495,14 +498,14 @@ __global__ __launch_bounds__(WAVE * TY_A) void k_pass_a(IterArgs a) {
     // past the interior compute without storing.
     __shared__ int stop_block;
     if (threadIdx.x < 64) {  // the whole first wave: the slot maximum is a wave reduction
-        const bool st = should_stop(a, blockIdx.x == 0 && threadIdx.x == 0);
+        const bool st = should_stop(a, blk == 0 && threadIdx.x == 0);
         if (threadIdx.x == 0) stop_block = st;
     }
     __syncthreads();
     if (stop_block) return;  // uniform over the block
     const Dims d = a.d;
     const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
-    const TileCoord tc = tile_of(a);
+    const TileCoord tc = tile_of(a, blk);
     const int x = tc.x, y = tc.y;
     const bool own = tc.own && y <= d.Y - 2;
     PassAView w;
@@ -576,11 +579,11 @@ __global__ __launch_bounds__(WAVE * TY_A) void k_pass_a(IterArgs a) {
 
 // One thread per (x, y) column of a 64 x 4 tile, marching `zc` z-slices: psi += alpha * v_new o
 // psi (gathered through L1/L2) and the running max of the update norm.
-__global__ __launch_bounds__(WAVE * TY_B) void k_pass_b(IterArgs a) {
+__device__ __forceinline__ void pass_b(const IterArgs &a, int blk) {
     if (__builtin_nontemporal_load(&a.ctl->stopped)) return;
     const Dims d = a.d;
     const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
-    const TileCoord tc = tile_of(a);
+    const TileCoord tc = tile_of(a, blk);
     const int x = tc.x, y = tc.y;
     unsigned long long pack = 0;
     if (tc.own && y <= d.Y - 2) {
@@ -612,9 +615,69 @@ __global__ __launch_bounds__(WAVE * TY_B) void k_pass_b(IterArgs a) {
     __syncthreads();
     if (tid == 0) {
         for (int w = 1; w < TY_B; ++w) pack = red[w] > pack ? red[w] : pack;
-        unsigned long long *slot = &a.maxpack[(size_t)(a.iter - 1) * NSLOT + (blockIdx.x % NSLOT)];
+        unsigned long long *slot = &a.maxpack[(size_t)(a.iter - 1) * NSLOT + (blk % NSLOT)];
         if (pack > __builtin_nontemporal_load(slot)) atomicMax(slot, pack);
     }
+}
+
+// --------------------------------------------------------------------------------------------
+// entry points: one volume, or a batch of volumes in one launch per pass
+// --------------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(WAVE * TY_L) void k_pass_l(IterArgs a) { pass_l(a, blockIdx.x); }
+__global__ __launch_bounds__(WAVE * TY_A) void k_pass_a(IterArgs a) { pass_a(a, blockIdx.x); }
+__global__ __launch_bounds__(WAVE * TY_B) void k_pass_b(IterArgs a) { pass_b(a, blockIdx.x); }
+
+// The volume a block of a batched launch works on: the last q with starts[q] <= b (starts has
+// npairs + 1 entries, increasing).
+__device__ __forceinline__ int batch_volume(const int *__restrict__ starts, int npairs, int b) {
+    int lo = 0, hi = npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// The arguments of volume q for pass `pass` at this iteration: the launch-time arguments (buffer
+// parity 0) with the old/new buffers exchanged on odd parities and the iteration's values set.
+__device__ __forceinline__ IterArgs batch_args(const IterArgs *__restrict__ base, int q, int pass, const BatchIter &it) {
+    IterArgs a = base[3 * q + pass];
+    if (it.cur) {
+        const float *r = a.rho_o, *v = a.v_o, *p = a.pinv_o;
+        a.rho_o = a.rho_n;
+        a.v_o = a.v_n;
+        a.pinv_o = a.pinv_n;
+        a.rho_n = const_cast<float *>(r);
+        a.v_n = const_cast<float *>(v);
+        a.pinv_n = const_cast<float *>(p);
+    }
+    a.iter = it.iter;
+    a.m1t = it.m1t;
+    a.m2 = it.m2;
+    a.check_energy = it.check_energy;
+    return a;
+}
+
+#define BATCH_ENTRY(pass_fn, pass_idx)                                                            \
+    const int q = batch_volume(starts, npairs, blockIdx.x);                                      \
+    const IterArgs a = batch_args(base, q, pass_idx, it);                                        \
+    const int blk = blockIdx.x - starts[q];                                                      \
+    if (blk >= a.nblocks) return; /* padding up to a multiple of 8 blocks: the whole block */   \
+    pass_fn(a, blk)
+
+__global__ __launch_bounds__(WAVE * TY_L) void k_pass_l_batch(const IterArgs *base, const int *starts, int npairs,
+                                                              BatchIter it) {
+    BATCH_ENTRY(pass_l, 0);
+}
+__global__ __launch_bounds__(WAVE * TY_A) void k_pass_a_batch(const IterArgs *base, const int *starts, int npairs,
+                                                              BatchIter it) {
+    BATCH_ENTRY(pass_a, 1);
+}
+__global__ __launch_bounds__(WAVE * TY_B) void k_pass_b_batch(const IterArgs *base, const int *starts, int npairs,
+                                                              BatchIter it) {
+    BATCH_ENTRY(pass_b, 2);
 }
 
 // --------------------------------------------------------------------------------------------
@@ -870,6 +933,22 @@ hipError_t launch_pass_a(const IterArgs &a0, int zc, hipStream_t s) {
 hipError_t launch_pass_b(const IterArgs &a0, int zc, hipStream_t s) {
     const IterArgs a = chunked(a0, zc, XS_B, TY_B);
     hipLaunchKernelGGL(k_pass_b, dim3(a.nblocks), dim3(WAVE * TY_B), 0, s, a);
+    return hipGetLastError();
+}
+
+IterArgs pass_args(const IterArgs &a, int pass, int zc) {
+    return pass == 0 ? chunked(a, zc, XS_L, TY_L) : pass == 1 ? chunked(a, zc, XS_A, TY_A) : chunked(a, zc, XS_B, TY_B);
+}
+
+hipError_t launch_pass_batch(int pass, const IterArgs *d_base, const int *d_starts, int npairs, int nblocks,
+                             const BatchIter &it, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    if (pass == 0)
+        hipLaunchKernelGGL(k_pass_l_batch, dim3(nblocks), dim3(WAVE * TY_L), 0, s, d_base, d_starts, npairs, it);
+    else if (pass == 1)
+        hipLaunchKernelGGL(k_pass_a_batch, dim3(nblocks), dim3(WAVE * TY_A), 0, s, d_base, d_starts, npairs, it);
+    else
+        hipLaunchKernelGGL(k_pass_b_batch, dim3(nblocks), dim3(WAVE * TY_B), 0, s, d_base, d_starts, npairs, it);
     return hipGetLastError();
 }
 

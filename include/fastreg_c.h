@@ -155,6 +155,28 @@ int fr_solver_get_state(fr_ctx *ctx, float *d_v4, float *d_rho, float *d_psi4, f
  * e_data = 0.5 sum (T - S o psi)^2, e_reg = 0.5 sum ||D(psi - id)||_F^2, summed in FP64. */
 int fr_solver_energy(fr_ctx *ctx, double *e_data, double *e_reg);
 
+/* ---- batches: many registrations advanced together ---------------------------------------- */
+
+/* A batch runs the iteration loops of n contexts together, each volume of its own size: one
+ * launch per pass per iteration covers the tiles of every volume, and the whole run is captured
+ * once into a hipGraph (per start parity) and replayed.  The contexts keep their own state, stop
+ * rules (each volume stops on its own), logs and statistics; fr_solver_init / fr_get_stats /
+ * fr_get_log / fr_solver_finalize are used on them as for a single run.  MAX_ITER, ALPHA,
+ * RESET_ITER, p, C and verbosity must be the same for all n (the iteration schedule is shared);
+ * TAU, W_REG, RHO_0 and MAX_UPDATE_NORM may differ.  No counterpart in the reference, which
+ * registers one pair per FastReg call (src/fastreg/fastreg.cpp:58-75): it is the batched form
+ * of that call.  The batch keeps the context pointers; destroy it before them. */
+typedef struct fr_batch fr_batch;
+int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out);
+int fr_batch_destroy(fr_batch *b);
+/* re-read each context's z-chunking (fr_set_chunking) into the batch's launch layout */
+int fr_batch_relayout(fr_batch *b);
+/* MAX_ITER iterations of every volume from its current state (all volumes must have been
+ * initialised and be at the same point: fresh, or after the same number of runs); asynchronous */
+int fr_batch_run(fr_batch *b);
+/* device time of the last fr_batch_run (waits for it) */
+int fr_batch_time(fr_batch *b, float *ms);
+
 /* ---- kernels ----------------------------------------------------------------------------- */
 
 /* apply (src/fastreg/cuda/vector_fields.cu:81-107): out_img = trilinear img at psi(p), out_lab =
