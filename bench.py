@@ -49,6 +49,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--zc", default="0,0,0",
                     help="interior z-slices per block of pass L, pass A, pass B (0 = automatic)")
+    ap.add_argument("--workload", choices=("plain", "batch"), default="plain",
+                    help="plain: the single-pair headline metric (default); batch: 64 pairs of uneven sizes, "
+                         "150 iterations each, batched form vs one pair at a time")
+    ap.add_argument("--batch-pairs", type=int, default=64)
+    ap.add_argument("--batch-iters", type=int, default=150)
+    ap.add_argument("--batch-sizes", choices=("mixed", "small"), default="mixed",
+                    help="mixed: uniform over the five sizes; small: uniform over 48^3 and 96^3")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="before timing, run one device-to-device copy of this many MiB (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -58,6 +65,8 @@ def main():
               f"--gpus {args.gpus} is not supported (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batch":
+        return batch_workload(args)
     zc = tuple(int(v) for v in args.zc.split(","))
     import numpy as np
     import fastreg_amd as fr
@@ -166,6 +175,106 @@ def main():
                    "z_chunk": {"pass_l": chunking[0], "pass_a": chunking[1], "pass_b": chunking[2]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+BATCH_SIZES = [(48, 48, 48), (96, 96, 96), (128, 160, 128), (160, 192, 160), (182, 218, 182)]  # (X, Y, Z)
+
+
+def _make_pair(job):
+    i, dims = job
+    from fastreg_amd import phantoms
+    X, Y, Z = dims
+    return i, phantoms.blobs(shape=(Z, Y, X), seed=1000 + i)
+
+
+def batch_workload(args):
+    """64 independent pairs of sizes drawn uniformly from BATCH_SIZES (seed 4), each the blob
+    phantom with its own seed (1000 + i), 150 iterations each with the params.ini weights.  Timed
+    twice on the same contexts: one pair at a time (fr_solver_run per pair, HIP events per pair)
+    and as one batch (fr_batch_run: one launch per pass per iteration for all pairs, replayed
+    from a hipGraph, z-chunks from the autotuner).  Both must give the same max-norm history for
+    every pair."""
+    import multiprocessing as mp
+    import numpy as np
+
+    rng = np.random.default_rng(4)
+    sizes = BATCH_SIZES if args.batch_sizes == "mixed" else BATCH_SIZES[:2]
+    jobs = [(i, sizes[int(k)]) for i, k in enumerate(rng.integers(0, len(sizes), args.batch_pairs))]
+    t0 = time.time()
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
+        pairs = dict(pool.map(_make_pair, jobs))
+    print(f"[bench] {len(jobs)} pairs generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    import fastreg_amd as fr
+    from fastreg_amd import autotune
+    solvers = []
+    for i, dims in jobs:
+        pair = pairs[i]
+        p = fr.default_params(vol_dims=dims, no_labels=pair.no_labels, max_iter=args.batch_iters)
+        s = fr.Solver(p)
+        s.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+        solvers.append(s)
+    voxels = sum(d[0] * d[1] * d[2] for _, d in jobs)
+
+    # one pair at a time (warm-up pass first)
+    for s in solvers[: min(4, len(solvers))]:
+        s.run()
+        s.stats()
+        s.reinit()
+    per_pair, hist = [], []
+    fr.check(fr.lib().fr_dev_sync())
+    tw = time.perf_counter()
+    for s in solvers:
+        s.run()
+        st = s.stats()
+        per_pair.append(st.solver_ms)
+        hist.append(s.history()[0][: st.iterations].copy())
+    plain_wall = (time.perf_counter() - tw) * 1e3
+    plain_ms = float(sum(per_pair))
+    for s in solvers:
+        s.reinit()
+
+    b = fr.Batch(solvers)
+    tuned, _ = autotune.tune_batch(b, log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
+    b.run()  # capture + first replay (warm-up)
+    b.time_ms()
+    for s in solvers:
+        s.stats()
+        s.reinit()
+    tw = time.perf_counter()
+    b.run()
+    batch_ms = b.time_ms()
+    batch_wall = (time.perf_counter() - tw) * 1e3
+    for k, s in enumerate(solvers):
+        st = s.stats()
+        assert np.array_equal(s.history()[0][: st.iterations], hist[k]), f"pair {k}: batch and single runs differ"
+    b.close()
+
+    n = len(solvers)
+    out = {
+        "metric": f"pairs/s, {n} pairs of uneven sizes x {args.batch_iters} iterations",
+        "value": round(n / (batch_ms / 1e3), 2),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "fp32",
+        "data": "synthetic (blob phantoms, one seed per pair)",
+        "config": {"workload": f"batched registrations ({args.batch_sizes} sizes)", "pairs": n,
+                   "iterations": args.batch_iters,
+                   "sizes": {"x".join(map(str, d)): sum(1 for _, e in jobs if e == d) for d in sizes},
+                   "voxels": voxels,
+                   "z_chunk": {"x".join(map(str, d)): list(v) for d, v in tuned.items()}},
+        "batch_ms": round(batch_ms, 2),
+        "batch_wall_ms": round(batch_wall, 2),
+        "batch_voxel_iterations_per_s": round(voxels * args.batch_iters / (batch_ms / 1e3), 1),
+        "one_at_a_time": {"pairs_per_s": round(n / (plain_ms / 1e3), 2), "sum_ms": round(plain_ms, 2),
+                          "wall_ms": round(plain_wall, 2),
+                          "p50_pair_ms": round(float(np.percentile(per_pair, 50)), 3),
+                          "p95_pair_ms": round(float(np.percentile(per_pair, 95)), 3)},
+        "speedup": round(plain_ms / batch_ms, 3),
+        "per_pair_latency_ms_batched": round(batch_ms, 2),
     }
     print(json.dumps(out), flush=True)
 

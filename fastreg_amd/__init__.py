@@ -90,6 +90,7 @@ def lib():
     sig("fr_batch_relayout", vp)
     sig("fr_batch_run", vp)
     sig("fr_batch_time", vp, P(f))
+    sig("fr_batch_time_iterations", vp, i, P(f))
     sig("fr_get_chunking", vp, P(i), P(i), P(i))
     sig("fr_register", vp, vp, vp, vp, vp)
     sig("fr_get_deformation", vp, vp)
@@ -370,6 +371,13 @@ class Batch:
     def time_ms(self):
         ms = C.c_float()
         check(lib().fr_batch_time(self.h, C.byref(ms)))
+        return ms.value
+
+    def time_iterations(self, iters):
+        """Device ms of `iters` iterations with the stop tests off; the volumes must be
+        re-initialised afterwards (Solver.reinit)."""
+        ms = C.c_float()
+        check(lib().fr_batch_time_iterations(self.h, iters, C.byref(ms)))
         return ms.value
 
     def close(self):

@@ -787,6 +787,7 @@ static int batch_launches(fr_batch *b, int par) {
         it.iter = iter;
         it.cur = par ^ ((iter - 1) & 1);
         it.check_energy = verbose_now(p, iter) ? 1 : 0;
+        it.no_stop = 0;
         vel_scalars(p, t, &it.m1t, &it.m2);
         if (it.check_energy) {
             const int mi = p.max_iter > 0 ? p.max_iter : 1;
@@ -851,6 +852,47 @@ extern "C" int fr_batch_run(fr_batch *b) {
         c->have_run = true;
         c->have_final = false;
         c->stats = fr_stats{};
+    }
+    return FR_OK;
+}
+
+extern "C" int fr_batch_time_iterations(fr_batch *b, int iters, float *ms) {
+    if (!b || iters < 1 || !ms) return fail(FR_ERR_INVALID, "bad arguments");
+    const int n = (int)b->ctx.size();
+    const int mi = b->ctx[0]->prm.max_iter;
+    if (iters > mi) return fail(FR_ERR_INVALID, "iters must be <= MAX_ITER");
+    int rc;
+    for (int q = 0; q < n; ++q) {
+        fr_ctx *c = b->ctx[q];
+        if (!c->initialised) return fail(FR_ERR_STATE, "batch volume %d not initialised", q);
+        if (c->have_run && (rc = collect_run(c))) return rc;
+        if ((rc = reset_control(c))) return rc;
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    const int par = b->ctx[0]->cur;
+    for (int q = 0; q < n; ++q)
+        if (b->ctx[q]->cur != par) return fail(FR_ERR_STATE, "batch volumes are at different buffer parities");
+    const fr_params &p = b->ctx[0]->prm;
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    float t = p.alpha;
+    for (int iter = 1; iter <= iters; ++iter) {
+        frk::BatchIter it;
+        it.iter = iter;
+        it.cur = par ^ ((iter - 1) & 1);
+        it.check_energy = 0;
+        it.no_stop = 1;
+        vel_scalars(p, t, &it.m1t, &it.m2);
+        for (int pass = 0; pass < 3; ++pass)
+            HIP_TRY(frk::launch_pass_batch(pass, b->d_base, b->d_starts + (size_t)pass * (n + 1), n, b->total[pass], it,
+                                           b->stream));
+        t += p.alpha;
+    }
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    HIP_TRY(hipEventSynchronize(b->ev1));
+    HIP_TRY(hipEventElapsedTime(ms, b->ev0, b->ev1));
+    for (int q = 0; q < n; ++q) {  // the volumes' state has advanced: they must be re-initialised
+        b->ctx[q]->cur = par ^ (iters & 1);
+        b->ctx[q]->initialised = false;
     }
     return FR_OK;
 }

@@ -78,6 +78,7 @@ struct BatchIter {
     int iter;          // 1-based iteration number
     int cur;           // buffer parity: 0 = the launch-time arguments' old/new buffers, 1 = exchanged
     int check_energy;  // stop on NaN energy of this iteration
+    int no_stop;       // 1: no stop test (timing runs of a fixed amount of work)
     float m1t, m2;
 };
 

@@ -657,6 +657,7 @@ __device__ __forceinline__ IterArgs batch_args(const IterArgs *__restrict__ base
     a.m1t = it.m1t;
     a.m2 = it.m2;
     a.check_energy = it.check_energy;
+    if (it.no_stop) a.check_conv = 0;
     return a;
 }
 

@@ -176,6 +176,10 @@ int fr_batch_relayout(fr_batch *b);
 int fr_batch_run(fr_batch *b);
 /* device time of the last fr_batch_run (waits for it) */
 int fr_batch_time(fr_batch *b, float *ms);
+/* Device time of `iters` iterations of every volume with the stop tests off (a fixed amount of
+ * work, for choosing launch shapes).  The volumes' state advances: they are marked uninitialised
+ * and must be initialised again (fr_solver_init) before any other use. */
+int fr_batch_time_iterations(fr_batch *b, int iters, float *ms);
 
 /* ---- kernels ----------------------------------------------------------------------------- */
 

@@ -102,3 +102,20 @@ def test_batch_rejects_mixed_schedules():
     g1, g2 = fr.Solver(p1), fr.Solver(p2)
     with pytest.raises(fr.FastRegError, match="MAX_ITER"):
         fr.Batch([g1, g2])
+
+
+def test_autotuned_batch_matches_oracle(oracle_mod):
+    """The autotuner's timing runs leave the volumes re-initialised; the tuned batch still equals
+    the oracle."""
+    from fastreg_amd import autotune
+    O = oracle_mod
+    vols = _volumes(O, 5)
+    b = fr.Batch([g for g, _, _ in vols])
+    try:
+        tuned, ms = autotune.tune_batch(b, iters=2, cand_a=(3, 8), cand_lb=(1, 2))
+        say("tuned", tuned, ms)
+        assert set(tuned) == set(DIMS) and ms > 0
+        b.run()
+        _check(vols, "tuned")
+    finally:
+        b.close()
