@@ -25,6 +25,7 @@ HEADER_PATH = os.path.join(ROOT_DIR, "include", "fastreg_c.h")
 FR_OK = 0
 FR_ERR_INVALID = 1
 FR_ERR_HIP = 2
+STORAGE_F32, STORAGE_BF16, STORAGE_F16 = 0, 1, 2
 FR_ERR_ALLOC = 3
 FR_ERR_IO = 4
 FR_ERR_PARSE = 5
@@ -85,6 +86,8 @@ def lib():
     sig("fr_get_params", vp, P(FrParams))
     sig("fr_set_stream", vp, vp)
     sig("fr_set_chunking", vp, i, i, i)
+    sig("fr_set_storage", vp, i)
+    sig("fr_get_storage", vp, P(i))
     sig("fr_batch_create", P(vp), i, P(vp))
     sig("fr_batch_destroy", vp)
     sig("fr_batch_relayout", vp)
@@ -332,6 +335,16 @@ class Solver:
         buf = C.create_string_buffer(size)
         check(lib().fr_get_log(self.h, buf, size))
         return buf.value.decode()
+
+    def set_storage(self, storage):
+        """STORAGE_F32 / STORAGE_BF16 / STORAGE_F16 storage of v and rho; discards the state
+        (init again)."""
+        check(lib().fr_set_storage(self.h, int(storage)))
+
+    def storage(self):
+        v = C.c_int()
+        check(lib().fr_get_storage(self.h, C.byref(v)))
+        return v.value
 
     def set_chunking(self, zc_l=0, zc_a=0, zc_b=0):
         """Interior z-slices per block of pass L, pass A, pass B (0 = the automatic choice)."""

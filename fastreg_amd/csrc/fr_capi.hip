@@ -61,6 +61,8 @@ struct fr_ctx {
     float *pinv[2] = {nullptr, nullptr};
     float *psi = nullptr;
     float *L = nullptr;  // -Laplacian(psi), boundary kept at 0
+    int st = frk::ST_F32;                  // storage of v and rho
+    unsigned *vr[2] = {nullptr, nullptr};  // 16-bit (v, rho) records, 2 unsigned per voxel
     int cur = 0;
     int zc[3] = {2, 8, 2};       // interior z-slices per block of pass L, pass A, pass B
     int zc_auto[3] = {2, 8, 2};  // the choice made at create
@@ -132,7 +134,8 @@ extern "C" int fr_params_default(fr_params *p) {
 static void free_ctx(fr_ctx *c) {
     if (!c) return;
     void *bufs[] = {c->T, c->S, c->LT, c->LS, c->rho[0], c->rho[1], c->v[0], c->v[1], c->pinv[0], c->pinv[1], c->psi,
-                    c->L, c->ctl, c->maxpack, c->energy, c->partials, c->dice_counts, c->det_parts, c->lab_tmp};
+                    c->L, c->ctl, c->maxpack, c->energy, c->partials, c->dice_counts, c->det_parts, c->lab_tmp,
+                    c->vr[0], c->vr[1]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -253,6 +256,27 @@ extern "C" int fr_get_chunking(const fr_ctx *c, int *zc_l, int *zc_a, int *zc_b)
     return FR_OK;
 }
 
+extern "C" int fr_set_storage(fr_ctx *c, int storage) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (storage != FR_STORAGE_F32 && storage != FR_STORAGE_BF16 && storage != FR_STORAGE_F16)
+        return fail(FR_ERR_INVALID, "storage must be FR_STORAGE_F32, FR_STORAGE_BF16 or FR_STORAGE_F16");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (storage != FR_STORAGE_F32 && !c->vr[0]) {
+        int rc;
+        if ((rc = dalloc(&c->vr[0], 2 * (size_t)c->n)) || (rc = dalloc(&c->vr[1], 2 * (size_t)c->n))) return rc;
+    }
+    c->st = storage == FR_STORAGE_BF16 ? frk::ST_BF16 : storage == FR_STORAGE_F16 ? frk::ST_F16 : frk::ST_F32;
+    c->initialised = false;  // the state must be initialised in the new storage
+    c->have_run = c->have_final = false;
+    return FR_OK;
+}
+
+extern "C" int fr_get_storage(const fr_ctx *c, int *storage) {
+    if (!c || !storage) return fail(FR_ERR_INVALID, "null argument");
+    *storage = c->st == frk::ST_BF16 ? FR_STORAGE_BF16 : c->st == frk::ST_F16 ? FR_STORAGE_F16 : FR_STORAGE_F32;
+    return FR_OK;
+}
+
 // ---------------------------------------------------------------------------------------------
 // solver
 // ---------------------------------------------------------------------------------------------
@@ -263,6 +287,7 @@ extern "C" int fr_solver_init(fr_ctx *c, const float *tgt, const int16_t *tgt_la
     HIP_TRY(frk::launch_init(c->d, tgt, tgt_lab, src, src_lab, c->T, c->LT, c->S, c->LS, c->rho[0], c->rho[1], c->v[0],
                              c->v[1], c->pinv[0], c->pinv[1], c->psi, c->prm.rho_0, c->stream));
     HIP_TRY(hipMemsetAsync(c->L, 0, (size_t)c->n * 3 * sizeof(float), c->stream));
+    if (c->st != frk::ST_F32) HIP_TRY(frk::launch_vr_init(c->st, c->vr[0], c->vr[1], c->n, c->prm.rho_0, c->stream));
     c->cur = 0;
     c->initialised = true;
     c->have_run = c->have_final = false;
@@ -276,6 +301,12 @@ static void iter_args(fr_ctx *c, frk::IterArgs &a) {
     a.rho_n = c->rho[c->cur ^ 1];
     a.v_o = c->v[c->cur];
     a.v_n = c->v[c->cur ^ 1];
+    if (c->st != frk::ST_F32) {  // the 16-bit (v, rho) records
+        a.v_o = reinterpret_cast<const float *>(c->vr[c->cur]);
+        a.v_n = reinterpret_cast<float *>(c->vr[c->cur ^ 1]);
+        a.rho_o = nullptr;
+        a.rho_n = nullptr;
+    }
     a.pinv_o = c->pinv[c->cur];
     a.pinv_n = c->pinv[c->cur ^ 1];
     a.psi = c->psi;
@@ -321,8 +352,8 @@ static int launch_iteration(fr_ctx *c, int iter, float t, bool with_energy) {
         a.check_energy = 1;
     }
     HIP_TRY(frk::launch_pass_l(a, c->zc[0], c->stream));
-    HIP_TRY(frk::launch_pass_a(a, c->zc[1], c->stream));
-    HIP_TRY(frk::launch_pass_b(a, c->zc[2], c->stream));
+    HIP_TRY(frk::launch_pass_a(a, c->zc[1], c->st, c->stream));
+    HIP_TRY(frk::launch_pass_b(a, c->zc[2], c->st, c->stream));
     c->cur ^= 1;
     return FR_OK;
 }
@@ -599,10 +630,14 @@ extern "C" int fr_solver_set_state(fr_ctx *c, const float *v4, const float *rho,
     int rc;
     if (c->have_run && (rc = collect_run(c))) return rc;
     size_t n = (size_t)c->n;
-    if (v4) HIP_TRY(frk::launch_pack4to3(v4, c->v[0], c->v[1], c->n, c->stream));
-    if (rho) {
-        HIP_TRY(hipMemcpyAsync(c->rho[0], rho, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-        HIP_TRY(hipMemcpyAsync(c->rho[1], rho, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    if (c->st != frk::ST_F32) {
+        if (v4 || rho) HIP_TRY(frk::launch_vr_set(c->st, c->vr[0], c->vr[1], v4, rho, c->n, c->stream));
+    } else {
+        if (v4) HIP_TRY(frk::launch_pack4to3(v4, c->v[0], c->v[1], c->n, c->stream));
+        if (rho) {
+            HIP_TRY(hipMemcpyAsync(c->rho[0], rho, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(c->rho[1], rho, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        }
     }
     if (psi4) HIP_TRY(frk::launch_pack4to3(psi4, c->psi, nullptr, c->n, c->stream));
     if (pinv4) HIP_TRY(frk::launch_pack4to3(pinv4, c->pinv[0], c->pinv[1], c->n, c->stream));
@@ -615,8 +650,13 @@ extern "C" int fr_solver_get_state(fr_ctx *c, float *v4, float *rho, float *psi4
     if (!c->initialised) return fail(FR_ERR_STATE, "fr_solver_get_state before fr_solver_init");
     int rc;
     if (c->have_run && (rc = collect_run(c))) return rc;
-    if (v4) HIP_TRY(frk::launch_unpack3to4(c->v[c->cur], v4, c->n, c->stream));
-    if (rho) HIP_TRY(hipMemcpyAsync(rho, c->rho[c->cur], (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    if (c->st != frk::ST_F32) {
+        if (v4 || rho) HIP_TRY(frk::launch_vr_get(c->st, c->vr[c->cur], v4, rho, c->n, c->stream));
+    } else {
+        if (v4) HIP_TRY(frk::launch_unpack3to4(c->v[c->cur], v4, c->n, c->stream));
+        if (rho)
+            HIP_TRY(hipMemcpyAsync(rho, c->rho[c->cur], (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    }
     if (psi4) HIP_TRY(frk::launch_unpack3to4(c->psi, psi4, c->n, c->stream));
     if (pinv4) HIP_TRY(frk::launch_unpack3to4(c->pinv[c->cur], pinv4, c->n, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -736,9 +776,9 @@ extern "C" int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
             if (ctxs[r] == ctxs[q]) return fail(FR_ERR_INVALID, "context %d appears twice", q);
         const fr_params &p = ctxs[q]->prm, &p0 = ctxs[0]->prm;
         if (p.max_iter != p0.max_iter || p.alpha != p0.alpha || p.reset_iter != p0.reset_iter || p.p != p0.p ||
-            p.C != p0.C || p.verbosity != p0.verbosity)
-            return fail(FR_ERR_INVALID, "context %d: MAX_ITER, ALPHA, RESET_ITER, p, C and verbosity must be the same "
-                                        "for every volume of a batch", q);
+            p.C != p0.C || p.verbosity != p0.verbosity || ctxs[q]->st != ctxs[0]->st)
+            return fail(FR_ERR_INVALID, "context %d: MAX_ITER, ALPHA, RESET_ITER, p, C, verbosity and the storage of v "
+                                        "and rho must be the same for every volume of a batch", q);
     }
     fr_batch *b = new fr_batch();
     b->ctx.assign(ctxs, ctxs + n);
@@ -801,7 +841,7 @@ static int batch_launches(fr_batch *b, int par) {
         }
         for (int pass = 0; pass < 3; ++pass)
             HIP_TRY(frk::launch_pass_batch(pass, b->d_base, b->d_starts + (size_t)pass * (n + 1), n, b->total[pass], it,
-                                           b->stream));
+                                           b->ctx[0]->st, b->stream));
         t += p.alpha;
         if (p.reset_iter > 0 && (iter + 1) % p.reset_iter == 0) t = p.alpha;
     }
@@ -884,7 +924,7 @@ extern "C" int fr_batch_time_iterations(fr_batch *b, int iters, float *ms) {
         vel_scalars(p, t, &it.m1t, &it.m2);
         for (int pass = 0; pass < 3; ++pass)
             HIP_TRY(frk::launch_pass_batch(pass, b->d_base, b->d_starts + (size_t)pass * (n + 1), n, b->total[pass], it,
-                                           b->stream));
+                                           b->ctx[0]->st, b->stream));
         t += p.alpha;
     }
     HIP_TRY(hipEventRecord(b->ev1, b->stream));
@@ -926,8 +966,8 @@ extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_l, float *ms_a, fl
     if (e == hipSuccess)
         e = hipMemcpyAsync(psi_save, c->psi, 3 * (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess) e = frk::launch_pass_l(a, c->zc[0], c->stream);  // warm-up iteration
-    if (e == hipSuccess) e = frk::launch_pass_a(a, c->zc[1], c->stream);
-    if (e == hipSuccess) e = frk::launch_pass_b(a, c->zc[2], c->stream);
+    if (e == hipSuccess) e = frk::launch_pass_a(a, c->zc[1], c->st, c->stream);
+    if (e == hipSuccess) e = frk::launch_pass_b(a, c->zc[2], c->st, c->stream);
     // each repeat reduces its max norm into its own (cleared) iteration slots, as the loop does;
     // the stop tests are off
     a.check_conv = 0;
@@ -938,9 +978,9 @@ extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_l, float *ms_a, fl
         e = hipEventRecord(ev[3 * r], c->stream);
         if (e == hipSuccess) e = frk::launch_pass_l(a, c->zc[0], c->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[3 * r + 1], c->stream);
-        if (e == hipSuccess) e = frk::launch_pass_a(a, c->zc[1], c->stream);
+        if (e == hipSuccess) e = frk::launch_pass_a(a, c->zc[1], c->st, c->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[3 * r + 2], c->stream);
-        if (e == hipSuccess) e = frk::launch_pass_b(a, c->zc[2], c->stream);
+        if (e == hipSuccess) e = frk::launch_pass_b(a, c->zc[2], c->st, c->stream);
     }
     if (e == hipSuccess) e = hipEventRecord(ev[3 * (size_t)reps], c->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev[3 * (size_t)reps]);

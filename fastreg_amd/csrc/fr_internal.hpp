@@ -30,6 +30,10 @@ constexpr int TY_L = 4, TY_A = 8, TY_B = 4;
 constexpr int XS_L = WAVE - 2;
 constexpr int XS_A = WAVE - 4;
 constexpr int XS_B = WAVE;
+// Storage of velocity and density: FP32 (3 + 1 floats in two buffers), or one 8-byte record
+// (v.x, v.y, v.z, rho) per voxel in bf16 or fp16.
+constexpr int ST_F32 = 0, ST_BF16 = 1, ST_F16 = 2;
+
 // Pass B spreads its per-iteration 64-bit atomic maxima over NSLOT addresses (a multiple of the
 // wave size: one wave reads them back).
 constexpr int NSLOT = 256;
@@ -65,8 +69,10 @@ hipError_t launch_init(Dims d, const float *tgt, const int16_t *tgt_lab, const f
 // The iteration kernels over blocks of XS_* x TY_* columns by `zc` interior z-slices (the
 // launch sets a's tile-grid fields).
 hipError_t launch_pass_l(const IterArgs &a, int zc, hipStream_t s);
-hipError_t launch_pass_a(const IterArgs &a, int zc, hipStream_t s);
-hipError_t launch_pass_b(const IterArgs &a, int zc, hipStream_t s);
+// st: storage of v and rho (ST_F32, ST_BF16, ST_F16); in the 16-bit forms a.v_o / a.v_n point to
+// the 8-byte (v, rho) records and a.rho_o / a.rho_n are unused
+hipError_t launch_pass_a(const IterArgs &a, int zc, int st, hipStream_t s);
+hipError_t launch_pass_b(const IterArgs &a, int zc, int st, hipStream_t s);
 // resident blocks per CU of pass L (0), pass A (1) or pass B (2)
 hipError_t pass_occupancy(int which, int *blocks_per_cu);
 
@@ -86,7 +92,7 @@ struct BatchIter {
 // grid set), d_starts the first block of each volume in this pass's grid (npairs + 1 entries,
 // multiples of 8), nblocks the grid size.
 hipError_t launch_pass_batch(int pass, const IterArgs *d_base, const int *d_starts, int npairs, int nblocks,
-                             const BatchIter &it, hipStream_t s);
+                             const BatchIter &it, int st, hipStream_t s);
 hipError_t launch_energy(Dims d, const float *S, const float *T, const float *psi, double *partials, int nparts,
                          hipStream_t s);
 hipError_t launch_energy_finalize(const double *partials, int nparts, double w_reg, double *e_data, double *e_reg,
@@ -100,6 +106,11 @@ hipError_t launch_dice(Dims d, const int16_t *la, const int16_t *lb, int no_labe
 int min_det_partials_count(Dims d);
 hipError_t launch_min_det(Dims d, const float *psi, float *part_true, float *part_ref, hipStream_t s);
 hipError_t launch_pack4to3(const float *src4, float *dst3a, float *dst3b, int n, hipStream_t s);
+// 16-bit (v, rho) records (2 unsigned per voxel): initial state (v = 0, rho = rho_0), replace v
+// (float4 per voxel) and/or rho in both buffers, read v (float4) and/or rho
+hipError_t launch_vr_init(int st, unsigned *r0, unsigned *r1, int n, float rho_0, hipStream_t s);
+hipError_t launch_vr_set(int st, unsigned *r0, unsigned *r1, const float *v4, const float *rho, int n, hipStream_t s);
+hipError_t launch_vr_get(int st, const unsigned *r, float *v4, float *rho, int n, hipStream_t s);
 hipError_t launch_unpack3to4(const float *src3, float *dst4, int n, hipStream_t s);
 
 }  // namespace frk

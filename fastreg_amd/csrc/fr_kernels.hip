@@ -283,15 +283,115 @@ __device__ __forceinline__ void pass_l(const IterArgs &a, int blk) {
     }
 }
 
+// Storage of velocity and density (ST_F32: v as 3 floats and rho as a float in separate buffers,
+// 12 + 4 B per voxel; ST_BF16 / ST_F16: one 8-byte record per voxel holding v.x, v.y, v.z, rho as
+// bf16 / fp16, low half first).  All arithmetic is FP32; a value of v or rho is rounded to the
+// storage format (nearest even) as soon as it is computed, and only the rounded value is used
+// afterwards, so the 16-bit forms are exact FP32 computations on representable values.
+__device__ __forceinline__ unsigned f32_to_bf16(float f) {
+    unsigned x = __builtin_bit_cast(unsigned, f);
+    if ((x & 0x7FFFFFFFu) > 0x7F800000u) return (x >> 16) | 0x40u;  // NaN stays a (quiet) NaN
+    x += 0x7FFFu + ((x >> 16) & 1u);
+    return x >> 16;
+}
+__device__ __forceinline__ float bf16_to_f32(unsigned h) { return __builtin_bit_cast(float, h << 16); }
+__device__ __forceinline__ unsigned f32_to_f16(float f) {
+    return __builtin_bit_cast(unsigned short, (_Float16)f);  // IEEE round to nearest even
+}
+__device__ __forceinline__ float f16_to_f32(unsigned h) {
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)h);
+}
+
+template <int ST> __device__ __forceinline__ unsigned h_of(float f) {
+    return ST == ST_BF16 ? f32_to_bf16(f) : f32_to_f16(f);
+}
+template <int ST> __device__ __forceinline__ float f_of(unsigned h) {
+    return ST == ST_BF16 ? bf16_to_f32(h) : f16_to_f32(h);
+}
+// the value after the storage's rounding.  The FP32 value is pinned in a register first: the
+// compiler would otherwise fuse an FMA that produces it with the fp16 conversion
+// (v_fma_mix*_f16), rounding once from the exact result instead of from the FP32 value.
+template <int ST> __device__ __forceinline__ float q16(float f) {
+    if constexpr (ST == ST_F32) {
+        return f;
+    } else {
+        asm volatile("" : "+v"(f));
+        return f_of<ST>(h_of<ST>(f));
+    }
+}
+
+// (v, rho) of one voxel as loaded: floats, or the raw 8-byte record
+template <int ST> struct VR {
+    unsigned lo, hi;
+};
+template <> struct VR<ST_F32> {
+    F3 v;
+    float r;
+};
+
+// the velocity/density buffers of one kernel: ST_F32 uses v and r, the 16-bit forms v only
+struct VRBuf {
+    Rsrc v, r;
+};
+
+template <int ST> __device__ __forceinline__ VRBuf vr_buf(const float *v, const float *r, int N) {
+    VRBuf b;
+    b.v = rsrc(v, (ST == ST_F32 ? 12 : 8) * N);
+    b.r = ST == ST_F32 ? rsrc(r, 4 * N) : b.v;
+    return b;
+}
+
+template <int ST> __device__ __forceinline__ VR<ST> vr_load(const VRBuf &b, int i) {
+    VR<ST> q;
+    if constexpr (ST == ST_F32) {
+        q.v = bld3(b.v, 12u * i);
+        q.r = bld1(b.r, 4u * i);
+    } else {
+        q.lo = __builtin_amdgcn_raw_buffer_load_b32(b.v, 8u * i, 0, 0);
+        q.hi = __builtin_amdgcn_raw_buffer_load_b32(b.v, 8u * i + 4u, 0, 0);
+    }
+    return q;
+}
+// v only (ST_F32 skips the density load)
+template <int ST> __device__ __forceinline__ VR<ST> v_load(const VRBuf &b, int i) {
+    if constexpr (ST == ST_F32) {
+        VR<ST> q;
+        q.v = bld3(b.v, 12u * i);
+        q.r = 0.f;
+        return q;
+    } else {
+        return vr_load<ST>(b, i);
+    }
+}
+template <int ST> __device__ __forceinline__ F3 vr_v(const VR<ST> &q) {
+    if constexpr (ST == ST_F32) return q.v;
+    else return f3(f_of<ST>(q.lo & 0xFFFFu), f_of<ST>(q.lo >> 16), f_of<ST>(q.hi & 0xFFFFu));
+}
+template <int ST> __device__ __forceinline__ float vr_r(const VR<ST> &q) {
+    if constexpr (ST == ST_F32) return q.r;
+    else return f_of<ST>(q.hi >> 16);
+}
+// store (v, rho), already rounded to the storage format; a dropped store when !store
+template <int ST> __device__ __forceinline__ void vr_store(const VRBuf &b, int i, bool store, F3 v, float r) {
+    if constexpr (ST == ST_F32) {
+        bst1(b.r, store ? 4u * i : NO_STORE, r);
+        bst3(b.v, store ? 12u * i : NO_STORE, v);
+    } else {
+        const unsigned o = store ? 8u * i : NO_STORE;
+        __builtin_amdgcn_raw_buffer_store_b32(h_of<ST>(v.x) | (h_of<ST>(v.y) << 16), b.v, o, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(h_of<ST>(v.z) | (h_of<ST>(r) << 16), b.v, o + 4u, 0, 0);
+    }
+}
+
 // pass A: steps 1-8 and 10 of one iteration per interior voxel.  One thread per (x, y) column,
-// marching the chunk's z range; the column's z-neighbours (v at z-2..z+2, rho, psi_inv and S at
-// z-1..z+1) slide through registers and the x-neighbours come from the adjacent lanes.  The loads
-// of plane z+1 (y-neighbours, and the L and T taps at psi_inv) are issued before plane z is
-// computed, so that their latency overlaps the arithmetic: software pipelining across the z-march.
-struct PlaneLoads {
-    float r_yp, r_ym;                // rho at y+-1
-    F3 v_yp, v_ym;                   // v at y+-1
-    F3 v_y2;                         // v at y+-2 on the side the upwind Laplacian reads
+// marching the chunk's z range; the column's z-neighbours (v at z-2..z+2, rho and psi_inv at
+// z-1..z+2, S at z-1..z+1) slide through registers and the x-neighbours come from the adjacent
+// lanes.  The loads of plane z+1 (y-neighbours, and the L and T taps at psi_inv) are issued before
+// plane z is computed, so that their latency overlaps the arithmetic: software pipelining across
+// the z-march.
+template <int ST> struct PlaneLoads {
+    VR<ST> vr_yp, vr_ym;             // (v, rho) at y+-1
+    VR<ST> v_y2;                     // v at y+-2 on the side the upwind Laplacian reads
     F3 p_yp, p_ym;                   // psi_inv at y+-1
     float s_yp, s_ym;                // S at y+-1
     Corner k;                        // corner and weights of psi_inv(p)
@@ -301,19 +401,20 @@ struct PlaneLoads {
 
 struct PassAView {
     Dims d;
-    Rsrc rv, rr, rp, rs, rt, rl;  // loads: v, rho, psi_inv (old), S, T, L
-    Rsrc wv, wr, wp;              // stores: v, rho, psi_inv (new)
+    VRBuf rvr, wvr;       // (v, rho): old (loads), new (stores)
+    Rsrc rp, rs, rt, rl;  // loads: psi_inv (old), S, T, L
+    Rsrc wp;              // stores: psi_inv (new)
+    int X;
     unsigned X12, XY12, X4, XY4;
 };
 
 // issue the loads of the voxel at (linear index i) whose own v and psi_inv are vc and pc
-__device__ __forceinline__ void fetch_plane(PlaneLoads &n, const PassAView &w, int i, F3 vc, F3 pc) {
+template <int ST>
+__device__ __forceinline__ void fetch_plane(PlaneLoads<ST> &n, const PassAView &w, int i, F3 vc, F3 pc) {
     const unsigned o12 = 12u * i, o4 = 4u * i;
-    n.r_yp = bld1(w.rr, o4 + w.X4);
-    n.r_ym = bld1(w.rr, o4 - w.X4);
-    n.v_yp = bld3(w.rv, o12 + w.X12);
-    n.v_ym = bld3(w.rv, o12 - w.X12);
-    n.v_y2 = bld3(w.rv, vc.y < 0.f ? o12 + 2u * w.X12 : o12 - 2u * w.X12);
+    n.vr_yp = vr_load<ST>(w.rvr, i + w.X);
+    n.vr_ym = vr_load<ST>(w.rvr, i - w.X);
+    n.v_y2 = v_load<ST>(w.rvr, vc.y < 0.f ? i + 2 * w.X : i - 2 * w.X);
     n.p_yp = bld3(w.rp, o12 + w.X12);
     n.p_ym = bld3(w.rp, o12 - w.X12);
     n.s_yp = bld1(w.rs, o4 + w.X4);
@@ -339,16 +440,17 @@ __device__ __forceinline__ void fetch_plane(PlaneLoads &n, const PassAView &w, i
     n.t000 = bld1(w.rt, g4);
 }
 
-// own-column z-windows; psi_inv runs one plane further ahead than it is used, so that the corner
-// of the plane being prefetched comes from a value loaded a whole step earlier
+// own-column z-windows; rho and psi_inv run one plane further ahead than they are used, so that
+// the corner of the plane being prefetched comes from a value loaded a whole step earlier
 struct ColumnWindow {
     F3 v_m2, v_m1, v_0, v_p1, v_p2, p_m1, p_0, p_p1, p_p2;
-    float r_m1, r_0, r_p1, s_m1, s_0, s_p1;
+    float r_m1, r_0, r_p1, r_p2, s_m1, s_0, s_p1;
 };
 
 // every lane of the wave runs this (the x-neighbours are exchanged between lanes); `store` is
 // false on halo lanes and on the voxel past the chunk
-__device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, const PlaneLoads &n,
+template <int ST>
+__device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, const PlaneLoads<ST> &n,
                                         const ColumnWindow &c, int i, bool lap, bool store) {
     // x-neighbours of rho, v, psi_inv, S from the adjacent lanes
     const float r_xp = lane_next(c.r_0), r_xm = lane_prev(c.r_0);
@@ -356,6 +458,9 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const F3 v_xpp = lane_next3(v_xp), v_xmm = lane_prev3(v_xm);
     const F3 p_xp = lane_next3(c.p_0), p_xm = lane_prev3(c.p_0);
     const float s_xp = lane_next(c.s_0), s_xm = lane_prev(c.s_0);
+    // y-neighbours of rho and v
+    const float r_yp = vr_r<ST>(n.vr_yp), r_ym = vr_r<ST>(n.vr_ym);
+    const F3 v_yp = vr_v<ST>(n.vr_yp), v_ym = vr_v<ST>(n.vr_ym), v_y2 = vr_v<ST>(n.v_y2);
 
     // 1. density (solver.cu:101-173)
     const F3 vc = c.v_0;
@@ -363,14 +468,14 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const float t11 = (vc.x > 0.f) ? -vc.x * rc : -vc.x * r_xp;
     const float vxm = v_xm.x;
     const float t12 = (vxm > 0.f) ? vxm * r_xm : vxm * rc;
-    const float t21 = (vc.y > 0.f) ? -vc.y * rc : -vc.y * n.r_yp;
-    const float vym = n.v_ym.y;
-    const float t22 = (vym > 0.f) ? vym * n.r_ym : vym * rc;
+    const float t21 = (vc.y > 0.f) ? -vc.y * rc : -vc.y * r_yp;
+    const float vym = v_ym.y;
+    const float t22 = (vym > 0.f) ? vym * r_ym : vym * rc;
     const float t31 = (vc.z > 0.f) ? -vc.z * rc : -vc.z * c.r_p1;
     const float vzm = c.v_m1.z;
     const float t32 = (vzm > 0.f) ? vzm * c.r_m1 : vzm * rc;
     const float div = ((((t11 + t12) + t21) + t22) + t31) + t32;
-    const float rho_n = fmaf(a.alpha, div, rc);
+    const float rho_n = q16<ST>(fmaf(a.alpha, div, rc));
 
     // 2. J_inv and det() (vector_fields.cu:503-535)
     const F3 jx = mul3(0.5f, sub3(p_xp, p_xm));
@@ -391,7 +496,7 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const F3 nU = mul3(det, add3(mul3(diff, gS), mul3(a.w_reg, Lo)));
 
     // 6. (Dv)v (vector_fields.cu:352-404)
-    const F3 xp = v_xp, xm = v_xm, yp = n.v_yp, ym = n.v_ym;
+    const F3 xp = v_xp, xm = v_xm, yp = v_yp, ym = v_ym;
     const F3 zp = c.v_p1, zm = c.v_m1;
 #define MX(q) fmaxf((q), 0.f)
 #define MN(q) fminf((q), 0.f)
@@ -415,7 +520,7 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const bool nx = vc.x < 0.f, ny = vc.y < 0.f, nz = vc.z < 0.f;
     const F3 v_x2 = nx ? v_xpp : v_xmm;
     const F3 Ax = nx ? v_x2 : vc, Bx = nx ? xp : xm, Cx = nx ? vc : v_x2;
-    const F3 Ay = ny ? n.v_y2 : vc, Cy = ny ? vc : n.v_y2;
+    const F3 Ay = ny ? v_y2 : vc, Cy = ny ? vc : v_y2;
     const F3 Az = nz ? c.v_p2 : vc, Cz = nz ? vc : c.v_m2;
     const F3 vxx = add3(sub3(Ax, mul3(2.f, Bx)), Cx);
     const F3 vyy = add3(sub3(Ay, mul3(2.f, ym)), Cy);
@@ -429,7 +534,7 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const float qv = a.m2 / (fabsf(rho_n) + 1e-5f);
     const F3 po = mul3(qv, nU);
     const F3 up = mul3(a.alpha, add3(sub3(add3(fr, dvv), df), po));
-    const F3 vn = f3(vc.x - up.x, vc.y - up.y, vc.z - up.z);
+    const F3 vn = f3(q16<ST>(vc.x - up.x), q16<ST>(vc.y - up.y), q16<ST>(vc.z - up.z));
 
     // 10. psi_inv -= alpha * (J_inv v_new) (solver.cu:263-279)
     const F3 pc = c.p_0;
@@ -438,39 +543,36 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const F3 ui = mul3(a.alpha, Jv);
     // a voxel outside the chunk (the odd step of the last trip) stores at an offset past every
     // buffer's range, which the buffer unit drops
-    const unsigned o4 = store ? 4u * i : NO_STORE, o12 = store ? 12u * i : NO_STORE;
-    bst1(w.wr, o4, rho_n);
-    bst3(w.wv, o12, vn);
-    bst3(w.wp, o12, f3(pc.x - ui.x, pc.y - ui.y, pc.z - ui.z));
+    vr_store<ST>(w.wvr, i, store, vn, rho_n);
+    bst3(w.wp, store ? 12u * i : NO_STORE, f3(pc.x - ui.x, pc.y - ui.y, pc.z - ui.z));
 }
 
 // the window entries entering at the next step, loaded at the start of a step (before its stores,
 // which the compiler cannot move loads across) so that the slide at its end does not wait
-struct WindowIn {
-    F3 v;      // v at z+3
-    F3 p;      // psi_inv at z+3
-    float r;   // rho at z+2
-    float s;   // S at z+2
+template <int ST> struct WindowIn {
+    VR<ST> vr;  // (v, rho) at z+3
+    F3 p;       // psi_inv at z+3
+    float s;    // S at z+2
 };
 
-__device__ __forceinline__ WindowIn window_in(const PassAView &w, int i, int XY) {
-    WindowIn n;
-    n.v = bld3(w.rv, 12u * (i + 3 * XY));
+template <int ST> __device__ __forceinline__ WindowIn<ST> window_in(const PassAView &w, int i, int XY) {
+    WindowIn<ST> n;
+    n.vr = vr_load<ST>(w.rvr, i + 3 * XY);
     n.p = bld3(w.rp, 12u * (i + 3 * XY));
-    n.r = bld1(w.rr, 4u * (i + 2 * XY));
     n.s = bld1(w.rs, 4u * (i + 2 * XY));
     return n;
 }
 
-__device__ __forceinline__ void slide(ColumnWindow &c, const WindowIn &n) {
+template <int ST> __device__ __forceinline__ void slide(ColumnWindow &c, const WindowIn<ST> &n) {
     c.v_m2 = c.v_m1;
     c.v_m1 = c.v_0;
     c.v_0 = c.v_p1;
     c.v_p1 = c.v_p2;
-    c.v_p2 = n.v;
+    c.v_p2 = vr_v<ST>(n.vr);
     c.r_m1 = c.r_0;
     c.r_0 = c.r_p1;
-    c.r_p1 = n.r;
+    c.r_p1 = c.r_p2;
+    c.r_p2 = vr_r<ST>(n.vr);
     c.p_m1 = c.p_0;
     c.p_0 = c.p_p1;
     c.p_p1 = c.p_p2;
@@ -490,7 +592,7 @@ __device__ __forceinline__ void slide(ColumnWindow &c, const WindowIn &n) {
 #define INTERLEAVE_LOADS() SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); \
     SG_PAIR4(); SG_PAIR4(); SG_PAIR(); SG_PAIR()
 
-__device__ __forceinline__ void pass_a(const IterArgs &a, int blk) {
+template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int blk) {
     // The block's waves step through z together (a bare s_barrier per half-step, no memory
     // fence), so that the rows one wave loads are still in L1 when the adjacent rows' waves read
     // them as y-neighbours, and adjacent rows gather from overlapping regions at the same time.
@@ -510,15 +612,14 @@ __device__ __forceinline__ void pass_a(const IterArgs &a, int blk) {
     const bool own = tc.own && y <= d.Y - 2;
     PassAView w;
     w.d = d;
-    w.rv = rsrc(a.v_o, 12 * N);
-    w.rr = rsrc(a.rho_o, 4 * N);
+    w.rvr = vr_buf<ST>(a.v_o, a.rho_o, N);
+    w.wvr = vr_buf<ST>(a.v_n, a.rho_n, N);
     w.rp = rsrc(a.pinv_o, 12 * N);
     w.rs = rsrc(a.S, 4 * N);
     w.rt = rsrc(a.T, 4 * N);
     w.rl = rsrc(a.L, 12 * N);
-    w.wv = rsrc(a.v_n, 12 * N);
-    w.wr = rsrc(a.rho_n, 4 * N);
     w.wp = rsrc(a.pinv_n, 12 * N);
+    w.X = d.X;
     w.X12 = 12u * d.X;
     w.XY12 = 12u * XY;
     w.X4 = 4u * d.X;
@@ -527,14 +628,19 @@ __device__ __forceinline__ void pass_a(const IterArgs &a, int blk) {
 
     int i = x + d.X * y + XY * tc.zs;
     ColumnWindow c;
-    c.v_m2 = bld3(w.rv, 12u * (i - 2 * XY));
-    c.v_m1 = bld3(w.rv, 12u * (i - XY));
-    c.v_0 = bld3(w.rv, 12u * i);
-    c.v_p1 = bld3(w.rv, 12u * (i + XY));
-    c.v_p2 = bld3(w.rv, 12u * (i + 2 * XY));
-    c.r_m1 = bld1(w.rr, 4u * (i - XY));
-    c.r_0 = bld1(w.rr, 4u * i);
-    c.r_p1 = bld1(w.rr, 4u * (i + XY));
+    c.v_m2 = vr_v<ST>(v_load<ST>(w.rvr, i - 2 * XY));
+    {
+        const VR<ST> q_m1 = vr_load<ST>(w.rvr, i - XY), q_0 = vr_load<ST>(w.rvr, i),
+                     q_p1 = vr_load<ST>(w.rvr, i + XY), q_p2 = vr_load<ST>(w.rvr, i + 2 * XY);
+        c.v_m1 = vr_v<ST>(q_m1);
+        c.v_0 = vr_v<ST>(q_0);
+        c.v_p1 = vr_v<ST>(q_p1);
+        c.v_p2 = vr_v<ST>(q_p2);
+        c.r_m1 = vr_r<ST>(q_m1);
+        c.r_0 = vr_r<ST>(q_0);
+        c.r_p1 = vr_r<ST>(q_p1);
+        c.r_p2 = vr_r<ST>(q_p2);
+    }
     c.p_m1 = bld3(w.rp, 12u * (i - XY));
     c.p_0 = bld3(w.rp, 12u * i);
     c.p_p1 = bld3(w.rp, 12u * (i + XY));
@@ -543,8 +649,8 @@ __device__ __forceinline__ void pass_a(const IterArgs &a, int blk) {
     c.s_0 = bld1(w.rs, 4u * i);
     c.s_p1 = bld1(w.rs, 4u * (i + XY));
 
-    PlaneLoads n0, n1;
-    fetch_plane(n0, w, i, c.v_0, c.p_0);
+    PlaneLoads<ST> n0, n1;
+    fetch_plane<ST>(n0, w, i, c.v_0, c.p_0);
     // drain the prologue's loads, so that the loop's memory-counter waits are set by the loop's
     // own steady state and not by the prologue's last loads
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
@@ -556,17 +662,17 @@ __device__ __forceinline__ void pass_a(const IterArgs &a, int blk) {
     for (int s = 0, z = tc.zs; s < nz; s += 2, z += 2, i += 2 * XY) {
         // (each half-step interleaves the next plane's loads with the current plane's arithmetic,
         // so that a wave keeps computing while its loads queue for the memory pipeline)
-        const WindowIn in0 = window_in(w, i, XY);
-        fetch_plane(n1, w, i + XY, c.v_p1, c.p_p1);
-        voxel_a(a, w, n0, c, i, lap_xy && z >= 2 && z <= d.Z - 3, own);
-        slide(c, in0);
+        const WindowIn<ST> in0 = window_in<ST>(w, i, XY);
+        fetch_plane<ST>(n1, w, i + XY, c.v_p1, c.p_p1);
+        voxel_a<ST>(a, w, n0, c, i, lap_xy && z >= 2 && z <= d.Z - 3, own);
+        slide<ST>(c, in0);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_barrier();
-        const WindowIn in1 = window_in(w, i + XY, XY);
-        fetch_plane(n0, w, i + 2 * XY, c.v_p1, c.p_p1);
-        voxel_a(a, w, n1, c, i + XY, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, own && s + 1 < nz);
-        slide(c, in1);
+        const WindowIn<ST> in1 = window_in<ST>(w, i + XY, XY);
+        fetch_plane<ST>(n0, w, i + 2 * XY, c.v_p1, c.p_p1);
+        voxel_a<ST>(a, w, n1, c, i + XY, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, own && s + 1 < nz);
+        slide<ST>(c, in1);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_barrier();
@@ -579,7 +685,7 @@ __device__ __forceinline__ void pass_a(const IterArgs &a, int blk) {
 
 // One thread per (x, y) column of a 64 x 4 tile, marching `zc` z-slices: psi += alpha * v_new o
 // psi (gathered through L1/L2) and the running max of the update norm.
-__device__ __forceinline__ void pass_b(const IterArgs &a, int blk) {
+template <int ST> __device__ __forceinline__ void pass_b(const IterArgs &a, int blk) {
     if (__builtin_nontemporal_load(&a.ctl->stopped)) return;
     const Dims d = a.d;
     const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
@@ -587,15 +693,20 @@ __device__ __forceinline__ void pass_b(const IterArgs &a, int blk) {
     const int x = tc.x, y = tc.y;
     unsigned long long pack = 0;
     if (tc.own && y <= d.Y - 2) {
-        const Rsrc rv = rsrc(a.v_n, 12 * N);
-        const unsigned X12 = 12u * d.X, XY12 = 12u * XY;
+        const VRBuf vb = vr_buf<ST>(a.v_n, a.rho_n, N);
         for (int z = tc.zs; z < tc.ze; ++z) {
             // psi += alpha * v_new o psi; updates = that increment (solver.cu:230-247)
             const int i = x + d.X * y + XY * z;
             const F3 p = ld3(a.psi, i);
             const Corner k = corner(d, p.x, p.y, p.z);
-            const unsigned g = corner_index(d, k);
-            const F3 w = tri_at3(rv, 12u * g, 12u, X12, XY12, k);
+            const int g = (int)corner_index(d, k);
+            const F3 f111 = vr_v<ST>(v_load<ST>(vb, g + 1 + d.X + XY)), f110 = vr_v<ST>(v_load<ST>(vb, g + 1 + d.X)),
+                     f101 = vr_v<ST>(v_load<ST>(vb, g + 1 + XY)), f100 = vr_v<ST>(v_load<ST>(vb, g + 1)),
+                     f011 = vr_v<ST>(v_load<ST>(vb, g + d.X + XY)), f010 = vr_v<ST>(v_load<ST>(vb, g + d.X)),
+                     f001 = vr_v<ST>(v_load<ST>(vb, g + XY)), f000 = vr_v<ST>(v_load<ST>(vb, g));
+            const F3 w = f3(tri(f111.x, f110.x, f101.x, f100.x, f011.x, f010.x, f001.x, f000.x, k.a, k.b, k.c),
+                            tri(f111.y, f110.y, f101.y, f100.y, f011.y, f010.y, f001.y, f000.y, k.a, k.b, k.c),
+                            tri(f111.z, f110.z, f101.z, f100.z, f011.z, f010.z, f001.z, f000.z, k.a, k.b, k.c));
             const F3 u = mul3(a.alpha, w);
             st3(a.psi, i, f3(p.x + u.x, p.y + u.y, p.z + u.z));
             // norm with a round-down sqrt (cuda_utils.hpp:251-253); NaN never wins (strict >,
@@ -625,8 +736,8 @@ __device__ __forceinline__ void pass_b(const IterArgs &a, int blk) {
 // --------------------------------------------------------------------------------------------
 
 __global__ __launch_bounds__(WAVE * TY_L) void k_pass_l(IterArgs a) { pass_l(a, blockIdx.x); }
-__global__ __launch_bounds__(WAVE * TY_A) void k_pass_a(IterArgs a) { pass_a(a, blockIdx.x); }
-__global__ __launch_bounds__(WAVE * TY_B) void k_pass_b(IterArgs a) { pass_b(a, blockIdx.x); }
+template <int ST> __global__ __launch_bounds__(WAVE * TY_A) void k_pass_a(IterArgs a) { pass_a<ST>(a, blockIdx.x); }
+template <int ST> __global__ __launch_bounds__(WAVE * TY_B) void k_pass_b(IterArgs a) { pass_b<ST>(a, blockIdx.x); }
 
 // The volume a block of a batched launch works on: the last q with starts[q] <= b (starts has
 // npairs + 1 entries, increasing).
@@ -672,13 +783,15 @@ __global__ __launch_bounds__(WAVE * TY_L) void k_pass_l_batch(const IterArgs *ba
                                                               BatchIter it) {
     BATCH_ENTRY(pass_l, 0);
 }
+template <int ST>
 __global__ __launch_bounds__(WAVE * TY_A) void k_pass_a_batch(const IterArgs *base, const int *starts, int npairs,
                                                               BatchIter it) {
-    BATCH_ENTRY(pass_a, 1);
+    BATCH_ENTRY(pass_a<ST>, 1);
 }
+template <int ST>
 __global__ __launch_bounds__(WAVE * TY_B) void k_pass_b_batch(const IterArgs *base, const int *starts, int npairs,
                                                               BatchIter it) {
-    BATCH_ENTRY(pass_b, 2);
+    BATCH_ENTRY(pass_b<ST>, 2);
 }
 
 // --------------------------------------------------------------------------------------------
@@ -889,6 +1002,46 @@ __global__ void k_unpack3to4(const float *__restrict__ src3, float *__restrict__
     }
 }
 
+// (v, rho) records of the 16-bit storage forms: initial state, set from and read into FP32 arrays
+template <int ST> __global__ void k_vr_init(unsigned *r0, unsigned *r1, int n, float rho_0) {
+    const unsigned hi = h_of<ST>(rho_0) << 16;  // v = 0, rho = rho_0 (rounded)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        r0[2 * i] = 0u;
+        r0[2 * i + 1] = hi;
+        r1[2 * i] = 0u;
+        r1[2 * i + 1] = hi;
+    }
+}
+
+// replace v (float4 per voxel) and/or rho (either may be null) in both buffers of records
+template <int ST> __global__ void k_vr_set(unsigned *r0, unsigned *r1, const float *v4, const float *rho, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        unsigned lo = r0[2 * i], hi = r0[2 * i + 1];
+        if (v4) {
+            lo = h_of<ST>(v4[4 * i]) | (h_of<ST>(v4[4 * i + 1]) << 16);
+            hi = (hi & 0xFFFF0000u) | h_of<ST>(v4[4 * i + 2]);
+        }
+        if (rho) hi = (hi & 0xFFFFu) | (h_of<ST>(rho[i]) << 16);
+        r0[2 * i] = lo;
+        r0[2 * i + 1] = hi;
+        r1[2 * i] = lo;
+        r1[2 * i + 1] = hi;
+    }
+}
+
+template <int ST> __global__ void k_vr_get(const unsigned *r, float *v4, float *rho, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned lo = r[2 * i], hi = r[2 * i + 1];
+        if (v4) {
+            v4[4 * i] = f_of<ST>(lo & 0xFFFFu);
+            v4[4 * i + 1] = f_of<ST>(lo >> 16);
+            v4[4 * i + 2] = f_of<ST>(hi & 0xFFFFu);
+            v4[4 * i + 3] = 0.f;
+        }
+        if (rho) rho[i] = f_of<ST>(hi >> 16);
+    }
+}
+
 // --------------------------------------------------------------------------------------------
 // launchers
 // --------------------------------------------------------------------------------------------
@@ -925,15 +1078,26 @@ hipError_t launch_pass_l(const IterArgs &a0, int zc, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_pass_a(const IterArgs &a0, int zc, hipStream_t s) {
+template <int ST> static void pass_a_kernel(const IterArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_pass_a<ST>, dim3(a.nblocks), dim3(WAVE * TY_A), 0, s, a);
+}
+template <int ST> static void pass_b_kernel(const IterArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_pass_b<ST>, dim3(a.nblocks), dim3(WAVE * TY_B), 0, s, a);
+}
+
+hipError_t launch_pass_a(const IterArgs &a0, int zc, int st, hipStream_t s) {
     const IterArgs a = chunked(a0, zc, XS_A, TY_A);
-    hipLaunchKernelGGL(k_pass_a, dim3(a.nblocks), dim3(WAVE * TY_A), 0, s, a);
+    if (st == ST_BF16) pass_a_kernel<ST_BF16>(a, s);
+    else if (st == ST_F16) pass_a_kernel<ST_F16>(a, s);
+    else pass_a_kernel<ST_F32>(a, s);
     return hipGetLastError();
 }
 
-hipError_t launch_pass_b(const IterArgs &a0, int zc, hipStream_t s) {
+hipError_t launch_pass_b(const IterArgs &a0, int zc, int st, hipStream_t s) {
     const IterArgs a = chunked(a0, zc, XS_B, TY_B);
-    hipLaunchKernelGGL(k_pass_b, dim3(a.nblocks), dim3(WAVE * TY_B), 0, s, a);
+    if (st == ST_BF16) pass_b_kernel<ST_BF16>(a, s);
+    else if (st == ST_F16) pass_b_kernel<ST_F16>(a, s);
+    else pass_b_kernel<ST_F32>(a, s);
     return hipGetLastError();
 }
 
@@ -941,20 +1105,30 @@ IterArgs pass_args(const IterArgs &a, int pass, int zc) {
     return pass == 0 ? chunked(a, zc, XS_L, TY_L) : pass == 1 ? chunked(a, zc, XS_A, TY_A) : chunked(a, zc, XS_B, TY_B);
 }
 
-hipError_t launch_pass_batch(int pass, const IterArgs *d_base, const int *d_starts, int npairs, int nblocks,
-                             const BatchIter &it, hipStream_t s) {
-    if (nblocks <= 0) return hipSuccess;
+template <int ST>
+static void pass_batch_kernel(int pass, const IterArgs *d_base, const int *d_starts, int npairs, int nblocks,
+                              const BatchIter &it, hipStream_t s) {
     if (pass == 0)
         hipLaunchKernelGGL(k_pass_l_batch, dim3(nblocks), dim3(WAVE * TY_L), 0, s, d_base, d_starts, npairs, it);
     else if (pass == 1)
-        hipLaunchKernelGGL(k_pass_a_batch, dim3(nblocks), dim3(WAVE * TY_A), 0, s, d_base, d_starts, npairs, it);
+        hipLaunchKernelGGL(k_pass_a_batch<ST>, dim3(nblocks), dim3(WAVE * TY_A), 0, s, d_base, d_starts, npairs, it);
     else
-        hipLaunchKernelGGL(k_pass_b_batch, dim3(nblocks), dim3(WAVE * TY_B), 0, s, d_base, d_starts, npairs, it);
+        hipLaunchKernelGGL(k_pass_b_batch<ST>, dim3(nblocks), dim3(WAVE * TY_B), 0, s, d_base, d_starts, npairs, it);
+}
+
+hipError_t launch_pass_batch(int pass, const IterArgs *d_base, const int *d_starts, int npairs, int nblocks,
+                             const BatchIter &it, int st, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    if (st == ST_BF16) pass_batch_kernel<ST_BF16>(pass, d_base, d_starts, npairs, nblocks, it, s);
+    else if (st == ST_F16) pass_batch_kernel<ST_F16>(pass, d_base, d_starts, npairs, nblocks, it, s);
+    else pass_batch_kernel<ST_F32>(pass, d_base, d_starts, npairs, nblocks, it, s);
     return hipGetLastError();
 }
 
 hipError_t pass_occupancy(int which, int *blocks_per_cu) {
-    const void *k = which == 0 ? (const void *)k_pass_l : which == 1 ? (const void *)k_pass_a : (const void *)k_pass_b;
+    const void *k = which == 0 ? (const void *)k_pass_l
+                    : which == 1 ? (const void *)k_pass_a<ST_F32>
+                                 : (const void *)k_pass_b<ST_F32>;
     const int ty = which == 0 ? TY_L : which == 1 ? TY_A : TY_B;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, WAVE * ty, 0);
 }
@@ -993,6 +1167,24 @@ int min_det_partials_count(Dims d) { return grid_for(d.X * d.Y * d.Z, RED_BLOCK,
 
 hipError_t launch_min_det(Dims d, const float *psi, float *part_true, float *part_ref, hipStream_t s) {
     hipLaunchKernelGGL(k_min_det, dim3(min_det_partials_count(d)), dim3(RED_BLOCK), 0, s, d, psi, part_true, part_ref);
+    return hipGetLastError();
+}
+
+hipError_t launch_vr_init(int st, unsigned *r0, unsigned *r1, int n, float rho_0, hipStream_t s) {
+    if (st == ST_BF16) hipLaunchKernelGGL(k_vr_init<ST_BF16>, dim3(grid_for(n, 256)), dim3(256), 0, s, r0, r1, n, rho_0);
+    else hipLaunchKernelGGL(k_vr_init<ST_F16>, dim3(grid_for(n, 256)), dim3(256), 0, s, r0, r1, n, rho_0);
+    return hipGetLastError();
+}
+
+hipError_t launch_vr_set(int st, unsigned *r0, unsigned *r1, const float *v4, const float *rho, int n, hipStream_t s) {
+    if (st == ST_BF16) hipLaunchKernelGGL(k_vr_set<ST_BF16>, dim3(grid_for(n, 256)), dim3(256), 0, s, r0, r1, v4, rho, n);
+    else hipLaunchKernelGGL(k_vr_set<ST_F16>, dim3(grid_for(n, 256)), dim3(256), 0, s, r0, r1, v4, rho, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_vr_get(int st, const unsigned *r, float *v4, float *rho, int n, hipStream_t s) {
+    if (st == ST_BF16) hipLaunchKernelGGL(k_vr_get<ST_BF16>, dim3(grid_for(n, 256)), dim3(256), 0, s, r, v4, rho, n);
+    else hipLaunchKernelGGL(k_vr_get<ST_F16>, dim3(grid_for(n, 256)), dim3(256), 0, s, r, v4, rho, n);
     return hipGetLastError();
 }
 

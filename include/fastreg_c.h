@@ -98,6 +98,19 @@ int fr_set_stream(fr_ctx *ctx, void *hip_stream);
 int fr_set_chunking(fr_ctx *ctx, int zc_l, int zc_a, int zc_b);
 int fr_get_chunking(const fr_ctx *ctx, int *zc_l, int *zc_a, int *zc_b);
 
+/* Storage of the velocity and density fields: FP32 (the default, the reference's precision), or
+ * one 8-byte record per voxel holding v.x, v.y, v.z, rho in bf16 or fp16.  All arithmetic stays
+ * FP32: every value of v and rho is rounded to the storage format (nearest even) when it is
+ * computed, and the rounded value is the one used afterwards; energies are summed in FP64.
+ * Setting the storage discards the state (call fr_solver_init / fr_register next).  No
+ * counterpart in the reference, which stores both as FP32 (include/cuda/vector_fields.hpp:21-78,
+ * include/cuda/scalar_fields.hpp:20-65): it is the reduced-precision form of those fields. */
+#define FR_STORAGE_F32 0
+#define FR_STORAGE_BF16 1
+#define FR_STORAGE_F16 2
+int fr_set_storage(fr_ctx *ctx, int storage);
+int fr_get_storage(const fr_ctx *ctx, int *storage);
+
 /* FastReg::operator() (src/fastreg/fastreg.cpp:58-75): register the source (img2, lab2) to the
  * target (img1, lab1), all device pointers of no_voxels elements, and overwrite them in place:
  * img1/lab1 <- target o psi_inv, img2/lab2 <- source o psi.  Every call starts from a fresh

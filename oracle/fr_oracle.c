@@ -57,6 +57,67 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Rounding of the 16-bit storage forms (velocity and density stored as bf16 or fp16, arithmetic
+ * in FP32): round to nearest even, as IEEE 754 conversion does.  Pinned in the tests against
+ * numpy's float16 and torch's bfloat16 conversions. */
+static uint16_t or_f32_to_f16(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    const uint32_t s = (x >> 16) & 0x8000u, a = x & 0x7FFFFFFFu;
+    if (a > 0x7F800000u) return (uint16_t) (s | 0x7E00u | ((a >> 13) & 0x3FFu)); /* NaN (quiet) */
+    if (a >= 0x477FF000u) return (uint16_t) (s | 0x7C00u); /* rounds to infinity (>= 65520) */
+    if (a >= 0x38800000u) {                                /* normal half: >= 2^-14 */
+        const uint32_t r = a - 0x38000000u;                /* exponent rebiased from 127 to 15 */
+        uint32_t h = r >> 13;
+        const uint32_t rem = r & 0x1FFFu;
+        if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+        return (uint16_t) (s | h);
+    }
+    const uint32_t e = a >> 23;
+    if (e < 102) return (uint16_t) s; /* below 2^-25: rounds to zero */
+    /* subnormal half: the value in units of 2^-24 is m * 2^(e - 126), m with its leading 1 */
+    const uint32_t m = (a & 0x7FFFFFu) | 0x800000u, sh = 126 - e;
+    uint32_t h = m >> sh;
+    const uint32_t rem = m & ((1u << sh) - 1u), half = 1u << (sh - 1u);
+    if (rem > half || (rem == half && (h & 1u))) ++h;
+    return (uint16_t) (s | h);
+}
+
+static float or_f16_to_f32(uint16_t h) {
+    const uint32_t s = (uint32_t) (h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+    uint32_t x;
+    if (e == 0x1Fu) x = s | 0x7F800000u | (m << 13);
+    else if (e != 0) x = s | ((e + 112u) << 23) | (m << 13);
+    else if (m == 0) x = s;
+    else { /* subnormal half: m * 2^-24, exact in float */
+        float f = (float) m * 0x1p-24f;
+        memcpy(&x, &f, 4);
+        x |= s;
+    }
+    float f;
+    memcpy(&f, &x, 4);
+    return f;
+}
+
+static float or_bf16_round(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    if ((x & 0x7FFFFFFFu) > 0x7F800000u) x = (x | 0x00400000u) & 0xFFFF0000u; /* NaN (quiet) */
+    else x = (x + 0x7FFFu + ((x >> 16) & 1u)) & 0xFFFF0000u;
+    memcpy(&f, &x, 4);
+    return f;
+}
+
+/* storage 0: FP32 (no rounding), 1: bf16, 2: fp16 */
+static float or_q16(int storage, float f) {
+    return storage == 1 ? or_bf16_round(f) : storage == 2 ? or_f16_to_f32(or_f32_to_f16(f)) : f;
+}
+
+/* the rounding applied to an array (tests pin it) */
+void or_round16(int storage, float *a, long n) {
+    for (long i = 0; i < n; ++i) a[i] = or_q16(storage, a[i]);
+}
+
 #define REAL float
 #define SFX _f32
 #include "fr_oracle_impl.h"

@@ -14,6 +14,7 @@ typedef struct {
     V4 *psi, *psi_inv, *v, *nabla_phi_n, *L, *L_o, *dv_v, *delta_v, *nabla_U, *updates;
     M4 *J_inv, *J;
     REAL *rho, *div;
+    int storage; /* 0: FP32 (or FP64) fields; 1 / 2: v and rho rounded to bf16 / fp16 when computed */
 } N_(or_ctx);
 #define CTX N_(or_ctx)
 
@@ -57,6 +58,22 @@ void N_(or_destroy)(void *h) {
     free(c);
 }
 
+/* v and rho rounded to the storage format (all voxels; identity for storage 0) */
+static void N_(round_v_rho)(CTX *c, int do_v, int do_rho) {
+    if (!c->storage) return;
+    for (long i = 0; i < c->n; ++i) {
+        if (do_v) {
+            c->v[i].x = (REAL) or_q16(c->storage, (float) c->v[i].x);
+            c->v[i].y = (REAL) or_q16(c->storage, (float) c->v[i].y);
+            c->v[i].z = (REAL) or_q16(c->storage, (float) c->v[i].z);
+        }
+        if (do_rho) c->rho[i] = (REAL) or_q16(c->storage, (float) c->rho[i]);
+    }
+}
+
+/* storage of v and rho for the next or_init (the 16-bit forms are meant for the FP32 build) */
+void N_(or_set_storage)(void *h, int storage) { ((CTX *) h)->storage = storage; }
+
 /* Fresh state for one registration: FastReg::operator() packs the four inputs
  * (fastreg.cpp:58-63), the constructors give psi = psi_inv = id and zero derived fields
  * (vector_fields.cpp:8-12,85-94), estimate_psi computes grad S and sets rho = rho_0
@@ -81,6 +98,7 @@ void N_(or_init)(void *h, const float *tgt, const int16_t *tgt_lab, const float 
     memset(c->J, 0, (size_t) c->n * sizeof(M4));
     memset(c->div, 0, (size_t) c->n * sizeof(REAL));
     for (long i = 0; i < c->n; ++i) c->rho[i] = rho_0;
+    N_(round_v_rho)(c, 0, 1);
     N_(k_volume_gradient)(c->d, c->phi_n, c->nabla_phi_n);
 }
 
@@ -95,6 +113,7 @@ static void N_(vel_scalars)(REAL t, REAL p, REAL C, REAL *m1t, REAL *m2) {
 static void N_(phase_a)(CTX *c, REAL alpha) {
     N_(k_divergence)(c->d, c->rho, c->v, c->div);
     N_(k_update_density)(c->d, c->rho, c->div, alpha);
+    N_(round_v_rho)(c, 0, 1);
     N_(k_jacobian)(c->d, c->psi_inv, c->J_inv);
     N_(k_laplacian)(c->d, c->psi, c->L);
     N_(k_interpolate_laplacian)(c->d, c->L, c->L_o, c->psi_inv);
@@ -108,6 +127,7 @@ static REAL N_(phase_pde)(CTX *c, REAL alpha, REAL t, REAL tau, REAL w_reg, REAL
     N_(k_dv_v)(c->d, c->v, c->dv_v);
     N_(k_laplacian_velocity)(c->d, c->v, c->delta_v);
     N_(k_update_velocity)(c->d, c->v, c->dv_v, c->delta_v, c->rho, c->nabla_U, alpha, m1t, tau, m2);
+    N_(round_v_rho)(c, 1, 0);
     N_(k_update_psi)(c->d, c->psi, c->v, c->updates, alpha);
     N_(k_update_psi_inv)(c->d, c->psi_inv, c->J_inv, c->v, alpha);
     N_(k_apply)(c->d, c->phi_global, c->phi_global_psi_inv, c->psi_inv);
@@ -222,6 +242,7 @@ void N_(or_set_state)(void *h, const REAL *v, const REAL *rho, const REAL *psi, 
     CTX *c = (CTX *) h;
     if (v) memcpy(c->v, v, (size_t) c->n * sizeof(V4));
     if (rho) memcpy(c->rho, rho, (size_t) c->n * sizeof(REAL));
+    N_(round_v_rho)(c, v != NULL, rho != NULL);
     if (psi) memcpy(c->psi, psi, (size_t) c->n * sizeof(V4));
     if (psi_inv) {
         memcpy(c->psi_inv, psi_inv, (size_t) c->n * sizeof(V4));

@@ -49,6 +49,7 @@ def lib():
                 fn.argtypes = list(args)
             i = C.c_int
             f("or_create", vp, i, i, i)
+            f("or_set_storage", None, vp, i)
             f("or_destroy", None, vp)
             f("or_init", None, vp, vp, vp, vp, vp, R)
             f("or_step", R, vp, R, R, R, R, R, R, C.POINTER(C.c_long))
@@ -80,6 +81,8 @@ def lib():
             f("ork_norm", R, vp)
             f("ork_lerp", R, R, R, R)
         L.or_num_threads.restype = C.c_int
+        L.or_round16.restype = None
+        L.or_round16.argtypes = [C.c_int, C.c_void_p, C.c_long]
         _lib = L
     return _lib
 
@@ -101,6 +104,16 @@ def _chk(a, dtype, shape):
 
 def num_threads() -> int:
     return lib().or_num_threads()
+
+
+STORAGE_F32, STORAGE_BF16, STORAGE_F16 = 0, 1, 2
+
+
+def round16(storage: int, a) -> np.ndarray:
+    """The oracle's storage rounding (bf16 / fp16, nearest even) of a float32 array."""
+    out = np.ascontiguousarray(a, dtype=np.float32).copy()
+    lib().or_round16(storage, _p(out), out.size)
+    return out
 
 
 @dataclass
@@ -126,7 +139,9 @@ class SolverParams:
 class Oracle:
     """One registration's state in the oracle (precision 'f32' or 'f64')."""
 
-    def __init__(self, dims, prec: str = "f32"):
+    def __init__(self, dims, prec: str = "f32", storage: int = 0):
+        """storage: 0 FP32 fields; 1 / 2 (FP32 build only): v and rho rounded to bf16 / fp16 when
+        computed, as the library's 16-bit storage forms do."""
         self.X, self.Y, self.Z = (int(d) for d in dims)
         self.n = self.X * self.Y * self.Z
         self.prec = prec
@@ -136,6 +151,9 @@ class Oracle:
         self.h = self._fn("or_create")(self.X, self.Y, self.Z)
         if not self.h:
             raise MemoryError("oracle context allocation failed")
+        if storage and prec != "f32":
+            raise ValueError("the 16-bit storage forms are modelled in the FP32 oracle only")
+        self._fn("or_set_storage")(self.h, int(storage))
 
     def _fn(self, name):
         return getattr(self._L, name + self.sfx)
