@@ -49,9 +49,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--zc", default="0,0,0",
                     help="interior z-slices per block of pass L, pass A, pass B (0 = automatic)")
-    ap.add_argument("--workload", choices=("plain", "batch"), default="plain",
+    ap.add_argument("--workload", choices=("plain", "batch", "precision"), default="plain",
                     help="plain: the single-pair headline metric (default); batch: 64 pairs of uneven sizes, "
-                         "150 iterations each, batched form vs one pair at a time")
+                         "150 iterations each, batched form vs one pair at a time; precision: FP32, BF16 and "
+                         "FP16 storage of v and rho against an FP64 oracle run (64^3, 300 iterations) and their "
+                         "throughput at 182x218x182")
     ap.add_argument("--batch-pairs", type=int, default=64)
     ap.add_argument("--batch-iters", type=int, default=150)
     ap.add_argument("--batch-sizes", choices=("mixed", "small"), default="mixed",
@@ -67,6 +69,8 @@ def main():
 
     if args.workload == "batch":
         return batch_workload(args)
+    if args.workload == "precision":
+        return precision_workload(args)
     zc = tuple(int(v) for v in args.zc.split(","))
     import numpy as np
     import fastreg_amd as fr
@@ -175,6 +179,124 @@ def main():
                    "z_chunk": {"pass_l": chunking[0], "pass_a": chunking[1], "pass_b": chunking[2]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def precision_workload(args):
+    """The 64^3 blob phantom (seed 1, 9 labels), 300 iterations with the params.ini weights and the
+    energy logged every iteration, run on the GPU with v and rho stored in FP32, BF16 and FP16, and
+    once by the FP64 oracle (the reference's algorithm in double).  Reported per storage: relative
+    L2 error of v, rho and the displacements psi - id, psi_inv - id against FP64; the largest
+    relative deviation of the energy curve; Dice (labels 1..8) and min det J next to FP64's; for the
+    16-bit forms also the same errors against the FP32 form (what the storage alone changes).  Then
+    the throughput of each storage at 182x218x182 (300 iterations, HIP events) and its kernel times.
+    (With this phantom and step the displacements stay below 1e-3 voxel, a few FP32 ulps of the
+    absolute coordinates psi holds, so FP32 itself is far from FP64 in psi and, through the floor
+    of the label warp, in Dice.)"""
+    import numpy as np
+    import fastreg_amd as fr
+    from fastreg_amd import phantoms
+    from oracle import oracle as O
+
+    O.build()
+    iters = 300
+    pair = phantoms.blobs()
+    dims = pair.dims
+    n = dims[0] * dims[1] * dims[2]
+    t0 = time.time()
+    o = O.Oracle(dims, "f64")
+    o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+    r = o.run(O.SolverParams(max_iter=iters, verbosity=2, no_labels=pair.no_labels))
+    ref = o.get_state()
+    o.outputs()
+    ov, vx, _ = o.dice_counts(pair.no_labels)
+    ref_dice = 2.0 * ov[1:].sum() / vx[1:].sum()
+    ref_det = o.min_det()[0]
+    ref_e = np.asarray(r["energy_hist"][:iters], np.float64)
+    o.close()
+    print(f"[bench] FP64 oracle, {iters} iterations at {dims}: {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    zz, yy, xx = np.meshgrid(np.arange(dims[2]), np.arange(dims[1]), np.arange(dims[0]), indexing="ij")
+    ident = np.stack([xx.ravel(), yy.ravel(), zz.ravel(), np.zeros(n)], 1).astype(np.float64)
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    names = {fr.STORAGE_F32: "fp32", fr.STORAGE_BF16: "bf16", fr.STORAGE_F16: "fp16"}
+    errors, states = {}, {}
+    for stg, nm in names.items():
+        p = fr.default_params(vol_dims=dims, no_labels=pair.no_labels, max_iter=iters, verbosity=2)
+        s = fr.Solver(p)
+        s.set_storage(stg)
+        s.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+        s.run()
+        st = s.stats()
+        _, eh = s.history()
+        g = s.get_state()
+        s.finalize(want_outputs=False)
+        st = s.stats()
+        eh = np.asarray(eh[:iters], np.float64)
+        errors[nm] = {
+            "rel_l2_v": rel(g["v"], ref["v"]),
+            "rel_l2_rho": rel(g["rho"], ref["rho"]),
+            "rel_l2_psi_disp": rel(np.asarray(g["psi"], np.float64) - ident, ref["psi"] - ident),
+            "rel_l2_psi_inv_disp": rel(np.asarray(g["psi_inv"], np.float64) - ident, ref["psi_inv"] - ident),
+            "energy_max_rel_dev": float(np.max(np.abs(eh - ref_e) / np.abs(ref_e))),
+            "dice_all": round(st.dice_all, 6),
+            "min_det_true": round(st.min_det_true, 6),
+        }
+        states[nm] = g
+        if nm != "fp32":  # what the storage alone changes: against the FP32 form
+            f32 = states["fp32"]
+            errors[nm]["vs_fp32"] = {
+                "rel_l2_v": rel(g["v"], f32["v"]),
+                "rel_l2_rho": rel(g["rho"], f32["rho"]),
+                "rel_l2_psi_disp": rel(np.asarray(g["psi"], np.float64) - ident,
+                                       np.asarray(f32["psi"], np.float64) - ident),
+                "dice_all_diff": round(errors[nm]["dice_all"] - errors["fp32"]["dice_all"], 6),
+            }
+        s.close()
+        print(f"[bench] {nm}: {errors[nm]}", file=sys.stderr, flush=True)
+
+    # throughput at the full size
+    big = phantoms.brain()
+    bd = big.dims
+    nb = bd[0] * bd[1] * bd[2]
+    speed = {}
+    for stg, nm in names.items():
+        p = fr.default_params(vol_dims=bd, no_labels=big.no_labels, max_iter=300)
+        s = fr.Solver(p)
+        s.set_storage(stg)
+        for rep in range(2):  # warm-up run, then the timed run
+            s.init(big.target, big.target_labels, big.source, big.source_labels)
+            s.run()
+            st = s.stats()
+        kl, ka, kb = s.time_kernels(20)
+        speed[nm] = {"voxel_iterations_per_s": round(nb * 300 / (st.solver_ms / 1e3), 1),
+                     "ms_per_iteration": round(st.solver_ms / 300, 4),
+                     "kernel_ms": {"pass_l": round(kl, 4), "pass_a": round(ka, 4), "pass_b": round(kb, 4)}}
+        s.close()
+        print(f"[bench] {nm} at {bd}: {speed[nm]}", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "storage precision of v and rho vs FP64 oracle (64^3, 300 iterations) and voxel-iterations/s "
+                  "at 182x218x182",
+        "value": speed["bf16"]["voxel_iterations_per_s"],
+        "unit": "voxel-iterations/s (bf16 storage)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "fp32 arithmetic; v, rho stored in fp32 / bf16 / fp16",
+        "data": "synthetic (blob phantom seed 1 for the errors, brain phantom seed 2 for the speed)",
+        "config": {"workload": "precision forms", "error_dims": list(dims), "iterations": iters,
+                   "speed_dims": list(bd)},
+        "fp64_reference": {"dice_all": round(float(ref_dice), 6), "min_det_true": round(float(ref_det), 6)},
+        "errors_vs_fp64": errors,
+        "speed": speed,
+        "speedup_vs_fp32": {nm: round(speed[nm]["voxel_iterations_per_s"] / speed["fp32"]["voxel_iterations_per_s"], 3)
+                            for nm in speed},
     }
     print(json.dumps(out), flush=True)
 

@@ -289,10 +289,7 @@ __device__ __forceinline__ void pass_l(const IterArgs &a, int blk) {
 // storage format (nearest even) as soon as it is computed, and only the rounded value is used
 // afterwards, so the 16-bit forms are exact FP32 computations on representable values.
 __device__ __forceinline__ unsigned f32_to_bf16(float f) {
-    unsigned x = __builtin_bit_cast(unsigned, f);
-    if ((x & 0x7FFFFFFFu) > 0x7F800000u) return (x >> 16) | 0x40u;  // NaN stays a (quiet) NaN
-    x += 0x7FFFu + ((x >> 16) & 1u);
-    return x >> 16;
+    return __builtin_bit_cast(unsigned short, (__bf16)f);  // v_cvt_pk_bf16_f32: IEEE round to nearest even
 }
 __device__ __forceinline__ float bf16_to_f32(unsigned h) { return __builtin_bit_cast(float, h << 16); }
 __device__ __forceinline__ unsigned f32_to_f16(float f) {
