@@ -111,6 +111,10 @@ def lib():
     sig("fr_solver_energy", vp, P(d), P(d))
     sig("fr_warp", i, i, i, vp, vp, vp, vp, vp, vp)
     sig("fr_time_kernels", vp, i, P(f), P(f), P(f))
+    sig("fr_get_inverse", i, i, i, vp, vp, vp)
+    sig("fr_interpolate_gradient", i, i, i, vp, vp, vp, vp)
+    sig("fr_voxel_max_energy", vp, P(f), P(ll))
+    sig("fr_sum", vp, ll, P(d), vp)
     sig("fr_dev_malloc", P(vp), sz)
     sig("fr_dev_free", vp)
     sig("fr_memcpy_htod", vp, vp, sz)
@@ -208,6 +212,37 @@ def warp(img, lab, psi4, dims):
     o_img, o_lab = DeviceArray(n, np.float32), DeviceArray(n, np.int16)
     check(lib().fr_warp(X, Y, Z, d_img.ptr, d_lab.ptr, d_psi.ptr, o_img.ptr, o_lab.ptr, None))
     return o_img.get(), o_lab.get()
+
+
+def get_inverse(psi4, psi_inv4, dims):
+    """DeformationField::get_inverse on the GPU: 48 sweeps of psi_inv = -trilerp(psi, psi_inv) from
+    the given psi_inv (float4 per voxel); returns the result."""
+    X, Y, Z = dims
+    n = X * Y * Z
+    d_psi = DeviceArray.from_host(np.asarray(psi4, np.float32).reshape(n, 4))
+    d_inv = DeviceArray.from_host(np.asarray(psi_inv4, np.float32).reshape(n, 4))
+    check(lib().fr_get_inverse(X, Y, Z, d_psi.ptr, d_inv.ptr, None))
+    return d_inv.get()
+
+
+def interpolate_gradient(field4, psi4, dims):
+    """interpolate_gradient on the GPU: field (float4 per voxel) trilinearly sampled at psi."""
+    X, Y, Z = dims
+    n = X * Y * Z
+    d_f = DeviceArray.from_host(np.asarray(field4, np.float32).reshape(n, 4))
+    d_psi = DeviceArray.from_host(np.asarray(psi4, np.float32).reshape(n, 4))
+    out = DeviceArray((n, 4), np.float32)
+    check(lib().fr_interpolate_gradient(X, Y, Z, d_f.ptr, out.ptr, d_psi.ptr, None))
+    return out.get()
+
+
+def field_sum(a):
+    """ScalarField::sum on the GPU (FP64 accumulation of a float32 array)."""
+    a = np.ascontiguousarray(a, np.float32).reshape(-1)
+    d_a = DeviceArray.from_host(a)
+    out = C.c_double()
+    check(lib().fr_sum(d_a.ptr, a.size, C.byref(out), None))
+    return out.value
 
 
 # ---- solver context ---------------------------------------------------------------------------
@@ -335,6 +370,12 @@ class Solver:
         buf = C.create_string_buffer(size)
         check(lib().fr_get_log(self.h, buf, size))
         return buf.value.decode()
+
+    def voxel_max_energy(self):
+        """Reductor::voxel_max_energy of the current state: (energy, voxel)."""
+        e, v = C.c_float(), C.c_longlong()
+        check(lib().fr_voxel_max_energy(self.h, C.byref(e), C.byref(v)))
+        return e.value, v.value
 
     def set_storage(self, storage):
         """STORAGE_F32 / STORAGE_BF16 / STORAGE_F16 storage of v and rho; discards the state

@@ -78,6 +78,7 @@ struct fr_ctx {
     float *det_parts = nullptr;
     int ndet = 0;
     int16_t *lab_tmp = nullptr;
+    unsigned long long *max_slot = nullptr;  // fr_voxel_max_energy
     // host copies of the last run
     std::vector<float> h_max;
     std::vector<long long> h_argmax;
@@ -135,7 +136,7 @@ static void free_ctx(fr_ctx *c) {
     if (!c) return;
     void *bufs[] = {c->T, c->S, c->LT, c->LS, c->rho[0], c->rho[1], c->v[0], c->v[1], c->pinv[0], c->pinv[1], c->psi,
                     c->L, c->ctl, c->maxpack, c->energy, c->partials, c->dice_counts, c->det_parts, c->lab_tmp,
-                    c->vr[0], c->vr[1]};
+                    c->vr[0], c->vr[1], c->max_slot};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -199,7 +200,7 @@ extern "C" int fr_create(const fr_params *p, fr_ctx **out) {
         (rc = dalloc(&c->psi, 3 * n)) || (rc = dalloc(&c->L, 3 * n)) || (rc = dalloc(&c->ctl, 1)) || (rc = dalloc(&c->maxpack, (size_t)mi * frk::NSLOT)) ||
         (rc = dalloc(&c->energy, 3 * (size_t)mi)) || (rc = dalloc(&c->partials, 2 * (size_t)c->nparts)) ||
         (rc = dalloc(&c->dice_counts, 2 * (size_t)p->no_labels + 1)) || (rc = dalloc(&c->det_parts, 2 * (size_t)c->ndet)) ||
-        (rc = dalloc(&c->lab_tmp, n))) {
+        (rc = dalloc(&c->lab_tmp, n)) || (rc = dalloc(&c->max_slot, 1))) {
         free_ctx(c);
         return rc;
     }
@@ -941,6 +942,65 @@ extern "C" int fr_batch_time(fr_batch *b, float *ms) {
     if (!b || !ms) return fail(FR_ERR_INVALID, "bad arguments");
     HIP_TRY(hipEventSynchronize(b->ev1));
     HIP_TRY(hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return FR_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// the reference's further public functions
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int fr_get_inverse(int X, int Y, int Z, const float *psi4, float *pinv4, void *stream) {
+    if (X < 3 || Y < 3 || Z < 3 || !psi4 || !pinv4) return fail(FR_ERR_INVALID, "bad fr_get_inverse arguments");
+    if (16LL * X * Y * Z >= (1LL << 31)) return fail(FR_ERR_INVALID, "volume too large (16*N must be < 2^31)");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(frk::launch_get_inverse(Dims{X, Y, Z}, psi4, pinv4, 48, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return FR_OK;
+}
+
+extern "C" int fr_interpolate_gradient(int X, int Y, int Z, const float *field4, float *out4, const float *psi4,
+                                       void *stream) {
+    if (X < 1 || Y < 1 || Z < 1 || !field4 || !out4 || !psi4 || field4 == out4)
+        return fail(FR_ERR_INVALID, "bad fr_interpolate_gradient arguments");
+    if (16LL * X * Y * Z >= (1LL << 31)) return fail(FR_ERR_INVALID, "volume too large (16*N must be < 2^31)");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(frk::launch_interpolate_field4(Dims{X, Y, Z}, field4, psi4, out4, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return FR_OK;
+}
+
+extern "C" int fr_voxel_max_energy(fr_ctx *c, float *max_energy, long long *voxel) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (!c->initialised) return fail(FR_ERR_STATE, "fr_voxel_max_energy before fr_solver_init");
+    int rc;
+    if (c->have_run && (rc = collect_run(c))) return rc;
+    HIP_TRY(hipMemsetAsync(c->max_slot, 0, sizeof(unsigned long long), c->stream));
+    HIP_TRY(frk::launch_voxel_max_energy(c->d, c->T, c->S, c->psi, c->prm.w_reg, c->max_slot, c->stream));
+    unsigned long long mp = 0;
+    HIP_TRY(hipMemcpyAsync(&mp, c->max_slot, sizeof mp, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float m;
+    long long idx;
+    unpack_max(mp, &m, &idx);
+    if (max_energy) *max_energy = m;
+    if (voxel) *voxel = idx;
+    return FR_OK;
+}
+
+extern "C" int fr_sum(const float *d, long long n, double *sum, void *stream) {
+    if (!d || n < 0 || !sum) return fail(FR_ERR_INVALID, "bad fr_sum arguments");
+    hipStream_t s = (hipStream_t)stream;
+    double *tmp = nullptr;
+    const int np = frk::sum_partials_count(n);
+    int rc;
+    if ((rc = dalloc(&tmp, (size_t)np + 1))) return rc;
+    hipError_t e = frk::launch_sum(d, n, tmp, tmp + np, s);
+    double h = 0.0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, tmp + np, sizeof h, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(FR_ERR_HIP, "fr_sum: %s", hipGetErrorString(e));
+    *sum = h;
     return FR_OK;
 }
 

@@ -106,6 +106,14 @@ hipError_t launch_dice(Dims d, const int16_t *la, const int16_t *lb, int no_labe
 int min_det_partials_count(Dims d);
 hipError_t launch_min_det(Dims d, const float *psi, float *part_true, float *part_ref, hipStream_t s);
 hipError_t launch_pack4to3(const float *src4, float *dst3a, float *dst3b, int n, hipStream_t s);
+// the reference's further public functions (float4 fields, 16 B per voxel)
+hipError_t launch_get_inverse(Dims d, const float *psi4, float *pinv4, int sweeps, hipStream_t s);
+hipError_t launch_interpolate_field4(Dims d, const float *f4, const float *psi4, float *out4, hipStream_t s);
+// psi xyz-packed; *slot receives the packed (energy bits, ~index) maximum (atomic max: clear it first)
+hipError_t launch_voxel_max_energy(Dims d, const float *T, const float *S, const float *psi, float w_reg,
+                                   unsigned long long *slot, hipStream_t s);
+int sum_partials_count(long long n);
+hipError_t launch_sum(const float *a, long long n, double *partials, double *out, hipStream_t s);
 // 16-bit (v, rho) records (2 unsigned per voxel): initial state (v = 0, rho = rho_0), replace v
 // (float4 per voxel) and/or rho in both buffers, read v (float4) and/or rho
 hipError_t launch_vr_init(int st, unsigned *r0, unsigned *r1, int n, float rho_0, hipStream_t s);

@@ -999,6 +999,109 @@ __global__ void k_unpack3to4(const float *__restrict__ src3, float *__restrict__
     }
 }
 
+// --------------------------------------------------------------------------------------------
+// the reference's further public functions: inverse by fixed point, field interpolation, the
+// largest per-voxel energy, sum of a scalar field
+// --------------------------------------------------------------------------------------------
+
+// trilinear blend of the x, y, z of a float4 field (16-byte records) at point (px, py, pz)
+__device__ __forceinline__ F3 tri_field4(Rsrc r, Dims d, float px, float py, float pz) {
+    const Corner k = corner(d, px, py, pz);
+    const unsigned g = 16u * corner_index(d, k), X16 = 16u * d.X, XY16 = 16u * d.X * d.Y;
+    return tri_at3(r, g, 16u, X16, XY16, k);
+}
+
+// one sweep of estimate_inverse_kernel: psi_inv(p) = -trilerp(psi, psi_inv(p)), all voxels, in
+// place (vector_fields.cu:108-122)
+__global__ void k_estimate_inverse(Dims d, const float *__restrict__ psi4, float *__restrict__ pinv4) {
+    const int n = d.X * d.Y * d.Z;
+    const Rsrc r = rsrc(psi4, 16 * n);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const F3 q = tri_field4(r, d, pinv4[4 * i], pinv4[4 * i + 1], pinv4[4 * i + 2]);
+        pinv4[4 * i] = -1.f * q.x;
+        pinv4[4 * i + 1] = -1.f * q.y;
+        pinv4[4 * i + 2] = -1.f * q.z;
+        pinv4[4 * i + 3] = 0.f;
+    }
+}
+
+// interpolate_gradient_kernel: out(p) = trilerp(field, psi(p)), all voxels (vector_fields.cu:194-225)
+__global__ void k_interpolate_field4(Dims d, const float *__restrict__ f4, const float *__restrict__ psi4,
+                                     float *__restrict__ out4) {
+    const int n = d.X * d.Y * d.Z;
+    const Rsrc r = rsrc(f4, 16 * n);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const F3 q = tri_field4(r, d, psi4[4 * i], psi4[4 * i + 1], psi4[4 * i + 2]);
+        out4[4 * i] = q.x;
+        out4[4 * i + 1] = q.y;
+        out4[4 * i + 2] = q.z;
+        out4[4 * i + 3] = 0.f;
+    }
+}
+
+// reduce_voxel_max_energy_kernel: per voxel (T - S o psi)^2 + w_reg * ((|r0|^2 + |r1|^2) + |r2|^2)
+// with r the rows of J = D(psi - id) (central differences, mirrored neighbours at the faces) and
+// |r|^2 = (x*x + y*y) + z*z; the largest, lowest voxel on ties, into *slot as a packed (energy bits,
+// ~index) key (reductor.cu:637-760, vector_fields.cu:537-585, cuda_utils.hpp:255-257)
+__global__ __launch_bounds__(256) void k_voxel_max_energy(Dims d, const float *__restrict__ T,
+                                                         const float *__restrict__ S,
+                                                         const float *__restrict__ psi, float w_reg,
+                                                         unsigned long long *slot) {
+    const int n = d.X * d.Y * d.Z;
+    const Rsrc rs = rsrc(S, 4 * n);
+    unsigned long long pack = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int x = i % d.X, r = i / d.X, y = r % d.Y, z = r / d.Y;
+        const F3 p = ld3(psi, i);
+        const Corner k = corner(d, p.x, p.y, p.z);
+        const float sp = tri_at1(rs, 4u * corner_index(d, k), 4u, 4u * d.X, 4u * d.X * d.Y, k);
+        const int x1 = x == d.X - 1 ? x - 1 : x + 1, x2 = x == 0 ? x + 1 : x - 1;
+        const int y1 = y == d.Y - 1 ? y - 1 : y + 1, y2 = y == 0 ? y + 1 : y - 1;
+        const int z1 = z == d.Z - 1 ? z - 1 : z + 1, z2 = z == 0 ? z + 1 : z - 1;
+        const F3 ax1 = sub3(ld3(psi, lin(d, x1, y, z)), f3((float)x1, (float)y, (float)z));
+        const F3 ax2 = sub3(ld3(psi, lin(d, x2, y, z)), f3((float)x2, (float)y, (float)z));
+        const F3 ay1 = sub3(ld3(psi, lin(d, x, y1, z)), f3((float)x, (float)y1, (float)z));
+        const F3 ay2 = sub3(ld3(psi, lin(d, x, y2, z)), f3((float)x, (float)y2, (float)z));
+        const F3 az1 = sub3(ld3(psi, lin(d, x, y, z1)), f3((float)x, (float)y, (float)z1));
+        const F3 az2 = sub3(ld3(psi, lin(d, x, y, z2)), f3((float)x, (float)y, (float)z2));
+        const F3 jx = mul3(0.5f, sub3(ax1, ax2)), jy = mul3(0.5f, sub3(ay1, ay2)), jz = mul3(0.5f, sub3(az1, az2));
+        const float n0 = (jx.x * jx.x + jy.x * jy.x) + jz.x * jz.x;
+        const float n1 = (jx.y * jx.y + jy.y * jy.y) + jz.y * jz.y;
+        const float n2 = (jx.z * jx.z + jy.z * jy.z) + jz.z * jz.z;
+        const float df = T[i] - sp;
+        const float e = df * df + w_reg * ((n0 + n1) + n2);
+        const unsigned long long pk =
+            ((unsigned long long)__float_as_uint(e) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)i);
+        if (!isnan(e) && pk > pack) pack = pk;
+    }
+    pack = wave_max_u64(pack);
+    __shared__ unsigned long long red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = pack;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) pack = red[w] > pack ? red[w] : pack;
+        atomicMax(slot, pack);
+    }
+}
+
+// sum of a float array in FP64: per-block partial sums, then one block over the partials
+__global__ __launch_bounds__(256) void k_sum_partials(const float *__restrict__ a, long long n, double *partials) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        acc += (double)a[i];
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_sum_final(const double *partials, int np, double *out) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < np; j += 256) acc += partials[j];
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) *out = acc;
+}
+
 // (v, rho) records of the 16-bit storage forms: initial state, set from and read into FP32 arrays
 template <int ST> __global__ void k_vr_init(unsigned *r0, unsigned *r1, int n, float rho_0) {
     const unsigned hi = h_of<ST>(rho_0) << 16;  // v = 0, rho = rho_0 (rounded)
@@ -1164,6 +1267,38 @@ int min_det_partials_count(Dims d) { return grid_for(d.X * d.Y * d.Z, RED_BLOCK,
 
 hipError_t launch_min_det(Dims d, const float *psi, float *part_true, float *part_ref, hipStream_t s) {
     hipLaunchKernelGGL(k_min_det, dim3(min_det_partials_count(d)), dim3(RED_BLOCK), 0, s, d, psi, part_true, part_ref);
+    return hipGetLastError();
+}
+
+hipError_t launch_get_inverse(Dims d, const float *psi4, float *pinv4, int sweeps, hipStream_t s) {
+    const int n = d.X * d.Y * d.Z;
+    for (int k = 0; k < sweeps; ++k) {
+        hipLaunchKernelGGL(k_estimate_inverse, dim3(grid_for(n, 256)), dim3(256), 0, s, d, psi4, pinv4);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_interpolate_field4(Dims d, const float *f4, const float *psi4, float *out4, hipStream_t s) {
+    const int n = d.X * d.Y * d.Z;
+    hipLaunchKernelGGL(k_interpolate_field4, dim3(grid_for(n, 256)), dim3(256), 0, s, d, f4, psi4, out4);
+    return hipGetLastError();
+}
+
+hipError_t launch_voxel_max_energy(Dims d, const float *T, const float *S, const float *psi, float w_reg,
+                                   unsigned long long *slot, hipStream_t s) {
+    const int n = d.X * d.Y * d.Z;
+    hipLaunchKernelGGL(k_voxel_max_energy, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, d, T, S, psi, w_reg, slot);
+    return hipGetLastError();
+}
+
+int sum_partials_count(long long n) { return (int)(n / 256 + 1 < 1024 ? n / 256 + 1 : 1024); }
+
+hipError_t launch_sum(const float *a, long long n, double *partials, double *out, hipStream_t s) {
+    const int np = sum_partials_count(n);
+    hipLaunchKernelGGL(k_sum_partials, dim3(np), dim3(256), 0, s, a, n, partials);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, partials, np, out);
     return hipGetLastError();
 }
 

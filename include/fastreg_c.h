@@ -202,6 +202,27 @@ int fr_batch_time_iterations(fr_batch *b, int iters, float *ms);
 int fr_warp(int X, int Y, int Z, const float *d_img, const int16_t *d_lab, const float *d_psi4, float *d_out_img,
             int16_t *d_out_lab, void *hip_stream);
 
+/* ---- the reference's further public functions (not used by its registration loop) -------- */
+
+/* DeformationField::get_inverse (src/fastreg/vector_fields.cpp:96-103, src/fastreg/cuda/
+ * vector_fields.cu:108-136): 48 sweeps of psi_inv(p) = -trilerp(psi, psi_inv(p)) over all voxels,
+ * in place, starting from the given d_psi_inv4; float4 (x, y, z, w) fields, w written 0.  Waits. */
+int fr_get_inverse(int X, int Y, int Z, const float *d_psi4, float *d_psi_inv4, void *hip_stream);
+
+/* interpolate_gradient (src/fastreg/cuda/vector_fields.cu:194-225): out(p) = trilerp(field,
+ * psi(p)) over all voxels; float4 fields (d_out4 distinct from d_field4).  Waits. */
+int fr_interpolate_gradient(int X, int Y, int Z, const float *d_field4, float *d_out4, const float *d_psi4,
+                            void *hip_stream);
+
+/* Reductor::voxel_max_energy (src/fastreg/reductor.cpp:82-89, src/fastreg/cuda/reductor.cu:637-760) on
+ * the context's current state: the largest per-voxel (T - S o psi)^2 + W_REG * (sum of the squared
+ * rows of D(psi - id)) and its voxel (the lowest index on equal energies).  Waits. */
+int fr_voxel_max_energy(fr_ctx *ctx, float *max_energy, long long *voxel);
+
+/* ScalarField::sum (src/fastreg/scalar_fields.cpp:29-33, src/fastreg/cuda/scalar_fields.cu:75-295): the
+ * sum of n device floats, accumulated in FP64 (the reference accumulates float block sums).  Waits. */
+int fr_sum(const float *d_data, long long n, double *sum, void *hip_stream);
+
 /* Average device time per launch of the solver's three iteration kernels (pass L: -Laplacian of
  * psi; pass A: density, velocity and psi_inv updates; pass B: psi update and max norm), from HIP
  * events between consecutive launches of `reps` iterations run in sequence on the context stream,

@@ -4,10 +4,12 @@
 // in fastreg_c.h.  Header-only; link libfastreg_hip.so.
 #pragma once
 
+#include <array>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 #include "fastreg_c.h"
 #include "params.hpp"
@@ -73,8 +75,37 @@ public:
     explicit DeformationField(const std::array<int, 3> &dims)
         : dims_(dims), buf_(4 * (size_t)dims[0] * dims[1] * dims[2]) {}
     float *data() { return buf_.data(); }
+    const float *data() const { return buf_.data(); }
     const std::array<int, 3> &get_dims() const { return dims_; }
     void download(float *host_xyzw) const { buf_.download(host_xyzw); }
+
+    // DeformationField::get_inverse (vector_fields.cpp:96-103): 48 fixed-point sweeps of
+    // psi_inv = -trilerp(*this, psi_inv), starting from psi_inv's current values
+    void get_inverse(DeformationField &psi_inv) const {
+        check(fr_get_inverse(dims_[0], dims_[1], dims_[2], data(), psi_inv.data(), nullptr));
+    }
+
+    // interpolate_gradient (vector_fields.cu:194-225): out = trilerp(field, *this) per voxel
+    void interpolate(const DeformationField &field, DeformationField &out) const {
+        check(fr_interpolate_gradient(dims_[0], dims_[1], dims_[2], field.data(), out.data(), data(), nullptr));
+    }
+
+private:
+    std::array<int, 3> dims_;
+    DeviceBuffer<float> buf_;
+};
+
+// a scalar field on the device (ScalarField, scalar_fields.hpp:20-65)
+class ScalarField {
+public:
+    explicit ScalarField(const std::array<int, 3> &dims) : dims_(dims), buf_((size_t)dims[0] * dims[1] * dims[2]) {}
+    float *data() { return buf_.data(); }
+    // ScalarField::sum (scalar_fields.cpp:29-33), accumulated in FP64
+    double sum() const {
+        double s = 0.0;
+        check(fr_sum(buf_.data(), (long long)buf_.size(), &s, nullptr));
+        return s;
+    }
 
 private:
     std::array<int, 3> dims_;
@@ -108,6 +139,14 @@ public:
                                  phi_n_psi->labels()));
         if (psi) check(fr_get_deformation(ctx_, psi->data()));
         if (psi_inv) check(fr_get_inverse_deformation(ctx_, psi_inv->data()));
+    }
+
+    // Reductor::voxel_max_energy (reductor.cpp:82-89) of the current state: (energy, voxel)
+    std::pair<float, long long> voxel_max_energy() {
+        float e = 0.f;
+        long long v = 0;
+        check(fr_voxel_max_energy(ctx_, &e, &v));
+        return {e, v};
     }
 
     fr_ctx *handle() { return ctx_; }

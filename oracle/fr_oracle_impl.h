@@ -457,6 +457,49 @@ static void N_(k_energy)(DIMS d, const V2 *tgt, const V2 *src_psi, const M4 *J, 
     *e_reg = 0.5 * B;
 }
 
+/* estimate_inverse_kernel, one sweep over all voxels: psi_inv(p) = -trilerp(psi, psi_inv(p)), in
+ * place (a voxel reads only its own psi_inv) — vector_fields.cu:108-122; DeformationField::
+ * get_inverse runs 48 sweeps (vector_fields.cu:124-136, vector_fields.cpp:96-103) */
+static void N_(k_estimate_inverse)(DIMS d, const V4 *psi, V4 *psi_inv) {
+    FOR_ALL(d) {
+        long i = IX(d, x, y, z);
+        V4 r = N_(interp_field)(d, psi, psi_inv[i].x, psi_inv[i].y, psi_inv[i].z);
+        psi_inv[i] = N_(mul4)((REAL) -1, r);
+    }
+}
+
+/* interpolate_gradient_kernel: out(p) = trilerp(grad, psi(p)), all voxels — vector_fields.cu:194-225 */
+static void N_(k_interpolate_gradient)(DIMS d, const V4 *grad, V4 *out, const V4 *psi) {
+    FOR_ALL(d) {
+        long i = IX(d, x, y, z);
+        out[i] = N_(interp_field)(d, grad, psi[i].x, psi[i].y, psi[i].z);
+    }
+}
+
+/* squared norm of a Jacobian row: (x*x + y*y) + z*z, products rounded separately —
+ * cuda_utils.hpp:255-257 */
+static inline REAL N_(norm_sq3)(V4 r) { return (r.x * r.x + r.y * r.y) + r.z * r.z; }
+
+/* reduce_voxel_max_energy: the largest per-voxel energy (T - S o psi)^2 + w_reg * ((|J r0|^2 +
+ * |J r1|^2) + |J r2|^2) over all voxels and its voxel; on equal energies the lowest index —
+ * reductor.cu:637-760, reductor.cpp:82-89, 107-120 */
+static REAL N_(k_voxel_max_energy)(DIMS d, const V2 *tgt, const V2 *src_psi, const M4 *J, REAL w_reg, long *arg) {
+    long n = (long) d.X * d.Y * d.Z, best_i = 0;
+    REAL best = 0;
+    int found = 0;
+    for (long i = 0; i < n; ++i) {
+        REAL df = tgt[i].x - src_psi[i].x;
+        REAL e = df * df + w_reg * ((N_(norm_sq3)(J[i].r[0]) + N_(norm_sq3)(J[i].r[1])) + N_(norm_sq3)(J[i].r[2]));
+        if (e == e && (!found || e > best)) {
+            best = e;
+            best_i = i;
+            found = 1;
+        }
+    }
+    *arg = best_i;
+    return best;
+}
+
 /* Dice counts over [0,X-2]x[0,Y-2]x[0,Z-2] (the last slice of each axis is skipped), labels
  * taken as array indices — solver.cu:9-50.  Labels outside [0, no_labels) are counted in
  * `bad` instead of indexing out of bounds. */

@@ -228,6 +228,15 @@ void N_(or_min_det)(void *h, double *min_true, double *min_ref) {
     N_(k_min_det)(c->d, c->psi, min_true, min_ref);
 }
 
+/* Reductor::voxel_max_energy on the current state: S o psi and J = D(psi - id) as for the energy
+ * (reductor.cpp:82-89) */
+REAL N_(or_voxel_max_energy)(void *h, REAL w_reg, long *arg) {
+    CTX *c = (CTX *) h;
+    N_(k_apply)(c->d, c->phi_n, c->phi_n_psi, c->psi);
+    N_(k_deformation_jacobian)(c->d, c->psi, c->J);
+    return N_(k_voxel_max_energy)(c->d, c->phi_global, c->phi_n_psi, c->J, w_reg, arg);
+}
+
 /* state access, REAL arrays: v (N,4), rho (N), psi (N,4), psi_inv (N,4); NULL skips a field */
 void N_(or_get_state)(void *h, REAL *v, REAL *rho, REAL *psi, REAL *psi_inv) {
     CTX *c = (CTX *) h;
@@ -277,6 +286,17 @@ int N_(or_get_field)(void *h, const char *name, REAL *out) {
 
 static DIMS N_(mkdims)(int X, int Y, int Z) { DIMS d; d.X = X; d.Y = Y; d.Z = Z; return d; }
 
+void N_(ork_get_inverse)(int X, int Y, int Z, const REAL *psi4, REAL *psi_inv4, int sweeps) {
+    for (int k = 0; k < sweeps; ++k) N_(k_estimate_inverse)(N_(mkdims)(X, Y, Z), (const V4 *) psi4, (V4 *) psi_inv4);
+}
+void N_(ork_interpolate_gradient)(int X, int Y, int Z, const REAL *grad4, REAL *out4, const REAL *psi4) {
+    N_(k_interpolate_gradient)(N_(mkdims)(X, Y, Z), (const V4 *) grad4, (V4 *) out4, (const V4 *) psi4);
+}
+REAL N_(ork_voxel_max_energy)(int X, int Y, int Z, const REAL *tgt2, const REAL *src_psi2, const REAL *J16, REAL w_reg,
+                              long *arg) {
+    return N_(k_voxel_max_energy)(N_(mkdims)(X, Y, Z), (const V2 *) tgt2, (const V2 *) src_psi2, (const M4 *) J16,
+                                  w_reg, arg);
+}
 void N_(ork_apply)(int X, int Y, int Z, const REAL *phi2, REAL *out2, const REAL *psi4) {
     N_(k_apply)(N_(mkdims)(X, Y, Z), (const V2 *) phi2, (V2 *) out2, (const V4 *) psi4);
 }

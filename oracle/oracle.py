@@ -62,6 +62,10 @@ def lib():
             f("or_set_state", None, vp, vp, vp, vp, vp)
             f("or_get_field", i, vp, C.c_char_p, vp)
             f("ork_apply", None, i, i, i, vp, vp, vp)
+            f("ork_get_inverse", None, i, i, i, vp, vp, i)
+            f("ork_interpolate_gradient", None, i, i, i, vp, vp, vp)
+            f("ork_voxel_max_energy", R, i, i, i, vp, vp, vp, R, C.POINTER(C.c_long))
+            f("or_voxel_max_energy", R, vp, R, C.POINTER(C.c_long))
             f("ork_volume_gradient", None, i, i, i, vp, vp)
             f("ork_jacobian", None, i, i, i, vp, vp)
             f("ork_deformation_jacobian", None, i, i, i, vp, vp)
@@ -186,6 +190,12 @@ class Oracle:
         self._fn("or_energy")(self.h, C.byref(a), C.byref(b))
         return a.value, b.value
 
+    def voxel_max_energy(self, w_reg):
+        """(max per-voxel energy, its voxel) of the current state (Reductor::voxel_max_energy)."""
+        arg = C.c_long(0)
+        e = self._fn("or_voxel_max_energy")(self.h, w_reg, C.byref(arg))
+        return e, arg.value
+
     def run(self, params: SolverParams, log: bool = False):
         prm = params.c_struct()
         mh = np.zeros(max(params.max_iter, 1), self.dtype)
@@ -241,6 +251,37 @@ class Oracle:
 
 def _k(prec, name):
     return getattr(lib(), name + "_" + prec)
+
+
+def get_inverse(dims, psi4, psi_inv4, sweeps=48, prec="f32"):
+    """DeformationField::get_inverse: `sweeps` sweeps of psi_inv = -trilerp(psi, psi_inv)."""
+    X, Y, Z = dims
+    n = X * Y * Z
+    dt = _dt(prec)
+    psi4 = _chk(psi4, dt, (n, 4))
+    out = _chk(psi_inv4, dt, (n, 4)).copy()
+    _k(prec, "ork_get_inverse")(X, Y, Z, _p(psi4), _p(out), sweeps)
+    return out
+
+
+def interpolate_gradient(dims, grad4, psi4, prec="f32"):
+    X, Y, Z = dims
+    n = X * Y * Z
+    dt = _dt(prec)
+    grad4, psi4 = _chk(grad4, dt, (n, 4)), _chk(psi4, dt, (n, 4))
+    out = np.zeros((n, 4), dt)
+    _k(prec, "ork_interpolate_gradient")(X, Y, Z, _p(grad4), _p(out), _p(psi4))
+    return out
+
+
+def voxel_max_energy(dims, tgt2, src_psi2, J16, w_reg, prec="f32"):
+    X, Y, Z = dims
+    n = X * Y * Z
+    dt = _dt(prec)
+    tgt2, src_psi2, J16 = _chk(tgt2, dt, (n, 2)), _chk(src_psi2, dt, (n, 2)), _chk(J16, dt, (n, 4, 4))
+    arg = C.c_long(0)
+    e = _k(prec, "ork_voxel_max_energy")(X, Y, Z, _p(tgt2), _p(src_psi2), _p(J16), w_reg, C.byref(arg))
+    return e, arg.value
 
 
 def apply(dims, phi2, psi4, prec="f32"):

@@ -549,3 +549,94 @@ def test_dice_skips_last_slice(oracle_mod):
     o.init(pair.target, lab, pair.target, lab2)
     ov2, vx2, _ = o.dice_counts(pair.no_labels)
     assert ov2.sum() == ov.sum() - 1
+
+
+# ---------------------------------------------------------------------------------------------
+# the reference's further public functions (get_inverse, interpolate_gradient,
+# voxel_max_energy, ScalarField::sum)
+# ---------------------------------------------------------------------------------------------
+
+def _map_field(F4, pos4, dims):
+    """map_coordinates(order=1, mode='nearest') of each of the 3 components of F4 at pos4."""
+    X, Y, Z = dims
+    out = np.zeros((F4.shape[0], 4))
+    for c in range(3):
+        out[:, c] = ndimage.map_coordinates(F4[:, c].reshape(Z, Y, X), [pos4[:, 2], pos4[:, 1], pos4[:, 0]],
+                                            order=1, mode="nearest")
+    return out
+
+
+def test_get_inverse_closed_forms(oracle_mod):
+    O = oracle_mod
+    dims = (9, 8, 7)
+    n = 9 * 8 * 7
+    rng = np.random.default_rng(21)
+    # a constant field c: one sweep gives -c everywhere, and it stays there
+    psi = np.zeros((n, 4))
+    psi[:, :3] = [0.75, -1.5, 2.25]
+    start = rng.uniform(-3, 12, (n, 4))
+    start[:, 3] = 0
+    for sweeps in (1, 48):
+        out = O.get_inverse(dims, psi, start, sweeps, "f64")
+        assert np.all(out[:, :3] == [-0.75, 1.5, -2.25]), sweeps
+    # psi.x = a * x: from psi_inv.x = 3 one sweep gives -3a, the next -a * clamp(-3a) = -0
+    a = 0.5
+    x, _, _ = grid(dims)
+    psi = np.zeros((n, 4))
+    psi[:, 0] = a * x
+    start = np.zeros((n, 4))
+    start[:, 0] = 3.0
+    assert np.all(O.get_inverse(dims, psi, start, 1, "f64")[:, 0] == -1.5)
+    assert np.all(O.get_inverse(dims, psi, start, 2, "f64")[:, 0] == 0.0)
+
+
+def test_get_inverse_matches_map_coordinates(oracle_mod):
+    O = oracle_mod
+    dims = (11, 9, 8)
+    n = 11 * 9 * 8
+    rng = np.random.default_rng(22)
+    psi = np.zeros((n, 4))
+    psi[:, :3] = rng.normal(0, 4.0, (n, 3))
+    inv = rng.uniform(-2, 12, (n, 4))
+    inv[:, 3] = 0
+    ref = inv.copy()
+    for _ in range(3):
+        ref = -_map_field(psi, ref, dims)
+    out = O.get_inverse(dims, psi, inv, 3, "f64")
+    np.testing.assert_allclose(out[:, :3], ref[:, :3], atol=1e-11)
+
+
+def test_interpolate_gradient_matches_map_coordinates(oracle_mod):
+    O = oracle_mod
+    dims = (10, 12, 7)
+    n = 10 * 12 * 7
+    rng = np.random.default_rng(23)
+    g = np.zeros((n, 4))
+    g[:, :3] = rng.normal(size=(n, 3))
+    assert np.array_equal(O.interpolate_gradient(dims, g, identity4(dims, np.float64), "f64"), g)
+    psi = identity4(dims, np.float64)
+    psi[:, :3] += rng.normal(0, 2.5, (n, 3))
+    out = O.interpolate_gradient(dims, g, psi, "f64")
+    np.testing.assert_allclose(out[:, :3], _map_field(g, psi, dims)[:, :3], atol=1e-12)
+    assert np.all(out[:, 3] == 0)
+
+
+def test_voxel_max_energy_brute_force(oracle_mod):
+    O = oracle_mod
+    dims = (9, 10, 8)
+    n = 9 * 10 * 8
+    rng = np.random.default_rng(24)
+    tgt = np.zeros((n, 2))
+    src = np.zeros((n, 2))
+    tgt[:, 0], src[:, 0] = rng.random(n), rng.random(n)
+    psi = identity4(dims, np.float64)
+    psi[:, :3] += rng.normal(0, 0.8, (n, 3))
+    J = O.deformation_jacobian(dims, psi, "f64")
+    w = 0.2
+    e_ref = (tgt[:, 0] - src[:, 0]) ** 2 + w * (J[:, :3, :3] ** 2).sum(axis=(1, 2))
+    e, arg = O.voxel_max_energy(dims, tgt, src, J, w, "f64")
+    assert arg == int(np.argmax(e_ref))
+    assert abs(e - e_ref.max()) <= 1e-12 * e_ref.max()
+    # equal energies everywhere: the lowest voxel
+    e, arg = O.voxel_max_energy(dims, np.zeros((n, 2)), np.zeros((n, 2)), np.zeros((n, 4, 4)), w, "f64")
+    assert (e, arg) == (0.0, 0)
