@@ -115,6 +115,13 @@ def lib():
     sig("fr_interpolate_gradient", i, i, i, vp, vp, vp, vp)
     sig("fr_voxel_max_energy", vp, P(f), P(ll))
     sig("fr_sum", vp, ll, P(d), vp)
+    sig("fr_adjoint_create", P(FrParams), i, i, P(vp))
+    sig("fr_adjoint_destroy", vp)
+    sig("fr_adjoint_set_images", vp, vp, vp, vp, vp)
+    sig("fr_adjoint_forward", vp, vp, d, d, d, P(d))
+    sig("fr_adjoint_backward", vp, vp, P(d), P(d))
+    sig("fr_adjoint_get_state", vp, i, vp, vp, vp, vp)
+    sig("fr_adjoint_info", vp, P(sz), P(f), P(f))
     sig("fr_dev_malloc", P(vp), sz)
     sig("fr_dev_free", vp)
     sig("fr_memcpy_htod", vp, vp, sz)
@@ -437,6 +444,88 @@ class Batch:
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             lib().fr_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Adjoint:
+    """fr_adjoint: K unrolled iterations of the solver in FP64 from (RHO_0, v0, identity,
+    identity), the objective E = 0.5 sum (T - S o psi)^2 + label_weight * (label-overlap
+    surrogate) of the final state, and its gradients with respect to v0, W_REG and TAU through
+    hand-written adjoint kernels, with checkpoints every `ckpt` iterations."""
+
+    def __init__(self, params: FrParams, K: int, ckpt: int):
+        self.params = params
+        self.dims = tuple(params.vol_dims)
+        self.n = self.dims[0] * self.dims[1] * self.dims[2]
+        self.K, self.ckpt = K, ckpt
+        h = C.c_void_p()
+        check(lib().fr_adjoint_create(C.byref(params), K, ckpt, C.byref(h)))
+        self.h = h
+        self._v0 = None
+        self.forward_calls = 0  # counts forward(); backward() differentiates the latest one
+
+    def set_images(self, tgt, tgt_lab, src, src_lab):
+        n = self.n
+        bufs = [DeviceArray.from_host(np.asarray(tgt, np.float32).reshape(n)),
+                DeviceArray.from_host(np.asarray(tgt_lab, np.int16).reshape(n)),
+                DeviceArray.from_host(np.asarray(src, np.float32).reshape(n)),
+                DeviceArray.from_host(np.asarray(src_lab, np.int16).reshape(n))]
+        check(lib().fr_adjoint_set_images(self.h, *[b.ptr for b in bufs]))
+
+    def forward(self, v0=None, w_reg=None, tau=None, label_weight=0.0) -> float:
+        """E after K iterations from v0 ((N, 3) float64 host array, a DeviceArray, or None for 0);
+        w_reg / tau default to the parameters' W_REG / TAU (as float32 values)."""
+        w = float(np.float32(self.params.w_reg)) if w_reg is None else float(w_reg)
+        t = float(np.float32(self.params.tau)) if tau is None else float(tau)
+        ptr = None
+        if isinstance(v0, DeviceArray):
+            ptr = v0.ptr
+        elif v0 is not None:
+            if self._v0 is None:
+                self._v0 = DeviceArray((self.n, 3), np.float64)
+            self._v0.upload(np.asarray(v0, np.float64).reshape(self.n, 3))
+            ptr = self._v0.ptr
+        e = C.c_double()
+        check(lib().fr_adjoint_forward(self.h, ptr, w, t, float(label_weight), C.byref(e)))
+        self.forward_calls += 1
+        return e.value
+
+    def backward(self):
+        """(dE/dv0 as (N, 3) float64, dE/dW_REG, dE/dTAU) of the last forward pass."""
+        g = DeviceArray((self.n, 3), np.float64)
+        gw, gt = C.c_double(), C.c_double()
+        check(lib().fr_adjoint_backward(self.h, g.ptr, C.byref(gw), C.byref(gt)))
+        out = g.get()
+        g.free()
+        return out, gw.value, gt.value
+
+    def state(self, k):
+        """The state after k iterations (k a multiple of ckpt below K, or K): dict of v (N, 3),
+        rho (N), psi (N, 3), psi_inv (N, 3), float64."""
+        n = self.n
+        b = {"v": DeviceArray((n, 3), np.float64), "rho": DeviceArray(n, np.float64),
+             "psi": DeviceArray((n, 3), np.float64), "psi_inv": DeviceArray((n, 3), np.float64)}
+        check(lib().fr_adjoint_get_state(self.h, k, b["v"].ptr, b["rho"].ptr, b["psi"].ptr, b["psi_inv"].ptr))
+        return {k_: a.get() for k_, a in b.items()}
+
+    def info(self):
+        """(device bytes held, device ms of the last forward, of the last backward)."""
+        by, f, b = C.c_size_t(), C.c_float(), C.c_float()
+        check(lib().fr_adjoint_info(self.h, C.byref(by), C.byref(f), C.byref(b)))
+        return by.value, f.value, b.value
+
+    def close(self):
+        if getattr(self, "_v0", None) is not None:
+            self._v0.free()
+            self._v0 = None
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().fr_adjoint_destroy(self.h)
         self.h = None
 
     def __del__(self):

@@ -23,7 +23,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libfastreg_hip.so")
 APP = os.path.join(PKG, "fastreg_app")
 
-HIP_SOURCES = ["fr_kernels.hip", "fr_capi.hip"]
+HIP_SOURCES = ["fr_kernels.hip", "fr_capi.hip", "fr_adjoint.hip"]
 APP_SOURCES = ["app.cpp", "nifti.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,1096 @@
+// The differentiated solver: K iterations of the registration solver in FP64 and their adjoint,
+// for gradients of a final objective with respect to the initial velocity and the regularisation
+// weights (w_reg, tau).
+//
+// Forward: the reference's iteration as separate steps (solver.cu:295-444), in FP64 and in the
+// reference's operation order, one thread per voxel:
+//   F1 rho1 = rho + alpha * div(rho, v)                           (solver.cu:101-173)
+//   F2 J = D psi_inv                                              (vector_fields.cu:503-535)
+//   F3 L = -Laplacian(psi)                                        (vector_fields.cu:275-300)
+//   F4 Lo = trilerp(L, psi_inv), Tp = trilerp(T, psi_inv)         (vector_fields.cu:231-247, 81-99)
+//   F5 gU = ((S - Tp) grad S + w_reg Lo) det_ref(J)               (solver.cu:56-80)
+//   F6 dvv = (Dv)v, dlv = upwind Laplacian of v                   (vector_fields.cu:307-404)
+//   F7 v1 = v - alpha (m1t v + dvv - tau dlv + q gU), q = m2/(|rho1| + 1e-5)   (solver.cu:188-211)
+//   F8 psi1 = psi + alpha trilerp(v1, psi)                        (solver.cu:230-247)
+//   F9 psi_inv1 = psi_inv - alpha J v1                            (solver.cu:263-279)
+// Objective of the final state: E = 0.5 sum (T - S o psi)^2 + lw * 0.5 sum_l sum (1[LT = l] -
+// trilerp(1[LS = l], psi))^2 over labels l = 1..NO_LABELS-1 (a label-overlap surrogate).
+//
+// Backward: one step B9..B1 per forward step in reverse order, each the transpose of its step's
+// derivative; the transposes of the gathers (trilerp, stencils) scatter with FP64 atomics.  The
+// derivative of trilerp with respect to its position is that of the piecewise-trilinear function
+// (zero where the position is clamped); the upwind / min / max branches are differentiated
+// within the branch the forward took.  Checkpoints of the state every `ckpt` iterations; the
+// backward pass recomputes each segment's states from its checkpoint.
+
+#include "../../include/fastreg_c.h"
+#include "fr_internal.hpp"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <new>
+#include <vector>
+
+namespace frd {
+
+struct D3 {
+    double x, y, z;
+};
+
+__device__ __forceinline__ D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+__device__ __forceinline__ D3 ld(const double *p, long i) { return d3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+__device__ __forceinline__ void st(double *p, long i, D3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+// FP64 atomic add in hardware (global_atomic_add_f64; every buffer here is ordinary device memory)
+__device__ __forceinline__ void aadd(double *p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void add_at(double *p, long i, D3 v) {
+    aadd(&p[3 * i], v.x);
+    aadd(&p[3 * i + 1], v.y);
+    aadd(&p[3 * i + 2], v.z);
+}
+__device__ __forceinline__ double comp(D3 v, int c) { return c == 0 ? v.x : c == 1 ? v.y : v.z; }
+
+struct Dims {
+    int X, Y, Z;
+};
+__device__ __forceinline__ long lin(Dims d, int x, int y, int z) { return x + (long)d.X * (y + (long)d.Y * z); }
+__device__ __forceinline__ bool interior(Dims d, int x, int y, int z) {
+    return x >= 1 && x <= d.X - 2 && y >= 1 && y <= d.Y - 2 && z >= 1 && z <= d.Z - 2;
+}
+#define FOR_VOXELS(d, i)                                                                                    \
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x, n_ = (long)(d).X * (d).Y * (d).Z; i < n_; \
+         i += (long)gridDim.x * blockDim.x)
+#define XYZ(d, i)                                     \
+    const int x = (int)((i) % (d).X);                 \
+    const int y = (int)(((i) / (d).X) % (d).Y);       \
+    const int z = (int)((i) / ((long)(d).X * (d).Y))
+
+// lerp(v0, v1, t) = fma(t, v0, fma(-t, v1, v1)) (cuda_utils.hpp:43-46), in FP64
+__device__ __forceinline__ double lerp(double v0, double v1, double t) { return fma(t, v0, fma(-t, v1, v1)); }
+
+// clamped corner and weights (cuda_utils.hpp:60-136); `in` marks the axes along which the
+// position is inside the grid (the derivative with respect to a clamped coordinate is 0)
+struct Corner {
+    long g[8];  // linear indices of the 8 corners, order 111, 110, 101, 100, 011, 010, 001, 000
+    double a, b, c;
+    bool inx, iny, inz;
+};
+__device__ __forceinline__ Corner corner(Dims d, double px, double py, double pz) {
+    Corner k;
+    const double cx = fmin(fmax(0.0, px), (double)(d.X - 1)), cy = fmin(fmax(0.0, py), (double)(d.Y - 1)),
+                 cz = fmin(fmax(0.0, pz), (double)(d.Z - 1));
+    const int gx = (int)floor(cx), gy = (int)floor(cy), gz = (int)floor(cz);
+    k.a = cx - gx;
+    k.b = cy - gy;
+    k.c = cz - gz;
+    const int hx = gx + 1 < d.X ? gx + 1 : gx, hy = gy + 1 < d.Y ? gy + 1 : gy, hz = gz + 1 < d.Z ? gz + 1 : gz;
+    k.g[0] = lin(d, hx, hy, hz);
+    k.g[1] = lin(d, hx, hy, gz);
+    k.g[2] = lin(d, hx, gy, hz);
+    k.g[3] = lin(d, hx, gy, gz);
+    k.g[4] = lin(d, gx, hy, hz);
+    k.g[5] = lin(d, gx, hy, gz);
+    k.g[6] = lin(d, gx, gy, hz);
+    k.g[7] = lin(d, gx, gy, gz);
+    k.inx = px > 0.0 && px < (double)(d.X - 1);
+    k.iny = py > 0.0 && py < (double)(d.Y - 1);
+    k.inz = pz > 0.0 && pz < (double)(d.Z - 1);
+    return k;
+}
+// trilerp of 8 corner values in the reference's nesting
+__device__ __forceinline__ double tri(const double f[8], double a, double b, double c) {
+    return lerp(lerp(lerp(f[0], f[1], c), lerp(f[2], f[3], c), b), lerp(lerp(f[4], f[5], c), lerp(f[6], f[7], c), b),
+                a);
+}
+// weights of the 8 corners (the derivative of trilerp with respect to the corner values)
+__device__ __forceinline__ void tri_weights(double a, double b, double c, double w[8]) {
+    w[0] = a * b * c;
+    w[1] = a * b * (1 - c);
+    w[2] = a * (1 - b) * c;
+    w[3] = a * (1 - b) * (1 - c);
+    w[4] = (1 - a) * b * c;
+    w[5] = (1 - a) * b * (1 - c);
+    w[6] = (1 - a) * (1 - b) * c;
+    w[7] = (1 - a) * (1 - b) * (1 - c);
+}
+// derivative of trilerp with respect to the position (zero along clamped axes)
+__device__ __forceinline__ D3 tri_dpos(const double f[8], const Corner &k) {
+    const double a = k.a, b = k.b, c = k.c;
+    const double c11 = c * f[0] + (1 - c) * f[1], c10 = c * f[2] + (1 - c) * f[3], c01 = c * f[4] + (1 - c) * f[5],
+                 c00 = c * f[6] + (1 - c) * f[7];
+    const double da = (b * c11 + (1 - b) * c10) - (b * c01 + (1 - b) * c00);
+    const double db = a * (c11 - c10) + (1 - a) * (c01 - c00);
+    const double dc = a * (b * (f[0] - f[1]) + (1 - b) * (f[2] - f[3])) + (1 - a) * (b * (f[4] - f[5]) + (1 - b) * (f[6] - f[7]));
+    return d3(k.inx ? da : 0.0, k.iny ? db : 0.0, k.inz ? dc : 0.0);
+}
+__device__ __forceinline__ void gather1(const double *f, const Corner &k, double out[8]) {
+    for (int j = 0; j < 8; ++j) out[j] = f[k.g[j]];
+}
+__device__ __forceinline__ void gather3(const double *f, const Corner &k, int c, double out[8]) {
+    for (int j = 0; j < 8; ++j) out[j] = f[3 * k.g[j] + c];
+}
+
+// det() of the reference (not the determinant; cuda_utils.hpp:389-396), m[r][c], and its gradient
+__device__ __forceinline__ double det_ref(const double m[3][3]) {
+    const double a = ((m[0][0] * m[1][1]) * m[2][2] + (m[1][0] * m[2][1]) * m[0][2]) + (m[2][0] * m[0][1]) * m[1][2];
+    const double b = ((m[0][2] * m[1][1]) + m[2][0]) * ((m[1][2] * m[2][1]) * m[0][0]) + (m[2][2] * m[0][1]) * m[1][0];
+    return a - b;
+}
+__device__ __forceinline__ void det_ref_grad(const double m[3][3], double g[3][3]) {
+    // a = m00 m11 m22 + m10 m21 m02 + m20 m01 m12; b = (m02 m11 + m20)(m12 m21 m00) + m22 m01 m10
+    const double P = m[0][2] * m[1][1] + m[2][0], Q = m[1][2] * m[2][1] * m[0][0];
+    g[0][0] = m[1][1] * m[2][2] - P * (m[1][2] * m[2][1]);
+    g[0][1] = m[2][0] * m[1][2] - m[2][2] * m[1][0];
+    g[0][2] = m[1][0] * m[2][1] - m[1][1] * Q;
+    g[1][0] = m[2][1] * m[0][2] - m[2][2] * m[0][1];
+    g[1][1] = m[0][0] * m[2][2] - m[0][2] * Q;
+    g[1][2] = m[2][0] * m[0][1] - P * (m[2][1] * m[0][0]);
+    g[2][0] = m[0][1] * m[1][2] - Q;
+    g[2][1] = m[1][0] * m[0][2] - P * (m[1][2] * m[0][0]);
+    g[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+}
+
+// ---- forward steps ----------------------------------------------------------------------------
+
+// F1 (+ copy of the boundary): rho1 = rho + alpha * div on the interior
+__global__ void f_density(Dims d, const double *rho, const double *v, double *rho1, double alpha) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        double r = rho[i];
+        if (interior(d, x, y, z)) {
+            const D3 vel = ld(v, i);
+            const double p = rho[i];
+            const double t11 = vel.x > 0 ? -vel.x * p : -vel.x * rho[lin(d, x + 1, y, z)];
+            const double vxm = v[3 * lin(d, x - 1, y, z)];
+            const double t12 = vxm > 0 ? vxm * rho[lin(d, x - 1, y, z)] : vxm * p;
+            const double t21 = vel.y > 0 ? -vel.y * p : -vel.y * rho[lin(d, x, y + 1, z)];
+            const double vym = v[3 * lin(d, x, y - 1, z) + 1];
+            const double t22 = vym > 0 ? vym * rho[lin(d, x, y - 1, z)] : vym * p;
+            const double t31 = vel.z > 0 ? -vel.z * p : -vel.z * rho[lin(d, x, y, z + 1)];
+            const double vzm = v[3 * lin(d, x, y, z - 1) + 2];
+            const double t32 = vzm > 0 ? vzm * rho[lin(d, x, y, z - 1)] : vzm * p;
+            const double div = ((((t11 + t12) + t21) + t22) + t31) + t32;
+            r = fma(alpha, div, p);
+        }
+        rho1[i] = r;
+    }
+}
+
+// F2: J = D psi_inv (rows = components, columns = d/dx, d/dy, d/dz), interior; 0 elsewhere
+__global__ void f_jacobian(Dims d, const double *pinv, double *J) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (interior(d, x, y, z)) {
+            const D3 xp = ld(pinv, lin(d, x + 1, y, z)), xm = ld(pinv, lin(d, x - 1, y, z));
+            const D3 yp = ld(pinv, lin(d, x, y + 1, z)), ym = ld(pinv, lin(d, x, y - 1, z));
+            const D3 zp = ld(pinv, lin(d, x, y, z + 1)), zm = ld(pinv, lin(d, x, y, z - 1));
+            const D3 jx = d3((xp.x - xm.x) * 0.5, (xp.y - xm.y) * 0.5, (xp.z - xm.z) * 0.5);
+            const D3 jy = d3((yp.x - ym.x) * 0.5, (yp.y - ym.y) * 0.5, (yp.z - ym.z) * 0.5);
+            const D3 jz = d3((zp.x - zm.x) * 0.5, (zp.y - zm.y) * 0.5, (zp.z - zm.z) * 0.5);
+            m[0] = jx.x; m[1] = jy.x; m[2] = jz.x;
+            m[3] = jx.y; m[4] = jy.y; m[5] = jz.y;
+            m[6] = jx.z; m[7] = jy.z; m[8] = jz.z;
+        }
+        for (int k = 0; k < 9; ++k) J[9 * i + k] = m[k];
+    }
+}
+
+// F3: L = -((((((-6 psi + psi(x+1)) + psi(x-1)) + psi(y+1)) + psi(y-1)) + psi(z+1)) + psi(z-1)), interior
+__global__ void f_laplacian(Dims d, const double *psi, double *L) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        D3 o = d3(0, 0, 0);
+        if (interior(d, x, y, z)) {
+            const long nb[6] = {lin(d, x + 1, y, z), lin(d, x - 1, y, z), lin(d, x, y + 1, z),
+                                lin(d, x, y - 1, z), lin(d, x, y, z + 1), lin(d, x, y, z - 1)};
+            const D3 c = ld(psi, i);
+            D3 s = d3(c.x * -6.0, c.y * -6.0, c.z * -6.0);
+            for (int k = 0; k < 6; ++k) {
+                const D3 q = ld(psi, nb[k]);
+                s = d3(s.x + q.x, s.y + q.y, s.z + q.z);
+            }
+            o = d3(s.x * -1.0, s.y * -1.0, s.z * -1.0);
+        }
+        st(L, i, o);
+    }
+}
+
+// F4: Lo = trilerp(L, psi_inv), Tp = trilerp(T, psi_inv), all voxels
+__global__ void f_interp(Dims d, const double *L, const double *T, const double *pinv, double *Lo, double *Tp) {
+    FOR_VOXELS(d, i) {
+        const D3 p = ld(pinv, i);
+        const Corner k = corner(d, p.x, p.y, p.z);
+        double f[8];
+        D3 o;
+        gather3(L, k, 0, f);
+        o.x = tri(f, k.a, k.b, k.c);
+        gather3(L, k, 1, f);
+        o.y = tri(f, k.a, k.b, k.c);
+        gather3(L, k, 2, f);
+        o.z = tri(f, k.a, k.b, k.c);
+        st(Lo, i, o);
+        gather1(T, k, f);
+        Tp[i] = tri(f, k.a, k.b, k.c);
+    }
+}
+
+// F5: gU = ((S - Tp) grad S + w_reg Lo) det_ref(J), interior
+__global__ void f_potential(Dims d, const double *S, const double *gS, const double *Tp, const double *Lo,
+                            const double *J, double *gU, double w_reg) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        D3 o = d3(0, 0, 0);
+        if (interior(d, x, y, z)) {
+            double m[3][3];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) m[r][c] = J[9 * i + 3 * r + c];
+            const double det = det_ref(m), diff = S[i] - Tp[i];
+            const D3 g = ld(gS, i), lo = ld(Lo, i);
+            o = d3((g.x * diff + lo.x * w_reg) * det, (g.y * diff + lo.y * w_reg) * det, (g.z * diff + lo.z * w_reg) * det);
+        }
+        st(gU, i, o);
+    }
+}
+
+__device__ __forceinline__ double mx(double q) { return fmax(q, 0.0); }
+__device__ __forceinline__ double mn(double q) { return fmin(q, 0.0); }
+
+// F6: dvv = (Dv)v on the interior, dlv = the reference's upwind Laplacian of v on its range
+__global__ void f_vel_terms(Dims d, const double *v, double *dvv, double *dlv) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        D3 o = d3(0, 0, 0), l = d3(0, 0, 0);
+        if (interior(d, x, y, z)) {
+            const D3 c = ld(v, i), xp = ld(v, lin(d, x + 1, y, z)), xm = ld(v, lin(d, x - 1, y, z)),
+                     yp = ld(v, lin(d, x, y + 1, z)), ym = ld(v, lin(d, x, y - 1, z)), zp = ld(v, lin(d, x, y, z + 1)),
+                     zm = ld(v, lin(d, x, y, z - 1));
+            const double ax1 = ((mx(c.x) * mx(c.x) - mn(c.x) * mn(c.x)) + mn(xp.x) * mn(xp.x)) - mx(xp.x) * mx(xp.x);
+            const double ax2 = c.y > 0 ? c.y * (c.x - ym.x) : c.y * (yp.x - c.x);
+            const double ax3 = c.z > 0 ? c.z * (c.x - zm.x) : c.z * (zp.x - c.x);
+            const double ay1 = c.x > 0 ? c.x * (c.y - xm.y) : c.x * (xp.y - c.y);
+            const double ay2 = ((mx(c.y) * mx(c.y) - mn(c.y) * mn(c.y)) + mn(yp.y) * mn(yp.y)) - mx(yp.y) * mx(yp.y);
+            const double ay3 = c.z > 0 ? c.z * (c.y - zm.y) : c.z * (zp.y - c.y);
+            const double az1 = c.x > 0 ? c.x * (c.z - xm.z) : c.x * (xp.z - c.z);
+            const double az2 = c.y > 0 ? c.y * (c.z - ym.z) : c.y * (yp.z - c.z);
+            const double az3 = ((mx(c.z) * mx(c.z) - mn(c.z) * mn(c.z)) + mn(zp.z) * mn(zp.z)) - mx(zp.z) * mx(zp.z);
+            o = d3((0.5 * ax1 + ax2) + ax3, (ay1 + 0.5 * ay2) + ay3, (az1 + az2) + 0.5 * az3);
+        }
+        if (x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2 && z >= 2 && z <= d.Z - 3) {
+            const D3 c = ld(v, i);
+            const long sx = 1, sy = d.X, sz = (long)d.X * d.Y;
+            // one-sided second differences with the reference's y-1 / z-1 where +1 is meant
+            const long ax = c.x < 0 ? i + 2 * sx : i - 2 * sx, bx = c.x < 0 ? i + sx : i - sx;
+            const long ay = c.y < 0 ? i + 2 * sy : i - 2 * sy, by = i - sy;
+            const long az = c.z < 0 ? i + 2 * sz : i - 2 * sz, bz = i - sz;
+            D3 s = d3(0, 0, 0);
+            for (int q = 0; q < 3; ++q) {
+                const double cv = comp(c, q);
+                const double vxx = c.x < 0 ? (v[3 * ax + q] - 2.0 * v[3 * bx + q]) + cv : (cv - 2.0 * v[3 * bx + q]) + v[3 * ax + q];
+                const double vyy = c.y < 0 ? (v[3 * ay + q] - 2.0 * v[3 * by + q]) + cv : (cv - 2.0 * v[3 * by + q]) + v[3 * ay + q];
+                const double vzz = c.z < 0 ? (v[3 * az + q] - 2.0 * v[3 * bz + q]) + cv : (cv - 2.0 * v[3 * bz + q]) + v[3 * az + q];
+                const double r = -1.0 * ((vxx + vyy) + vzz);
+                if (q == 0) s.x = r; else if (q == 1) s.y = r; else s.z = r;
+            }
+            l = s;
+        }
+        st(dvv, i, o);
+        st(dlv, i, l);
+    }
+}
+
+// F7: v1 = v - alpha (((m1t v + dvv) - tau dlv) + q gU), q = m2 / (|rho1| + 1e-5f), interior
+__global__ void f_velocity(Dims d, const double *v, const double *dvv, const double *dlv, const double *rho1,
+                           const double *gU, double *v1, double alpha, double m1t, double tau, double m2) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        D3 o = ld(v, i);
+        if (interior(d, x, y, z)) {
+            const D3 vv = ld(v, i), a = ld(dvv, i), l = ld(dlv, i), g = ld(gU, i);
+            const double q = m2 / (fabs(rho1[i]) + (double)1e-5f);
+            const D3 fr = d3(vv.x * m1t, vv.y * m1t, vv.z * m1t), df = d3(l.x * tau, l.y * tau, l.z * tau),
+                     po = d3(g.x * q, g.y * q, g.z * q);
+            const D3 up = d3((((fr.x + a.x) - df.x) + po.x) * alpha, (((fr.y + a.y) - df.y) + po.y) * alpha,
+                             (((fr.z + a.z) - df.z) + po.z) * alpha);
+            o = d3(vv.x - up.x, vv.y - up.y, vv.z - up.z);
+        }
+        st(v1, i, o);
+    }
+}
+
+// F8: psi1 = psi + alpha trilerp(v1, psi), interior
+__global__ void f_psi(Dims d, const double *psi, const double *v1, double *psi1, double alpha) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        D3 o = ld(psi, i);
+        if (interior(d, x, y, z)) {
+            const Corner k = corner(d, o.x, o.y, o.z);
+            double f[8];
+            gather3(v1, k, 0, f);
+            const double ux = tri(f, k.a, k.b, k.c) * alpha;
+            gather3(v1, k, 1, f);
+            const double uy = tri(f, k.a, k.b, k.c) * alpha;
+            gather3(v1, k, 2, f);
+            const double uz = tri(f, k.a, k.b, k.c) * alpha;
+            o = d3(o.x + ux, o.y + uy, o.z + uz);
+        }
+        st(psi1, i, o);
+    }
+}
+
+// F9: psi_inv1 = psi_inv - alpha J v1, interior
+__global__ void f_psi_inv(Dims d, const double *pinv, const double *J, const double *v1, double *pinv1, double alpha) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        D3 o = ld(pinv, i);
+        if (interior(d, x, y, z)) {
+            const D3 w = ld(v1, i);
+            const double *m = J + 9 * i;
+            const D3 u = d3(((m[0] * w.x + m[1] * w.y) + m[2] * w.z) * alpha, ((m[3] * w.x + m[4] * w.y) + m[5] * w.z) * alpha,
+                            ((m[6] * w.x + m[7] * w.y) + m[8] * w.z) * alpha);
+            o = d3(o.x - u.x, o.y - u.y, o.z - u.z);
+        }
+        st(pinv1, i, o);
+    }
+}
+
+// objective terms per voxel, summed in FP64 per block
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+    __syncthreads();
+    return s;
+}
+
+// E = 0.5 sum (T - S o psi)^2 + lw * 0.5 sum_l (1[LT = l] - trilerp(1[LS = l], psi))^2; block partial sums
+__global__ __launch_bounds__(256) void f_objective(Dims d, const double *T, const double *S, const int16_t *LT,
+                                                   const int16_t *LS, int no_labels, double lw, const double *psi,
+                                                   double *partials) {
+    __shared__ double sh[4];
+    double acc = 0;
+    FOR_VOXELS(d, i) {
+        const D3 p = ld(psi, i);
+        const Corner k = corner(d, p.x, p.y, p.z);
+        double f[8];
+        gather1(S, k, f);
+        const double r = T[i] - tri(f, k.a, k.b, k.c);
+        acc += 0.5 * r * r;
+        if (lw != 0.0) {
+            int lab[8];
+            for (int j = 0; j < 8; ++j) lab[j] = LS[k.g[j]];
+            for (int l = 1; l < no_labels; ++l) {
+                for (int j = 0; j < 8; ++j) f[j] = lab[j] == l ? 1.0 : 0.0;
+                const double rl = (LT[i] == l ? 1.0 : 0.0) - tri(f, k.a, k.b, k.c);
+                acc += lw * 0.5 * rl * rl;
+            }
+        }
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// ---- backward steps ---------------------------------------------------------------------------
+
+// dE/dpsi of the objective
+__global__ void b_objective(Dims d, const double *T, const double *S, const int16_t *LT, const int16_t *LS,
+                            int no_labels, double lw, const double *psi, double *gpsi) {
+    FOR_VOXELS(d, i) {
+        const D3 p = ld(psi, i);
+        const Corner k = corner(d, p.x, p.y, p.z);
+        double f[8];
+        gather1(S, k, f);
+        const double r = T[i] - tri(f, k.a, k.b, k.c);
+        const D3 dp = tri_dpos(f, k);
+        D3 g = d3(-r * dp.x, -r * dp.y, -r * dp.z);
+        if (lw != 0.0) {
+            int lab[8];
+            for (int j = 0; j < 8; ++j) lab[j] = LS[k.g[j]];
+            for (int l = 1; l < no_labels; ++l) {
+                for (int j = 0; j < 8; ++j) f[j] = lab[j] == l ? 1.0 : 0.0;
+                const double rl = (LT[i] == l ? 1.0 : 0.0) - tri(f, k.a, k.b, k.c);
+                const D3 dl = tri_dpos(f, k);
+                g = d3(g.x - lw * rl * dl.x, g.y - lw * rl * dl.y, g.z - lw * rl * dl.z);
+            }
+        }
+        st(gpsi, i, g);
+    }
+}
+
+// B9: psi_inv1 = psi_inv - alpha J v1: gpinv = gpinv1 (all), gJ = -alpha gpinv1 (x) v1 (0 off the interior),
+// gv1 += -alpha J^T gpinv1 (interior)
+__global__ void b_psi_inv(Dims d, const double *J, const double *v1, const double *gpinv1, double *gpinv, double *gJ,
+                          double *gv1, double alpha) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        const D3 g = ld(gpinv1, i);
+        st(gpinv, i, g);
+        if (interior(d, x, y, z)) {
+            const D3 w = ld(v1, i);
+            const double *m = J + 9 * i;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) gJ[9 * i + 3 * r + c] = -alpha * comp(g, r) * comp(w, c);
+            const D3 t = d3(-alpha * (m[0] * g.x + m[3] * g.y + m[6] * g.z), -alpha * (m[1] * g.x + m[4] * g.y + m[7] * g.z),
+                            -alpha * (m[2] * g.x + m[5] * g.y + m[8] * g.z));
+            gv1[3 * i] += t.x;
+            gv1[3 * i + 1] += t.y;
+            gv1[3 * i + 2] += t.z;
+        } else {
+            for (int k = 0; k < 9; ++k) gJ[9 * i + k] = 0.0;
+        }
+    }
+}
+
+// B8: psi1 = psi + alpha trilerp(v1, psi): gpsi = gpsi1 + alpha (d trilerp / d pos)^T gpsi1,
+// gv1(corners) += alpha w gpsi1 (interior)
+__global__ void b_psi(Dims d, const double *psi, const double *v1, const double *gpsi1, double *gpsi, double *gv1,
+                      double alpha) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        const D3 g = ld(gpsi1, i);
+        D3 o = g;
+        if (interior(d, x, y, z)) {
+            const D3 p = ld(psi, i);
+            const Corner k = corner(d, p.x, p.y, p.z);
+            double w[8], f[8];
+            tri_weights(k.a, k.b, k.c, w);
+            for (int j = 0; j < 8; ++j) add_at(gv1, k.g[j], d3(alpha * w[j] * g.x, alpha * w[j] * g.y, alpha * w[j] * g.z));
+            for (int q = 0; q < 3; ++q) {
+                gather3(v1, k, q, f);
+                const D3 dp = tri_dpos(f, k);
+                const double gq = alpha * comp(g, q);
+                o = d3(o.x + gq * dp.x, o.y + gq * dp.y, o.z + gq * dp.z);
+            }
+        }
+        st(gpsi, i, o);
+    }
+}
+
+// B7: v1 = v - alpha (m1t v + dvv - tau dlv + q gU): gv (boundary gv1; interior (1 - alpha m1t) gv1),
+// gdvv = -alpha gv1, gdlv = alpha tau gv1, ggU = -alpha q gv1, grho1 += -alpha (gv1 . gU) dq/drho1,
+// the tau adjoint summed into *gtau (per block, atomically)
+__global__ __launch_bounds__(256) void b_velocity(Dims d, const double *rho1, const double *dlv, const double *gU,
+                                                  const double *gv1, double *gv, double *gdvv, double *gdlv, double *ggU,
+                                                  double *grho1, double alpha, double m1t, double tau, double m2,
+                                                  double *gtau) {
+    __shared__ double sh[4];
+    double acc = 0;
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        const D3 g = ld(gv1, i);
+        if (interior(d, x, y, z)) {
+            const double den = fabs(rho1[i]) + (double)1e-5f, q = m2 / den;
+            const D3 u = ld(gU, i), l = ld(dlv, i);
+            st(gv, i, d3((1 - alpha * m1t) * g.x, (1 - alpha * m1t) * g.y, (1 - alpha * m1t) * g.z));
+            st(gdvv, i, d3(-alpha * g.x, -alpha * g.y, -alpha * g.z));
+            st(gdlv, i, d3(alpha * tau * g.x, alpha * tau * g.y, alpha * tau * g.z));
+            st(ggU, i, d3(-alpha * q * g.x, -alpha * q * g.y, -alpha * q * g.z));
+            const double gq = -alpha * (g.x * u.x + g.y * u.y + g.z * u.z);
+            const double sgn = rho1[i] > 0 ? 1.0 : rho1[i] < 0 ? -1.0 : 0.0;
+            grho1[i] += gq * (-m2 * sgn / (den * den));
+            acc += alpha * (g.x * l.x + g.y * l.y + g.z * l.z);
+        } else {
+            st(gv, i, g);
+            st(gdvv, i, d3(0, 0, 0));
+            st(gdlv, i, d3(0, 0, 0));
+            st(ggU, i, d3(0, 0, 0));
+        }
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) aadd(gtau, s);
+}
+
+// B6: transposes of dvv = (Dv)v and dlv (upwind Laplacian), scattered into gv
+__global__ void b_vel_terms(Dims d, const double *v, const double *gdvv, const double *gdlv, double *gv) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        if (interior(d, x, y, z)) {
+            const long ixp = lin(d, x + 1, y, z), ixm = lin(d, x - 1, y, z), iyp = lin(d, x, y + 1, z),
+                       iym = lin(d, x, y - 1, z), izp = lin(d, x, y, z + 1), izm = lin(d, x, y, z - 1);
+            const D3 c = ld(v, i), xp = ld(v, ixp), xm = ld(v, ixm), yp = ld(v, iyp), ym = ld(v, iym), zp = ld(v, izp),
+                     zm = ld(v, izm);
+            const D3 G = ld(gdvv, i);
+            // dvv.x = 0.5 ax1 + ax2 + ax3; dvv.y = ay1 + 0.5 ay2 + ay3; dvv.z = az1 + az2 + 0.5 az3
+            D3 gc = d3(0, 0, 0), gxp = d3(0, 0, 0), gxm = d3(0, 0, 0), gyp = d3(0, 0, 0), gym = d3(0, 0, 0),
+               gzp = d3(0, 0, 0), gzm = d3(0, 0, 0);
+            // ax1 (x-component of c and xp), weight 0.5
+            gc.x += 0.5 * G.x * (2 * mx(c.x) - 2 * mn(c.x));
+            gxp.x += 0.5 * G.x * (2 * mn(xp.x) - 2 * mx(xp.x));
+            // ax2 = c.y>0 ? c.y (c.x - ym.x) : c.y (yp.x - c.x)
+            if (c.y > 0) { gc.y += G.x * (c.x - ym.x); gc.x += G.x * c.y; gym.x -= G.x * c.y; }
+            else { gc.y += G.x * (yp.x - c.x); gyp.x += G.x * c.y; gc.x -= G.x * c.y; }
+            if (c.z > 0) { gc.z += G.x * (c.x - zm.x); gc.x += G.x * c.z; gzm.x -= G.x * c.z; }
+            else { gc.z += G.x * (zp.x - c.x); gzp.x += G.x * c.z; gc.x -= G.x * c.z; }
+            // y-component
+            if (c.x > 0) { gc.x += G.y * (c.y - xm.y); gc.y += G.y * c.x; gxm.y -= G.y * c.x; }
+            else { gc.x += G.y * (xp.y - c.y); gxp.y += G.y * c.x; gc.y -= G.y * c.x; }
+            gc.y += 0.5 * G.y * (2 * mx(c.y) - 2 * mn(c.y));
+            gyp.y += 0.5 * G.y * (2 * mn(yp.y) - 2 * mx(yp.y));
+            if (c.z > 0) { gc.z += G.y * (c.y - zm.y); gc.y += G.y * c.z; gzm.y -= G.y * c.z; }
+            else { gc.z += G.y * (zp.y - c.y); gzp.y += G.y * c.z; gc.y -= G.y * c.z; }
+            // z-component
+            if (c.x > 0) { gc.x += G.z * (c.z - xm.z); gc.z += G.z * c.x; gxm.z -= G.z * c.x; }
+            else { gc.x += G.z * (xp.z - c.z); gxp.z += G.z * c.x; gc.z -= G.z * c.x; }
+            if (c.y > 0) { gc.y += G.z * (c.z - ym.z); gc.z += G.z * c.y; gym.z -= G.z * c.y; }
+            else { gc.y += G.z * (yp.z - c.z); gyp.z += G.z * c.y; gc.z -= G.z * c.y; }
+            gc.z += 0.5 * G.z * (2 * mx(c.z) - 2 * mn(c.z));
+            gzp.z += 0.5 * G.z * (2 * mn(zp.z) - 2 * mx(zp.z));
+            add_at(gv, i, gc);
+            add_at(gv, ixp, gxp);
+            add_at(gv, ixm, gxm);
+            add_at(gv, iyp, gyp);
+            add_at(gv, iym, gym);
+            add_at(gv, izp, gzp);
+            add_at(gv, izm, gzm);
+        }
+        if (x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2 && z >= 2 && z <= d.Z - 3) {
+            const D3 c = ld(v, i), g = ld(gdlv, i);
+            const long sx = 1, sy = d.X, sz = (long)d.X * d.Y;
+            const long ax = c.x < 0 ? i + 2 * sx : i - 2 * sx, bx = c.x < 0 ? i + sx : i - sx;
+            const long ay = c.y < 0 ? i + 2 * sy : i - 2 * sy, by = i - sy;
+            const long az = c.z < 0 ? i + 2 * sz : i - 2 * sz, bz = i - sz;
+            // dlv = -(vxx + vyy + vzz), each of the form A - 2 B + C with the centre once per axis
+            const D3 mg = d3(-g.x, -g.y, -g.z);
+            add_at(gv, i, d3(3 * mg.x, 3 * mg.y, 3 * mg.z));
+            add_at(gv, ax, mg);
+            add_at(gv, ay, mg);
+            add_at(gv, az, mg);
+            add_at(gv, bx, d3(-2 * mg.x, -2 * mg.y, -2 * mg.z));
+            add_at(gv, by, d3(-2 * mg.x, -2 * mg.y, -2 * mg.z));
+            add_at(gv, bz, d3(-2 * mg.x, -2 * mg.y, -2 * mg.z));
+        }
+    }
+}
+
+// B5: gU = ((S - Tp) gS + w Lo) det: gLo = w det ggU, gTp = -det (ggU . gS), gJ += (ggU . (diff gS + w Lo)) ddet/dJ,
+// the w_reg adjoint summed into *gw
+__global__ __launch_bounds__(256) void b_potential(Dims d, const double *S, const double *gS, const double *Tp,
+                                                   const double *Lo, const double *J, const double *ggU, double *gLo,
+                                                   double *gTp, double *gJ, double w_reg, double *gw) {
+    __shared__ double sh[4];
+    double acc = 0;
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        if (interior(d, x, y, z)) {
+            double m[3][3], dg[3][3];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) m[r][c] = J[9 * i + 3 * r + c];
+            const double det = det_ref(m), diff = S[i] - Tp[i];
+            const D3 g = ld(ggU, i), s = ld(gS, i), lo = ld(Lo, i);
+            const double gdet = g.x * (diff * s.x + w_reg * lo.x) + g.y * (diff * s.y + w_reg * lo.y) +
+                                g.z * (diff * s.z + w_reg * lo.z);
+            st(gLo, i, d3(w_reg * det * g.x, w_reg * det * g.y, w_reg * det * g.z));
+            gTp[i] = -det * (g.x * s.x + g.y * s.y + g.z * s.z);
+            det_ref_grad(m, dg);
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) gJ[9 * i + 3 * r + c] += gdet * dg[r][c];
+            acc += det * (g.x * lo.x + g.y * lo.y + g.z * lo.z);
+        } else {
+            st(gLo, i, d3(0, 0, 0));
+            gTp[i] = 0;
+        }
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) aadd(gw, s);
+}
+
+// B4: Lo = trilerp(L, psi_inv), Tp = trilerp(T, psi_inv): gL(corners) += w gLo, gpinv += (d/dpos)^T (gLo, gTp)
+__global__ void b_interp(Dims d, const double *L, const double *T, const double *pinv, const double *gLo,
+                         const double *gTp, double *gL, double *gpinv) {
+    FOR_VOXELS(d, i) {
+        const D3 p = ld(pinv, i), g = ld(gLo, i);
+        const double gt = gTp[i];
+        if (g.x == 0 && g.y == 0 && g.z == 0 && gt == 0) continue;
+        const Corner k = corner(d, p.x, p.y, p.z);
+        double w[8], f[8];
+        tri_weights(k.a, k.b, k.c, w);
+        for (int j = 0; j < 8; ++j) add_at(gL, k.g[j], d3(w[j] * g.x, w[j] * g.y, w[j] * g.z));
+        D3 o = d3(0, 0, 0);
+        for (int q = 0; q < 3; ++q) {
+            gather3(L, k, q, f);
+            const D3 dp = tri_dpos(f, k);
+            const double gq = comp(g, q);
+            o = d3(o.x + gq * dp.x, o.y + gq * dp.y, o.z + gq * dp.z);
+        }
+        gather1(T, k, f);
+        const D3 dt = tri_dpos(f, k);
+        o = d3(o.x + gt * dt.x, o.y + gt * dt.y, o.z + gt * dt.z);
+        gpinv[3 * i] += o.x;
+        gpinv[3 * i + 1] += o.y;
+        gpinv[3 * i + 2] += o.z;
+    }
+}
+
+// B3: L = 6 psi - sum of the 6 neighbours (interior): gpsi(p) += 6 gL(p), gpsi(nb) -= gL(p)
+__global__ void b_laplacian(Dims d, const double *gL, double *gpsi) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        if (!interior(d, x, y, z)) continue;
+        const D3 g = ld(gL, i);
+        add_at(gpsi, i, d3(6 * g.x, 6 * g.y, 6 * g.z));
+        const D3 m = d3(-g.x, -g.y, -g.z);
+        add_at(gpsi, lin(d, x + 1, y, z), m);
+        add_at(gpsi, lin(d, x - 1, y, z), m);
+        add_at(gpsi, lin(d, x, y + 1, z), m);
+        add_at(gpsi, lin(d, x, y - 1, z), m);
+        add_at(gpsi, lin(d, x, y, z + 1), m);
+        add_at(gpsi, lin(d, x, y, z - 1), m);
+    }
+}
+
+// B2: J = D psi_inv (interior): gpinv(x+-1)[r] += +-0.5 gJ[r][0], and y, z alike
+__global__ void b_jacobian(Dims d, const double *gJ, double *gpinv) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        if (!interior(d, x, y, z)) continue;
+        const double *g = gJ + 9 * i;
+        const D3 cx = d3(0.5 * g[0], 0.5 * g[3], 0.5 * g[6]), cy = d3(0.5 * g[1], 0.5 * g[4], 0.5 * g[7]),
+                 cz = d3(0.5 * g[2], 0.5 * g[5], 0.5 * g[8]);
+        add_at(gpinv, lin(d, x + 1, y, z), cx);
+        add_at(gpinv, lin(d, x - 1, y, z), d3(-cx.x, -cx.y, -cx.z));
+        add_at(gpinv, lin(d, x, y + 1, z), cy);
+        add_at(gpinv, lin(d, x, y - 1, z), d3(-cy.x, -cy.y, -cy.z));
+        add_at(gpinv, lin(d, x, y, z + 1), cz);
+        add_at(gpinv, lin(d, x, y, z - 1), d3(-cz.x, -cz.y, -cz.z));
+    }
+}
+
+// B1: rho1 = rho + alpha div(rho, v) (interior): grho += grho1 (all) + alpha (d div/d rho)^T grho1,
+// gv += alpha (d div / d v)^T grho1
+__global__ void b_density(Dims d, const double *rho, const double *v, const double *grho1, double *grho, double *gv,
+                          double alpha) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        aadd(&grho[i], grho1[i]);
+        if (!interior(d, x, y, z)) continue;
+        const double g = alpha * grho1[i];
+        if (g == 0) continue;
+        const D3 vel = ld(v, i);
+        const long ixp = lin(d, x + 1, y, z), ixm = lin(d, x - 1, y, z), iyp = lin(d, x, y + 1, z),
+                   iym = lin(d, x, y - 1, z), izp = lin(d, x, y, z + 1), izm = lin(d, x, y, z - 1);
+        const double p = rho[i];
+        // t11 = vel.x > 0 ? -vel.x p : -vel.x rho(x+1)
+        if (vel.x > 0) { aadd(&gv[3 * i], -g * p); aadd(&grho[i], -g * vel.x); }
+        else { aadd(&gv[3 * i], -g * rho[ixp]); aadd(&grho[ixp], -g * vel.x); }
+        const double vxm = v[3 * ixm];
+        if (vxm > 0) { aadd(&gv[3 * ixm], g * rho[ixm]); aadd(&grho[ixm], g * vxm); }
+        else { aadd(&gv[3 * ixm], g * p); aadd(&grho[i], g * vxm); }
+        if (vel.y > 0) { aadd(&gv[3 * i + 1], -g * p); aadd(&grho[i], -g * vel.y); }
+        else { aadd(&gv[3 * i + 1], -g * rho[iyp]); aadd(&grho[iyp], -g * vel.y); }
+        const double vym = v[3 * iym + 1];
+        if (vym > 0) { aadd(&gv[3 * iym + 1], g * rho[iym]); aadd(&grho[iym], g * vym); }
+        else { aadd(&gv[3 * iym + 1], g * p); aadd(&grho[i], g * vym); }
+        if (vel.z > 0) { aadd(&gv[3 * i + 2], -g * p); aadd(&grho[i], -g * vel.z); }
+        else { aadd(&gv[3 * i + 2], -g * rho[izp]); aadd(&grho[izp], -g * vel.z); }
+        const double vzm = v[3 * izm + 2];
+        if (vzm > 0) { aadd(&gv[3 * izm + 2], g * rho[izm]); aadd(&grho[izm], g * vzm); }
+        else { aadd(&gv[3 * izm + 2], g * p); aadd(&grho[i], g * vzm); }
+    }
+}
+
+// grad S on the interior ((S(+1) - S(-1)) / 2 per axis), 0 elsewhere; FP64 copies of the inputs
+__global__ void f_setup(Dims d, const float *tgt, const float *src, double *T, double *S, double *gS) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        T[i] = tgt[i];
+        S[i] = src[i];
+        D3 g = d3(0, 0, 0);
+        if (interior(d, x, y, z)) {
+            g.x = ((double)src[lin(d, x + 1, y, z)] - (double)src[lin(d, x - 1, y, z)]) / 2.0;
+            g.y = ((double)src[lin(d, x, y + 1, z)] - (double)src[lin(d, x, y - 1, z)]) / 2.0;
+            g.z = ((double)src[lin(d, x, y, z + 1)] - (double)src[lin(d, x, y, z - 1)]) / 2.0;
+        }
+        st(gS, i, g);
+    }
+}
+
+// initial state: rho = rho_0, v = v0 (or 0), psi = psi_inv = identity
+__global__ void f_init_state(Dims d, double *rho, double *v, double *psi, double *pinv, const double *v0, double rho_0) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        rho[i] = rho_0;
+        st(v, i, v0 ? ld(v0, i) : d3(0, 0, 0));
+        st(psi, i, d3(x, y, z));
+        st(pinv, i, d3(x, y, z));
+    }
+}
+
+__global__ void k_sum_final(const double *partials, int np, double *out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0;
+        for (int j = 0; j < np; ++j) s += partials[j];
+        *out = s;
+    }
+}
+
+}  // namespace frd
+
+// ---- host driver --------------------------------------------------------------------------------
+
+namespace {
+
+using frd::Dims;
+
+constexpr int TPB = 256;  // threads per block of every kernel above (the block sums assume 256)
+
+// One solver state: density, velocity, psi, psi_inv (1 + 3 + 3 + 3 doubles per voxel).
+struct State {
+    double *rho, *v, *psi, *pinv;
+};
+// The intermediates of one iteration that its adjoint reads.
+struct Inter {
+    double *J, *L, *Lo, *Tp, *gU, *dvv, *dlv;
+};
+
+int err(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int err(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    return frk::set_error(code, buf);
+}
+
+#define ADJ_TRY(expr)                                                                                        \
+    do {                                                                                                     \
+        hipError_t e_ = (expr);                                                                              \
+        if (e_ != hipSuccess) return err(FR_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                                         __LINE__);                                                          \
+    } while (0)
+
+}  // namespace
+
+struct fr_adjoint {
+    Dims d{};
+    long n = 0;
+    int K = 0, ckpt = 0, nseg = 0;
+    double alpha = 0, rho_0 = 0, p = 0, C = 0;
+    int reset_iter = -1, no_labels = 1;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    size_t bytes = 0;
+    // inputs (FP64 copies of the images, grad S, labels)
+    double *T = nullptr, *S = nullptr, *gS = nullptr;
+    int16_t *LT = nullptr, *LS = nullptr;
+    // checkpoints: ck[s] is the state before iteration s*ckpt + 1; fin the state after K iterations
+    std::vector<State> ck;
+    State fin{};
+    // the segment being differentiated: nxt[j] is the state after its (j+1)-th iteration,
+    // inter[j] that iteration's intermediates (the plain forward uses nxt[0..1] and inter[0])
+    std::vector<State> nxt;
+    std::vector<Inter> inter;
+    // adjoint of the state (grho double-buffered: its kernel scatters into the older iteration's)
+    double *grho[2] = {nullptr, nullptr}, *gv = nullptr, *gpsi = nullptr, *gpinv = nullptr;
+    // adjoints of one iteration's intermediates
+    double *gJ = nullptr, *gL = nullptr, *gLo = nullptr, *gTp = nullptr, *ggU = nullptr, *gdvv = nullptr,
+           *gdlv = nullptr;
+    double *scal = nullptr;  // [0] objective, [1] dE/dw_reg, [2] dE/dtau
+    double *partials = nullptr;
+    int grid = 0;
+    // per-iteration scalars of the last forward (index k-1 for iteration k)
+    std::vector<double> m1t, m2;
+    double w_reg = 0, tau = 0, lw = 0, energy = 0;
+    bool have_inputs = false, have_forward = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    float fwd_ms = 0, bwd_ms = 0;
+
+    template <class T>
+    int alloc(T **ptr, size_t count) {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, (count ? count : 1) * sizeof(T));
+        if (e != hipSuccess) {
+            *ptr = nullptr;
+            return err(FR_ERR_ALLOC, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+        }
+        allocs.push_back(q);
+        bytes += count * sizeof(T);
+        *ptr = static_cast<T *>(q);
+        return FR_OK;
+    }
+    int alloc_state(State *s) {
+        double *b = nullptr;
+        if (int rc = alloc(&b, 10 * (size_t)n)) return rc;
+        s->rho = b;
+        s->v = b + n;
+        s->psi = b + 4 * n;
+        s->pinv = b + 7 * n;
+        return FR_OK;
+    }
+    int alloc_inter(Inter *t) {
+        double *b = nullptr;
+        if (int rc = alloc(&b, 25 * (size_t)n)) return rc;
+        t->J = b;
+        t->L = b + 9 * n;
+        t->Lo = b + 12 * n;
+        t->Tp = b + 15 * n;
+        t->gU = b + 16 * n;
+        t->dvv = b + 19 * n;
+        t->dlv = b + 22 * n;
+        return FR_OK;
+    }
+    ~fr_adjoint() {
+        for (void *q : allocs) (void)hipFree(q);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// t of iteration k (1-based) as the solver loop advances it (solver.cu:423-428): t_1 = alpha,
+// t_k = t_{k-1} + alpha, reset to alpha when k is a multiple of RESET_ITER; then
+// m1t = (p + 1) / t and m2 = C p^2 t^(p - 2) (solver.cu:198-207), all in double.
+void schedule(fr_adjoint *h) {
+    h->m1t.assign(h->K, 0.0);
+    h->m2.assign(h->K, 0.0);
+    double t = h->alpha;
+    for (int k = 1; k <= h->K; ++k) {
+        if (k >= 2) {
+            t += h->alpha;
+            if (h->reset_iter > 0 && k % h->reset_iter == 0) t = h->alpha;
+        }
+        const double m1 = h->p + 1.0;
+        h->m1t[k - 1] = m1 / t;
+        h->m2[k - 1] = h->C * pow(h->p, 2.0) * pow(t, h->p - 2.0);
+    }
+}
+
+// one iteration k (1-based): state `in` -> state `out`, intermediates into `it`
+int forward_iteration(fr_adjoint *h, const State &in, const State &out, const Inter &it, int k) {
+    const Dims d = h->d;
+    const int g = h->grid;
+    hipStream_t s = h->stream;
+    const double a = h->alpha;
+    frd::f_density<<<g, TPB, 0, s>>>(d, in.rho, in.v, out.rho, a);
+    frd::f_jacobian<<<g, TPB, 0, s>>>(d, in.pinv, it.J);
+    frd::f_laplacian<<<g, TPB, 0, s>>>(d, in.psi, it.L);
+    frd::f_interp<<<g, TPB, 0, s>>>(d, it.L, h->T, in.pinv, it.Lo, it.Tp);
+    frd::f_potential<<<g, TPB, 0, s>>>(d, h->S, h->gS, it.Tp, it.Lo, it.J, it.gU, h->w_reg);
+    frd::f_vel_terms<<<g, TPB, 0, s>>>(d, in.v, it.dvv, it.dlv);
+    frd::f_velocity<<<g, TPB, 0, s>>>(d, in.v, it.dvv, it.dlv, out.rho, it.gU, out.v, a, h->m1t[k - 1], h->tau,
+                                      h->m2[k - 1]);
+    frd::f_psi<<<g, TPB, 0, s>>>(d, in.psi, out.v, out.psi, a);
+    frd::f_psi_inv<<<g, TPB, 0, s>>>(d, in.pinv, it.J, out.v, out.pinv, a);
+    ADJ_TRY(hipGetLastError());
+    return FR_OK;
+}
+
+// adjoint of iteration k: the adjoint of state `nx` (grho[cur], gv, gpsi, gpinv) becomes that of `in`
+int backward_iteration(fr_adjoint *h, const State &in, const State &nx, const Inter &it, int k, int *cur) {
+    const Dims d = h->d;
+    const int g = h->grid;
+    hipStream_t s = h->stream;
+    const double a = h->alpha;
+    const size_t n = (size_t)h->n;
+    double *gr1 = h->grho[*cur], *gr0 = h->grho[1 - *cur];
+    frd::b_psi_inv<<<g, TPB, 0, s>>>(d, it.J, nx.v, h->gpinv, h->gpinv, h->gJ, h->gv, a);
+    frd::b_psi<<<g, TPB, 0, s>>>(d, in.psi, nx.v, h->gpsi, h->gpsi, h->gv, a);
+    frd::b_velocity<<<g, TPB, 0, s>>>(d, nx.rho, it.dlv, it.gU, h->gv, h->gv, h->gdvv, h->gdlv, h->ggU, gr1, a,
+                                      h->m1t[k - 1], h->tau, h->m2[k - 1], h->scal + 2);
+    frd::b_vel_terms<<<g, TPB, 0, s>>>(d, in.v, h->gdvv, h->gdlv, h->gv);
+    frd::b_potential<<<g, TPB, 0, s>>>(d, h->S, h->gS, it.Tp, it.Lo, it.J, h->ggU, h->gLo, h->gTp, h->gJ, h->w_reg,
+                                       h->scal + 1);
+    ADJ_TRY(hipMemsetAsync(h->gL, 0, 3 * n * sizeof(double), s));
+    frd::b_interp<<<g, TPB, 0, s>>>(d, it.L, h->T, in.pinv, h->gLo, h->gTp, h->gL, h->gpinv);
+    frd::b_laplacian<<<g, TPB, 0, s>>>(d, h->gL, h->gpsi);
+    frd::b_jacobian<<<g, TPB, 0, s>>>(d, h->gJ, h->gpinv);
+    ADJ_TRY(hipMemsetAsync(gr0, 0, n * sizeof(double), s));
+    frd::b_density<<<g, TPB, 0, s>>>(d, in.rho, in.v, gr1, gr0, h->gv, a);
+    ADJ_TRY(hipGetLastError());
+    *cur = 1 - *cur;
+    return FR_OK;
+}
+
+int copy_state(fr_adjoint *h, const State &st, double *v, double *rho, double *psi, double *pinv) {
+    const size_t n = (size_t)h->n;
+    if (v) ADJ_TRY(hipMemcpyAsync(v, st.v, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    if (rho) ADJ_TRY(hipMemcpyAsync(rho, st.rho, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    if (psi) ADJ_TRY(hipMemcpyAsync(psi, st.psi, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    if (pinv) ADJ_TRY(hipMemcpyAsync(pinv, st.pinv, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    ADJ_TRY(hipStreamSynchronize(h->stream));
+    return FR_OK;
+}
+
+}  // namespace
+
+extern "C" int fr_adjoint_create(const fr_params *p, int K, int ckpt, fr_adjoint **out) {
+    if (!p || !out) return err(FR_ERR_INVALID, "null argument");
+    *out = nullptr;
+    for (int k = 0; k < 3; ++k)
+        if (p->vol_dims[k] < 3)
+            return err(FR_ERR_INVALID, "VOL_DIMS must be >= 3 along every axis (got %d %d %d)", p->vol_dims[0],
+                       p->vol_dims[1], p->vol_dims[2]);
+    if (K < 1) return err(FR_ERR_INVALID, "K (unrolled iterations) must be >= 1 (got %d)", K);
+    if (ckpt < 1 || ckpt > K) return err(FR_ERR_INVALID, "checkpoint interval must be in [1, K] (got %d, K = %d)", ckpt, K);
+    if (p->no_labels < 1 || p->no_labels > 32768) return err(FR_ERR_INVALID, "NO_LABELS must be in [1, 32768]");
+    fr_adjoint *h = new (std::nothrow) fr_adjoint;
+    if (!h) return err(FR_ERR_ALLOC, "out of host memory");
+    h->d = Dims{p->vol_dims[0], p->vol_dims[1], p->vol_dims[2]};
+    h->n = (long)p->vol_dims[0] * p->vol_dims[1] * p->vol_dims[2];
+    h->K = K;
+    h->ckpt = ckpt;
+    h->nseg = (K + ckpt - 1) / ckpt;
+    h->alpha = (double)p->alpha;
+    h->rho_0 = (double)p->rho_0;
+    h->p = (double)p->p;
+    h->C = (double)p->C;
+    h->reset_iter = p->reset_iter;
+    h->no_labels = p->no_labels;
+    const long nb = (h->n + TPB - 1) / TPB;
+    h->grid = (int)(nb < 8192 ? nb : 8192);
+    const size_t n = (size_t)h->n;
+    int rc = FR_OK;
+    auto fail = [&](int code) {
+        delete h;
+        return code;
+    };
+    if (hipStreamCreate(&h->stream) != hipSuccess || hipEventCreate(&h->ev[0]) != hipSuccess ||
+        hipEventCreate(&h->ev[1]) != hipSuccess)
+        return fail(err(FR_ERR_HIP, "stream / event creation failed"));
+    if ((rc = h->alloc(&h->T, n)) || (rc = h->alloc(&h->S, n)) || (rc = h->alloc(&h->gS, 3 * n)) ||
+        (rc = h->alloc(&h->LT, n)) || (rc = h->alloc(&h->LS, n)))
+        return fail(rc);
+    h->ck.resize(h->nseg);
+    for (State &st : h->ck)
+        if ((rc = h->alloc_state(&st))) return fail(rc);
+    if ((rc = h->alloc_state(&h->fin))) return fail(rc);
+    h->nxt.resize(ckpt < 2 ? 2 : ckpt);
+    for (State &st : h->nxt)
+        if ((rc = h->alloc_state(&st))) return fail(rc);
+    h->inter.resize(ckpt);
+    for (Inter &it : h->inter)
+        if ((rc = h->alloc_inter(&it))) return fail(rc);
+    if ((rc = h->alloc(&h->grho[0], n)) || (rc = h->alloc(&h->grho[1], n)) || (rc = h->alloc(&h->gv, 3 * n)) ||
+        (rc = h->alloc(&h->gpsi, 3 * n)) || (rc = h->alloc(&h->gpinv, 3 * n)) || (rc = h->alloc(&h->gJ, 9 * n)) ||
+        (rc = h->alloc(&h->gL, 3 * n)) || (rc = h->alloc(&h->gLo, 3 * n)) || (rc = h->alloc(&h->gTp, n)) ||
+        (rc = h->alloc(&h->ggU, 3 * n)) || (rc = h->alloc(&h->gdvv, 3 * n)) || (rc = h->alloc(&h->gdlv, 3 * n)) ||
+        (rc = h->alloc(&h->scal, 4)) || (rc = h->alloc(&h->partials, (size_t)h->grid)))
+        return fail(rc);
+    *out = h;
+    return FR_OK;
+}
+
+extern "C" int fr_adjoint_destroy(fr_adjoint *h) {
+    if (h) {
+        (void)hipStreamSynchronize(h->stream);
+        delete h;
+    }
+    return FR_OK;
+}
+
+extern "C" int fr_adjoint_set_images(fr_adjoint *h, const float *d_tgt, const int16_t *d_tgt_lab, const float *d_src,
+                                     const int16_t *d_src_lab) {
+    if (!h || !d_tgt || !d_tgt_lab || !d_src || !d_src_lab) return err(FR_ERR_INVALID, "null argument");
+    ADJ_TRY(hipDeviceSynchronize());  // the inputs may have been written on any stream
+    frd::f_setup<<<h->grid, TPB, 0, h->stream>>>(h->d, d_tgt, d_src, h->T, h->S, h->gS);
+    ADJ_TRY(hipGetLastError());
+    ADJ_TRY(hipMemcpyAsync(h->LT, d_tgt_lab, (size_t)h->n * sizeof(int16_t), hipMemcpyDeviceToDevice, h->stream));
+    ADJ_TRY(hipMemcpyAsync(h->LS, d_src_lab, (size_t)h->n * sizeof(int16_t), hipMemcpyDeviceToDevice, h->stream));
+    ADJ_TRY(hipStreamSynchronize(h->stream));
+    h->have_inputs = true;
+    h->have_forward = false;
+    return FR_OK;
+}
+
+extern "C" int fr_adjoint_forward(fr_adjoint *h, const double *d_v0, double w_reg, double tau, double label_weight,
+                                  double *energy) {
+    if (!h) return err(FR_ERR_INVALID, "null handle");
+    if (!h->have_inputs) return err(FR_ERR_STATE, "fr_adjoint_forward before fr_adjoint_set_images");
+    if (!std::isfinite(w_reg) || !std::isfinite(tau) || !std::isfinite(label_weight))
+        return err(FR_ERR_INVALID, "w_reg, tau and label_weight must be finite");
+    if (d_v0) ADJ_TRY(hipDeviceSynchronize());  // v0 may have been written on any stream
+    h->w_reg = w_reg;
+    h->tau = tau;
+    h->lw = label_weight;
+    h->have_forward = false;
+    schedule(h);
+    hipStream_t s = h->stream;
+    ADJ_TRY(hipEventRecord(h->ev[0], s));
+    frd::f_init_state<<<h->grid, TPB, 0, s>>>(h->d, h->ck[0].rho, h->ck[0].v, h->ck[0].psi, h->ck[0].pinv, d_v0,
+                                              h->rho_0);
+    ADJ_TRY(hipGetLastError());
+    State in = h->ck[0];
+    for (int k = 1; k <= h->K; ++k) {
+        const State out = k == h->K ? h->fin : (k % h->ckpt == 0 ? h->ck[k / h->ckpt] : h->nxt[(k - 1) % 2]);
+        if (int rc = forward_iteration(h, in, out, h->inter[0], k)) return rc;
+        in = out;
+    }
+    frd::f_objective<<<h->grid, TPB, 0, s>>>(h->d, h->T, h->S, h->LT, h->LS, h->no_labels, h->lw, h->fin.psi,
+                                             h->partials);
+    frd::k_sum_final<<<1, 64, 0, s>>>(h->partials, h->grid, h->scal);
+    ADJ_TRY(hipGetLastError());
+    ADJ_TRY(hipEventRecord(h->ev[1], s));
+    ADJ_TRY(hipMemcpyAsync(&h->energy, h->scal, sizeof(double), hipMemcpyDeviceToHost, s));
+    ADJ_TRY(hipStreamSynchronize(s));
+    ADJ_TRY(hipEventElapsedTime(&h->fwd_ms, h->ev[0], h->ev[1]));
+    if (energy) *energy = h->energy;
+    h->have_forward = true;
+    return FR_OK;
+}
+
+extern "C" int fr_adjoint_backward(fr_adjoint *h, double *d_grad_v0, double *grad_w_reg, double *grad_tau) {
+    if (!h) return err(FR_ERR_INVALID, "null handle");
+    if (!h->have_forward) return err(FR_ERR_STATE, "fr_adjoint_backward before fr_adjoint_forward");
+    hipStream_t s = h->stream;
+    const size_t n = (size_t)h->n;
+    ADJ_TRY(hipEventRecord(h->ev[0], s));
+    ADJ_TRY(hipMemsetAsync(h->scal + 1, 0, 2 * sizeof(double), s));
+    int cur = 0;
+    ADJ_TRY(hipMemsetAsync(h->grho[cur], 0, n * sizeof(double), s));
+    ADJ_TRY(hipMemsetAsync(h->gv, 0, 3 * n * sizeof(double), s));
+    ADJ_TRY(hipMemsetAsync(h->gpinv, 0, 3 * n * sizeof(double), s));
+    frd::b_objective<<<h->grid, TPB, 0, s>>>(h->d, h->T, h->S, h->LT, h->LS, h->no_labels, h->lw, h->fin.psi, h->gpsi);
+    ADJ_TRY(hipGetLastError());
+    for (int sg = h->nseg - 1; sg >= 0; --sg) {
+        const int start = sg * h->ckpt;
+        const int len = h->K - start < h->ckpt ? h->K - start : h->ckpt;
+        // recompute the segment's states and intermediates from its checkpoint
+        for (int j = 0; j < len; ++j) {
+            const State in = j == 0 ? h->ck[sg] : h->nxt[j - 1];
+            if (int rc = forward_iteration(h, in, h->nxt[j], h->inter[j], start + j + 1)) return rc;
+        }
+        for (int j = len - 1; j >= 0; --j) {
+            const State in = j == 0 ? h->ck[sg] : h->nxt[j - 1];
+            if (int rc = backward_iteration(h, in, h->nxt[j], h->inter[j], start + j + 1, &cur)) return rc;
+        }
+    }
+    ADJ_TRY(hipEventRecord(h->ev[1], s));
+    if (d_grad_v0) ADJ_TRY(hipMemcpyAsync(d_grad_v0, h->gv, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    double sc[2] = {0, 0};
+    ADJ_TRY(hipMemcpyAsync(sc, h->scal + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    ADJ_TRY(hipStreamSynchronize(s));
+    ADJ_TRY(hipEventElapsedTime(&h->bwd_ms, h->ev[0], h->ev[1]));
+    if (grad_w_reg) *grad_w_reg = sc[0];
+    if (grad_tau) *grad_tau = sc[1];
+    return FR_OK;
+}
+
+extern "C" int fr_adjoint_get_state(fr_adjoint *h, int k, double *d_v, double *d_rho, double *d_psi, double *d_psi_inv) {
+    if (!h) return err(FR_ERR_INVALID, "null handle");
+    if (!h->have_forward) return err(FR_ERR_STATE, "fr_adjoint_get_state before fr_adjoint_forward");
+    if (k == h->K) return copy_state(h, h->fin, d_v, d_rho, d_psi, d_psi_inv);
+    if (k < 0 || k > h->K || k % h->ckpt != 0)
+        return err(FR_ERR_INVALID, "state %d is not kept (kept: multiples of %d below %d, and %d)", k, h->ckpt, h->K,
+                   h->K);
+    return copy_state(h, h->ck[k / h->ckpt], d_v, d_rho, d_psi, d_psi_inv);
+}
+
+extern "C" int fr_adjoint_info(fr_adjoint *h, size_t *device_bytes, float *forward_ms, float *backward_ms) {
+    if (!h) return err(FR_ERR_INVALID, "null handle");
+    if (device_bytes) *device_bytes = h->bytes;
+    if (forward_ms) *forward_ms = h->fwd_ms;
+    if (backward_ms) *backward_ms = h->bwd_ms;
+    return FR_OK;
+}

@@ -34,6 +34,11 @@ static int fail(int code, const char *fmt, ...) {
     return code;
 }
 
+int frk::set_error(int code, const char *msg) {
+    g_err = msg;
+    return code;
+}
+
 #define HIP_TRY(expr)                                                                                   \
     do {                                                                                                \
         hipError_t e_ = (expr);                                                                         \

@@ -1,4 +1,5 @@
-// Internal declarations shared by the HIP kernels (fr_kernels.hip) and the C-ABI (fr_capi.hip).
+// Internal declarations shared by the HIP kernels (fr_kernels.hip), the C-ABI (fr_capi.hip) and
+// the differentiated solver (fr_adjoint.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -61,6 +62,9 @@ struct IterArgs {
     unsigned long long *maxpack; // per-iteration packed (norm bits, ~index) maxima, NSLOT each
     const double *energy;        // per-iteration energy
 };
+
+// Record `msg` as the calling thread's last error (fr_last_error) and return `code`.
+int set_error(int code, const char *msg);
 
 // launchers (all asynchronous on `s`)
 hipError_t launch_init(Dims d, const float *tgt, const int16_t *tgt_lab, const float *src, const int16_t *src_lab,

@@ -230,6 +230,42 @@ int fr_sum(const float *d_data, long long n, double *sum, void *hip_stream);
  * NULL.  No counterpart in the reference: it is the per-kernel measurement of this solver. */
 int fr_time_kernels(fr_ctx *ctx, int reps, float *ms_pass_l, float *ms_pass_a, float *ms_pass_b);
 
+/* ---- the differentiated solver: K unrolled iterations in FP64 and their adjoint ---------------
+ *
+ * No counterpart in the reference, which has no gradients: these differentiate its iteration loop
+ * (src/fastreg/cuda/solver.cu:295-444, one iteration = density update solver.cu:101-173, J_inv,
+ * -Laplacian(psi) and its warp vector_fields.cu:231-300,503-535, potential gradient solver.cu:
+ * 56-80, (Dv)v and the velocity Laplacian vector_fields.cu:307-404, velocity / psi / psi_inv
+ * updates solver.cu:188-279) run for exactly K iterations (no convergence test) from
+ * rho = RHO_0, v = v0, psi = psi_inv = identity, in FP64 with the reference's operation order.
+ * The objective of the final state is
+ *     E = 0.5 sum_p (T(p) - S o psi(p))^2
+ *       + label_weight * 0.5 sum_{l=1}^{NO_LABELS-1} sum_p (1[LT(p) = l] - trilerp(1[LS = l], psi(p)))^2
+ * (the SSD data energy of solver.cu:357-363 plus a label-overlap surrogate).  The backward pass
+ * gives dE/dv0 (per voxel) and dE/dW_REG, dE/dTAU, with checkpoints of the state every `ckpt`
+ * iterations (the segments are recomputed during the backward pass).  Branches (upwind, min/max,
+ * floor, clamp) are differentiated within the branch the forward pass took.  Vector fields are
+ * 3 doubles per voxel (x, y, z), device pointers; every call waits for its work. */
+typedef struct fr_adjoint fr_adjoint;
+
+/* Uses VOL_DIMS, ALPHA, RHO_0, p, C, RESET_ITER and NO_LABELS of `p` (W_REG and TAU are arguments
+ * of the forward pass).  1 <= ckpt <= K. */
+int fr_adjoint_create(const fr_params *p, int K, int ckpt, fr_adjoint **out);
+int fr_adjoint_destroy(fr_adjoint *h);
+/* target / source image and labels (device pointers, N voxels each), copied into the context */
+int fr_adjoint_set_images(fr_adjoint *h, const float *d_tgt, const int16_t *d_tgt_lab, const float *d_src,
+                          const int16_t *d_src_lab);
+/* K iterations from v0 (d_v0: N*3 doubles, or NULL for v0 = 0); *energy receives E */
+int fr_adjoint_forward(fr_adjoint *h, const double *d_v0, double w_reg, double tau, double label_weight,
+                       double *energy);
+/* gradients of E of the last forward pass: d_grad_v0 (N*3 doubles, may be NULL), *grad_w_reg, *grad_tau */
+int fr_adjoint_backward(fr_adjoint *h, double *d_grad_v0, double *grad_w_reg, double *grad_tau);
+/* the state after k iterations of the last forward pass, k a multiple of ckpt below K or k = K:
+ * v and psi, psi_inv N*3 doubles, rho N doubles; any pointer may be NULL */
+int fr_adjoint_get_state(fr_adjoint *h, int k, double *d_v, double *d_rho, double *d_psi, double *d_psi_inv);
+/* device bytes held by the context; device time of the last forward and backward calls */
+int fr_adjoint_info(fr_adjoint *h, size_t *device_bytes, float *forward_ms, float *backward_ms);
+
 /* ---- device memory plumbing (for callers without a HIP binding of their own) -------------- */
 
 int fr_dev_malloc(void **d_ptr, size_t bytes);

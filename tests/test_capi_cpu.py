@@ -102,3 +102,25 @@ def test_null_arguments_are_errors_not_crashes():
     assert L.fr_register(None, None, None, None, None) == fr.FR_ERR_INVALID
     assert L.fr_solver_step(None, C.c_float(0.1), None, None) == fr.FR_ERR_INVALID
     assert L.fr_destroy(None) == fr.FR_OK
+
+
+@pytest.mark.parametrize("K,ckpt,kw", [(0, 1, {}), (8, 0, {}), (8, 9, {}), (4, 2, dict(vol_dims=(2, 8, 8))),
+                                       (4, 2, dict(no_labels=0))])
+def test_adjoint_create_rejects_invalid_arguments_before_touching_the_gpu(K, ckpt, kw):
+    p = fr.default_params(**kw)
+    h = C.c_void_p()
+    rc = fr.lib().fr_adjoint_create(C.byref(p), K, ckpt, C.byref(h))
+    assert rc == fr.FR_ERR_INVALID and not h.value
+    assert fr.lib().fr_last_error()
+
+
+def test_adjoint_null_handles_are_errors_not_crashes():
+    L = fr.lib()
+    d = C.c_double()
+    assert L.fr_adjoint_create(None, 4, 2, None) == fr.FR_ERR_INVALID
+    assert L.fr_adjoint_forward(None, None, 0.2, 0.2, 0.0, C.byref(d)) == fr.FR_ERR_INVALID
+    assert L.fr_adjoint_backward(None, None, None, None) == fr.FR_ERR_INVALID
+    assert L.fr_adjoint_get_state(None, 0, None, None, None, None) == fr.FR_ERR_INVALID
+    assert L.fr_adjoint_info(None, None, None, None) == fr.FR_ERR_INVALID
+    assert L.fr_adjoint_set_images(None, None, None, None, None) == fr.FR_ERR_INVALID
+    assert L.fr_adjoint_destroy(None) == fr.FR_OK

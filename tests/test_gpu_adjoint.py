@@ -1,0 +1,211 @@
+"""The differentiated solver (fr_adjoint_*) on the GPU: its FP64 forward pass bit-exact against the
+FP64 oracle iteration by iteration, its gradient (hand-written adjoint kernels, checkpointed)
+against torch autograd of the unrolled restatement (oracle/unrolled.py) and against central
+finite differences of its own forward pass, independent of the checkpoint interval."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+from scipy import ndimage
+
+import fastreg_amd as fr
+from fastreg_amd import phantoms
+
+pytestmark = pytest.mark.gpu
+
+ALPHA = float(np.float32(0.004))
+TAU = float(np.float32(0.2))
+W_REG = float(np.float32(0.2))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _pair(dims, seed):
+    X, Y, Z = dims
+    return phantoms.blobs(shape=(Z, Y, X), seed=seed, n_blobs=5, n_labels=4, n_rbf=3, rbf_sigma=4.0, max_v=1.5,
+                          steps=4)
+
+
+def _v0(dims, seed, amp=1.0, off=0.1):
+    """A smooth velocity, nonzero on the boundary too (v = 0 would sit on every upwind kink)."""
+    X, Y, Z = dims
+    rng = np.random.default_rng(seed)
+    f = np.stack([ndimage.gaussian_filter(rng.normal(size=(Z, Y, X)), 1.5) for _ in range(3)], -1).reshape(-1, 3)
+    return f * (amp / np.abs(f).max()) + off
+
+
+def _adjoint(dims, pair, K, ckpt, **kw):
+    p = fr.default_params(vol_dims=dims, no_labels=pair.no_labels, alpha=ALPHA, **kw)
+    a = fr.Adjoint(p, K, ckpt)
+    a.set_images(pair.target, pair.target_labels, pair.source, pair.source_labels)
+    return a
+
+
+@pytest.mark.parametrize("dims,seed,K,reset_iter", [((24, 20, 18), 3, 5, -1), ((19, 13, 11), 4, 6, 3)])
+def test_forward_is_bit_exact_against_the_fp64_oracle(oracle_mod, dims, seed, K, reset_iter):
+    O = oracle_mod
+    say("adjoint forward", dims, K, reset_iter)
+    n = dims[0] * dims[1] * dims[2]
+    pair = _pair(dims, seed)
+    v0 = _v0(dims, seed)
+    a = _adjoint(dims, pair, K, 1, reset_iter=reset_iter)
+    try:
+        a.forward(v0, W_REG, TAU, 0.0)
+        o = O.Oracle(dims, "f64")
+        o.init(pair.target, pair.target_labels, pair.source, pair.source_labels, 1.0)
+        v4 = np.zeros((n, 4))
+        v4[:, :3] = v0
+        o.set_state(v=v4)
+        g0 = a.state(0)
+        assert np.array_equal(g0["v"], v0) and np.all(g0["rho"] == 1.0)
+        t = ALPHA
+        for k in range(1, K + 1):
+            if k >= 2:
+                t += ALPHA
+                if reset_iter > 0 and k % reset_iter == 0:
+                    t = ALPHA
+            o.step(t, ALPHA, TAU, W_REG, 2.0, 0.25)
+            ref, got = o.get_state(), a.state(k)
+            for key in ("v", "psi", "psi_inv"):
+                assert np.array_equal(got[key], ref[key][:, :3]), (k, key)
+            assert np.array_equal(got["rho"], ref["rho"]), (k, "rho")
+    finally:
+        a.close()
+
+
+def test_gradient_matches_autograd_of_the_unrolled_restatement():
+    from oracle import unrolled as U
+    dims, K, ckpt, lw = (20, 18, 16), 7, 3, 0.5
+    say("adjoint vs autograd", dims, K, ckpt)
+    pair = _pair(dims, 5)
+    v0 = _v0(dims, 5)
+    a = _adjoint(dims, pair, K, ckpt)
+    try:
+        e = a.forward(v0, W_REG, TAU, lw)
+        g, gw, gt = a.backward()
+    finally:
+        a.close()
+    v0t = torch.tensor(v0, requires_grad=True)
+    wt = torch.tensor(W_REG, dtype=torch.float64, requires_grad=True)
+    tt = torch.tensor(TAU, dtype=torch.float64, requires_grad=True)
+    E = U.unrolled(dims, pair.target, pair.target_labels, pair.source, pair.source_labels, v0t, wt, tt, K,
+                   alpha=ALPHA, no_labels=pair.no_labels, label_weight=lw)
+    E.backward()
+    assert abs(e - E.item()) <= 1e-12 * abs(E.item())
+    gr = v0t.grad.numpy()
+    rel = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    say("rel L2 of dE/dv0", rel, "w_reg", gw, wt.grad.item(), "tau", gt, tt.grad.item())
+    assert rel <= 1e-10
+    assert abs(gw - wt.grad.item()) <= 1e-10 * abs(wt.grad.item())
+    assert abs(gt - tt.grad.item()) <= 1e-10 * abs(tt.grad.item())
+
+
+def test_gradient_matches_central_finite_differences():
+    """Central differences of the library's own forward pass.  The solver is piecewise smooth: its
+    upwind schemes switch branch where a velocity component changes sign, and the trilinear warps
+    are piecewise linear between grid nodes, which psi (starting at the identity) stays close to.
+    Steps of 1e-6 and more already cross such kinks at this size, so the steps are 1e-7 (v0) and
+    1e-6 (W_REG, TAU); the tolerance includes the rounding of E over the step."""
+    dims, K, ckpt, lw = (28, 24, 20), 10, 4, 0.5
+    say("adjoint vs FD", dims, K, ckpt)
+    n = dims[0] * dims[1] * dims[2]
+    pair = _pair(dims, 6)
+    v0 = _v0(dims, 6)
+    a = _adjoint(dims, pair, K, ckpt)
+    try:
+        e = a.forward(v0, W_REG, TAU, lw)
+        g, gw, gt = a.backward()
+        rng = np.random.default_rng(8)
+
+        def close(fd, an, eps):
+            return abs(fd - an) <= 1e-4 * abs(an) + 8 * abs(e) * 2.2e-16 / eps
+
+        eps = 1e-7
+        for _ in range(4):
+            d = rng.normal(size=(n, 3))
+            fd = (a.forward(v0 + eps * d, W_REG, TAU, lw) - a.forward(v0 - eps * d, W_REG, TAU, lw)) / (2 * eps)
+            an = float((g * d).sum())
+            say("direction", fd, an)
+            assert close(fd, an, eps), (fd, an)
+        eps = 1e-6
+        fd = (a.forward(v0, W_REG + eps, TAU, lw) - a.forward(v0, W_REG - eps, TAU, lw)) / (2 * eps)
+        say("w_reg", fd, gw)
+        assert close(fd, gw, eps), (fd, gw)
+        fd = (a.forward(v0, W_REG, TAU + eps, lw) - a.forward(v0, W_REG, TAU - eps, lw)) / (2 * eps)
+        say("tau", fd, gt)
+        assert close(fd, gt, eps), (fd, gt)
+    finally:
+        a.close()
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    dims, K, lw = (22, 17, 15), 9, 0.25
+    say("adjoint checkpoint intervals", dims, K)
+    pair = _pair(dims, 9)
+    v0 = _v0(dims, 9)
+    res = {}
+    for ckpt in (1, 4, 9):
+        a = _adjoint(dims, pair, K, ckpt)
+        try:
+            e = a.forward(v0, W_REG, TAU, lw)
+            g, gw, gt = a.backward()
+            st = a.state(K)
+            res[ckpt] = (e, g, gw, gt, st)
+        finally:
+            a.close()
+    e1, g1, gw1, gt1, st1 = res[1]
+    for ckpt in (4, 9):
+        e, g, gw, gt, st = res[ckpt]
+        assert e == e1
+        for key in st1:
+            assert np.array_equal(st[key], st1[key])
+        # the adjoint's scatters are FP64 atomics, whose order varies
+        assert np.linalg.norm(g - g1) <= 1e-12 * np.linalg.norm(g1)
+        assert abs(gw - gw1) <= 1e-12 * abs(gw1) and abs(gt - gt1) <= 1e-12 * abs(gt1)
+
+
+def test_autograd_function_wrapper_and_repeat_calls():
+    from adjoint_function import SolverObjective
+    dims, K, ckpt, lw = (12, 10, 9), 4, 2, 0.5
+    say("adjoint autograd.Function", dims)
+    pair = _pair(dims, 10)
+    v0 = _v0(dims, 10)
+    a = _adjoint(dims, pair, K, ckpt)
+    try:
+        e_direct = a.forward(v0, W_REG, TAU, lw)
+        g_direct, gw_direct, gt_direct = a.backward()
+        v = torch.tensor(v0, requires_grad=True)
+        w = torch.tensor(W_REG, dtype=torch.float64, requires_grad=True)
+        t = torch.tensor(TAU, dtype=torch.float64, requires_grad=True)
+        E = SolverObjective.apply(v, w, t, a, lw)
+        assert E.item() == e_direct
+        (2.0 * E).backward()
+        assert np.linalg.norm(v.grad.numpy() - 2 * g_direct) <= 1e-12 * np.linalg.norm(g_direct)
+        assert abs(w.grad.item() - 2 * gw_direct) <= 1e-12 * abs(gw_direct)
+        assert abs(t.grad.item() - 2 * gt_direct) <= 1e-12 * abs(gt_direct)
+        ok = torch.autograd.gradcheck(lambda vv: SolverObjective.apply(vv, w.detach(), t.detach(), a, lw),
+                                      (torch.tensor(v0, requires_grad=True),), eps=1e-6, atol=1e-8, rtol=1e-5,
+                                      nondet_tol=1e-12, fast_mode=True)
+        assert ok
+        by, fwd_ms, bwd_ms = a.info()
+        assert by > 0 and fwd_ms > 0 and bwd_ms > 0
+        with pytest.raises(fr.FastRegError):
+            a.state(1)  # not a kept state (ckpt 2)
+    finally:
+        a.close()
+
+
+def test_backward_before_forward_is_a_state_error():
+    dims = (10, 9, 8)
+    pair = _pair(dims, 11)
+    a = _adjoint(dims, pair, 3, 2)
+    try:
+        with pytest.raises(fr.FastRegError) as e:
+            a.backward()
+        assert e.value.code == fr.FR_ERR_STATE
+    finally:
+        a.close()
