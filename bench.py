@@ -49,11 +49,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--zc", default="0,0,0",
                     help="interior z-slices per block of pass L, pass A, pass B (0 = automatic)")
-    ap.add_argument("--workload", choices=("plain", "batch", "precision"), default="plain",
+    ap.add_argument("--workload", choices=("plain", "batch", "precision", "adjoint"), default="plain",
                     help="plain: the single-pair headline metric (default); batch: 64 pairs of uneven sizes, "
                          "150 iterations each, batched form vs one pair at a time; precision: FP32, BF16 and "
                          "FP16 storage of v and rho against an FP64 oracle run (64^3, 300 iterations) and their "
-                         "throughput at 182x218x182")
+                         "throughput at 182x218x182; adjoint: the differentiated solver at 128^3, K unrolled "
+                         "iterations with checkpoints (gradient check, time ratio, device memory)")
+    ap.add_argument("--adj-k", type=int, default=64, help="adjoint workload: unrolled iterations")
+    ap.add_argument("--adj-ckpt", type=int, default=8, help="adjoint workload: checkpoint interval")
+    ap.add_argument("--adj-dirs", type=int, default=32, help="adjoint workload: finite-difference directions")
+    ap.add_argument("--cpu-adj-k", type=int, default=2,
+                    help="adjoint workload: unrolled iterations of the CPU reference-path gradient sample")
     ap.add_argument("--batch-pairs", type=int, default=64)
     ap.add_argument("--batch-iters", type=int, default=150)
     ap.add_argument("--batch-sizes", choices=("mixed", "small"), default="mixed",
@@ -71,6 +77,8 @@ def main():
         return batch_workload(args)
     if args.workload == "precision":
         return precision_workload(args)
+    if args.workload == "adjoint":
+        return adjoint_workload(args)
     zc = tuple(int(v) for v in args.zc.split(","))
     import numpy as np
     import fastreg_amd as fr
@@ -299,6 +307,151 @@ def precision_workload(args):
                             for nm in speed},
     }
     print(json.dumps(out), flush=True)
+
+
+def adjoint_workload(args):
+    """The differentiated solver on the 128^3 blob pair (the 64^3 generator with every length
+    doubled, seed 5, 9 labels), K unrolled FP64 iterations (default 64) with checkpoints every
+    `--adj-ckpt` (default 8), objective = SSD of the final state + 1.0 x the label-overlap
+    surrogate, evaluated at a smooth random v0 (amplitude 0.5, offset 0.05: v0 = 0 would put every
+    upwind branch on its kink).  Reported: device time of the forward pass (K iterations + the
+    objective) and of the backward pass (segment recompute + adjoint kernels) and their ratio;
+    device memory of the context at checkpoint intervals 1, ckpt and K; central finite differences
+    of the forward pass along `--adj-dirs` random directions at eps 1e-4 and at eps 1e-7 against the
+    adjoint's directional derivative, and for W_REG and TAU; and, in the CPU leg, the reference-path
+    gradient (torch autograd of the unrolled restatement, FP64, host cores) at 128^3 with
+    `--cpu-adj-k` iterations against the adjoint at the same K."""
+    import numpy as np
+    import fastreg_amd as fr
+    from fastreg_amd import phantoms
+
+    dims = (128, 128, 128)
+    X, Y, Z = dims
+    n = X * Y * Z
+    K, ckpt, lw = args.adj_k, args.adj_ckpt, 1.0
+    alpha, w_reg, tau = (float(np.float32(x)) for x in (0.004, 0.2, 0.2))
+    t0 = time.time()
+    pair = phantoms.blobs(shape=(Z, Y, X), seed=5, scale=2.0)
+    v0 = phantoms.smooth_velocity(dims, 5, amp=0.5, offset=0.05, sigma=4.0)
+    print(f"[bench] adjoint pair {dims} ready ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+
+    def make(k, c):
+        p = fr.default_params(vol_dims=dims, no_labels=pair.no_labels, alpha=alpha)
+        a = fr.Adjoint(p, k, c)
+        a.set_images(pair.target, pair.target_labels, pair.source, pair.source_labels)
+        return a
+
+    runs = {}
+    for c in sorted({1, ckpt, K}):
+        a = make(K, c)
+        a.forward(v0, w_reg, tau, lw)  # warm-up
+        a.backward()
+        fwd, bwd = [], []
+        for _ in range(3):
+            e = a.forward(v0, w_reg, tau, lw)
+            g, gw, gt = a.backward()
+            by, fm, bm = a.info()
+            fwd.append(fm)
+            bwd.append(bm)
+        runs[c] = {"device_bytes": by, "forward_ms": round(float(np.median(fwd)), 3),
+                   "backward_ms": round(float(np.median(bwd)), 3),
+                   "backward_over_forward": round(float(np.median(bwd) / np.median(fwd)), 3)}
+        print(f"[bench] ckpt {c}: {runs[c]}", file=sys.stderr, flush=True)
+        if c == ckpt:
+            main_ctx, E, G, GW, GT = a, e, g, gw, gt
+        else:
+            a.close()
+    a = main_ctx
+
+    def fd_check(eps, ndir, seed):
+        r = np.random.default_rng(seed)
+        rels = []
+        for _ in range(ndir):
+            d = r.normal(size=(n, 3))
+            fd = (a.forward(v0 + eps * d, w_reg, tau, lw) - a.forward(v0 - eps * d, w_reg, tau, lw)) / (2 * eps)
+            an = float((G * d).sum())
+            rels.append(abs(fd - an) / abs(an))
+        rels = np.asarray(rels)
+        return {"eps": eps, "directions": ndir, "median_rel_err": float(np.median(rels)),
+                "max_rel_err": float(rels.max()), "frac_below_1e-3": float(np.mean(rels < 1e-3))}
+
+    fd = {}
+    for eps in (1e-4, 1e-7):
+        fd[f"v0_eps_{eps:g}"] = fd_check(eps, args.adj_dirs, 55)
+        print(f"[bench] FD {fd[f'v0_eps_{eps:g}']}", file=sys.stderr, flush=True)
+    for eps in (1e-4, 1e-6):
+        fw = (a.forward(v0, w_reg + eps, tau, lw) - a.forward(v0, w_reg - eps, tau, lw)) / (2 * eps)
+        ft = (a.forward(v0, w_reg, tau + eps, lw) - a.forward(v0, w_reg, tau - eps, lw)) / (2 * eps)
+        fd[f"scalars_eps_{eps:g}"] = {"w_reg": {"fd": fw, "adjoint": GW, "rel_err": abs(fw - GW) / abs(GW)},
+                                      "tau": {"fd": ft, "adjoint": GT, "rel_err": abs(ft - GT) / abs(GT)}}
+    a.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = adjoint_cpu_baseline(dims, pair, v0, args.cpu_adj_k, alpha, w_reg, tau, lw, make)
+    tot_ms = runs[ckpt]["forward_ms"] + runs[ckpt]["backward_ms"]
+    no_ckpt_bytes = runs[K]["device_bytes"]
+    out = {
+        "metric": "differentiated solver at 128^3: voxel-iterations/s of forward + backward, adjoint/forward time, "
+                  "device memory, gradient error against finite differences and the reference-path gradient",
+        "value": round(n * K / (tot_ms / 1e3), 1),
+        "unit": "voxel-iterations/s (forward + backward, FP64)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "ms_per_step": round(tot_ms, 3),
+        "dtype": "f64",
+        "data": "synthetic (blob phantom at 128^3, seed 5; smooth random v0)",
+        "config": {"workload": f"differentiated solver, 128x128x128, K={K} unrolled iterations, checkpoint every "
+                               f"{ckpt}", "dims": list(dims), "K": K, "ckpt": ckpt, "label_weight": lw},
+        "objective": E,
+        "grad_w_reg": GW,
+        "grad_tau": GT,
+        "backward_over_forward": runs[ckpt]["backward_over_forward"],
+        "device_bytes": runs[ckpt]["device_bytes"],
+        "device_bytes_by_ckpt": {str(c): r["device_bytes"] for c, r in runs.items()},
+        "times_by_ckpt": runs,
+        "memory_saving_vs_keeping_every_iteration": round(no_ckpt_bytes / runs[ckpt]["device_bytes"], 2),
+        "finite_differences": fd,
+        "vs_baseline": None,
+    }
+    if cpu:
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+
+
+def adjoint_cpu_baseline(dims, pair, v0, k, alpha, w_reg, tau, lw, make):
+    """CPU leg of the adjoint workload: the reference-path gradient (oracle/unrolled.py, torch FP64
+    autograd on the host cores) of `k` unrolled iterations on the same pair, timed, and the
+    adjoint's gradient at the same k compared with it."""
+    import numpy as np
+    import torch
+    from oracle import unrolled as U
+
+    n = dims[0] * dims[1] * dims[2]
+    t0 = time.time()
+    vt = torch.tensor(v0, requires_grad=True)
+    wt = torch.tensor(w_reg, dtype=torch.float64, requires_grad=True)
+    tt = torch.tensor(tau, dtype=torch.float64, requires_grad=True)
+    E = U.unrolled(dims, pair.target, pair.target_labels, pair.source, pair.source_labels, vt, wt, tt, k,
+                   alpha=alpha, no_labels=pair.no_labels, label_weight=lw)
+    E.backward()
+    cs = time.time() - t0
+    a = make(k, 1)
+    e = a.forward(v0, w_reg, tau, lw)
+    g, gw, gt = a.backward()
+    a.close()
+    gr = vt.grad.numpy()
+    res = {"value": round(n * k / cs, 1), "unit": "voxel-iterations/s (forward + backward, FP64)",
+           "cores": torch.get_num_threads(), "kind": "port",
+           "sample": f"torch FP64 autograd of the unrolled restatement, {dims[0]}x{dims[1]}x{dims[2]}, K={k}, "
+                     f"host cores ({cs:.1f} s)",
+           "adjoint_vs_reference_path_gradient": {
+               "K": k, "objective_rel_diff": abs(e - E.item()) / abs(E.item()),
+               "rel_l2_grad_v0": float(np.linalg.norm(g - gr) / np.linalg.norm(gr)),
+               "rel_diff_grad_w_reg": abs(gw - wt.grad.item()) / abs(wt.grad.item()),
+               "rel_diff_grad_tau": abs(gt - tt.grad.item()) / abs(tt.grad.item())}}
+    print(f"[bench] cpu reference-path gradient: {res}", file=sys.stderr, flush=True)
+    return res
 
 
 BATCH_SIZES = [(48, 48, 48), (96, 96, 96), (128, 160, 128), (160, 192, 160), (182, 218, 182)]  # (X, Y, Z)

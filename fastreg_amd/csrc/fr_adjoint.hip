@@ -725,12 +725,18 @@ __global__ void f_init_state(Dims d, double *rho, double *v, double *psi, double
     }
 }
 
-__global__ void k_sum_final(const double *partials, int np, double *out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double s = 0;
-        for (int j = 0; j < np; ++j) s += partials[j];
-        *out = s;
+// *out = sum of np partials, one block of 256 threads, in a fixed order (deterministic)
+__global__ __launch_bounds__(256) void k_sum_final(const double *partials, int np, double *out) {
+    __shared__ double sh[256];
+    double s = 0;
+    for (int j = threadIdx.x; j < np; j += 256) s += partials[j];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *out = sh[0];
 }
 
 }  // namespace frd
@@ -1029,7 +1035,7 @@ extern "C" int fr_adjoint_forward(fr_adjoint *h, const double *d_v0, double w_re
     }
     frd::f_objective<<<h->grid, TPB, 0, s>>>(h->d, h->T, h->S, h->LT, h->LS, h->no_labels, h->lw, h->fin.psi,
                                              h->partials);
-    frd::k_sum_final<<<1, 64, 0, s>>>(h->partials, h->grid, h->scal);
+    frd::k_sum_final<<<1, 256, 0, s>>>(h->partials, h->grid, h->scal);
     ADJ_TRY(hipGetLastError());
     ADJ_TRY(hipEventRecord(h->ev[1], s));
     ADJ_TRY(hipMemcpyAsync(&h->energy, h->scal, sizeof(double), hipMemcpyDeviceToHost, s));

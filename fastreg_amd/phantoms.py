@@ -87,14 +87,15 @@ def warp(img, u, order):
 
 
 def blobs(shape=(64, 64, 64), seed=1, n_blobs=12, n_labels=8, n_rbf=6, rbf_sigma=8.0, max_v=3.0, steps=6,
-          noise=0.01) -> Pair:
-    """The blob phantom pair; `shape` is (Z, Y, X)."""
+          noise=0.01, scale=1.0) -> Pair:
+    """The blob phantom pair; `shape` is (Z, Y, X).  `scale` multiplies every length (blob sigmas,
+    RBF sigma, max |v|), e.g. 2 for the 64^3 phantom drawn at 128^3."""
     rng = np.random.default_rng(seed)
     zs, ys, xs = _grid(shape)
     t = np.zeros(shape, np.float64)
     for _ in range(n_blobs):
         c = rng.uniform([0, 0, 0], [shape[2] - 1, shape[1] - 1, shape[0] - 1])
-        s = rng.uniform(4.0, 10.0)
+        s = rng.uniform(4.0, 10.0) * scale
         a = rng.uniform(0.3, 1.0)
         gx = np.exp(-(xs - c[0]) ** 2 / (2 * s * s))
         gy = np.exp(-(ys - c[1]) ** 2 / (2 * s * s))
@@ -104,7 +105,7 @@ def blobs(shape=(64, 64, 64), seed=1, n_blobs=12, n_labels=8, n_rbf=6, rbf_sigma
     pts = rng.uniform([0, 0, 0], [shape[2] - 1, shape[1] - 1, shape[0] - 1], size=(n_labels, 3))
     lab = _voronoi(shape, pts) + 1
     lab[t < 0.05] = 0
-    v = rbf_velocity(shape, rng, n_rbf, rbf_sigma, max_v)
+    v = rbf_velocity(shape, rng, n_rbf, rbf_sigma * scale, max_v * scale)
     u = exp_velocity(v, steps)
     src = warp(t, u, 1) + rng.normal(0.0, noise, size=shape)
     src_lab = warp(lab.astype(np.float64), u, 0)
@@ -188,3 +189,13 @@ def random_state(dims, rng, amp=1.5, smooth=2.0):
     rho[~interior] = 1.0
     return {"v": v.astype(np.float32), "rho": rho.reshape(-1).astype(np.float32),
             "psi": psi.astype(np.float32), "psi_inv": psi_inv.astype(np.float32)}
+
+
+def smooth_velocity(dims, seed, amp=1.0, offset=0.1, sigma=1.5):
+    """A smooth random velocity (N, 3) float64 for the differentiated solver, x-fastest: each
+    component Gaussian-filtered N(0, 1) noise scaled to max |.| = amp, plus `offset` (so that no
+    component sits at 0, where the upwind schemes switch branch)."""
+    X, Y, Z = dims
+    rng = np.random.default_rng(seed)
+    f = np.stack([ndimage.gaussian_filter(rng.normal(size=(Z, Y, X)), sigma) for _ in range(3)], -1).reshape(-1, 3)
+    return f * (amp / np.abs(f).max()) + offset

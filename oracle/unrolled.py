@@ -47,20 +47,20 @@ def schedule(K, alpha, p, C, reset_iter=-1):
 
 
 class _Grid:
-    def __init__(self, dims):
+    def __init__(self, dims, device="cpu"):
         X, Y, Z = (int(d) for d in dims)
         self.X, self.Y, self.Z, self.n = X, Y, Z, X * Y * Z
         idx = np.arange(self.n)
         x, y, z = idx % X, (idx // X) % Y, idx // (X * Y)
         inner = (x >= 1) & (x <= X - 2) & (y >= 1) & (y <= Y - 2) & (z >= 1) & (z <= Z - 2)
-        self.I = torch.from_numpy(idx[inner])
+        self.device = device
+        self.I = torch.from_numpy(idx[inner]).to(device)
         self.sx, self.sy, self.sz = 1, X, X * Y
         lap = (x >= 2) & (x <= X - 3) & (y >= 2) & (y <= Y - 2) & (z >= 2) & (z <= Z - 3)
-        self.IL = torch.from_numpy(idx[lap])
-        self.mask = torch.from_numpy(inner)
+        self.IL = torch.from_numpy(idx[lap]).to(device)
 
     def scatter(self, idx, vals, like_shape):
-        out = torch.zeros(like_shape, dtype=torch.float64)
+        out = torch.zeros(like_shape, dtype=torch.float64, device=self.device)
         return out.index_put((idx,), vals)
 
 
@@ -193,25 +193,40 @@ def objective(g: _Grid, psi, T, S, LT, LS, no_labels, lw):
 
 
 def unrolled(dims, tgt, tgt_lab, src, src_lab, v0, w_reg, tau, K, alpha=0.004, rho_0=1.0, p=2.0, C=0.25,
-             reset_iter=-1, no_labels=1, label_weight=0.0, keep_states=False):
+             reset_iter=-1, no_labels=1, label_weight=0.0, keep_states=False, checkpoint=False):
     """E after K iterations from (rho_0, v0, identity, identity).  tgt/src: float32 arrays of N
-    voxels (x-fastest); labels int16; v0 (N, 3) float64 tensor; w_reg, tau float64 tensors or
-    floats.  Returns E (and the list of states after each iteration if keep_states)."""
-    g = _Grid(dims)
+    voxels (x-fastest); labels int16; v0 (N, 3) float64 tensor (its device is used throughout);
+    w_reg, tau float64 tensors or floats.  checkpoint: recompute each iteration in the backward
+    pass (torch.utils.checkpoint) instead of keeping its graph.  Returns E (and the list of states
+    after each iteration if keep_states)."""
+    dev = v0.device
+    g = _Grid(dims, dev)
     n = g.n
-    T = torch.from_numpy(np.asarray(tgt, np.float32).reshape(n).astype(np.float64))
-    S = torch.from_numpy(np.asarray(src, np.float32).reshape(n).astype(np.float64))
-    LT = torch.from_numpy(np.asarray(tgt_lab, np.int16).reshape(n).astype(np.int64))
-    LS = torch.from_numpy(np.asarray(src_lab, np.int16).reshape(n).astype(np.int64))
+
+    def load(a, dt, npdt):
+        return torch.from_numpy(np.asarray(a, dt).reshape(n).astype(npdt)).to(dev)
+
+    T = load(tgt, np.float32, np.float64)
+    S = load(src, np.float32, np.float64)
+    LT = load(tgt_lab, np.int16, np.int64)
+    LS = load(src_lab, np.int16, np.int64)
     I = g.I
     gS = g.scatter(I, torch.stack([(S[I + 1] - S[I - 1]) / 2.0, (S[I + g.sy] - S[I - g.sy]) / 2.0,
                                    (S[I + g.sz] - S[I - g.sz]) / 2.0], dim=1), (n, 3))
     idx = np.arange(n)
     ident = torch.from_numpy(np.stack([idx % g.X, (idx // g.X) % g.Y, idx // (g.X * g.Y)], 1).astype(np.float64))
-    st = (torch.full((n,), float(rho_0), dtype=torch.float64), v0, ident.clone(), ident.clone())
+    ident = ident.to(dev)
+    st = (torch.full((n,), float(rho_0), dtype=torch.float64, device=dev), v0, ident.clone(), ident.clone())
     states = []
     for m1t, m2 in schedule(K, float(alpha), float(p), float(C), reset_iter):
-        st = iterate(g, st, T, S, gS, float(alpha), m1t, m2, tau, w_reg)
+        if checkpoint:
+            from torch.utils.checkpoint import checkpoint as ckpt
+
+            st = ckpt(lambda r, v, ps, pi, w, t, m1t=m1t, m2=m2: iterate(g, (r, v, ps, pi), T, S, gS, float(alpha),
+                                                                         m1t, m2, t, w),
+                      *st, w_reg, tau, use_reentrant=False)
+        else:
+            st = iterate(g, st, T, S, gS, float(alpha), m1t, m2, tau, w_reg)
         if keep_states:
             states.append(st)
     e = objective(g, st[2], T, S, LT, LS, no_labels, label_weight)

@@ -7,7 +7,6 @@ import sys
 import numpy as np
 import pytest
 import torch
-from scipy import ndimage
 
 import fastreg_amd as fr
 from fastreg_amd import phantoms
@@ -32,10 +31,7 @@ def _pair(dims, seed):
 
 def _v0(dims, seed, amp=1.0, off=0.1):
     """A smooth velocity, nonzero on the boundary too (v = 0 would sit on every upwind kink)."""
-    X, Y, Z = dims
-    rng = np.random.default_rng(seed)
-    f = np.stack([ndimage.gaussian_filter(rng.normal(size=(Z, Y, X)), 1.5) for _ in range(3)], -1).reshape(-1, 3)
-    return f * (amp / np.abs(f).max()) + off
+    return phantoms.smooth_velocity(dims, seed, amp, off)
 
 
 def _adjoint(dims, pair, K, ckpt, **kw):
@@ -102,6 +98,65 @@ def test_gradient_matches_autograd_of_the_unrolled_restatement():
     assert rel <= 1e-10
     assert abs(gw - wt.grad.item()) <= 1e-10 * abs(wt.grad.item())
     assert abs(gt - tt.grad.item()) <= 1e-10 * abs(tt.grad.item())
+
+
+def _autograd(dims, pair, v0, K, lw, device="cpu", checkpoint=False):
+    from oracle import unrolled as U
+    v0t = torch.tensor(v0, device=device, requires_grad=True)
+    wt = torch.tensor(W_REG, dtype=torch.float64, device=device, requires_grad=True)
+    tt = torch.tensor(TAU, dtype=torch.float64, device=device, requires_grad=True)
+    E = U.unrolled(dims, pair.target, pair.target_labels, pair.source, pair.source_labels, v0t, wt, tt, K,
+                   alpha=ALPHA, no_labels=pair.no_labels, label_weight=lw, checkpoint=checkpoint)
+    E.backward()
+    return E.item(), v0t.grad.cpu().numpy(), wt.grad.item(), tt.grad.item()
+
+
+def _compare(res, ref, tol):
+    e, g, gw, gt = res
+    E, G, GW, GT = ref
+    rel = np.linalg.norm(g - G) / np.linalg.norm(G)
+    say("E", e, E, "rel L2 dE/dv0", rel, "w_reg", gw, GW, "tau", gt, GT)
+    assert abs(e - E) <= 1e-12 * abs(E)
+    assert rel <= tol
+    assert abs(gw - GW) <= tol * abs(GW) and abs(gt - GT) <= tol * abs(GT)
+
+
+def test_gradient_matches_autograd_at_depth_64():
+    """K = 64 unrolled iterations with checkpoints every 8 (the workload's depth) on a small volume."""
+    dims, K, ckpt, lw = (24, 20, 18), 64, 8, 1.0
+    say("adjoint vs autograd, depth", dims, K, ckpt)
+    pair = _pair(dims, 12)
+    v0 = _v0(dims, 12, amp=0.5, off=0.05)
+    a = _adjoint(dims, pair, K, ckpt)
+    try:
+        e = a.forward(v0, W_REG, TAU, lw)
+        g, gw, gt = a.backward()
+    finally:
+        a.close()
+    _compare((e, g, gw, gt), _autograd(dims, pair, v0, K, lw), 1e-9)
+
+
+def test_full_size_gradient_matches_autograd_on_the_gpu():
+    """The workload itself: the 128^3 blob pair, K = 64, checkpoints every 8, against torch autograd
+    of the unrolled restatement run on the GPU in FP64 (each iteration recomputed in its backward
+    pass, torch.utils.checkpoint, to bound memory)."""
+    if not torch.cuda.is_available():
+        pytest.fail("torch sees no GPU")
+    dims, K, ckpt, lw = (128, 128, 128), 64, 8, 1.0
+    say("adjoint vs autograd, full size", dims, K, ckpt)
+    X, Y, Z = dims
+    pair = phantoms.blobs(shape=(Z, Y, X), seed=5, scale=2.0)
+    v0 = phantoms.smooth_velocity(dims, 5, amp=0.5, offset=0.05, sigma=4.0)
+    a = _adjoint(dims, pair, K, ckpt)
+    try:
+        e = a.forward(v0, W_REG, TAU, lw)
+        g, gw, gt = a.backward()
+    finally:
+        a.close()
+    say("adjoint done")
+    ref = _autograd(dims, pair, v0, K, lw, device="cuda", checkpoint=True)
+    torch.cuda.synchronize()
+    _compare((e, g, gw, gt), ref, 1e-9)
 
 
 def test_gradient_matches_central_finite_differences():
