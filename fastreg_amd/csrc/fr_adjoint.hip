@@ -16,12 +16,13 @@
 // Objective of the final state: E = 0.5 sum (T - S o psi)^2 + lw * 0.5 sum_l sum (1[LT = l] -
 // trilerp(1[LS = l], psi))^2 over labels l = 1..NO_LABELS-1 (a label-overlap surrogate).
 //
-// Backward: one step B9..B1 per forward step in reverse order, each the transpose of its step's
-// derivative; the transposes of the gathers (trilerp, stencils) scatter with FP64 atomics.  The
-// derivative of trilerp with respect to its position is that of the piecewise-trilinear function
-// (zero where the position is clamped); the upwind / min / max branches are differentiated
-// within the branch the forward took.  Checkpoints of the state every `ckpt` iterations; the
-// backward pass recomputes each segment's states from its checkpoint.
+// Backward: the transposes of F9..F1 in reverse order, fused into 8 kernels per iteration, each
+// written as a gather (every thread writes only its own voxel); only sources displaced by a voxel
+// or more fall back to FP64 atomic scatters.  The derivative of trilerp with respect to its
+// position is that of the piecewise-trilinear function (zero where the position is clamped);
+// the upwind / min / max branches are differentiated within the branch the forward took.
+// Checkpoints of the state every `ckpt` iterations; the backward pass recomputes each segment's
+// states and intermediates from its checkpoint, then runs the segment's adjoint.
 
 #include "../../include/fastreg_c.h"
 #include "fr_internal.hpp"
@@ -80,6 +81,7 @@ __device__ __forceinline__ double lerp(double v0, double v1, double t) { return 
 struct Corner {
     long g[8];  // linear indices of the 8 corners, order 111, 110, 101, 100, 011, 010, 001, 000
     double a, b, c;
+    int gx, gy, gz;  // the floor corner
     bool inx, iny, inz;
 };
 __device__ __forceinline__ Corner corner(Dims d, double px, double py, double pz) {
@@ -87,6 +89,9 @@ __device__ __forceinline__ Corner corner(Dims d, double px, double py, double pz
     const double cx = fmin(fmax(0.0, px), (double)(d.X - 1)), cy = fmin(fmax(0.0, py), (double)(d.Y - 1)),
                  cz = fmin(fmax(0.0, pz), (double)(d.Z - 1));
     const int gx = (int)floor(cx), gy = (int)floor(cy), gz = (int)floor(cz);
+    k.gx = gx;
+    k.gy = gy;
+    k.gz = gz;
     k.a = cx - gx;
     k.b = cy - gy;
     k.c = cz - gz;
@@ -427,67 +432,123 @@ __global__ void b_objective(Dims d, const double *T, const double *S, const int1
     }
 }
 
-// B9: psi_inv1 = psi_inv - alpha J v1: gpinv = gpinv1 (all), gJ = -alpha gpinv1 (x) v1 (0 off the interior),
-// gv1 += -alpha J^T gpinv1 (interior)
-__global__ void b_psi_inv(Dims d, const double *J, const double *v1, const double *gpinv1, double *gpinv, double *gJ,
-                          double *gv1, double alpha) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        const D3 g = ld(gpinv1, i);
-        st(gpinv, i, g);
-        if (interior(d, x, y, z)) {
-            const D3 w = ld(v1, i);
-            const double *m = J + 9 * i;
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) gJ[9 * i + 3 * r + c] = -alpha * comp(g, r) * comp(w, c);
-            const D3 t = d3(-alpha * (m[0] * g.x + m[3] * g.y + m[6] * g.z), -alpha * (m[1] * g.x + m[4] * g.y + m[7] * g.z),
-                            -alpha * (m[2] * g.x + m[5] * g.y + m[8] * g.z));
-            gv1[3 * i] += t.x;
-            gv1[3 * i + 1] += t.y;
-            gv1[3 * i + 2] += t.z;
-        } else {
-            for (int k = 0; k < 9; ++k) gJ[9 * i + k] = 0.0;
-        }
-    }
+// Every transpose below is a gather: voxel i sums, over the voxels j whose forward stencil reads
+// i, the partial of j's output with respect to i (j's branch re-evaluated from the forward state),
+// so each thread writes only its own voxel and the result does not depend on thread order.  The
+// transpose of a trilinear sample at a moved position (a scatter to the 8 corners of the
+// position's cell) takes two steps: each source voxel writes a record (its cell relative to
+// itself, the cell fractions, the adjoint to spread), then each target voxel gathers from the 27
+// sources around it whose cell it is a corner of.  A source whose cell is not within one voxel of
+// it (a displacement of a voxel or more) is marked irregular and scattered with FP64 atomics by a
+// separate pass that runs before the gather.
+
+constexpr int RC_VALID = 1, RC_IRREG = 2;  // record code bits; bits 2, 3, 4: cell offset + 1 along x, y, z
+
+__device__ __forceinline__ int rec_code(const Corner &k, int x, int y, int z) {
+    const int ox = k.gx - x, oy = k.gy - y, oz = k.gz - z;
+    if (ox < -1 || ox > 0 || oy < -1 || oy > 0 || oz < -1 || oz > 0) return RC_VALID | RC_IRREG;
+    return RC_VALID | ((ox + 1) << 2) | ((oy + 1) << 3) | ((oz + 1) << 4);
+}
+__device__ __forceinline__ void put_rec(int *code, double *abc, double *gw, long i, int c, const Corner &k, D3 g) {
+    code[i] = c;
+    st(abc, i, d3(k.a, k.b, k.c));
+    st(gw, i, g);
 }
 
-// B8: psi1 = psi + alpha trilerp(v1, psi): gpsi = gpsi1 + alpha (d trilerp / d pos)^T gpsi1,
-// gv1(corners) += alpha w gpsi1 (interior)
-__global__ void b_psi(Dims d, const double *psi, const double *v1, const double *gpsi1, double *gpsi, double *gv1,
-                      double alpha) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        const D3 g = ld(gpsi1, i);
-        D3 o = g;
-        if (interior(d, x, y, z)) {
-            const D3 p = ld(psi, i);
-            const Corner k = corner(d, p.x, p.y, p.z);
-            double w[8], f[8];
-            tri_weights(k.a, k.b, k.c, w);
-            for (int j = 0; j < 8; ++j) add_at(gv1, k.g[j], d3(alpha * w[j] * g.x, alpha * w[j] * g.y, alpha * w[j] * g.z));
-            for (int q = 0; q < 3; ++q) {
-                gather3(v1, k, q, f);
-                const D3 dp = tri_dpos(f, k);
-                const double gq = alpha * comp(g, q);
-                o = d3(o.x + gq * dp.x, o.y + gq * dp.y, o.z + gq * dp.z);
+// sum over the regular sources within one voxel of target (x, y, z) of weight x adjoint
+__device__ __forceinline__ D3 gather_cells(Dims d, int x, int y, int z, const int *code, const double *abc,
+                                           const double *gw) {
+    D3 acc = d3(0, 0, 0);
+    for (int sz = -1; sz <= 1; ++sz) {
+        const int zz = z + sz;
+        if (zz < 0 || zz >= d.Z) continue;
+        for (int sy = -1; sy <= 1; ++sy) {
+            const int yy = y + sy;
+            if (yy < 0 || yy >= d.Y) continue;
+            for (int sx = -1; sx <= 1; ++sx) {
+                const int xx = x + sx;
+                if (xx < 0 || xx >= d.X) continue;
+                const long i = lin(d, xx, yy, zz);
+                const int c = code[i];
+                if ((c & (RC_VALID | RC_IRREG)) != RC_VALID) continue;
+                // the target is the source's (x, y, z) + t; corner sides: t == o (floor), t == o + 1
+                const int ox = ((c >> 2) & 1) - 1, oy = ((c >> 3) & 1) - 1, oz = ((c >> 4) & 1) - 1;
+                const int tx = -sx, ty = -sy, tz = -sz;
+                if ((tx != ox && tx != ox + 1) || (ty != oy && ty != oy + 1) || (tz != oz && tz != oz + 1)) continue;
+                const D3 f = ld(abc, i), g = ld(gw, i);
+                const double wa = tx == ox ? 1 - f.x : f.x, wb = ty == oy ? 1 - f.y : f.y, wc = tz == oz ? 1 - f.z : f.z;
+                const double w = wa * wb * wc;
+                acc = d3(acc.x + w * g.x, acc.y + w * g.y, acc.z + w * g.z);
             }
         }
-        st(gpsi, i, o);
+    }
+    return acc;
+}
+
+// the irregular sources' scatter: out(corners of pos(i)) += weights x gw(i)
+__global__ void b_scatter_irregular(Dims d, const int *code, const double *pos, const double *gw, double *out) {
+    FOR_VOXELS(d, i) {
+        if ((code[i] & RC_IRREG) == 0) continue;
+        const D3 p = ld(pos, i), g = ld(gw, i);
+        const Corner k = corner(d, p.x, p.y, p.z);
+        double w[8];
+        tri_weights(k.a, k.b, k.c, w);
+        for (int j = 0; j < 8; ++j) add_at(out, k.g[j], d3(w[j] * g.x, w[j] * g.y, w[j] * g.z));
     }
 }
 
-// B7: v1 = v - alpha (m1t v + dvv - tau dlv + q gU): gv (boundary gv1; interior (1 - alpha m1t) gv1),
-// gdvv = -alpha gv1, gdlv = alpha tau gv1, ggU = -alpha q gv1, grho1 += -alpha (gv1 . gU) dq/drho1,
-// the tau adjoint summed into *gtau (per block, atomically)
-__global__ __launch_bounds__(256) void b_velocity(Dims d, const double *rho1, const double *dlv, const double *gU,
-                                                  const double *gv1, double *gv, double *gdvv, double *gdlv, double *ggU,
-                                                  double *grho1, double alpha, double m1t, double tau, double m2,
-                                                  double *gtau) {
+// Step 1 (adjoints of F9 and F8, own voxel), interior:
+//   psi_inv1 = psi_inv - alpha J v1:         gJ = -alpha gpinv1 (x) v1, gv1 += -alpha J^T gpinv1
+//   psi1 = psi + alpha trilerp(v1, psi):      gpsi += alpha (d trilerp / d pos)^T gpsi1,
+//                                              record (cell of psi, alpha gpsi1) for the corners of v1
+// gpinv, gpsi, gv1 are updated in place (off the interior they pass through); gJ is written.
+__global__ void b_step1(Dims d, const double *J, const double *v1, const double *psi, double *gpinv, double *gJ,
+                        double *gv1, double *gpsi, int *code, double *abc, double *gw, double alpha) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        if (!interior(d, x, y, z)) {
+            for (int k = 0; k < 9; ++k) gJ[9 * i + k] = 0.0;
+            code[i] = 0;
+            continue;
+        }
+        const D3 g9 = ld(gpinv, i), w = ld(v1, i);
+        const double *m = J + 9 * i;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) gJ[9 * i + 3 * r + c] = -alpha * comp(g9, r) * comp(w, c);
+        const D3 gv = ld(gv1, i);
+        st(gv1, i, d3(gv.x - alpha * (m[0] * g9.x + m[3] * g9.y + m[6] * g9.z),
+                      gv.y - alpha * (m[1] * g9.x + m[4] * g9.y + m[7] * g9.z),
+                      gv.z - alpha * (m[2] * g9.x + m[5] * g9.y + m[8] * g9.z)));
+        const D3 g8 = ld(gpsi, i), p = ld(psi, i);
+        const Corner k = corner(d, p.x, p.y, p.z);
+        double f[8];
+        D3 o = g8;
+        for (int q = 0; q < 3; ++q) {
+            gather3(v1, k, q, f);
+            const D3 dp = tri_dpos(f, k);
+            const double gq = alpha * comp(g8, q);
+            o = d3(o.x + gq * dp.x, o.y + gq * dp.y, o.z + gq * dp.z);
+        }
+        st(gpsi, i, o);
+        put_rec(code, abc, gw, i, rec_code(k, x, y, z), k, d3(alpha * g8.x, alpha * g8.y, alpha * g8.z));
+    }
+}
+
+// Step 3: gv1 completed by the records' gather, then the adjoint of F7
+//   v1 = v - alpha (m1t v + dvv - tau dlv + q gU), q = m2 / (|rho1| + 1e-5f)   (interior; v1 = v elsewhere):
+//   gv = (1 - alpha m1t) gv1, gdvv = -alpha gv1, gdlv = alpha tau gv1, ggU = -alpha q gv1,
+//   grho1 += -alpha (gv1 . gU) dq/drho1, dE/dtau += alpha gv1 . dlv (block sums, one atomic per block).
+// gv holds gv1 on entry and gv on exit.
+__global__ __launch_bounds__(256) void b_step3(Dims d, const int *code, const double *abc, const double *gw,
+                                               const double *rho1, const double *dlv, const double *gU, double *gv,
+                                               double *gdvv, double *gdlv, double *ggU, double *grho1, double alpha,
+                                               double m1t, double tau, double m2, double *gtau) {
     __shared__ double sh[4];
     double acc = 0;
     FOR_VOXELS(d, i) {
         XYZ(d, i);
-        const D3 g = ld(gv1, i);
+        const D3 own = ld(gv, i), sc = gather_cells(d, x, y, z, code, abc, gw);
+        const D3 g = d3(own.x + sc.x, own.y + sc.y, own.z + sc.z);
         if (interior(d, x, y, z)) {
             const double den = fabs(rho1[i]) + (double)1e-5f, q = m2 / den;
             const D3 u = ld(gU, i), l = ld(dlv, i);
@@ -510,191 +571,239 @@ __global__ __launch_bounds__(256) void b_velocity(Dims d, const double *rho1, co
     if (threadIdx.x == 0) aadd(gtau, s);
 }
 
-// B6: transposes of dvv = (Dv)v and dlv (upwind Laplacian), scattered into gv
-__global__ void b_vel_terms(Dims d, const double *v, const double *gdvv, const double *gdlv, double *gv) {
+// voxel (x, y, z) in the range of the velocity Laplacian (x in [2, X-3], y in [2, Y-2], z in [2, Z-3])
+__device__ __forceinline__ bool in_lap(Dims d, int x, int y, int z) {
+    return x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2 && z >= 2 && z <= d.Z - 3;
+}
+
+// Step 4: transposes of F6 gathered into gv: dvv = (Dv)v (interior) and dlv, the velocity
+// Laplacian (its range; centre 3 times, "far" neighbours (+-2) once, "near" ones (+-1) -2 times,
+// chosen by the sign of the centre's component, with y-1 / z-1 as the near neighbour always).
+__global__ void b_step4(Dims d, const double *v, const double *gdvv, const double *gdlv, double *gv) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
+        const long sy = d.X, sz = (long)d.X * d.Y;
+        const D3 vi = ld(v, i);
+        D3 a = d3(0, 0, 0);
+        // (Dv)v, voxel i as the centre
         if (interior(d, x, y, z)) {
-            const long ixp = lin(d, x + 1, y, z), ixm = lin(d, x - 1, y, z), iyp = lin(d, x, y + 1, z),
-                       iym = lin(d, x, y - 1, z), izp = lin(d, x, y, z + 1), izm = lin(d, x, y, z - 1);
-            const D3 c = ld(v, i), xp = ld(v, ixp), xm = ld(v, ixm), yp = ld(v, iyp), ym = ld(v, iym), zp = ld(v, izp),
-                     zm = ld(v, izm);
-            const D3 G = ld(gdvv, i);
-            // dvv.x = 0.5 ax1 + ax2 + ax3; dvv.y = ay1 + 0.5 ay2 + ay3; dvv.z = az1 + az2 + 0.5 az3
-            D3 gc = d3(0, 0, 0), gxp = d3(0, 0, 0), gxm = d3(0, 0, 0), gyp = d3(0, 0, 0), gym = d3(0, 0, 0),
-               gzp = d3(0, 0, 0), gzm = d3(0, 0, 0);
-            // ax1 (x-component of c and xp), weight 0.5
-            gc.x += 0.5 * G.x * (2 * mx(c.x) - 2 * mn(c.x));
-            gxp.x += 0.5 * G.x * (2 * mn(xp.x) - 2 * mx(xp.x));
-            // ax2 = c.y>0 ? c.y (c.x - ym.x) : c.y (yp.x - c.x)
-            if (c.y > 0) { gc.y += G.x * (c.x - ym.x); gc.x += G.x * c.y; gym.x -= G.x * c.y; }
-            else { gc.y += G.x * (yp.x - c.x); gyp.x += G.x * c.y; gc.x -= G.x * c.y; }
-            if (c.z > 0) { gc.z += G.x * (c.x - zm.x); gc.x += G.x * c.z; gzm.x -= G.x * c.z; }
-            else { gc.z += G.x * (zp.x - c.x); gzp.x += G.x * c.z; gc.x -= G.x * c.z; }
-            // y-component
-            if (c.x > 0) { gc.x += G.y * (c.y - xm.y); gc.y += G.y * c.x; gxm.y -= G.y * c.x; }
-            else { gc.x += G.y * (xp.y - c.y); gxp.y += G.y * c.x; gc.y -= G.y * c.x; }
-            gc.y += 0.5 * G.y * (2 * mx(c.y) - 2 * mn(c.y));
-            gyp.y += 0.5 * G.y * (2 * mn(yp.y) - 2 * mx(yp.y));
-            if (c.z > 0) { gc.z += G.y * (c.y - zm.y); gc.y += G.y * c.z; gzm.y -= G.y * c.z; }
-            else { gc.z += G.y * (zp.y - c.y); gzp.y += G.y * c.z; gc.y -= G.y * c.z; }
-            // z-component
-            if (c.x > 0) { gc.x += G.z * (c.z - xm.z); gc.z += G.z * c.x; gxm.z -= G.z * c.x; }
-            else { gc.x += G.z * (xp.z - c.z); gxp.z += G.z * c.x; gc.z -= G.z * c.x; }
-            if (c.y > 0) { gc.y += G.z * (c.z - ym.z); gc.z += G.z * c.y; gym.z -= G.z * c.y; }
-            else { gc.y += G.z * (yp.z - c.z); gyp.z += G.z * c.y; gc.z -= G.z * c.y; }
-            gc.z += 0.5 * G.z * (2 * mx(c.z) - 2 * mn(c.z));
-            gzp.z += 0.5 * G.z * (2 * mn(zp.z) - 2 * mx(zp.z));
-            add_at(gv, i, gc);
-            add_at(gv, ixp, gxp);
-            add_at(gv, ixm, gxm);
-            add_at(gv, iyp, gyp);
-            add_at(gv, iym, gym);
-            add_at(gv, izp, gzp);
-            add_at(gv, izm, gzm);
+            const D3 c = vi, xp = ld(v, i + 1), xm = ld(v, i - 1), yp = ld(v, i + sy), ym = ld(v, i - sy),
+                     zp = ld(v, i + sz), zm = ld(v, i - sz), G = ld(gdvv, i);
+            a.x += 0.5 * G.x * (2 * mx(c.x) - 2 * mn(c.x));
+            if (c.y > 0) { a.y += G.x * (c.x - ym.x); a.x += G.x * c.y; }
+            else { a.y += G.x * (yp.x - c.x); a.x -= G.x * c.y; }
+            if (c.z > 0) { a.z += G.x * (c.x - zm.x); a.x += G.x * c.z; }
+            else { a.z += G.x * (zp.x - c.x); a.x -= G.x * c.z; }
+            if (c.x > 0) { a.x += G.y * (c.y - xm.y); a.y += G.y * c.x; }
+            else { a.x += G.y * (xp.y - c.y); a.y -= G.y * c.x; }
+            a.y += 0.5 * G.y * (2 * mx(c.y) - 2 * mn(c.y));
+            if (c.z > 0) { a.z += G.y * (c.y - zm.y); a.y += G.y * c.z; }
+            else { a.z += G.y * (zp.y - c.y); a.y -= G.y * c.z; }
+            if (c.x > 0) { a.x += G.z * (c.z - xm.z); a.z += G.z * c.x; }
+            else { a.x += G.z * (xp.z - c.z); a.z -= G.z * c.x; }
+            if (c.y > 0) { a.y += G.z * (c.z - ym.z); a.z += G.z * c.y; }
+            else { a.y += G.z * (yp.z - c.z); a.z -= G.z * c.y; }
+            a.z += 0.5 * G.z * (2 * mx(c.z) - 2 * mn(c.z));
         }
-        if (x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2 && z >= 2 && z <= d.Z - 3) {
-            const D3 c = ld(v, i), g = ld(gdlv, i);
-            const long sx = 1, sy = d.X, sz = (long)d.X * d.Y;
-            const long ax = c.x < 0 ? i + 2 * sx : i - 2 * sx, bx = c.x < 0 ? i + sx : i - sx;
-            const long ay = c.y < 0 ? i + 2 * sy : i - 2 * sy, by = i - sy;
-            const long az = c.z < 0 ? i + 2 * sz : i - 2 * sz, bz = i - sz;
-            // dlv = -(vxx + vyy + vzz), each of the form A - 2 B + C with the centre once per axis
-            const D3 mg = d3(-g.x, -g.y, -g.z);
-            add_at(gv, i, d3(3 * mg.x, 3 * mg.y, 3 * mg.z));
-            add_at(gv, ax, mg);
-            add_at(gv, ay, mg);
-            add_at(gv, az, mg);
-            add_at(gv, bx, d3(-2 * mg.x, -2 * mg.y, -2 * mg.z));
-            add_at(gv, by, d3(-2 * mg.x, -2 * mg.y, -2 * mg.z));
-            add_at(gv, bz, d3(-2 * mg.x, -2 * mg.y, -2 * mg.z));
+        // voxel i as the x+1 / x-1 / y+1 / y-1 / z+1 / z-1 neighbour of an interior voxel j
+        if (interior(d, x - 1, y, z)) {
+            const long j = i - 1;
+            const D3 cj = ld(v, j), G = ld(gdvv, j);
+            a.x += 0.5 * G.x * (2 * mn(vi.x) - 2 * mx(vi.x));
+            if (!(cj.x > 0)) { a.y += G.y * cj.x; a.z += G.z * cj.x; }
         }
+        if (interior(d, x + 1, y, z)) {
+            const long j = i + 1;
+            const D3 cj = ld(v, j), G = ld(gdvv, j);
+            if (cj.x > 0) { a.y -= G.y * cj.x; a.z -= G.z * cj.x; }
+        }
+        if (interior(d, x, y - 1, z)) {
+            const long j = i - sy;
+            const D3 cj = ld(v, j), G = ld(gdvv, j);
+            if (!(cj.y > 0)) { a.x += G.x * cj.y; a.z += G.z * cj.y; }
+            a.y += 0.5 * G.y * (2 * mn(vi.y) - 2 * mx(vi.y));
+        }
+        if (interior(d, x, y + 1, z)) {
+            const long j = i + sy;
+            const D3 cj = ld(v, j), G = ld(gdvv, j);
+            if (cj.y > 0) { a.x -= G.x * cj.y; a.z -= G.z * cj.y; }
+        }
+        if (interior(d, x, y, z - 1)) {
+            const long j = i - sz;
+            const D3 cj = ld(v, j), G = ld(gdvv, j);
+            if (!(cj.z > 0)) { a.x += G.x * cj.z; a.y += G.y * cj.z; }
+            a.z += 0.5 * G.z * (2 * mn(vi.z) - 2 * mx(vi.z));
+        }
+        if (interior(d, x, y, z + 1)) {
+            const long j = i + sz;
+            const D3 cj = ld(v, j), G = ld(gdvv, j);
+            if (cj.z > 0) { a.x -= G.x * cj.z; a.y -= G.y * cj.z; }
+        }
+        // the velocity Laplacian: dlv(j) = -(sum over axes of far - 2 near + centre)
+        const auto add = [&](long j, double f) {
+            const D3 g = ld(gdlv, j);
+            a = d3(a.x + f * g.x, a.y + f * g.y, a.z + f * g.z);
+        };
+        if (in_lap(d, x, y, z)) add(i, -3.0);
+        if (in_lap(d, x - 2, y, z) && v[3 * (i - 2)] < 0) add(i - 2, -1.0);
+        if (in_lap(d, x + 2, y, z) && !(v[3 * (i + 2)] < 0)) add(i + 2, -1.0);
+        if (in_lap(d, x - 1, y, z) && v[3 * (i - 1)] < 0) add(i - 1, 2.0);
+        if (in_lap(d, x + 1, y, z) && !(v[3 * (i + 1)] < 0)) add(i + 1, 2.0);
+        // far y neighbour: j = i - 2 sy by linear address (for y = 0 that is (x, Y-2, z-1))
+        if (y >= 2 ? in_lap(d, x, y - 2, z) : (y == 0 && in_lap(d, x, d.Y - 2, z - 1)))
+            if (v[3 * (i - 2 * sy) + 1] < 0) add(i - 2 * sy, -1.0);
+        if (in_lap(d, x, y + 2, z) && !(v[3 * (i + 2 * sy) + 1] < 0)) add(i + 2 * sy, -1.0);
+        if (in_lap(d, x, y + 1, z)) add(i + sy, 2.0);
+        if (in_lap(d, x, y, z - 2) && v[3 * (i - 2 * sz) + 2] < 0) add(i - 2 * sz, -1.0);
+        if (in_lap(d, x, y, z + 2) && !(v[3 * (i + 2 * sz) + 2] < 0)) add(i + 2 * sz, -1.0);
+        if (in_lap(d, x, y, z + 1)) add(i + sz, 2.0);
+        const D3 o = ld(gv, i);
+        st(gv, i, d3(o.x + a.x, o.y + a.y, o.z + a.z));
     }
 }
 
-// B5: gU = ((S - Tp) gS + w Lo) det: gLo = w det ggU, gTp = -det (ggU . gS), gJ += (ggU . (diff gS + w Lo)) ddet/dJ,
-// the w_reg adjoint summed into *gw
-__global__ __launch_bounds__(256) void b_potential(Dims d, const double *S, const double *gS, const double *Tp,
-                                                   const double *Lo, const double *J, const double *ggU, double *gLo,
-                                                   double *gTp, double *gJ, double w_reg, double *gw) {
+// Step 5 (own voxel, interior): the adjoint of F5, gU = ((S - Tp) gS + w Lo) det_ref(J):
+//   gLo = w det ggU, gTp = -det (ggU . gS), gJ += (ggU . ((S - Tp) gS + w Lo)) d det_ref / dJ,
+//   dE/dw_reg += det (ggU . Lo) (block sums, one atomic per block);
+// then of F4 at this voxel's position psi_inv: gpinv += (d trilerp(L) / d pos)^T gLo +
+// (d trilerp(T) / d pos)^T gTp, and the record (cell of psi_inv, gLo) for the corners of L.
+// gL is cleared here (the irregular scatter and the gather fill it).
+__global__ __launch_bounds__(256) void b_step5(Dims d, const double *S, const double *gS, const double *T,
+                                               const double *Tp, const double *Lo, const double *L, const double *J,
+                                               const double *pinv, const double *ggU, double *gJ, double *gpinv,
+                                               double *gL, int *code, double *abc, double *gw, double w_reg,
+                                               double *gwsum) {
     __shared__ double sh[4];
     double acc = 0;
     FOR_VOXELS(d, i) {
         XYZ(d, i);
-        if (interior(d, x, y, z)) {
-            double m[3][3], dg[3][3];
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) m[r][c] = J[9 * i + 3 * r + c];
-            const double det = det_ref(m), diff = S[i] - Tp[i];
-            const D3 g = ld(ggU, i), s = ld(gS, i), lo = ld(Lo, i);
-            const double gdet = g.x * (diff * s.x + w_reg * lo.x) + g.y * (diff * s.y + w_reg * lo.y) +
-                                g.z * (diff * s.z + w_reg * lo.z);
-            st(gLo, i, d3(w_reg * det * g.x, w_reg * det * g.y, w_reg * det * g.z));
-            gTp[i] = -det * (g.x * s.x + g.y * s.y + g.z * s.z);
-            det_ref_grad(m, dg);
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) gJ[9 * i + 3 * r + c] += gdet * dg[r][c];
-            acc += det * (g.x * lo.x + g.y * lo.y + g.z * lo.z);
-        } else {
-            st(gLo, i, d3(0, 0, 0));
-            gTp[i] = 0;
+        st(gL, i, d3(0, 0, 0));
+        if (!interior(d, x, y, z)) {
+            code[i] = 0;
+            continue;
         }
-    }
-    const double s = block_sum(acc, sh);
-    if (threadIdx.x == 0) aadd(gw, s);
-}
-
-// B4: Lo = trilerp(L, psi_inv), Tp = trilerp(T, psi_inv): gL(corners) += w gLo, gpinv += (d/dpos)^T (gLo, gTp)
-__global__ void b_interp(Dims d, const double *L, const double *T, const double *pinv, const double *gLo,
-                         const double *gTp, double *gL, double *gpinv) {
-    FOR_VOXELS(d, i) {
-        const D3 p = ld(pinv, i), g = ld(gLo, i);
-        const double gt = gTp[i];
-        if (g.x == 0 && g.y == 0 && g.z == 0 && gt == 0) continue;
+        double m[3][3], dg[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) m[r][c] = J[9 * i + 3 * r + c];
+        const double det = det_ref(m), diff = S[i] - Tp[i];
+        const D3 g = ld(ggU, i), s = ld(gS, i), lo = ld(Lo, i);
+        const double gdet = g.x * (diff * s.x + w_reg * lo.x) + g.y * (diff * s.y + w_reg * lo.y) +
+                            g.z * (diff * s.z + w_reg * lo.z);
+        const D3 glo = d3(w_reg * det * g.x, w_reg * det * g.y, w_reg * det * g.z);
+        const double gtp = -det * (g.x * s.x + g.y * s.y + g.z * s.z);
+        det_ref_grad(m, dg);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) gJ[9 * i + 3 * r + c] += gdet * dg[r][c];
+        acc += det * (g.x * lo.x + g.y * lo.y + g.z * lo.z);
+        const D3 p = ld(pinv, i);
         const Corner k = corner(d, p.x, p.y, p.z);
-        double w[8], f[8];
-        tri_weights(k.a, k.b, k.c, w);
-        for (int j = 0; j < 8; ++j) add_at(gL, k.g[j], d3(w[j] * g.x, w[j] * g.y, w[j] * g.z));
+        double f[8];
         D3 o = d3(0, 0, 0);
         for (int q = 0; q < 3; ++q) {
             gather3(L, k, q, f);
             const D3 dp = tri_dpos(f, k);
-            const double gq = comp(g, q);
+            const double gq = comp(glo, q);
             o = d3(o.x + gq * dp.x, o.y + gq * dp.y, o.z + gq * dp.z);
         }
         gather1(T, k, f);
         const D3 dt = tri_dpos(f, k);
-        o = d3(o.x + gt * dt.x, o.y + gt * dt.y, o.z + gt * dt.z);
-        gpinv[3 * i] += o.x;
-        gpinv[3 * i + 1] += o.y;
-        gpinv[3 * i + 2] += o.z;
+        o = d3(o.x + gtp * dt.x, o.y + gtp * dt.y, o.z + gtp * dt.z);
+        const D3 gp = ld(gpinv, i);
+        st(gpinv, i, d3(gp.x + o.x, gp.y + o.y, gp.z + o.z));
+        put_rec(code, abc, gw, i, rec_code(k, x, y, z), k, glo);
+    }
+    const double sm = block_sum(acc, sh);
+    if (threadIdx.x == 0) aadd(gwsum, sm);
+}
+
+// Step 7: gL += the records' gather (the transpose of Lo = trilerp(L, psi_inv))
+__global__ void b_step7(Dims d, const int *code, const double *abc, const double *gw, double *gL) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        const D3 o = ld(gL, i), g = gather_cells(d, x, y, z, code, abc, gw);
+        st(gL, i, d3(o.x + g.x, o.y + g.y, o.z + g.z));
     }
 }
 
-// B3: L = 6 psi - sum of the 6 neighbours (interior): gpsi(p) += 6 gL(p), gpsi(nb) -= gL(p)
-__global__ void b_laplacian(Dims d, const double *gL, double *gpsi) {
+// Step 8: transposes of F3, F2 and F1 gathered into voxel i:
+//   L(j) = 6 psi(j) - (sum of its 6 neighbours), interior j:       gpsi(i)
+//   J(j)[r][a] = (psi_inv(j + e_a) - psi_inv(j - e_a))[r] / 2:      gpinv(i)
+//   rho1(j) = rho(j) + alpha div(rho, v)(j) (upwind fluxes):        grho(i) = grho1(i) + ..., gv(i)
+__global__ void b_step8(Dims d, const double *gL, const double *gJ, const double *rho, const double *v,
+                        const double *grho1, double *gpsi, double *gpinv, double *grho, double *gv, double alpha) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
-        if (!interior(d, x, y, z)) continue;
-        const D3 g = ld(gL, i);
-        add_at(gpsi, i, d3(6 * g.x, 6 * g.y, 6 * g.z));
-        const D3 m = d3(-g.x, -g.y, -g.z);
-        add_at(gpsi, lin(d, x + 1, y, z), m);
-        add_at(gpsi, lin(d, x - 1, y, z), m);
-        add_at(gpsi, lin(d, x, y + 1, z), m);
-        add_at(gpsi, lin(d, x, y - 1, z), m);
-        add_at(gpsi, lin(d, x, y, z + 1), m);
-        add_at(gpsi, lin(d, x, y, z - 1), m);
-    }
-}
-
-// B2: J = D psi_inv (interior): gpinv(x+-1)[r] += +-0.5 gJ[r][0], and y, z alike
-__global__ void b_jacobian(Dims d, const double *gJ, double *gpinv) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        if (!interior(d, x, y, z)) continue;
-        const double *g = gJ + 9 * i;
-        const D3 cx = d3(0.5 * g[0], 0.5 * g[3], 0.5 * g[6]), cy = d3(0.5 * g[1], 0.5 * g[4], 0.5 * g[7]),
-                 cz = d3(0.5 * g[2], 0.5 * g[5], 0.5 * g[8]);
-        add_at(gpinv, lin(d, x + 1, y, z), cx);
-        add_at(gpinv, lin(d, x - 1, y, z), d3(-cx.x, -cx.y, -cx.z));
-        add_at(gpinv, lin(d, x, y + 1, z), cy);
-        add_at(gpinv, lin(d, x, y - 1, z), d3(-cy.x, -cy.y, -cy.z));
-        add_at(gpinv, lin(d, x, y, z + 1), cz);
-        add_at(gpinv, lin(d, x, y, z - 1), d3(-cz.x, -cz.y, -cz.z));
-    }
-}
-
-// B1: rho1 = rho + alpha div(rho, v) (interior): grho += grho1 (all) + alpha (d div/d rho)^T grho1,
-// gv += alpha (d div / d v)^T grho1
-__global__ void b_density(Dims d, const double *rho, const double *v, const double *grho1, double *grho, double *gv,
-                          double alpha) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        aadd(&grho[i], grho1[i]);
-        if (!interior(d, x, y, z)) continue;
-        const double g = alpha * grho1[i];
-        if (g == 0) continue;
-        const D3 vel = ld(v, i);
-        const long ixp = lin(d, x + 1, y, z), ixm = lin(d, x - 1, y, z), iyp = lin(d, x, y + 1, z),
-                   iym = lin(d, x, y - 1, z), izp = lin(d, x, y, z + 1), izm = lin(d, x, y, z - 1);
-        const double p = rho[i];
-        // t11 = vel.x > 0 ? -vel.x p : -vel.x rho(x+1)
-        if (vel.x > 0) { aadd(&gv[3 * i], -g * p); aadd(&grho[i], -g * vel.x); }
-        else { aadd(&gv[3 * i], -g * rho[ixp]); aadd(&grho[ixp], -g * vel.x); }
-        const double vxm = v[3 * ixm];
-        if (vxm > 0) { aadd(&gv[3 * ixm], g * rho[ixm]); aadd(&grho[ixm], g * vxm); }
-        else { aadd(&gv[3 * ixm], g * p); aadd(&grho[i], g * vxm); }
-        if (vel.y > 0) { aadd(&gv[3 * i + 1], -g * p); aadd(&grho[i], -g * vel.y); }
-        else { aadd(&gv[3 * i + 1], -g * rho[iyp]); aadd(&grho[iyp], -g * vel.y); }
-        const double vym = v[3 * iym + 1];
-        if (vym > 0) { aadd(&gv[3 * iym + 1], g * rho[iym]); aadd(&grho[iym], g * vym); }
-        else { aadd(&gv[3 * iym + 1], g * p); aadd(&grho[i], g * vym); }
-        if (vel.z > 0) { aadd(&gv[3 * i + 2], -g * p); aadd(&grho[i], -g * vel.z); }
-        else { aadd(&gv[3 * i + 2], -g * rho[izp]); aadd(&grho[izp], -g * vel.z); }
-        const double vzm = v[3 * izm + 2];
-        if (vzm > 0) { aadd(&gv[3 * izm + 2], g * rho[izm]); aadd(&grho[izm], g * vzm); }
-        else { aadd(&gv[3 * izm + 2], g * p); aadd(&grho[i], g * vzm); }
+        const long sy = d.X, sz = (long)d.X * d.Y;
+        const long nb[6] = {i + 1, i - 1, i + sy, i - sy, i + sz, i - sz};
+        const bool nin[6] = {interior(d, x + 1, y, z), interior(d, x - 1, y, z), interior(d, x, y + 1, z),
+                             interior(d, x, y - 1, z), interior(d, x, y, z + 1), interior(d, x, y, z - 1)};
+        const bool in = interior(d, x, y, z);
+        // F3
+        D3 gp = ld(gpsi, i);
+        if (in) {
+            const D3 g = ld(gL, i);
+            gp = d3(gp.x + 6 * g.x, gp.y + 6 * g.y, gp.z + 6 * g.z);
+        }
+        for (int k = 0; k < 6; ++k)
+            if (nin[k]) {
+                const D3 g = ld(gL, nb[k]);
+                gp = d3(gp.x - g.x, gp.y - g.y, gp.z - g.z);
+            }
+        st(gpsi, i, gp);
+        // F2: i = j - e_a (j = nb[2a]) gives -1/2, i = j + e_a (j = nb[2a + 1]) gives +1/2 of gJ(j)[.][a]
+        D3 gq = ld(gpinv, i);
+        for (int ax = 0; ax < 3; ++ax) {
+            if (nin[2 * ax]) {
+                const double *g = gJ + 9 * nb[2 * ax];
+                gq = d3(gq.x - 0.5 * g[ax], gq.y - 0.5 * g[3 + ax], gq.z - 0.5 * g[6 + ax]);
+            }
+            if (nin[2 * ax + 1]) {
+                const double *g = gJ + 9 * nb[2 * ax + 1];
+                gq = d3(gq.x + 0.5 * g[ax], gq.y + 0.5 * g[3 + ax], gq.z + 0.5 * g[6 + ax]);
+            }
+        }
+        st(gpinv, i, gq);
+        // F1: the fluxes t_a1(j) = v(j).a > 0 ? -v(j).a rho(j) : -v(j).a rho(j + e_a) and
+        // t_a2(j) = w > 0 ? w rho(j - e_a) : w rho(j), w = v(j - e_a).a
+        double gr = grho1[i];
+        D3 ga = d3(0, 0, 0);
+        const D3 vi = ld(v, i);
+        const double ri = rho[i];
+        const long step[3] = {1, sy, sz};
+        for (int ax = 0; ax < 3; ++ax) {
+            const double via = comp(vi, ax);
+            double gax = 0;
+            if (in) {  // j = i
+                const double g = alpha * grho1[i];
+                if (via > 0) {
+                    gr += -g * via;
+                    gax += -g * ri;
+                } else {
+                    gax += -g * rho[i + step[ax]];
+                }
+                const double w = v[3 * (i - step[ax]) + ax];
+                if (!(w > 0)) gr += g * w;
+            }
+            if (nin[2 * ax + 1]) {  // j = i - e_a: i is j + e_a in t_a1(j)
+                const long j = i - step[ax];
+                const double g = alpha * grho1[j], vj = v[3 * j + ax];
+                if (!(vj > 0)) gr += -g * vj;
+            }
+            if (nin[2 * ax]) {  // j = i + e_a: i is j - e_a in t_a2(j), w = v(i).a
+                const long j = i + step[ax];
+                const double g = alpha * grho1[j];
+                if (via > 0) {
+                    gr += g * via;
+                    gax += g * ri;
+                } else {
+                    gax += g * rho[j];
+                }
+            }
+            if (ax == 0) ga.x = gax; else if (ax == 1) ga.y = gax; else ga.z = gax;
+        }
+        grho[i] = gr;
+        const D3 o = ld(gv, i);
+        st(gv, i, d3(o.x + ga.x, o.y + ga.y, o.z + ga.z));
     }
 }
 
@@ -799,8 +908,10 @@ struct fr_adjoint {
     // adjoint of the state (grho double-buffered: its kernel scatters into the older iteration's)
     double *grho[2] = {nullptr, nullptr}, *gv = nullptr, *gpsi = nullptr, *gpinv = nullptr;
     // adjoints of one iteration's intermediates
-    double *gJ = nullptr, *gL = nullptr, *gLo = nullptr, *gTp = nullptr, *ggU = nullptr, *gdvv = nullptr,
-           *gdlv = nullptr;
+    double *gJ = nullptr, *gL = nullptr, *ggU = nullptr, *gdvv = nullptr, *gdlv = nullptr;
+    // records of the trilinear transposes: per source voxel a code, the cell fractions, the adjoint
+    int *code = nullptr;
+    double *abc = nullptr, *gw = nullptr;
     double *scal = nullptr;  // [0] objective, [1] dE/dw_reg, [2] dE/dtau
     double *partials = nullptr;
     int grid = 0;
@@ -899,21 +1010,17 @@ int backward_iteration(fr_adjoint *h, const State &in, const State &nx, const In
     const int g = h->grid;
     hipStream_t s = h->stream;
     const double a = h->alpha;
-    const size_t n = (size_t)h->n;
     double *gr1 = h->grho[*cur], *gr0 = h->grho[1 - *cur];
-    frd::b_psi_inv<<<g, TPB, 0, s>>>(d, it.J, nx.v, h->gpinv, h->gpinv, h->gJ, h->gv, a);
-    frd::b_psi<<<g, TPB, 0, s>>>(d, in.psi, nx.v, h->gpsi, h->gpsi, h->gv, a);
-    frd::b_velocity<<<g, TPB, 0, s>>>(d, nx.rho, it.dlv, it.gU, h->gv, h->gv, h->gdvv, h->gdlv, h->ggU, gr1, a,
-                                      h->m1t[k - 1], h->tau, h->m2[k - 1], h->scal + 2);
-    frd::b_vel_terms<<<g, TPB, 0, s>>>(d, in.v, h->gdvv, h->gdlv, h->gv);
-    frd::b_potential<<<g, TPB, 0, s>>>(d, h->S, h->gS, it.Tp, it.Lo, it.J, h->ggU, h->gLo, h->gTp, h->gJ, h->w_reg,
-                                       h->scal + 1);
-    ADJ_TRY(hipMemsetAsync(h->gL, 0, 3 * n * sizeof(double), s));
-    frd::b_interp<<<g, TPB, 0, s>>>(d, it.L, h->T, in.pinv, h->gLo, h->gTp, h->gL, h->gpinv);
-    frd::b_laplacian<<<g, TPB, 0, s>>>(d, h->gL, h->gpsi);
-    frd::b_jacobian<<<g, TPB, 0, s>>>(d, h->gJ, h->gpinv);
-    ADJ_TRY(hipMemsetAsync(gr0, 0, n * sizeof(double), s));
-    frd::b_density<<<g, TPB, 0, s>>>(d, in.rho, in.v, gr1, gr0, h->gv, a);
+    frd::b_step1<<<g, TPB, 0, s>>>(d, it.J, nx.v, in.psi, h->gpinv, h->gJ, h->gv, h->gpsi, h->code, h->abc, h->gw, a);
+    frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.psi, h->gw, h->gv);
+    frd::b_step3<<<g, TPB, 0, s>>>(d, h->code, h->abc, h->gw, nx.rho, it.dlv, it.gU, h->gv, h->gdvv, h->gdlv, h->ggU,
+                                   gr1, a, h->m1t[k - 1], h->tau, h->m2[k - 1], h->scal + 2);
+    frd::b_step4<<<g, TPB, 0, s>>>(d, in.v, h->gdvv, h->gdlv, h->gv);
+    frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.Tp, it.Lo, it.L, it.J, in.pinv, h->ggU, h->gJ, h->gpinv,
+                                   h->gL, h->code, h->abc, h->gw, h->w_reg, h->scal + 1);
+    frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.pinv, h->gw, h->gL);
+    frd::b_step7<<<g, TPB, 0, s>>>(d, h->code, h->abc, h->gw, h->gL);
+    frd::b_step8<<<g, TPB, 0, s>>>(d, h->gL, h->gJ, in.rho, in.v, gr1, h->gpsi, h->gpinv, gr0, h->gv, a);
     ADJ_TRY(hipGetLastError());
     *cur = 1 - *cur;
     return FR_OK;
@@ -980,7 +1087,8 @@ extern "C" int fr_adjoint_create(const fr_params *p, int K, int ckpt, fr_adjoint
         if ((rc = h->alloc_inter(&it))) return fail(rc);
     if ((rc = h->alloc(&h->grho[0], n)) || (rc = h->alloc(&h->grho[1], n)) || (rc = h->alloc(&h->gv, 3 * n)) ||
         (rc = h->alloc(&h->gpsi, 3 * n)) || (rc = h->alloc(&h->gpinv, 3 * n)) || (rc = h->alloc(&h->gJ, 9 * n)) ||
-        (rc = h->alloc(&h->gL, 3 * n)) || (rc = h->alloc(&h->gLo, 3 * n)) || (rc = h->alloc(&h->gTp, n)) ||
+        (rc = h->alloc(&h->gL, 3 * n)) || (rc = h->alloc(&h->code, n)) || (rc = h->alloc(&h->abc, 3 * n)) ||
+        (rc = h->alloc(&h->gw, 3 * n)) ||
         (rc = h->alloc(&h->ggU, 3 * n)) || (rc = h->alloc(&h->gdvv, 3 * n)) || (rc = h->alloc(&h->gdlv, 3 * n)) ||
         (rc = h->alloc(&h->scal, 4)) || (rc = h->alloc(&h->partials, (size_t)h->grid)))
         return fail(rc);

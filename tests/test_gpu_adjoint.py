@@ -136,6 +136,30 @@ def test_gradient_matches_autograd_at_depth_64():
     _compare((e, g, gw, gt), _autograd(dims, pair, v0, K, lw), 1e-9)
 
 
+def test_gradient_with_displacements_of_several_voxels():
+    """A large v0 moves psi and psi_inv by more than a voxel from the first iteration on, so that the
+    transposes of the trilinear samples take their atomic path for the displaced sources."""
+    dims, K, lw = (20, 18, 16), 3, 0.5
+    say("adjoint vs autograd, large displacements", dims, K)
+    n = dims[0] * dims[1] * dims[2]
+    pair = _pair(dims, 13)
+    v0 = phantoms.smooth_velocity(dims, 13, amp=200.0, offset=20.0)
+    a = _adjoint(dims, pair, K, 1)
+    try:
+        e = a.forward(v0, W_REG, TAU, lw)
+        g, gw, gt = a.backward()
+        idx = np.arange(n)
+        ident = np.stack([idx % dims[0], (idx // dims[0]) % dims[1], idx // (dims[0] * dims[1])], 1)
+        for k in (1, 2):
+            st = a.state(k)
+            say("state", k, "max displacement psi", np.abs(st["psi"] - ident).max(),
+                "psi_inv", np.abs(st["psi_inv"] - ident).max())
+            assert np.abs(st["psi"] - ident).max() > 1.0 and np.abs(st["psi_inv"] - ident).max() > 1.0
+    finally:
+        a.close()
+    _compare((e, g, gw, gt), _autograd(dims, pair, v0, K, lw), 1e-9)
+
+
 def test_full_size_gradient_matches_autograd_on_the_gpu():
     """The workload itself: the 128^3 blob pair, K = 64, checkpoints every 8, against torch autograd
     of the unrolled restatement run on the GPU in FP64 (each iteration recomputed in its backward
