@@ -392,8 +392,16 @@ template <int ST> struct PlaneLoads {
     F3 p_yp, p_ym;                   // psi_inv at y+-1
     float s_yp, s_ym;                // S at y+-1
     Corner k;                        // corner and weights of psi_inv(p)
-    F3 l111, l110, l101, l100, l011, l010, l001, l000;   // L taps
-    float t111, t110, t101, t100, t011, t010, t001, t000; // T taps
+};
+
+// L and T of the planes around the one being computed, staged in LDS for the whole block: a ring
+// of 4 planes (z-1, z, z+1 are read while z+2 is written), 10 rows (the block's 8 rows and one on
+// either side), 64 columns (the lanes of a wave), (L.x, L.y, L.z, T) per voxel.  The 8 taps of a
+// voxel whose psi_inv lies within one voxel of it are all in the stage, so the trilinear taps
+// become LDS reads and the block loads each plane of L and T once, in whole rows.
+constexpr int LT_ROWS = TY_A + 2, LT_PLANES = 4;
+struct LTStage {
+    float4 v[LT_PLANES][LT_ROWS][WAVE];
 };
 
 struct PassAView {
@@ -401,9 +409,55 @@ struct PassAView {
     VRBuf rvr, wvr;       // (v, rho): old (loads), new (stores)
     Rsrc rp, rs, rt, rl;  // loads: psi_inv (old), S, T, L
     Rsrc wp;              // stores: psi_inv (new)
-    int X;
+    int X, XY;
     unsigned X12, XY12, X4, XY4;
+    int xb, yb;           // the stage's first column (x of lane 0) and first row (the block's y - 1)
+    LTStage *lt;
 };
+
+// one row of L and T, loaded by a wave for the stage (its lanes' columns; positions outside the
+// grid read other voxels of the fields or, past the buffers, 0: finite values that no in-stage
+// tap of a stored voxel uses)
+struct LTRow {
+    F3 l;
+    float t;
+};
+__device__ __forceinline__ LTRow lt_load(const PassAView &w, int x, int y, int zz) {
+    const int i = x + w.X * y + w.XY * zz;
+    LTRow r;
+    r.l = bld3(w.rl, 12u * i);
+    r.t = bld1(w.rt, 4u * i);
+    return r;
+}
+__device__ __forceinline__ void lt_store(const PassAView &w, int zz, int row, int lane, const LTRow &r) {
+    w.lt->v[zz & (LT_PLANES - 1)][row][lane] = make_float4(r.l.x, r.l.y, r.l.z, r.t);
+}
+
+// L and T at psi_inv(p) (corner k) of the voxel at plane z.  The taps come from the stage; a lane
+// whose cell is not in it (a displacement of a voxel or more) and that stores its result takes the
+// global gathers instead, in a branch that waits for its own loads only.
+__device__ __forceinline__ void lt_at(const PassAView &w, const Corner &k, int z, bool store, F3 &Lo, float &Tv) {
+    int cx = k.gx - w.xb, ry = k.gy - w.yb;
+    const int dz = k.gz - z;
+    const bool in = cx >= 0 && cx <= WAVE - 2 && ry >= 0 && ry <= LT_ROWS - 2 && (dz == -1 || dz == 0);
+    cx = min(max(cx, 0), WAVE - 2);
+    ry = min(max(ry, 0), LT_ROWS - 2);
+    const int zg = dz == -1 ? z - 1 : z;
+    const int s0 = zg & (LT_PLANES - 1), s1 = (zg + 1) & (LT_PLANES - 1);
+    const LTStage &st = *w.lt;
+    const float4 f000 = st.v[s0][ry][cx], f001 = st.v[s1][ry][cx], f010 = st.v[s0][ry + 1][cx],
+                 f011 = st.v[s1][ry + 1][cx], f100 = st.v[s0][ry][cx + 1], f101 = st.v[s1][ry][cx + 1],
+                 f110 = st.v[s0][ry + 1][cx + 1], f111 = st.v[s1][ry + 1][cx + 1];
+    Lo = f3(tri(f111.x, f110.x, f101.x, f100.x, f011.x, f010.x, f001.x, f000.x, k.a, k.b, k.c),
+            tri(f111.y, f110.y, f101.y, f100.y, f011.y, f010.y, f001.y, f000.y, k.a, k.b, k.c),
+            tri(f111.z, f110.z, f101.z, f100.z, f011.z, f010.z, f001.z, f000.z, k.a, k.b, k.c));
+    Tv = tri(f111.w, f110.w, f101.w, f100.w, f011.w, f010.w, f001.w, f000.w, k.a, k.b, k.c);
+    if (store && !in) {
+        const unsigned g = corner_index(w.d, k);
+        Lo = tri_at3(w.rl, 12u * g, 12u, w.X12, w.XY12, k);
+        Tv = tri_at1(w.rt, 4u * g, 4u, w.X4, w.XY4, k);
+    }
+}
 
 // issue the loads of the voxel at (linear index i) whose own v and psi_inv are vc and pc
 template <int ST>
@@ -417,24 +471,6 @@ __device__ __forceinline__ void fetch_plane(PlaneLoads<ST> &n, const PassAView &
     n.s_yp = bld1(w.rs, o4 + w.X4);
     n.s_ym = bld1(w.rs, o4 - w.X4);
     n.k = corner(w.d, pc.x, pc.y, pc.z);
-    const unsigned g = corner_index(w.d, n.k);
-    const unsigned g12 = 12u * g, g4 = 4u * g;
-    n.l111 = bld3(w.rl, g12 + 12u + w.X12 + w.XY12);
-    n.l110 = bld3(w.rl, g12 + 12u + w.X12);
-    n.l101 = bld3(w.rl, g12 + 12u + w.XY12);
-    n.l100 = bld3(w.rl, g12 + 12u);
-    n.l011 = bld3(w.rl, g12 + w.X12 + w.XY12);
-    n.l010 = bld3(w.rl, g12 + w.X12);
-    n.l001 = bld3(w.rl, g12 + w.XY12);
-    n.l000 = bld3(w.rl, g12);
-    n.t111 = bld1(w.rt, g4 + 4u + w.X4 + w.XY4);
-    n.t110 = bld1(w.rt, g4 + 4u + w.X4);
-    n.t101 = bld1(w.rt, g4 + 4u + w.XY4);
-    n.t100 = bld1(w.rt, g4 + 4u);
-    n.t011 = bld1(w.rt, g4 + w.X4 + w.XY4);
-    n.t010 = bld1(w.rt, g4 + w.X4);
-    n.t001 = bld1(w.rt, g4 + w.XY4);
-    n.t000 = bld1(w.rt, g4);
 }
 
 // own-column z-windows; rho and psi_inv run one plane further ahead than they are used, so that
@@ -448,7 +484,7 @@ struct ColumnWindow {
 // false on halo lanes and on the voxel past the chunk
 template <int ST>
 __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, const PlaneLoads<ST> &n,
-                                        const ColumnWindow &c, int i, bool lap, bool store) {
+                                        const ColumnWindow &c, int i, int z, bool lap, bool store) {
     // x-neighbours of rho, v, psi_inv, S from the adjacent lanes
     const float r_xp = lane_next(c.r_0), r_xm = lane_prev(c.r_0);
     const F3 v_xp = lane_next3(c.v_0), v_xm = lane_prev3(c.v_0);
@@ -481,11 +517,9 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const float det = det_ref(jx.x, jy.x, jz.x, jx.y, jy.y, jz.y, jx.z, jy.z, jz.z);
 
     // 3-4. L o psi_inv and T o psi_inv (vector_fields.cu:231-247, 81-99)
-    const Corner &k = n.k;
-    const F3 Lo = f3(tri(n.l111.x, n.l110.x, n.l101.x, n.l100.x, n.l011.x, n.l010.x, n.l001.x, n.l000.x, k.a, k.b, k.c),
-                     tri(n.l111.y, n.l110.y, n.l101.y, n.l100.y, n.l011.y, n.l010.y, n.l001.y, n.l000.y, k.a, k.b, k.c),
-                     tri(n.l111.z, n.l110.z, n.l101.z, n.l100.z, n.l011.z, n.l010.z, n.l001.z, n.l000.z, k.a, k.b, k.c));
-    const float Tv = tri(n.t111, n.t110, n.t101, n.t100, n.t011, n.t010, n.t001, n.t000, k.a, k.b, k.c);
+    F3 Lo;
+    float Tv;
+    lt_at(w, n.k, z, store, Lo, Tv);
 
     // 5. grad S and the potential gradient (vector_fields.cu:155-192, solver.cu:56-80)
     const F3 gS = f3((s_xp - s_xm) * 0.5f, (n.s_yp - n.s_ym) * 0.5f, (c.s_p1 - c.s_m1) * 0.5f);
@@ -617,11 +651,25 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     w.rl = rsrc(a.L, 12 * N);
     w.wp = rsrc(a.pinv_n, 12 * N);
     w.X = d.X;
+    w.XY = XY;
     w.X12 = 12u * d.X;
     w.XY12 = 12u * XY;
     w.X4 = 4u * d.X;
     w.XY4 = 4u * XY;
     const bool lap_xy = x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2;
+    // the L/T stage: this wave loads rows `wave` and (waves 0 and 1) `wave + 8` of each plane; the
+    // other waves load row 9 as well and write the same values, so that no wave branches
+    __shared__ LTStage lt_stage;
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    w.lt = &lt_stage;
+    w.xb = x - lane;
+    w.yb = y - wave - 1;
+    const int row0 = wave, row1 = min(wave + TY_A, LT_ROWS - 1);
+    for (int zz = tc.zs - 1; zz <= tc.zs + 1; ++zz) {
+        const LTRow r0 = lt_load(w, x, w.yb + row0, zz), r1 = lt_load(w, x, w.yb + row1, zz);
+        lt_store(w, zz, row0, lane, r0);
+        lt_store(w, zz, row1, lane, r1);
+    }
 
     int i = x + d.X * y + XY * tc.zs;
     ColumnWindow c;
@@ -649,8 +697,9 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     PlaneLoads<ST> n0, n1;
     fetch_plane<ST>(n0, w, i, c.v_0, c.p_0);
     // drain the prologue's loads, so that the loop's memory-counter waits are set by the loop's
-    // own steady state and not by the prologue's last loads
+    // own steady state and not by the prologue's last loads; the stage's first planes are written
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    __syncthreads();
     // Two steps per trip, alternating the staged plane loads between n0 and n1.  The trip has no
     // branch: the loads run ahead of the chunk unconditionally (range-checked buffer loads; the
     // values are never stored) and the odd step of the last trip drops its stores, so the memory
@@ -659,19 +708,29 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     for (int s = 0, z = tc.zs; s < nz; s += 2, z += 2, i += 2 * XY) {
         // (each half-step interleaves the next plane's loads with the current plane's arithmetic,
         // so that a wave keeps computing while its loads queue for the memory pipeline)
+        // (the stage's plane z+2 is loaded first and written into the ring at the end of the
+        // half-step; the LDS writes complete before the barrier that precedes their readers)
+        const LTRow a0 = lt_load(w, x, w.yb + row0, z + 2), a1 = lt_load(w, x, w.yb + row1, z + 2);
         const WindowIn<ST> in0 = window_in<ST>(w, i, XY);
         fetch_plane<ST>(n1, w, i + XY, c.v_p1, c.p_p1);
-        voxel_a<ST>(a, w, n0, c, i, lap_xy && z >= 2 && z <= d.Z - 3, own);
+        voxel_a<ST>(a, w, n0, c, i, z, lap_xy && z >= 2 && z <= d.Z - 3, own);
+        lt_store(w, z + 2, row0, lane, a0);
+        lt_store(w, z + 2, row1, lane, a1);
         slide<ST>(c, in0);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
+        const LTRow b0 = lt_load(w, x, w.yb + row0, z + 3), b1 = lt_load(w, x, w.yb + row1, z + 3);
         const WindowIn<ST> in1 = window_in<ST>(w, i + XY, XY);
         fetch_plane<ST>(n0, w, i + 2 * XY, c.v_p1, c.p_p1);
-        voxel_a<ST>(a, w, n1, c, i + XY, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, own && s + 1 < nz);
+        voxel_a<ST>(a, w, n1, c, i + XY, z + 1, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, own && s + 1 < nz);
+        lt_store(w, z + 3, row0, lane, b0);
+        lt_store(w, z + 3, row1, lane, b1);
         slide<ST>(c, in1);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
     }
 }
