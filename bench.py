@@ -47,8 +47,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=120,
                     help="oracle iterations for the CPU baseline sample (about 10-20 s on 16 host cores)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--zc", default="0,0,0",
-                    help="interior z-slices per block of pass L, pass A, pass B (0 = automatic)")
+    ap.add_argument("--zc", default="0,0",
+                    help="interior z-slices per block of pass A, pass B (0 = automatic)")
     ap.add_argument("--workload", choices=("plain", "batch", "precision", "adjoint"), default="plain",
                     help="plain: the single-pair headline metric (default); batch: 64 pairs of uneven sizes, "
                          "150 iterations each, batched form vs one pair at a time; precision: FP32, BF16 and "
@@ -129,14 +129,13 @@ def main():
 
     # per-kernel durations: HIP events between the launches of iterations run in sequence on the
     # context stream (the caches as the iteration loop leaves them)
-    ms_l, ms_a, ms_b = s.time_kernels(args.kernel_reps)
-    # algorithmic bytes per interior voxel: pass L reads psi 12 and writes L 12; pass A reads
-    # rho 4, v 12, psi_inv 12, S 4, L 12 and T 4 (gathered at psi_inv) and writes rho 4, v 12,
+    ms_a, ms_b = s.time_kernels(args.kernel_reps)
+    # algorithmic bytes per interior voxel: pass A reads rho 4, v 12, psi_inv 12, S 4, psi 12 (for
+    # -Laplacian(psi), formed in the kernel) and T 4 (sampled at psi_inv) and writes rho 4, v 12,
     # psi_inv 12; pass B reads psi 12 and v 12 (gathered at psi) and writes psi 12
-    bytes_l = 24 * n_int
     bytes_a = 76 * n_int
     bytes_b = 36 * n_int
-    kern = {"pass_l": (ms_l, bytes_l), "pass_a": (ms_a, bytes_a), "pass_b": (ms_b, bytes_b)}
+    kern = {"pass_a": (ms_a, bytes_a), "pass_b": (ms_b, bytes_b)}
     dom = max(kern, key=lambda k: kern[k][0])
     dms, dbytes = kern[dom]
     achieved = dbytes / (dms / 1e3) / 1e9
@@ -145,8 +144,8 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": prof.get(dom) if prof else None,
                 "traffic_source": os.path.relpath(PROFILE, ROOT) if prof else None,
-                "kernel_ms": {"pass_l": round(ms_l, 4), "pass_a": round(ms_a, 4), "pass_b": round(ms_b, 4)},
-                "alg_bytes_per_launch": {"pass_l": bytes_l, "pass_a": bytes_a, "pass_b": bytes_b},
+                "kernel_ms": {"pass_a": round(ms_a, 4), "pass_b": round(ms_b, 4)},
+                "alg_bytes_per_launch": {"pass_a": bytes_a, "pass_b": bytes_b},
                 "iteration_frac_88B": round(value * 88 / (HBM_PEAK_GBS * 1e9), 4)}
     fin = s.finalize(want_outputs=False)
     st2 = s.stats()
@@ -184,7 +183,7 @@ def main():
         "config": {"workload": "plain solver, 182x218x182 fp32, params.ini weights", "dims": list(dims),
                    "iterations": args.steps, "voxels": n, "dice_all": round(st2.dice_all, 6),
                    "min_det_true": st2.min_det_true, "wall_ms": round(wall_ms, 2),
-                   "z_chunk": {"pass_l": chunking[0], "pass_a": chunking[1], "pass_b": chunking[2]}},
+                   "z_chunk": {"pass_a": chunking[0], "pass_b": chunking[1]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -282,10 +281,10 @@ def precision_workload(args):
             s.init(big.target, big.target_labels, big.source, big.source_labels)
             s.run()
             st = s.stats()
-        kl, ka, kb = s.time_kernels(20)
+        ka, kb = s.time_kernels(20)
         speed[nm] = {"voxel_iterations_per_s": round(nb * 300 / (st.solver_ms / 1e3), 1),
                      "ms_per_iteration": round(st.solver_ms / 300, 4),
-                     "kernel_ms": {"pass_l": round(kl, 4), "pass_a": round(ka, 4), "pass_b": round(kb, 4)}}
+                     "kernel_ms": {"pass_a": round(ka, 4), "pass_b": round(kb, 4)}}
         s.close()
         print(f"[bench] {nm} at {bd}: {speed[nm]}", file=sys.stderr, flush=True)
 

@@ -85,7 +85,7 @@ def lib():
     sig("fr_destroy", vp)
     sig("fr_get_params", vp, P(FrParams))
     sig("fr_set_stream", vp, vp)
-    sig("fr_set_chunking", vp, i, i, i)
+    sig("fr_set_chunking", vp, i, i)
     sig("fr_set_storage", vp, i)
     sig("fr_get_storage", vp, P(i))
     sig("fr_batch_create", P(vp), i, P(vp))
@@ -94,7 +94,7 @@ def lib():
     sig("fr_batch_run", vp)
     sig("fr_batch_time", vp, P(f))
     sig("fr_batch_time_iterations", vp, i, P(f))
-    sig("fr_get_chunking", vp, P(i), P(i), P(i))
+    sig("fr_get_chunking", vp, P(i), P(i))
     sig("fr_register", vp, vp, vp, vp, vp)
     sig("fr_get_deformation", vp, vp)
     sig("fr_get_inverse_deformation", vp, vp)
@@ -110,7 +110,7 @@ def lib():
     sig("fr_solver_get_state", vp, vp, vp, vp, vp)
     sig("fr_solver_energy", vp, P(d), P(d))
     sig("fr_warp", i, i, i, vp, vp, vp, vp, vp, vp)
-    sig("fr_time_kernels", vp, i, P(f), P(f), P(f))
+    sig("fr_time_kernels", vp, i, P(f), P(f))
     sig("fr_get_inverse", i, i, i, vp, vp, vp)
     sig("fr_interpolate_gradient", i, i, i, vp, vp, vp, vp)
     sig("fr_voxel_max_energy", vp, P(f), P(ll))
@@ -394,22 +394,22 @@ class Solver:
         check(lib().fr_get_storage(self.h, C.byref(v)))
         return v.value
 
-    def set_chunking(self, zc_l=0, zc_a=0, zc_b=0):
-        """Interior z-slices per block of pass L, pass A, pass B (0 = the automatic choice)."""
-        check(lib().fr_set_chunking(self.h, zc_l, zc_a, zc_b))
+    def set_chunking(self, zc_a=0, zc_b=0):
+        """Interior z-slices per block of pass A and pass B (0 = the automatic choice)."""
+        check(lib().fr_set_chunking(self.h, zc_a, zc_b))
 
     def chunking(self):
-        """(zc_l, zc_a, zc_b) in use."""
-        v = [C.c_int() for _ in range(3)]
+        """(zc_a, zc_b) in use."""
+        v = [C.c_int() for _ in range(2)]
         check(lib().fr_get_chunking(self.h, *[C.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
     def time_kernels(self, reps=20):
-        """Average device ms per launch of (pass L, pass A, pass B), timed inside `reps`
-        iterations run in sequence."""
-        l, a, b = C.c_float(), C.c_float(), C.c_float()
-        check(lib().fr_time_kernels(self.h, reps, C.byref(l), C.byref(a), C.byref(b)))
-        return l.value, a.value, b.value
+        """Average device ms per launch of (pass A, pass B), timed inside `reps` iterations run
+        in sequence."""
+        a, b = C.c_float(), C.c_float()
+        check(lib().fr_time_kernels(self.h, reps, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
 
 class Batch:

@@ -65,12 +65,11 @@ struct fr_ctx {
     float *v[2] = {nullptr, nullptr};
     float *pinv[2] = {nullptr, nullptr};
     float *psi = nullptr;
-    float *L = nullptr;  // -Laplacian(psi), boundary kept at 0
     int st = frk::ST_F32;                  // storage of v and rho
     unsigned *vr[2] = {nullptr, nullptr};  // 16-bit (v, rho) records, 2 unsigned per voxel
     int cur = 0;
-    int zc[3] = {2, 8, 2};       // interior z-slices per block of pass L, pass A, pass B
-    int zc_auto[3] = {2, 8, 2};  // the choice made at create
+    int zc[3] = {0, 8, 2};       // interior z-slices per block of pass A ([1]) and pass B ([2])
+    int zc_auto[3] = {0, 8, 2};  // the choice made at create
     bool initialised = false;
     // loop control and history
     frk::Ctl *ctl = nullptr;
@@ -140,7 +139,7 @@ extern "C" int fr_params_default(fr_params *p) {
 static void free_ctx(fr_ctx *c) {
     if (!c) return;
     void *bufs[] = {c->T, c->S, c->LT, c->LS, c->rho[0], c->rho[1], c->v[0], c->v[1], c->pinv[0], c->pinv[1], c->psi,
-                    c->L, c->ctl, c->maxpack, c->energy, c->partials, c->dice_counts, c->det_parts, c->lab_tmp,
+                    c->ctl, c->maxpack, c->energy, c->partials, c->dice_counts, c->det_parts, c->lab_tmp,
                     c->vr[0], c->vr[1], c->max_slot};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -165,8 +164,8 @@ static int dalloc(T **p, size_t count) {
 // plane of staged loads in registers (two blocks per CU), so its chunks are sized for the whole
 // grid to be resident at once: every block then marches one long column chunk, the halo planes
 // a chunk re-reads (z-2..z+2 of v) are amortised over it and no second wave of blocks runs
-// behind a partly idle first one.  Pass L and pass B are light streaming kernels with many
-// blocks per CU; they use chunks of 2 slices (the fastest of 2..32 at 182x218x182 on MI355X).
+// behind a partly idle first one.  Pass B is a light streaming kernel with many blocks per CU;
+// it uses chunks of 2 slices (the fastest of 2..32 at 182x218x182 on MI355X).
 static int auto_chunking(fr_ctx *c) {
     int dev = 0, cus = 0, occ = 0;
     HIP_TRY(hipGetDevice(&dev));
@@ -179,7 +178,7 @@ static int auto_chunking(fr_ctx *c) {
     long long chunks = slots / tiles;
     if (chunks < 1) chunks = 1;
     if (chunks > interior) chunks = interior;
-    c->zc_auto[0] = interior < 2 ? interior : 2;
+    c->zc_auto[0] = 0;
     c->zc_auto[1] = (int)((interior + chunks - 1) / chunks);
     c->zc_auto[2] = interior < 2 ? interior : 2;
     for (int k = 0; k < 3; ++k) c->zc[k] = c->zc_auto[k];
@@ -202,7 +201,7 @@ extern "C" int fr_create(const fr_params *p, fr_ctx **out) {
     if ((rc = dalloc(&c->T, n)) || (rc = dalloc(&c->S, n)) || (rc = dalloc(&c->LT, n)) || (rc = dalloc(&c->LS, n)) ||
         (rc = dalloc(&c->rho[0], n)) || (rc = dalloc(&c->rho[1], n)) || (rc = dalloc(&c->v[0], 3 * n)) ||
         (rc = dalloc(&c->v[1], 3 * n)) || (rc = dalloc(&c->pinv[0], 3 * n)) || (rc = dalloc(&c->pinv[1], 3 * n)) ||
-        (rc = dalloc(&c->psi, 3 * n)) || (rc = dalloc(&c->L, 3 * n)) || (rc = dalloc(&c->ctl, 1)) || (rc = dalloc(&c->maxpack, (size_t)mi * frk::NSLOT)) ||
+        (rc = dalloc(&c->psi, 3 * n)) || (rc = dalloc(&c->ctl, 1)) || (rc = dalloc(&c->maxpack, (size_t)mi * frk::NSLOT)) ||
         (rc = dalloc(&c->energy, 3 * (size_t)mi)) || (rc = dalloc(&c->partials, 2 * (size_t)c->nparts)) ||
         (rc = dalloc(&c->dice_counts, 2 * (size_t)p->no_labels + 1)) || (rc = dalloc(&c->det_parts, 2 * (size_t)c->ndet)) ||
         (rc = dalloc(&c->lab_tmp, n)) || (rc = dalloc(&c->max_slot, 1))) {
@@ -245,18 +244,16 @@ extern "C" int fr_set_stream(fr_ctx *c, void *s) {
     return FR_OK;
 }
 
-extern "C" int fr_set_chunking(fr_ctx *c, int zc_l, int zc_a, int zc_b) {
+extern "C" int fr_set_chunking(fr_ctx *c, int zc_a, int zc_b) {
     if (!c) return fail(FR_ERR_INVALID, "null context");
-    const int in[3] = {zc_l, zc_a, zc_b};
-    for (int k = 0; k < 3; ++k)
-        if (in[k] < 0) return fail(FR_ERR_INVALID, "z-chunk sizes must be >= 0 (0 = automatic)");
-    for (int k = 0; k < 3; ++k) c->zc[k] = in[k] ? in[k] : c->zc_auto[k];
+    if (zc_a < 0 || zc_b < 0) return fail(FR_ERR_INVALID, "z-chunk sizes must be >= 0 (0 = automatic)");
+    c->zc[1] = zc_a ? zc_a : c->zc_auto[1];
+    c->zc[2] = zc_b ? zc_b : c->zc_auto[2];
     return FR_OK;
 }
 
-extern "C" int fr_get_chunking(const fr_ctx *c, int *zc_l, int *zc_a, int *zc_b) {
+extern "C" int fr_get_chunking(const fr_ctx *c, int *zc_a, int *zc_b) {
     if (!c) return fail(FR_ERR_INVALID, "null context");
-    if (zc_l) *zc_l = c->zc[0];
     if (zc_a) *zc_a = c->zc[1];
     if (zc_b) *zc_b = c->zc[2];
     return FR_OK;
@@ -292,7 +289,6 @@ extern "C" int fr_solver_init(fr_ctx *c, const float *tgt, const int16_t *tgt_la
     if (!c || !tgt || !tgt_lab || !src || !src_lab) return fail(FR_ERR_INVALID, "null argument");
     HIP_TRY(frk::launch_init(c->d, tgt, tgt_lab, src, src_lab, c->T, c->LT, c->S, c->LS, c->rho[0], c->rho[1], c->v[0],
                              c->v[1], c->pinv[0], c->pinv[1], c->psi, c->prm.rho_0, c->stream));
-    HIP_TRY(hipMemsetAsync(c->L, 0, (size_t)c->n * 3 * sizeof(float), c->stream));
     if (c->st != frk::ST_F32) HIP_TRY(frk::launch_vr_init(c->st, c->vr[0], c->vr[1], c->n, c->prm.rho_0, c->stream));
     c->cur = 0;
     c->initialised = true;
@@ -316,7 +312,6 @@ static void iter_args(fr_ctx *c, frk::IterArgs &a) {
     a.pinv_o = c->pinv[c->cur];
     a.pinv_n = c->pinv[c->cur ^ 1];
     a.psi = c->psi;
-    a.L = c->L;
     a.S = c->S;
     a.T = c->T;
     a.alpha = c->prm.alpha;
@@ -357,7 +352,6 @@ static int launch_iteration(fr_ctx *c, int iter, float t, bool with_energy) {
                                             e + 2 * mi + (iter - 1), e + (iter - 1), c->stream));
         a.check_energy = 1;
     }
-    HIP_TRY(frk::launch_pass_l(a, c->zc[0], c->stream));
     HIP_TRY(frk::launch_pass_a(a, c->zc[1], c->st, c->stream));
     HIP_TRY(frk::launch_pass_b(a, c->zc[2], c->st, c->stream));
     c->cur ^= 1;
@@ -716,7 +710,7 @@ extern "C" int fr_warp(int X, int Y, int Z, const float *img, const int16_t *lab
 
 struct fr_batch {
     std::vector<fr_ctx *> ctx;
-    frk::IterArgs *d_base = nullptr;  // 3 per volume: pass L, A, B arguments at buffer parity 0
+    frk::IterArgs *d_base = nullptr;  // 3 per volume: (unused), pass A, pass B arguments at buffer parity 0
     int *d_starts = nullptr;          // 3 x (n + 1): first block of each volume in each pass's grid
     int total[3] = {0, 0, 0};
     hipStream_t stream = nullptr;
@@ -744,7 +738,9 @@ static int batch_layout(fr_batch *b) {
     const int n = (int)b->ctx.size();
     std::vector<frk::IterArgs> base(3 * (size_t)n);
     std::vector<int> starts(3 * (size_t)(n + 1));
-    for (int pass = 0; pass < 3; ++pass) {
+    starts[(size_t)n] = 0;  // slot 0 (no pass) is empty
+    b->total[0] = 0;
+    for (int pass = 1; pass < 3; ++pass) {
         long long at = 0;
         for (int q = 0; q < n; ++q) {
             fr_ctx *c = b->ctx[q];
@@ -845,7 +841,7 @@ static int batch_launches(fr_batch *b, int par) {
                                                     e + 2 * mi + (iter - 1), e + (iter - 1), b->stream));
             }
         }
-        for (int pass = 0; pass < 3; ++pass)
+        for (int pass = 1; pass < 3; ++pass)
             HIP_TRY(frk::launch_pass_batch(pass, b->d_base, b->d_starts + (size_t)pass * (n + 1), n, b->total[pass], it,
                                            b->ctx[0]->st, b->stream));
         t += p.alpha;
@@ -928,7 +924,7 @@ extern "C" int fr_batch_time_iterations(fr_batch *b, int iters, float *ms) {
         it.check_energy = 0;
         it.no_stop = 1;
         vel_scalars(p, t, &it.m1t, &it.m2);
-        for (int pass = 0; pass < 3; ++pass)
+        for (int pass = 1; pass < 3; ++pass)
             HIP_TRY(frk::launch_pass_batch(pass, b->d_base, b->d_starts + (size_t)pass * (n + 1), n, b->total[pass], it,
                                            b->ctx[0]->st, b->stream));
         t += p.alpha;
@@ -1009,7 +1005,7 @@ extern "C" int fr_sum(const float *d, long long n, double *sum, void *stream) {
     return FR_OK;
 }
 
-extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_l, float *ms_a, float *ms_b) {
+extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_a, float *ms_b) {
     if (!c || reps < 1 || reps > 10000) return fail(FR_ERR_INVALID, "bad arguments");
     if (!c->initialised) return fail(FR_ERR_STATE, "fr_time_kernels before fr_solver_init");
     int rc;
@@ -1021,17 +1017,16 @@ extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_l, float *ms_a, fl
     iter_args(c, a);
     a.iter = 1;
     vel_scalars(c->prm, c->prm.alpha, &a.m1t, &a.m2);
-    // reps iterations in sequence (L, A, B, L, A, B, ...) with an event between consecutive
-    // launches, so each kernel runs with the caches as the iteration loop leaves them.  The
-    // iteration is not advanced: pass A re-reads the same old buffers, pass B accumulates into psi,
-    // which is restored afterwards.
-    std::vector<hipEvent_t> ev(3 * (size_t)reps + 1, nullptr);
+    // reps iterations in sequence (A, B, A, B, ...) with an event between consecutive launches, so
+    // each kernel runs with the caches as the iteration loop leaves them.  The iteration is not
+    // advanced: pass A re-reads the same old buffers, pass B accumulates into psi, which is
+    // restored afterwards.
+    std::vector<hipEvent_t> ev(2 * (size_t)reps + 1, nullptr);
     hipError_t e = hipSuccess;
     for (size_t k = 0; k < ev.size() && e == hipSuccess; ++k) e = hipEventCreate(&ev[k]);
     if (e == hipSuccess)
         e = hipMemcpyAsync(psi_save, c->psi, 3 * (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
-    if (e == hipSuccess) e = frk::launch_pass_l(a, c->zc[0], c->stream);  // warm-up iteration
-    if (e == hipSuccess) e = frk::launch_pass_a(a, c->zc[1], c->st, c->stream);
+    if (e == hipSuccess) e = frk::launch_pass_a(a, c->zc[1], c->st, c->stream);  // warm-up iteration
     if (e == hipSuccess) e = frk::launch_pass_b(a, c->zc[2], c->st, c->stream);
     // each repeat reduces its max norm into its own (cleared) iteration slots, as the loop does;
     // the stop tests are off
@@ -1040,21 +1035,19 @@ extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_l, float *ms_a, fl
     const int mi = c->prm.max_iter > 0 ? c->prm.max_iter : 1;
     for (int r = 0; r < reps && e == hipSuccess; ++r) {
         a.iter = 1 + (r + 1) % mi;
-        e = hipEventRecord(ev[3 * r], c->stream);
-        if (e == hipSuccess) e = frk::launch_pass_l(a, c->zc[0], c->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[3 * r + 1], c->stream);
+        e = hipEventRecord(ev[2 * r], c->stream);
         if (e == hipSuccess) e = frk::launch_pass_a(a, c->zc[1], c->st, c->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[3 * r + 2], c->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[2 * r + 1], c->stream);
         if (e == hipSuccess) e = frk::launch_pass_b(a, c->zc[2], c->st, c->stream);
     }
-    if (e == hipSuccess) e = hipEventRecord(ev[3 * (size_t)reps], c->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(ev[3 * (size_t)reps]);
+    if (e == hipSuccess) e = hipEventRecord(ev[2 * (size_t)reps], c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[2 * (size_t)reps]);
     double sum[3] = {0, 0, 0};
     for (int r = 0; r < reps && e == hipSuccess; ++r)
-        for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
             float ms = 0.f;
-            e = hipEventElapsedTime(&ms, ev[3 * r + k], ev[3 * r + k + 1]);
-            sum[k] += ms;
+            e = hipEventElapsedTime(&ms, ev[2 * r + k], ev[2 * r + k + 1]);
+            sum[k + 1] += ms;
         }
     if (e == hipSuccess)
         e = hipMemcpyAsync(c->psi, psi_save, 3 * (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
@@ -1063,7 +1056,6 @@ extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_l, float *ms_a, fl
         if (x) (void)hipEventDestroy(x);
     (void)hipFree(psi_save);
     if (e != hipSuccess) return fail(FR_ERR_HIP, "fr_time_kernels: %s", hipGetErrorString(e));
-    if (ms_l) *ms_l = (float)(sum[0] / reps);
     if (ms_a) *ms_a = (float)(sum[1] / reps);
     if (ms_b) *ms_b = (float)(sum[2] / reps);
     return FR_OK;

@@ -22,13 +22,12 @@ struct Ctl {
 
 // Tile of the iteration kernels: one 64-lane wave per x-row and TY_* rows per block, marching
 // over a chunk of z-slices.  Pass A runs 8 rows per block (one block of 8 waves per CU), so that
-// the rows a wave loads as y-neighbours and the region it gathers from are loaded into the CU's
-// L1 once for all eight waves.  A wave whose kernel takes x-neighbours from the adjacent lanes
-// (pass L: x+-1, pass A: x+-2) computes the XS_* interior lanes of its row and keeps the edge
-// lanes as a halo; consecutive tiles are XS_* columns apart.
+// the rows a wave loads as y-neighbours are loaded into the CU's L1 once for all eight waves and
+// the block can stage the region its trilinear taps read in LDS.  Pass A takes x-neighbours from
+// the adjacent lanes (x+-2), so a wave computes the XS_A interior lanes of its row and keeps the
+// edge lanes as a halo; consecutive tiles are XS_* columns apart.
 constexpr int WAVE = 64;
-constexpr int TY_L = 4, TY_A = 8, TY_B = 4;
-constexpr int XS_L = WAVE - 2;
+constexpr int TY_A = 8, TY_B = 4;
 constexpr int XS_A = WAVE - 4;
 constexpr int XS_B = WAVE;
 // Storage of velocity and density: FP32 (3 + 1 floats in two buffers), or one 8-byte record
@@ -52,7 +51,6 @@ struct IterArgs {
     const float *pinv_o;         // psi_inv xyz-packed, read
     float *pinv_n;               // psi_inv xyz-packed, written
     float *psi;                  // psi xyz-packed (pass B updates in place)
-    float *L;                    // -Laplacian(psi) xyz-packed (pass L writes the interior, boundary 0)
     const float *S, *T;          // source / target intensity
     float alpha, m1t, m2, tau, w_reg, thr;
     int iter;                    // 1-based iteration number
@@ -72,15 +70,14 @@ hipError_t launch_init(Dims d, const float *tgt, const int16_t *tgt_lab, const f
                        float *pinv0, float *pinv1, float *psi, float rho_0, hipStream_t s);
 // The iteration kernels over blocks of XS_* x TY_* columns by `zc` interior z-slices (the
 // launch sets a's tile-grid fields).
-hipError_t launch_pass_l(const IterArgs &a, int zc, hipStream_t s);
 // st: storage of v and rho (ST_F32, ST_BF16, ST_F16); in the 16-bit forms a.v_o / a.v_n point to
 // the 8-byte (v, rho) records and a.rho_o / a.rho_n are unused
 hipError_t launch_pass_a(const IterArgs &a, int zc, int st, hipStream_t s);
 hipError_t launch_pass_b(const IterArgs &a, int zc, int st, hipStream_t s);
-// resident blocks per CU of pass L (0), pass A (1) or pass B (2)
+// resident blocks per CU of pass A (1) or pass B (2)
 hipError_t pass_occupancy(int which, int *blocks_per_cu);
 
-// The arguments of one pass (0 = L, 1 = A, 2 = B) of a volume with its tile grid set for zc.
+// The arguments of one pass (1 = A, 2 = B) of a volume with its tile grid set for zc.
 IterArgs pass_args(const IterArgs &a, int pass, int zc);
 
 // Values of one iteration shared by every volume of a batch.
@@ -92,9 +89,9 @@ struct BatchIter {
     float m1t, m2;
 };
 
-// One pass over a batch of volumes: d_base holds 3 arguments per volume (pass L, A, B; parity 0,
-// grid set), d_starts the first block of each volume in this pass's grid (npairs + 1 entries,
-// multiples of 8), nblocks the grid size.
+// One pass (1 = A, 2 = B) over a batch of volumes: d_base holds 3 arguments per volume (slot 0
+// unused, then pass A, pass B; parity 0, grid set), d_starts the first block of each volume in this
+// pass's grid (npairs + 1 entries, multiples of 8), nblocks the grid size.
 hipError_t launch_pass_batch(int pass, const IterArgs *d_base, const int *d_starts, int npairs, int nblocks,
                              const BatchIter &it, int st, hipStream_t s);
 hipError_t launch_energy(Dims d, const float *S, const float *T, const float *psi, double *partials, int nparts,

@@ -1,18 +1,18 @@
 // HIP kernels of the fastreg registration solver for gfx950 (MI355X).
 //
-// One solver iteration is three launches over 64 x 4-column tiles, each thread marching a chunk
+// One solver iteration is two launches over tiles of 64-lane rows, each thread marching a chunk
 // of z-slices with its column's z-neighbours in registers:
-//   pass L  L = -Laplacian(psi) on the interior, into a buffer (boundary 0).
 //   pass A  (reads the old state, writes the new one into the other buffer of a ping-pong pair):
-//           upwind divergence and density update, J_inv = D psi_inv and the reference's det(),
-//           L and T interpolated at psi_inv(p), grad S, the potential gradient, (Dv)v, the upwind
-//           velocity Laplacian, the velocity update and the psi_inv update.
+//           L = -Laplacian(psi) and T on the block's neighbourhood, staged in LDS, interpolated at
+//           psi_inv(p); upwind divergence and density update, J_inv = D psi_inv and the
+//           reference's det(), grad S, the potential gradient, (Dv)v, the upwind velocity
+//           Laplacian, the velocity update and the psi_inv update.
 //   pass B  (after all of v_new exists): psi += alpha * v_new o psi, and the max update norm of the
 //           iteration, reduced in the wave, the block and one 64-bit atomic max per block into
 //           one of NSLOT per-iteration slots.
 // The reference runs the same iteration as 13 launches on 5 streams with 12 float4 fields and
 // two 64-byte Jacobians in HBM (src/fastreg/cuda/solver.cu:295-444); here the iterated state is
-// rho, v, psi, psi_inv (xyz-packed, 12 B per vector), plus L and the read-only S and T.
+// rho, v, psi, psi_inv (xyz-packed, 12 B per vector), plus the read-only S and T.
 //
 // Arithmetic follows the reference kernel by kernel in the same operation order.  The library
 // is compiled with -ffp-contract=off, so a multiply-add rounds twice unless written fmaf():
@@ -170,7 +170,7 @@ __device__ __forceinline__ bool should_stop(const IterArgs &a, bool record) {
 }
 
 // --------------------------------------------------------------------------------------------
-// pass L and pass A
+// pass A
 // --------------------------------------------------------------------------------------------
 
 // Buffer resources (raw, stride 0): loads at byte offsets >= num_records return 0.
@@ -253,34 +253,6 @@ __device__ __forceinline__ TileCoord tile_of(const IterArgs &a, int blk) {
     t.zs = 1 + bz * a.zc;
     t.ze = min(t.zs + a.zc, a.d.Z - 1);
     return t;
-}
-
-// pass L: L = -Laplacian(psi) on the interior (vector_fields.cu:275-300), into a buffer whose
-// boundary voxels stay 0.  One thread per (x, y) column, marching the chunk's z range with
-// psi(z-1), psi(z), psi(z+1) of the column in registers and psi(x+-1) from the adjacent lanes.
-__device__ __forceinline__ void pass_l(const IterArgs &a, int blk) {
-    if (__builtin_nontemporal_load(&a.ctl->stopped)) return;
-    const Dims d = a.d;
-    const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
-    const TileCoord tc = tile_of(a, blk);
-    if (tc.y > d.Y - 2) return;  // uniform over the wave
-    const Rsrc rq = rsrc(a.psi, 12 * N), wl = rsrc(a.L, 12 * N);
-    const unsigned sy = 12u * d.X, sz = 12u * XY;
-    unsigned o = 12u * (tc.x + d.X * tc.y + XY * tc.zs);
-    F3 zm = bld3(rq, o - sz), c = bld3(rq, o);
-    for (int z = tc.zs; z < tc.ze; ++z, o += sz) {
-        const F3 zp = bld3(rq, o + sz), yp = bld3(rq, o + sy), ym = bld3(rq, o - sy);
-        F3 s = mul3(-6.f, c);
-        s = add3(s, lane_next3(c));
-        s = add3(s, lane_prev3(c));
-        s = add3(s, yp);
-        s = add3(s, ym);
-        s = add3(s, zp);
-        s = add3(s, zm);
-        bst3(wl, tc.own ? o : NO_STORE, mul3(-1.f, s));
-        zm = c;
-        c = zp;
-    }
 }
 
 // Storage of velocity and density (ST_F32: v as 3 floats and rho as a float in separate buffers,
@@ -383,9 +355,9 @@ template <int ST> __device__ __forceinline__ void vr_store(const VRBuf &b, int i
 // pass A: steps 1-8 and 10 of one iteration per interior voxel.  One thread per (x, y) column,
 // marching the chunk's z range; the column's z-neighbours (v at z-2..z+2, rho and psi_inv at
 // z-1..z+2, S at z-1..z+1) slide through registers and the x-neighbours come from the adjacent
-// lanes.  The loads of plane z+1 (y-neighbours, and the L and T taps at psi_inv) are issued before
-// plane z is computed, so that their latency overlaps the arithmetic: software pipelining across
-// the z-march.
+// lanes.  The loads of plane z+1 (y-neighbours, the corner of psi_inv) and the stage's rows of the
+// planes ahead are issued before plane z is computed, so that their latency overlaps the
+// arithmetic: software pipelining across the z-march.
 template <int ST> struct PlaneLoads {
     VR<ST> vr_yp, vr_ym;             // (v, rho) at y+-1
     VR<ST> v_y2;                     // v at y+-2 on the side the upwind Laplacian reads
@@ -394,53 +366,96 @@ template <int ST> struct PlaneLoads {
     Corner k;                        // corner and weights of psi_inv(p)
 };
 
-// L and T of the planes around the one being computed, staged in LDS for the whole block: a ring
-// of 4 planes (z-1, z, z+1 are read while z+2 is written), 10 rows (the block's 8 rows and one on
-// either side), 64 columns (the lanes of a wave), (L.x, L.y, L.z, T) per voxel.  The 8 taps of a
-// voxel whose psi_inv lies within one voxel of it are all in the stage, so the trilinear taps
-// become LDS reads and the block loads each plane of L and T once, in whole rows.
+// L = -Laplacian(psi) and T on the planes around the one being computed, staged in LDS for the
+// whole block: a ring of 4 planes (z-1, z, z+1 are read while z+2 is written), 10 rows (the
+// block's 8 rows and one on either side), 64 columns (the lanes of a wave), (L.x, L.y, L.z, T)
+// per voxel.  L is not a field in memory: the block computes it in the stage, from psi staged the
+// same way (a ring of 4 planes of the 12 rows y-2 .. y+9 around the block), in the reference's
+// operation order (vector_fields.cu:275-300), 0 off the interior.  The 8 taps of a voxel whose
+// psi_inv lies within one voxel of it are all in the stage, so the trilinear taps are LDS reads
+// and the block loads each plane of psi and T once, in whole rows.
 constexpr int LT_ROWS = TY_A + 2, LT_PLANES = 4;
+constexpr int PS_ROWS = TY_A + 4, PS_PLANES = 4;
 struct LTStage {
     float4 v[LT_PLANES][LT_ROWS][WAVE];
+};
+struct PsiStage {
+    float4 v[PS_PLANES][PS_ROWS][WAVE];  // (x, y, z, unused): 16-byte entries for aligned LDS access
 };
 
 struct PassAView {
     Dims d;
     VRBuf rvr, wvr;       // (v, rho): old (loads), new (stores)
-    Rsrc rp, rs, rt, rl;  // loads: psi_inv (old), S, T, L
+    Rsrc rp, rs, rt, rq;  // loads: psi_inv (old), S, T, psi
     Rsrc wp;              // stores: psi_inv (new)
     int X, XY;
     unsigned X12, XY12, X4, XY4;
-    int xb, yb;           // the stage's first column (x of lane 0) and first row (the block's y - 1)
+    int xb, yb;           // the L/T stage's first column (x of lane 0) and first row (the block's y - 1)
     LTStage *lt;
+    PsiStage *ps;         // row j holds y = yb - 1 + j
 };
 
-// one row of L and T, loaded by a wave for the stage (its lanes' columns; positions outside the
-// grid read other voxels of the fields or, past the buffers, 0: finite values that no in-stage
-// tap of a stored voxel uses)
-struct LTRow {
-    F3 l;
-    float t;
-};
-__device__ __forceinline__ LTRow lt_load(const PassAView &w, int x, int y, int zz) {
-    const int i = x + w.X * y + w.XY * zz;
-    LTRow r;
-    r.l = bld3(w.rl, 12u * i);
-    r.t = bld1(w.rt, 4u * i);
-    return r;
+// Rows for the stages, loaded by a wave at its lanes' columns.  Positions outside the grid read
+// other voxels of the fields or, past the buffers, 0: finite values that no in-stage tap of a
+// stored voxel uses.
+__device__ __forceinline__ F3 psi_row_load(const PassAView &w, int x, int y, int zz) {
+    return bld3(w.rq, 12u * (x + w.X * y + w.XY * zz));
 }
-__device__ __forceinline__ void lt_store(const PassAView &w, int zz, int row, int lane, const LTRow &r) {
-    w.lt->v[zz & (LT_PLANES - 1)][row][lane] = make_float4(r.l.x, r.l.y, r.l.z, r.t);
+__device__ __forceinline__ float t_row_load(const PassAView &w, int x, int y, int zz) {
+    return bld1(w.rt, 4u * (x + w.X * y + w.XY * zz));
+}
+__device__ __forceinline__ void psi_store(const PassAView &w, int zz, int row, int lane, F3 p) {
+    w.ps->v[zz & (PS_PLANES - 1)][row][lane] = make_float4(p.x, p.y, p.z, 0.f);
+}
+__device__ __forceinline__ F3 ps_at(const PassAView &w, int zz, int row, int lane) {
+    const float4 q = w.ps->v[zz & (PS_PLANES - 1)][row][lane];
+    return f3(q.x, q.y, q.z);
+}
+
+// L of row r (y = yb + r) of plane zz from the psi stage (planes zz-1..zz+1, rows r..r+2) and the
+// x-neighbours of the adjacent lanes, with T of that row, into the L/T stage.  Every lane of the
+// wave runs it; the end lanes' L (x-neighbours missing) is never used.
+__device__ __forceinline__ void lt_compute(const PassAView &w, int zz, int r, int lane, float t) {
+    const F3 c = ps_at(w, zz, r + 1, lane), ym = ps_at(w, zz, r, lane), yp = ps_at(w, zz, r + 2, lane),
+             zm = ps_at(w, zz - 1, r + 1, lane), zp = ps_at(w, zz + 1, r + 1, lane);
+    F3 s = mul3(-6.f, c);
+    s = add3(s, lane_next3(c));
+    s = add3(s, lane_prev3(c));
+    s = add3(s, yp);
+    s = add3(s, ym);
+    s = add3(s, zp);
+    s = add3(s, zm);
+    const F3 L = mul3(-1.f, s);
+    const int x = w.xb + lane, y = w.yb + r;
+    const Dims d = w.d;
+    const bool in = x >= 1 && x <= d.X - 2 && y >= 1 && y <= d.Y - 2 && zz >= 1 && zz <= d.Z - 2;
+    w.lt->v[zz & (LT_PLANES - 1)][r][lane] = in ? make_float4(L.x, L.y, L.z, t) : make_float4(0.f, 0.f, 0.f, t);
+}
+
+// L at grid point (x, y, z) from psi in memory (0 off the interior and outside the grid)
+__device__ __forceinline__ F3 l_global(const PassAView &w, int x, int y, int z) {
+    const Dims d = w.d;
+    if (!(x >= 1 && x <= d.X - 2 && y >= 1 && y <= d.Y - 2 && z >= 1 && z <= d.Z - 2)) return f3(0.f, 0.f, 0.f);
+    const unsigned o = 12u * (x + w.X * y + w.XY * z);
+    F3 s = mul3(-6.f, bld3(w.rq, o));
+    s = add3(s, bld3(w.rq, o + 12u));
+    s = add3(s, bld3(w.rq, o - 12u));
+    s = add3(s, bld3(w.rq, o + w.X12));
+    s = add3(s, bld3(w.rq, o - w.X12));
+    s = add3(s, bld3(w.rq, o + w.XY12));
+    s = add3(s, bld3(w.rq, o - w.XY12));
+    return mul3(-1.f, s);
 }
 
 // L and T at psi_inv(p) (corner k) of the voxel at plane z.  The taps come from the stage; a lane
-// whose cell is not in it (a displacement of a voxel or more) and that stores its result takes the
-// global gathers instead, in a branch that waits for its own loads only.
+// whose cell is not in it (a displacement of a voxel or more) and that stores its result computes
+// them from memory instead, in a branch that waits for its own loads only.
 __device__ __forceinline__ void lt_at(const PassAView &w, const Corner &k, int z, bool store, F3 &Lo, float &Tv) {
     int cx = k.gx - w.xb, ry = k.gy - w.yb;
     const int dz = k.gz - z;
-    const bool in = cx >= 0 && cx <= WAVE - 2 && ry >= 0 && ry <= LT_ROWS - 2 && (dz == -1 || dz == 0);
-    cx = min(max(cx, 0), WAVE - 2);
+    // columns 1..62 of the stage hold valid L (the end lanes lack an x-neighbour)
+    const bool in = cx >= 1 && cx <= WAVE - 3 && ry >= 0 && ry <= LT_ROWS - 2 && (dz == -1 || dz == 0);
+    cx = min(max(cx, 1), WAVE - 3);
     ry = min(max(ry, 0), LT_ROWS - 2);
     const int zg = dz == -1 ? z - 1 : z;
     const int s0 = zg & (LT_PLANES - 1), s1 = (zg + 1) & (LT_PLANES - 1);
@@ -453,8 +468,16 @@ __device__ __forceinline__ void lt_at(const PassAView &w, const Corner &k, int z
             tri(f111.z, f110.z, f101.z, f100.z, f011.z, f010.z, f001.z, f000.z, k.a, k.b, k.c));
     Tv = tri(f111.w, f110.w, f101.w, f100.w, f011.w, f010.w, f001.w, f000.w, k.a, k.b, k.c);
     if (store && !in) {
+        // the +1 corners are not clamped: where the reference clamps one its weight is exactly 0,
+        // and its L here is 0 (outside the interior), a finite value
+        const F3 l111 = l_global(w, k.gx + 1, k.gy + 1, k.gz + 1), l110 = l_global(w, k.gx + 1, k.gy + 1, k.gz),
+                 l101 = l_global(w, k.gx + 1, k.gy, k.gz + 1), l100 = l_global(w, k.gx + 1, k.gy, k.gz),
+                 l011 = l_global(w, k.gx, k.gy + 1, k.gz + 1), l010 = l_global(w, k.gx, k.gy + 1, k.gz),
+                 l001 = l_global(w, k.gx, k.gy, k.gz + 1), l000 = l_global(w, k.gx, k.gy, k.gz);
+        Lo = f3(tri(l111.x, l110.x, l101.x, l100.x, l011.x, l010.x, l001.x, l000.x, k.a, k.b, k.c),
+                tri(l111.y, l110.y, l101.y, l100.y, l011.y, l010.y, l001.y, l000.y, k.a, k.b, k.c),
+                tri(l111.z, l110.z, l101.z, l100.z, l011.z, l010.z, l001.z, l000.z, k.a, k.b, k.c));
         const unsigned g = corner_index(w.d, k);
-        Lo = tri_at3(w.rl, 12u * g, 12u, w.X12, w.XY12, k);
         Tv = tri_at1(w.rt, 4u * g, 4u, w.X4, w.XY4, k);
     }
 }
@@ -648,7 +671,7 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     w.rp = rsrc(a.pinv_o, 12 * N);
     w.rs = rsrc(a.S, 4 * N);
     w.rt = rsrc(a.T, 4 * N);
-    w.rl = rsrc(a.L, 12 * N);
+    w.rq = rsrc(a.psi, 12 * N);
     w.wp = rsrc(a.pinv_n, 12 * N);
     w.X = d.X;
     w.XY = XY;
@@ -657,19 +680,47 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     w.X4 = 4u * d.X;
     w.XY4 = 4u * XY;
     const bool lap_xy = x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2;
-    // the L/T stage: this wave loads rows `wave` and (waves 0 and 1) `wave + 8` of each plane; the
-    // other waves load row 9 as well and write the same values, so that no wave branches
+    // The stages.  This wave loads psi rows `wave` and `wave + 8` of each plane (rows 8..11 come
+    // from waves 0..3; the other waves load row 11 as well) and computes L/T rows `wave` and
+    // `wave + 8` (rows 8 and 9 from waves 0 and 1; the others compute row 9 as well), writing the
+    // same values, so that no wave branches.
     __shared__ LTStage lt_stage;
+    __shared__ PsiStage psi_stage;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
     w.lt = &lt_stage;
+    w.ps = &psi_stage;
     w.xb = x - lane;
     w.yb = y - wave - 1;
     const int row0 = wave, row1 = min(wave + TY_A, LT_ROWS - 1);
-    for (int zz = tc.zs - 1; zz <= tc.zs + 1; ++zz) {
-        const LTRow r0 = lt_load(w, x, w.yb + row0, zz), r1 = lt_load(w, x, w.yb + row1, zz);
-        lt_store(w, zz, row0, lane, r0);
-        lt_store(w, zz, row1, lane, r1);
+    const int prow0 = wave, prow1 = min(wave + TY_A, PS_ROWS - 1);
+    const int ty0 = w.yb + row0, ty1 = w.yb + row1, py0 = w.yb - 1 + prow0, py1 = w.yb - 1 + prow1;
+    const int zs = tc.zs;
+    // prologue: psi planes zs-2..zs+1, L/T planes zs-1 and zs; psi planes zs+2 and zs+3 (into the
+    // slots of zs-2 and zs-1), L/T plane zs+1; T rows of plane zs+2 for the first step
+    for (int zz = zs - 2; zz <= zs + 1; ++zz) {
+        const F3 p0 = psi_row_load(w, x, py0, zz), p1 = psi_row_load(w, x, py1, zz);
+        psi_store(w, zz, prow0, lane, p0);
+        psi_store(w, zz, prow1, lane, p1);
     }
+    __syncthreads();
+    for (int zz = zs - 1; zz <= zs; ++zz) {
+        const float t0 = t_row_load(w, x, ty0, zz), t1 = t_row_load(w, x, ty1, zz);
+        lt_compute(w, zz, row0, lane, t0);
+        lt_compute(w, zz, row1, lane, t1);
+    }
+    __syncthreads();
+    for (int zz = zs + 2; zz <= zs + 3; ++zz) {
+        const F3 p0 = psi_row_load(w, x, py0, zz), p1 = psi_row_load(w, x, py1, zz);
+        psi_store(w, zz, prow0, lane, p0);
+        psi_store(w, zz, prow1, lane, p1);
+    }
+    __syncthreads();
+    {
+        const float t0 = t_row_load(w, x, ty0, zs + 1), t1 = t_row_load(w, x, ty1, zs + 1);
+        lt_compute(w, zs + 1, row0, lane, t0);
+        lt_compute(w, zs + 1, row1, lane, t1);
+    }
+    float tn0 = t_row_load(w, x, ty0, zs + 2), tn1 = t_row_load(w, x, ty1, zs + 2);
 
     int i = x + d.X * y + XY * tc.zs;
     ColumnWindow c;
@@ -707,26 +758,35 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     const int nz = tc.ze - tc.zs;
     for (int s = 0, z = tc.zs; s < nz; s += 2, z += 2, i += 2 * XY) {
         // (each half-step interleaves the next plane's loads with the current plane's arithmetic,
-        // so that a wave keeps computing while its loads queue for the memory pipeline)
-        // (the stage's plane z+2 is loaded first and written into the ring at the end of the
-        // half-step; the LDS writes complete before the barrier that precedes their readers)
-        const LTRow a0 = lt_load(w, x, w.yb + row0, z + 2), a1 = lt_load(w, x, w.yb + row1, z + 2);
+        // so that a wave keeps computing while its loads queue for the memory pipeline.  It first
+        // computes L/T of plane z+2 from psi planes z+1..z+3, then loads psi of plane z+4 (written
+        // into the ring at its end) and T of plane z+3 (used by the next half-step); the LDS writes
+        // complete before the barrier that precedes their readers.)
+        lt_compute(w, z + 2, row0, lane, tn0);
+        lt_compute(w, z + 2, row1, lane, tn1);
+        const F3 qa0 = psi_row_load(w, x, py0, z + 4), qa1 = psi_row_load(w, x, py1, z + 4);
+        tn0 = t_row_load(w, x, ty0, z + 3);
+        tn1 = t_row_load(w, x, ty1, z + 3);
         const WindowIn<ST> in0 = window_in<ST>(w, i, XY);
         fetch_plane<ST>(n1, w, i + XY, c.v_p1, c.p_p1);
         voxel_a<ST>(a, w, n0, c, i, z, lap_xy && z >= 2 && z <= d.Z - 3, own);
-        lt_store(w, z + 2, row0, lane, a0);
-        lt_store(w, z + 2, row1, lane, a1);
+        psi_store(w, z + 4, prow0, lane, qa0);
+        psi_store(w, z + 4, prow1, lane, qa1);
         slide<ST>(c, in0);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
-        const LTRow b0 = lt_load(w, x, w.yb + row0, z + 3), b1 = lt_load(w, x, w.yb + row1, z + 3);
+        lt_compute(w, z + 3, row0, lane, tn0);
+        lt_compute(w, z + 3, row1, lane, tn1);
+        const F3 qb0 = psi_row_load(w, x, py0, z + 5), qb1 = psi_row_load(w, x, py1, z + 5);
+        tn0 = t_row_load(w, x, ty0, z + 4);
+        tn1 = t_row_load(w, x, ty1, z + 4);
         const WindowIn<ST> in1 = window_in<ST>(w, i + XY, XY);
         fetch_plane<ST>(n0, w, i + 2 * XY, c.v_p1, c.p_p1);
         voxel_a<ST>(a, w, n1, c, i + XY, z + 1, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, own && s + 1 < nz);
-        lt_store(w, z + 3, row0, lane, b0);
-        lt_store(w, z + 3, row1, lane, b1);
+        psi_store(w, z + 5, prow0, lane, qb0);
+        psi_store(w, z + 5, prow1, lane, qb1);
         slide<ST>(c, in1);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
@@ -791,7 +851,6 @@ template <int ST> __device__ __forceinline__ void pass_b(const IterArgs &a, int 
 // entry points: one volume, or a batch of volumes in one launch per pass
 // --------------------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(WAVE * TY_L) void k_pass_l(IterArgs a) { pass_l(a, blockIdx.x); }
 template <int ST> __global__ __launch_bounds__(WAVE * TY_A) void k_pass_a(IterArgs a) { pass_a<ST>(a, blockIdx.x); }
 template <int ST> __global__ __launch_bounds__(WAVE * TY_B) void k_pass_b(IterArgs a) { pass_b<ST>(a, blockIdx.x); }
 
@@ -835,10 +894,6 @@ __device__ __forceinline__ IterArgs batch_args(const IterArgs *__restrict__ base
     if (blk >= a.nblocks) return; /* padding up to a multiple of 8 blocks: the whole block */   \
     pass_fn(a, blk)
 
-__global__ __launch_bounds__(WAVE * TY_L) void k_pass_l_batch(const IterArgs *base, const int *starts, int npairs,
-                                                              BatchIter it) {
-    BATCH_ENTRY(pass_l, 0);
-}
 template <int ST>
 __global__ __launch_bounds__(WAVE * TY_A) void k_pass_a_batch(const IterArgs *base, const int *starts, int npairs,
                                                               BatchIter it) {
@@ -1231,11 +1286,6 @@ static IterArgs chunked(IterArgs a, int zc, int xs, int ty) {
     return a;
 }
 
-hipError_t launch_pass_l(const IterArgs &a0, int zc, hipStream_t s) {
-    const IterArgs a = chunked(a0, zc, XS_L, TY_L);
-    hipLaunchKernelGGL(k_pass_l, dim3(a.nblocks), dim3(WAVE * TY_L), 0, s, a);
-    return hipGetLastError();
-}
 
 template <int ST> static void pass_a_kernel(const IterArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_pass_a<ST>, dim3(a.nblocks), dim3(WAVE * TY_A), 0, s, a);
@@ -1261,15 +1311,13 @@ hipError_t launch_pass_b(const IterArgs &a0, int zc, int st, hipStream_t s) {
 }
 
 IterArgs pass_args(const IterArgs &a, int pass, int zc) {
-    return pass == 0 ? chunked(a, zc, XS_L, TY_L) : pass == 1 ? chunked(a, zc, XS_A, TY_A) : chunked(a, zc, XS_B, TY_B);
+    return pass == 1 ? chunked(a, zc, XS_A, TY_A) : chunked(a, zc, XS_B, TY_B);
 }
 
 template <int ST>
 static void pass_batch_kernel(int pass, const IterArgs *d_base, const int *d_starts, int npairs, int nblocks,
                               const BatchIter &it, hipStream_t s) {
-    if (pass == 0)
-        hipLaunchKernelGGL(k_pass_l_batch, dim3(nblocks), dim3(WAVE * TY_L), 0, s, d_base, d_starts, npairs, it);
-    else if (pass == 1)
+    if (pass == 1)
         hipLaunchKernelGGL(k_pass_a_batch<ST>, dim3(nblocks), dim3(WAVE * TY_A), 0, s, d_base, d_starts, npairs, it);
     else
         hipLaunchKernelGGL(k_pass_b_batch<ST>, dim3(nblocks), dim3(WAVE * TY_B), 0, s, d_base, d_starts, npairs, it);
@@ -1285,10 +1333,9 @@ hipError_t launch_pass_batch(int pass, const IterArgs *d_base, const int *d_star
 }
 
 hipError_t pass_occupancy(int which, int *blocks_per_cu) {
-    const void *k = which == 0 ? (const void *)k_pass_l
-                    : which == 1 ? (const void *)k_pass_a<ST_F32>
-                                 : (const void *)k_pass_b<ST_F32>;
-    const int ty = which == 0 ? TY_L : which == 1 ? TY_A : TY_B;
+    if (which != 1 && which != 2) return hipErrorInvalidValue;
+    const void *k = which == 1 ? (const void *)k_pass_a<ST_F32> : (const void *)k_pass_b<ST_F32>;
+    const int ty = which == 1 ? TY_A : TY_B;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, WAVE * ty, 0);
 }
 

@@ -91,12 +91,12 @@ int fr_get_params(const fr_ctx *ctx, fr_params *p);
  * the context's own stream. */
 int fr_set_stream(fr_ctx *ctx, void *hip_stream);
 
-/* Interior z-slices per thread block of the three iteration kernels (pass L, pass A, pass B); 0
- * restores the choice fr_create made for the device and volume.  Results do not depend on it (every
- * voxel is computed the same way whatever the chunking).  No counterpart in the reference: it is a
+/* Interior z-slices per thread block of the two iteration kernels (pass A, pass B); 0 restores
+ * the choice fr_create made for the device and volume.  Results do not depend on it (every voxel
+ * is computed the same way whatever the chunking).  No counterpart in the reference: it is a
  * launch-shape setting of this implementation, for tuning. */
-int fr_set_chunking(fr_ctx *ctx, int zc_l, int zc_a, int zc_b);
-int fr_get_chunking(const fr_ctx *ctx, int *zc_l, int *zc_a, int *zc_b);
+int fr_set_chunking(fr_ctx *ctx, int zc_a, int zc_b);
+int fr_get_chunking(const fr_ctx *ctx, int *zc_a, int *zc_b);
 
 /* Storage of the velocity and density fields: FP32 (the default, the reference's precision), or
  * one 8-byte record per voxel holding v.x, v.y, v.z, rho in bf16 or fp16.  All arithmetic stays
@@ -223,12 +223,12 @@ int fr_voxel_max_energy(fr_ctx *ctx, float *max_energy, long long *voxel);
  * sum of n device floats, accumulated in FP64 (the reference accumulates float block sums).  Waits. */
 int fr_sum(const float *d_data, long long n, double *sum, void *hip_stream);
 
-/* Average device time per launch of the solver's three iteration kernels (pass L: -Laplacian of
- * psi; pass A: density, velocity and psi_inv updates; pass B: psi update and max norm), from HIP
+/* Average device time per launch of the solver's two iteration kernels (pass A: -Laplacian of psi
+ * and its warp, density, velocity and psi_inv updates; pass B: psi update and max norm), from HIP
  * events between consecutive launches of `reps` iterations run in sequence on the context stream,
  * as the iteration loop runs them (the state is restored afterwards).  Any output pointer may be
  * NULL.  No counterpart in the reference: it is the per-kernel measurement of this solver. */
-int fr_time_kernels(fr_ctx *ctx, int reps, float *ms_pass_l, float *ms_pass_a, float *ms_pass_b);
+int fr_time_kernels(fr_ctx *ctx, int reps, float *ms_pass_a, float *ms_pass_b);
 
 /* ---- the differentiated solver: K unrolled iterations in FP64 and their adjoint ---------------
  *

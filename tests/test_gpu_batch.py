@@ -84,7 +84,7 @@ def test_batch_relayout_and_verbose_energy(oracle_mod):
     b = fr.Batch([g for g, _, _ in vols])
     try:
         for k, (g, _, _) in enumerate(vols):
-            g.set_chunking(1 + k % 3, 3 + 2 * k, 1 + (k + 1) % 4)
+            g.set_chunking(3 + 2 * k, 1 + (k + 1) % 4)
         b.relayout()
         b.run()
         _check(vols, "relayout")
@@ -112,7 +112,7 @@ def test_autotuned_batch_matches_oracle(oracle_mod):
     vols = _volumes(O, 5)
     b = fr.Batch([g for g, _, _ in vols])
     try:
-        tuned, ms = autotune.tune_batch(b, iters=2, cand_a=(3, 8), cand_lb=(1, 2))
+        tuned, ms = autotune.tune_batch(b, iters=2, cand_a=(3, 8), cand_b=(1, 2))
         say("tuned", tuned, ms)
         assert set(tuned) == set(DIMS) and ms > 0
         b.run()

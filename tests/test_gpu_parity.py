@@ -214,9 +214,9 @@ def test_full_size_brain_three_iterations(oracle_mod):
     _assert_state_equal(g.get_state(), o.get_state(), "brain")
 
 
-@pytest.mark.parametrize("zc", [(1, 1, 1), (3, 5, 2), (0, 0, 0), (40, 7, 40), (8, 60, 16)])
+@pytest.mark.parametrize("zc", [(1, 1), (5, 2), (0, 0), (7, 40), (60, 16), (2, 3)])
 def test_chunking_does_not_change_results(oracle_mod, zc):
-    """Every z-chunking of the three iteration kernels gives the oracle's iteration bit for bit
+    """Every z-chunking of the two iteration kernels gives the oracle's iteration bit for bit
     (chunk edges fall on different slices for each choice, including a single chunk and chunks
     of one slice)."""
     O = oracle_mod
@@ -225,7 +225,7 @@ def test_chunking_does_not_change_results(oracle_mod, zc):
     g, o, pair, rng = _both(dims, O, 13)
     g.set_chunking(*zc)
     got = g.chunking()
-    assert all(got[k] == zc[k] for k in range(3) if zc[k]), got
+    assert all(got[k] == zc[k] for k in range(2) if zc[k]), got
     st = phantoms.random_state(dims, rng, amp=1.5)
     g.set_state(**st)
     o.set_state(**st)
