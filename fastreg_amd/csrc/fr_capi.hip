@@ -160,27 +160,30 @@ static int dalloc(T **p, size_t count) {
     return FR_OK;
 }
 
-// z-slices per block of the iteration kernels.  Pass A holds its column windows and a second
-// plane of staged loads in registers (two blocks per CU), so its chunks are sized for the whole
-// grid to be resident at once: every block then marches one long column chunk, the halo planes
-// a chunk re-reads (z-2..z+2 of v) are amortised over it and no second wave of blocks runs
-// behind a partly idle first one.  Pass B is a light streaming kernel with many blocks per CU;
-// it uses chunks of 2 slices (the fastest of 2..32 at 182x218x182 on MI355X).
-static int auto_chunking(fr_ctx *c) {
-    int dev = 0, cus = 0, occ = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_TRY(frk::pass_occupancy(1, &occ));
-    const long long tiles = (long long)((c->d.X - 2 + frk::XS_A - 1) / frk::XS_A) *
-                            ((c->d.Y - 2 + frk::TY_A - 1) / frk::TY_A);
-    const long long slots = (long long)(cus > 0 ? cus : 1) * (occ > 0 ? occ : 1);
-    const int interior = c->d.Z - 2;
+// z-slices per block of the iteration kernels.  Both passes march long column chunks with the
+// block's rows in lockstep and stage the planes around the current one in LDS, so their chunks
+// are sized for the whole grid to be resident at once: every block then marches one long chunk,
+// the halo planes a chunk re-reads are amortised over it and no second wave of blocks runs
+// behind a partly idle first one.
+static int chunk_for(Dims d, int xs, int ty, long long slots) {
+    const long long tiles = (long long)((d.X - 2 + xs - 1) / xs) * ((d.Y - 2 + ty - 1) / ty);
+    const int interior = d.Z - 2;
     long long chunks = slots / tiles;
     if (chunks < 1) chunks = 1;
     if (chunks > interior) chunks = interior;
+    return (int)((interior + chunks - 1) / chunks);
+}
+
+static int auto_chunking(fr_ctx *c) {
+    int dev = 0, cus = 0, occ_a = 0, occ_b = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(frk::pass_occupancy(1, &occ_a));
+    HIP_TRY(frk::pass_occupancy(2, &occ_b));
+    const long long n_cu = cus > 0 ? cus : 1;
     c->zc_auto[0] = 0;
-    c->zc_auto[1] = (int)((interior + chunks - 1) / chunks);
-    c->zc_auto[2] = interior < 2 ? interior : 2;
+    c->zc_auto[1] = chunk_for(c->d, frk::XS_A, frk::TY_A, n_cu * (occ_a > 0 ? occ_a : 1));
+    c->zc_auto[2] = chunk_for(c->d, frk::XS_B, frk::TY_B, n_cu * (occ_b > 0 ? occ_b : 1));
     for (int k = 0; k < 3; ++k) c->zc[k] = c->zc_auto[k];
     return FR_OK;
 }

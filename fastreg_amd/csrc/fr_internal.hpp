@@ -23,13 +23,14 @@ struct Ctl {
 // Tile of the iteration kernels: one 64-lane wave per x-row and TY_* rows per block, marching
 // over a chunk of z-slices.  Pass A runs 8 rows per block (one block of 8 waves per CU), so that
 // the rows a wave loads as y-neighbours are loaded into the CU's L1 once for all eight waves and
-// the block can stage the region its trilinear taps read in LDS.  Pass A takes x-neighbours from
-// the adjacent lanes (x+-2), so a wave computes the XS_A interior lanes of its row and keeps the
-// edge lanes as a halo; consecutive tiles are XS_* columns apart.
+// the block can stage the region its trilinear taps read in LDS (pass B likewise).  Pass A takes
+// x-neighbours from the adjacent lanes (x+-2) and pass B's taps reach x-1..x+1, so a wave computes
+// the XS_* interior lanes of its row and keeps the edge lanes as a halo; consecutive tiles are
+// XS_* columns apart.
 constexpr int WAVE = 64;
-constexpr int TY_A = 8, TY_B = 4;
+constexpr int TY_A = 8, TY_B = 8;
 constexpr int XS_A = WAVE - 4;
-constexpr int XS_B = WAVE;
+constexpr int XS_B = WAVE - 2;
 // Storage of velocity and density: FP32 (3 + 1 floats in two buffers), or one 8-byte record
 // (v.x, v.y, v.z, rho) per voxel in bf16 or fp16.
 constexpr int ST_F32 = 0, ST_BF16 = 1, ST_F16 = 2;

@@ -799,44 +799,93 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
 // pass B
 // --------------------------------------------------------------------------------------------
 
-// One thread per (x, y) column of a 64 x 4 tile, marching `zc` z-slices: psi += alpha * v_new o
-// psi (gathered through L1/L2) and the running max of the update norm.
+// v_new of the planes around the one being computed, staged in LDS for the block: a ring of 4
+// planes (z-1, z, z+1 are read while z+2 is written), TY_B + 2 rows (the block's rows and one on
+// either side), 64 columns (the lanes of a wave), (v.x, v.y, v.z, unused) per voxel.  The 8 taps
+// of a voxel whose psi lies within one voxel of it are all in the stage.
+constexpr int VS_ROWS = TY_B + 2, VS_PLANES = 4;
+struct VStage {
+    float4 v[VS_PLANES][VS_ROWS][WAVE];
+};
+
+// One thread per (x, y) column of a 64 x TY_B tile (XS_B computed columns), the block's waves
+// marching `zc` z-slices together: psi += alpha * v_new o psi (solver.cu:230-247), the trilinear
+// taps read from the stage, and the running max of the update norm.
 template <int ST> __device__ __forceinline__ void pass_b(const IterArgs &a, int blk) {
-    if (__builtin_nontemporal_load(&a.ctl->stopped)) return;
+    __shared__ VStage vs;
+    __shared__ unsigned long long red[TY_B];
+    if (__builtin_nontemporal_load(&a.ctl->stopped)) return;  // set before this launch: uniform
     const Dims d = a.d;
     const int N = d.X * d.Y * d.Z, XY = d.X * d.Y;
     const TileCoord tc = tile_of(a, blk);
     const int x = tc.x, y = tc.y;
+    const bool own = tc.own && y <= d.Y - 2;
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    const int xb = x - lane, yb = y - wave - 1;
+    const VRBuf vb = vr_buf<ST>(a.v_n, a.rho_n, N);
+    const Rsrc rq = rsrc(a.psi, 12 * N), wq = rq;
+    // this wave stages rows `wave` and (waves 0 and 1) `wave + TY_B`; the others stage the last row
+    // as well, writing the same values, so that no wave branches
+    const int row0 = wave, row1 = min(wave + TY_B, VS_ROWS - 1);
+    const int sy0 = yb + row0, sy1 = yb + row1;
+    auto stage_load = [&](int yy, int zz) { return vr_v<ST>(v_load<ST>(vb, x + d.X * yy + XY * zz)); };
+    auto stage_store = [&](int zz, int row, F3 v) {
+        vs.v[zz & (VS_PLANES - 1)][row][lane] = make_float4(v.x, v.y, v.z, 0.f);
+    };
+    for (int zz = tc.zs - 1; zz <= tc.zs + 1; ++zz) {
+        const F3 v0 = stage_load(sy0, zz), v1 = stage_load(sy1, zz);
+        stage_store(zz, row0, v0);
+        stage_store(zz, row1, v1);
+    }
+    const unsigned o0 = 12u * (x + d.X * y + XY * tc.zs);
+    F3 pn = bld3(rq, o0);  // psi of the next plane to compute
+    __syncthreads();
     unsigned long long pack = 0;
-    if (tc.own && y <= d.Y - 2) {
-        const VRBuf vb = vr_buf<ST>(a.v_n, a.rho_n, N);
-        for (int z = tc.zs; z < tc.ze; ++z) {
-            // psi += alpha * v_new o psi; updates = that increment (solver.cu:230-247)
-            const int i = x + d.X * y + XY * z;
-            const F3 p = ld3(a.psi, i);
-            const Corner k = corner(d, p.x, p.y, p.z);
+    for (int z = tc.zs; z < tc.ze; ++z) {
+        const unsigned o = 12u * (x + d.X * y + XY * z);
+        const F3 v0 = stage_load(sy0, z + 2), v1 = stage_load(sy1, z + 2);
+        const F3 p = pn;
+        pn = bld3(rq, o + 12u * XY);
+        const Corner k = corner(d, p.x, p.y, p.z);
+        int cx = k.gx - xb, ry = k.gy - yb;
+        const int dz = k.gz - z;
+        const bool in = cx >= 0 && cx <= WAVE - 2 && ry >= 0 && ry <= VS_ROWS - 2 && (dz == -1 || dz == 0);
+        cx = min(max(cx, 0), WAVE - 2);
+        ry = min(max(ry, 0), VS_ROWS - 2);
+        const int zg = dz == -1 ? z - 1 : z;
+        const int s0 = zg & (VS_PLANES - 1), s1 = (zg + 1) & (VS_PLANES - 1);
+        const float4 f000 = vs.v[s0][ry][cx], f001 = vs.v[s1][ry][cx], f010 = vs.v[s0][ry + 1][cx],
+                     f011 = vs.v[s1][ry + 1][cx], f100 = vs.v[s0][ry][cx + 1], f101 = vs.v[s1][ry][cx + 1],
+                     f110 = vs.v[s0][ry + 1][cx + 1], f111 = vs.v[s1][ry + 1][cx + 1];
+        F3 w = f3(tri(f111.x, f110.x, f101.x, f100.x, f011.x, f010.x, f001.x, f000.x, k.a, k.b, k.c),
+                  tri(f111.y, f110.y, f101.y, f100.y, f011.y, f010.y, f001.y, f000.y, k.a, k.b, k.c),
+                  tri(f111.z, f110.z, f101.z, f100.z, f011.z, f010.z, f001.z, f000.z, k.a, k.b, k.c));
+        if (own && !in) {  // a voxel displaced out of the stage: gather v_new from memory
             const int g = (int)corner_index(d, k);
-            const F3 f111 = vr_v<ST>(v_load<ST>(vb, g + 1 + d.X + XY)), f110 = vr_v<ST>(v_load<ST>(vb, g + 1 + d.X)),
-                     f101 = vr_v<ST>(v_load<ST>(vb, g + 1 + XY)), f100 = vr_v<ST>(v_load<ST>(vb, g + 1)),
-                     f011 = vr_v<ST>(v_load<ST>(vb, g + d.X + XY)), f010 = vr_v<ST>(v_load<ST>(vb, g + d.X)),
-                     f001 = vr_v<ST>(v_load<ST>(vb, g + XY)), f000 = vr_v<ST>(v_load<ST>(vb, g));
-            const F3 w = f3(tri(f111.x, f110.x, f101.x, f100.x, f011.x, f010.x, f001.x, f000.x, k.a, k.b, k.c),
-                            tri(f111.y, f110.y, f101.y, f100.y, f011.y, f010.y, f001.y, f000.y, k.a, k.b, k.c),
-                            tri(f111.z, f110.z, f101.z, f100.z, f011.z, f010.z, f001.z, f000.z, k.a, k.b, k.c));
-            const F3 u = mul3(a.alpha, w);
-            st3(a.psi, i, f3(p.x + u.x, p.y + u.y, p.z + u.z));
-            // norm with a round-down sqrt (cuda_utils.hpp:251-253); NaN never wins (strict >,
-            // reductor.cu:445-456)
-            const float nr = sqrt_rd((u.x * u.x + u.y * u.y) + u.z * u.z);
-            const unsigned long long pk =
-                ((unsigned long long)__float_as_uint(nr) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)i);
-            if (!isnan(nr) && pk > pack) pack = pk;
+            const F3 g111 = vr_v<ST>(v_load<ST>(vb, g + 1 + d.X + XY)), g110 = vr_v<ST>(v_load<ST>(vb, g + 1 + d.X)),
+                     g101 = vr_v<ST>(v_load<ST>(vb, g + 1 + XY)), g100 = vr_v<ST>(v_load<ST>(vb, g + 1)),
+                     g011 = vr_v<ST>(v_load<ST>(vb, g + d.X + XY)), g010 = vr_v<ST>(v_load<ST>(vb, g + d.X)),
+                     g001 = vr_v<ST>(v_load<ST>(vb, g + XY)), g000 = vr_v<ST>(v_load<ST>(vb, g));
+            w = f3(tri(g111.x, g110.x, g101.x, g100.x, g011.x, g010.x, g001.x, g000.x, k.a, k.b, k.c),
+                   tri(g111.y, g110.y, g101.y, g100.y, g011.y, g010.y, g001.y, g000.y, k.a, k.b, k.c),
+                   tri(g111.z, g110.z, g101.z, g100.z, g011.z, g010.z, g001.z, g000.z, k.a, k.b, k.c));
         }
+        const F3 u = mul3(a.alpha, w);
+        bst3(wq, own ? o : NO_STORE, f3(p.x + u.x, p.y + u.y, p.z + u.z));
+        // norm with a round-down sqrt (cuda_utils.hpp:251-253); NaN never wins (strict >,
+        // reductor.cu:445-456)
+        const float nr = sqrt_rd((u.x * u.x + u.y * u.y) + u.z * u.z);
+        const int i = x + d.X * y + XY * z;
+        const unsigned long long pk =
+            ((unsigned long long)__float_as_uint(nr) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)i);
+        if (own && !isnan(nr) && pk > pack) pack = pk;
+        stage_store(z + 2, row0, v0);
+        stage_store(z + 2, row1, v1);
+        __syncthreads();
     }
     // max (norm, lowest index) over the block, then one atomic per block into one of the
     // iteration's NSLOT slots
     pack = wave_max_u64(pack);
-    __shared__ unsigned long long red[TY_B];
     const int tid = threadIdx.x;
     if ((tid & 63) == 0) red[tid >> 6] = pack;
     __syncthreads();
