@@ -358,12 +358,27 @@ template <int ST> __device__ __forceinline__ void vr_store(const VRBuf &b, int i
 // lanes.  The loads of plane z+1 (y-neighbours, the corner of psi_inv) and the stage's rows of the
 // planes ahead are issued before plane z is computed, so that their latency overlaps the
 // arithmetic: software pipelining across the z-march.
-template <int ST> struct PlaneLoads {
-    VR<ST> vr_yp, vr_ym;             // (v, rho) at y+-1
-    VR<ST> v_y2;                     // v at y+-2 on the side the upwind Laplacian reads
-    F3 p_yp, p_ym;                   // psi_inv at y+-1
-    float s_yp, s_ym;                // S at y+-1
-    Corner k;                        // corner and weights of psi_inv(p)
+// the y-neighbours of the voxel being computed
+struct PlaneNb {
+    F3 v_yp, v_ym;  // v at y+-1
+    F3 v_y2;        // v at y+-2 on the side the upwind Laplacian reads
+    F3 p_yp, p_ym;  // psi_inv at y+-1
+    float r_yp, r_ym, s_yp, s_ym;  // rho and S at y+-1
+};
+
+// The y-neighbours, exchanged through LDS: a ring of 2 planes (the plane being computed is read
+// while the next one is written) of the 12 rows y-2 .. y+9 around the block's 8 rows, holding
+// (v, rho) and (psi_inv, S) per voxel.  Every wave writes its own row from its column window;
+// the 4 halo rows are loaded, one per wave of waves 0-3 (waves 4-7 load the same rows again and
+// write the same values, so that no wave branches).
+constexpr int YS_ROWS = TY_A + 4, YS_PLANES = 2;
+struct YStage {
+    float4 vr[YS_PLANES][YS_ROWS][WAVE];
+    float4 ps[YS_PLANES][YS_ROWS][WAVE];
+};
+struct YRow {
+    F3 v, p;
+    float r, s;
 };
 
 // L = -Laplacian(psi) and T on the planes around the one being computed, staged in LDS for the
@@ -393,7 +408,40 @@ struct PassAView {
     int xb, yb;           // the L/T stage's first column (x of lane 0) and first row (the block's y - 1)
     LTStage *lt;
     PsiStage *ps;         // row j holds y = yb - 1 + j
+    YStage *ys;           // row j holds y = yb - 1 + j
 };
+
+template <int ST> __device__ __forceinline__ YRow yrow_load(const PassAView &w, int i) {
+    const VR<ST> q = vr_load<ST>(w.rvr, i);
+    YRow h;
+    h.v = vr_v<ST>(q);
+    h.r = vr_r<ST>(q);
+    h.p = bld3(w.rp, 12u * i);
+    h.s = bld1(w.rs, 4u * i);
+    return h;
+}
+__device__ __forceinline__ void ys_store(const PassAView &w, int zz, int row, int lane, F3 v, float r, F3 p, float s) {
+    w.ys->vr[zz & (YS_PLANES - 1)][row][lane] = make_float4(v.x, v.y, v.z, r);
+    w.ys->ps[zz & (YS_PLANES - 1)][row][lane] = make_float4(p.x, p.y, p.z, s);
+}
+// the y-neighbours of plane zz for the wave whose row is j, the upwind side chosen by vneg (v.y < 0)
+__device__ __forceinline__ PlaneNb ys_read(const PassAView &w, int zz, int j, int lane, bool vneg) {
+    const int sl = zz & (YS_PLANES - 1);
+    const float4 a = w.ys->vr[sl][j + 1][lane], b = w.ys->vr[sl][j - 1][lane],
+                 c = w.ys->vr[sl][vneg ? j + 2 : j - 2][lane], e = w.ys->ps[sl][j + 1][lane],
+                 f = w.ys->ps[sl][j - 1][lane];
+    PlaneNb n;
+    n.v_yp = f3(a.x, a.y, a.z);
+    n.r_yp = a.w;
+    n.v_ym = f3(b.x, b.y, b.z);
+    n.r_ym = b.w;
+    n.v_y2 = f3(c.x, c.y, c.z);
+    n.p_yp = f3(e.x, e.y, e.z);
+    n.s_yp = e.w;
+    n.p_ym = f3(f.x, f.y, f.z);
+    n.s_ym = f.w;
+    return n;
+}
 
 // Rows for the stages, loaded by a wave at its lanes' columns.  Positions outside the grid read
 // other voxels of the fields or, past the buffers, 0: finite values that no in-stage tap of a
@@ -482,20 +530,6 @@ __device__ __forceinline__ void lt_at(const PassAView &w, const Corner &k, int z
     }
 }
 
-// issue the loads of the voxel at (linear index i) whose own v and psi_inv are vc and pc
-template <int ST>
-__device__ __forceinline__ void fetch_plane(PlaneLoads<ST> &n, const PassAView &w, int i, F3 vc, F3 pc) {
-    const unsigned o12 = 12u * i, o4 = 4u * i;
-    n.vr_yp = vr_load<ST>(w.rvr, i + w.X);
-    n.vr_ym = vr_load<ST>(w.rvr, i - w.X);
-    n.v_y2 = v_load<ST>(w.rvr, vc.y < 0.f ? i + 2 * w.X : i - 2 * w.X);
-    n.p_yp = bld3(w.rp, o12 + w.X12);
-    n.p_ym = bld3(w.rp, o12 - w.X12);
-    n.s_yp = bld1(w.rs, o4 + w.X4);
-    n.s_ym = bld1(w.rs, o4 - w.X4);
-    n.k = corner(w.d, pc.x, pc.y, pc.z);
-}
-
 // own-column z-windows; rho and psi_inv run one plane further ahead than they are used, so that
 // the corner of the plane being prefetched comes from a value loaded a whole step earlier
 struct ColumnWindow {
@@ -506,7 +540,7 @@ struct ColumnWindow {
 // every lane of the wave runs this (the x-neighbours are exchanged between lanes); `store` is
 // false on halo lanes and on the voxel past the chunk
 template <int ST>
-__device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, const PlaneLoads<ST> &n,
+__device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, const PlaneNb &n,
                                         const ColumnWindow &c, int i, int z, bool lap, bool store) {
     // x-neighbours of rho, v, psi_inv, S from the adjacent lanes
     const float r_xp = lane_next(c.r_0), r_xm = lane_prev(c.r_0);
@@ -515,8 +549,8 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const F3 p_xp = lane_next3(c.p_0), p_xm = lane_prev3(c.p_0);
     const float s_xp = lane_next(c.s_0), s_xm = lane_prev(c.s_0);
     // y-neighbours of rho and v
-    const float r_yp = vr_r<ST>(n.vr_yp), r_ym = vr_r<ST>(n.vr_ym);
-    const F3 v_yp = vr_v<ST>(n.vr_yp), v_ym = vr_v<ST>(n.vr_ym), v_y2 = vr_v<ST>(n.v_y2);
+    const float r_yp = n.r_yp, r_ym = n.r_ym;
+    const F3 v_yp = n.v_yp, v_ym = n.v_ym, v_y2 = n.v_y2;
 
     // 1. density (solver.cu:101-173)
     const F3 vc = c.v_0;
@@ -542,7 +576,7 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     // 3-4. L o psi_inv and T o psi_inv (vector_fields.cu:231-247, 81-99)
     F3 Lo;
     float Tv;
-    lt_at(w, n.k, z, store, Lo, Tv);
+    lt_at(w, corner(w.d, c.p_0.x, c.p_0.y, c.p_0.z), z, store, Lo, Tv);
 
     // 5. grad S and the potential gradient (vector_fields.cu:155-192, solver.cu:56-80)
     const F3 gS = f3((s_xp - s_xm) * 0.5f, (n.s_yp - n.s_ym) * 0.5f, (c.s_p1 - c.s_m1) * 0.5f);
@@ -686,9 +720,11 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     // same values, so that no wave branches.
     __shared__ LTStage lt_stage;
     __shared__ PsiStage psi_stage;
+    __shared__ YStage y_stage;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
     w.lt = &lt_stage;
     w.ps = &psi_stage;
+    w.ys = &y_stage;
     w.xb = x - lane;
     w.yb = y - wave - 1;
     const int row0 = wave, row1 = min(wave + TY_A, LT_ROWS - 1);
@@ -745,13 +781,21 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     c.s_0 = bld1(w.rs, 4u * i);
     c.s_p1 = bld1(w.rs, 4u * (i + XY));
 
-    PlaneLoads<ST> n0, n1;
-    fetch_plane<ST>(n0, w, i, c.v_0, c.p_0);
+    // the y-neighbour stage: this wave's row is jo; its halo row jh is y-1, y+8, y-2 or y+9 of the
+    // block (waves 4-7 repeat waves 0-3); plane zs now, the halo of plane zs+1 into registers
+    const int jo = wave + 2, jh = (wave & 3) == 0 ? 1 : (wave & 3) == 1 ? TY_A + 2 : (wave & 3) == 2 ? 0 : TY_A + 3;
+    const int ih = x + d.X * (w.yb - 1 + jh);
+    {
+        const YRow h0 = yrow_load<ST>(w, ih + XY * zs);
+        ys_store(w, zs, jo, lane, c.v_0, c.r_0, c.p_0, c.s_0);
+        ys_store(w, zs, jh, lane, h0.v, h0.r, h0.p, h0.s);
+    }
+    YRow hA = yrow_load<ST>(w, ih + XY * (zs + 1)), hB;
     // drain the prologue's loads, so that the loop's memory-counter waits are set by the loop's
     // own steady state and not by the prologue's last loads; the stage's first planes are written
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
-    // Two steps per trip, alternating the staged plane loads between n0 and n1.  The trip has no
+    // Two steps per trip, alternating the halo-row registers between hA and hB.  The trip has no
     // branch: the loads run ahead of the chunk unconditionally (range-checked buffer loads; the
     // values are never stored) and the odd step of the last trip drops its stores, so the memory
     // counter waits in the loop are the exact distances of one step of pipelining.
@@ -762,31 +806,37 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         // computes L/T of plane z+2 from psi planes z+1..z+3, then loads psi of plane z+4 (written
         // into the ring at its end) and T of plane z+3 (used by the next half-step); the LDS writes
         // complete before the barrier that precedes their readers.)
+        const PlaneNb n0 = ys_read(w, z, jo, lane, c.v_0.y < 0.f);
         lt_compute(w, z + 2, row0, lane, tn0);
         lt_compute(w, z + 2, row1, lane, tn1);
         const F3 qa0 = psi_row_load(w, x, py0, z + 4), qa1 = psi_row_load(w, x, py1, z + 4);
         tn0 = t_row_load(w, x, ty0, z + 3);
         tn1 = t_row_load(w, x, ty1, z + 3);
+        hB = yrow_load<ST>(w, ih + XY * (z + 2));
         const WindowIn<ST> in0 = window_in<ST>(w, i, XY);
-        fetch_plane<ST>(n1, w, i + XY, c.v_p1, c.p_p1);
         voxel_a<ST>(a, w, n0, c, i, z, lap_xy && z >= 2 && z <= d.Z - 3, own);
         psi_store(w, z + 4, prow0, lane, qa0);
         psi_store(w, z + 4, prow1, lane, qa1);
+        ys_store(w, z + 1, jo, lane, c.v_p1, c.r_p1, c.p_p1, c.s_p1);
+        ys_store(w, z + 1, jh, lane, hA.v, hA.r, hA.p, hA.s);
         slide<ST>(c, in0);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
+        const PlaneNb n1 = ys_read(w, z + 1, jo, lane, c.v_0.y < 0.f);
         lt_compute(w, z + 3, row0, lane, tn0);
         lt_compute(w, z + 3, row1, lane, tn1);
         const F3 qb0 = psi_row_load(w, x, py0, z + 5), qb1 = psi_row_load(w, x, py1, z + 5);
         tn0 = t_row_load(w, x, ty0, z + 4);
         tn1 = t_row_load(w, x, ty1, z + 4);
+        hA = yrow_load<ST>(w, ih + XY * (z + 3));
         const WindowIn<ST> in1 = window_in<ST>(w, i + XY, XY);
-        fetch_plane<ST>(n0, w, i + 2 * XY, c.v_p1, c.p_p1);
         voxel_a<ST>(a, w, n1, c, i + XY, z + 1, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, own && s + 1 < nz);
         psi_store(w, z + 5, prow0, lane, qb0);
         psi_store(w, z + 5, prow1, lane, qb1);
+        ys_store(w, z + 2, jo, lane, c.v_p1, c.r_p1, c.p_p1, c.s_p1);
+        ys_store(w, z + 2, jh, lane, hB.v, hB.r, hB.p, hB.s);
         slide<ST>(c, in1);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
