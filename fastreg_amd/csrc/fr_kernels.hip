@@ -369,8 +369,8 @@ struct PlaneNb {
 // The y-neighbours, exchanged through LDS: a ring of 2 planes (the plane being computed is read
 // while the next one is written) of the 12 rows y-2 .. y+9 around the block's 8 rows, holding
 // (v, rho) and (psi_inv, S) per voxel.  Every wave writes its own row from its column window;
-// the 4 halo rows are loaded, one per wave of waves 0-3 (waves 4-7 load the same rows again and
-// write the same values, so that no wave branches).
+// the 4 halo rows are loaded one per wave by waves 4-7 (waves 0-3 carry the stages' extra psi and
+// L/T rows; this split keeps the waves, which step in lockstep, evenly loaded).
 constexpr int YS_ROWS = TY_A + 4, YS_PLANES = 2;
 struct YStage {
     float4 vr[YS_PLANES][YS_ROWS][WAVE];
@@ -781,16 +781,18 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     c.s_0 = bld1(w.rs, 4u * i);
     c.s_p1 = bld1(w.rs, 4u * (i + XY));
 
-    // the y-neighbour stage: this wave's row is jo; its halo row jh is y-1, y+8, y-2 or y+9 of the
-    // block (waves 4-7 repeat waves 0-3); plane zs now, the halo of plane zs+1 into registers
+    // the y-neighbour stage: this wave's row is jo; the halo row jh (y-1, y+8, y-2 or y+9 of the
+    // block) is loaded by waves 4-7 (has_h, uniform over the wave); plane zs now (every wave loads
+    // a halo row here), the halo of plane zs+1 into registers
     const int jo = wave + 2, jh = (wave & 3) == 0 ? 1 : (wave & 3) == 1 ? TY_A + 2 : (wave & 3) == 2 ? 0 : TY_A + 3;
+    const bool has_h = __builtin_amdgcn_readfirstlane(wave) >= 4;
     const int ih = x + d.X * (w.yb - 1 + jh);
     {
         const YRow h0 = yrow_load<ST>(w, ih + XY * zs);
         ys_store(w, zs, jo, lane, c.v_0, c.r_0, c.p_0, c.s_0);
         ys_store(w, zs, jh, lane, h0.v, h0.r, h0.p, h0.s);
     }
-    YRow hA = yrow_load<ST>(w, ih + XY * (zs + 1)), hB;
+    YRow hA = yrow_load<ST>(w, ih + XY * (zs + 1)), hB = hA;
     // drain the prologue's loads, so that the loop's memory-counter waits are set by the loop's
     // own steady state and not by the prologue's last loads; the stage's first planes are written
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
@@ -812,13 +814,13 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         const F3 qa0 = psi_row_load(w, x, py0, z + 4), qa1 = psi_row_load(w, x, py1, z + 4);
         tn0 = t_row_load(w, x, ty0, z + 3);
         tn1 = t_row_load(w, x, ty1, z + 3);
-        hB = yrow_load<ST>(w, ih + XY * (z + 2));
+        if (has_h) hB = yrow_load<ST>(w, ih + XY * (z + 2));
         const WindowIn<ST> in0 = window_in<ST>(w, i, XY);
         voxel_a<ST>(a, w, n0, c, i, z, lap_xy && z >= 2 && z <= d.Z - 3, own);
         psi_store(w, z + 4, prow0, lane, qa0);
         psi_store(w, z + 4, prow1, lane, qa1);
         ys_store(w, z + 1, jo, lane, c.v_p1, c.r_p1, c.p_p1, c.s_p1);
-        ys_store(w, z + 1, jh, lane, hA.v, hA.r, hA.p, hA.s);
+        if (has_h) ys_store(w, z + 1, jh, lane, hA.v, hA.r, hA.p, hA.s);
         slide<ST>(c, in0);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
@@ -830,13 +832,13 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         const F3 qb0 = psi_row_load(w, x, py0, z + 5), qb1 = psi_row_load(w, x, py1, z + 5);
         tn0 = t_row_load(w, x, ty0, z + 4);
         tn1 = t_row_load(w, x, ty1, z + 4);
-        hA = yrow_load<ST>(w, ih + XY * (z + 3));
+        if (has_h) hA = yrow_load<ST>(w, ih + XY * (z + 3));
         const WindowIn<ST> in1 = window_in<ST>(w, i + XY, XY);
         voxel_a<ST>(a, w, n1, c, i + XY, z + 1, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, own && s + 1 < nz);
         psi_store(w, z + 5, prow0, lane, qb0);
         psi_store(w, z + 5, prow1, lane, qb1);
         ys_store(w, z + 2, jo, lane, c.v_p1, c.r_p1, c.p_p1, c.s_p1);
-        ys_store(w, z + 2, jh, lane, hB.v, hB.r, hB.p, hB.s);
+        if (has_h) ys_store(w, z + 2, jh, lane, hB.v, hB.r, hB.p, hB.s);
         slide<ST>(c, in1);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
