@@ -122,6 +122,14 @@ def lib():
     sig("fr_adjoint_backward", vp, vp, P(d), P(d))
     sig("fr_adjoint_get_state", vp, i, vp, vp, vp, vp)
     sig("fr_adjoint_info", vp, P(sz), P(f), P(f))
+    sig("fr_step_fwd", vp, i, d, d, vp, vp, vp, vp, vp, vp, vp, vp)
+    sig("fr_step_bwd", vp, i, d, d, vp, vp, vp, vp, vp, vp, vp, vp, P(d), P(d))
+    sig("fr_warp_fwd", i, i, i, vp, vp, vp)
+    sig("fr_warp_bwd", i, i, i, vp, vp, vp, vp, vp)
+    sig("fr_gradient_fwd", i, i, i, vp, vp)
+    sig("fr_gradient_bwd", i, i, i, vp, vp)
+    sig("fr_laplacian_fwd", i, i, i, vp, vp)
+    sig("fr_laplacian_bwd", i, i, i, vp, vp)
     sig("fr_dev_malloc", P(vp), sz)
     sig("fr_dev_free", vp)
     sig("fr_memcpy_htod", vp, vp, sz)
@@ -514,6 +522,31 @@ class Adjoint:
         check(lib().fr_adjoint_get_state(self.h, k, b["v"].ptr, b["rho"].ptr, b["psi"].ptr, b["psi_inv"].ptr))
         return {k_: a.get() for k_, a in b.items()}
 
+    def _state_bufs(self, st):
+        n = self.n
+        return [DeviceArray.from_host(np.asarray(st["rho"], np.float64).reshape(n)),
+                DeviceArray.from_host(np.asarray(st["v"], np.float64).reshape(n, 3)),
+                DeviceArray.from_host(np.asarray(st["psi"], np.float64).reshape(n, 3)),
+                DeviceArray.from_host(np.asarray(st["psi_inv"], np.float64).reshape(n, 3))]
+
+    def step_fwd(self, k, state, w_reg, tau):
+        """One iteration (number k) from `state` (dict rho, v, psi, psi_inv): the next state."""
+        ins = self._state_bufs(state)
+        outs = [DeviceArray(b.shape, np.float64) for b in ins]
+        check(lib().fr_step_fwd(self.h, k, float(w_reg), float(tau), *[b.ptr for b in ins], *[b.ptr for b in outs]))
+        return dict(zip(("rho", "v", "psi", "psi_inv"), (b.get() for b in outs)))
+
+    def step_bwd(self, k, state, grad_next, w_reg, tau):
+        """Adjoint of one iteration: (adjoint of `state` as a dict, dE/dW_REG, dE/dTAU) from the
+        adjoint of the next state `grad_next` (dict rho, v, psi, psi_inv)."""
+        ins = self._state_bufs(state)
+        gs = self._state_bufs(grad_next)
+        gw, gt = C.c_double(), C.c_double()
+        check(lib().fr_step_bwd(self.h, k, float(w_reg), float(tau), *[b.ptr for b in ins], *[b.ptr for b in gs],
+                                C.byref(gw), C.byref(gt)))
+        self.forward_calls += 1  # the handle's last forward pass is no longer differentiable
+        return dict(zip(("rho", "v", "psi", "psi_inv"), (b.get() for b in gs))), gw.value, gt.value
+
     def info(self):
         """(device bytes held, device ms of the last forward, of the last backward)."""
         by, f, b = C.c_size_t(), C.c_float(), C.c_float()
@@ -533,3 +566,60 @@ class Adjoint:
             self.close()
         except Exception:
             pass
+
+
+# ---- single operators as FP64 forward/backward pairs ---------------------------------------------
+
+def _dev(a, shape):
+    return DeviceArray.from_host(np.ascontiguousarray(np.asarray(a, np.float64).reshape(shape)))
+
+
+def warp_fwd(f, pos, dims):
+    """trilerp(f, pos(p)) for every voxel (f: N, pos: (N, 3) float64)."""
+    X, Y, Z = dims
+    n = X * Y * Z
+    fb, pb, ob = _dev(f, n), _dev(pos, (n, 3)), DeviceArray(n, np.float64)
+    check(lib().fr_warp_fwd(X, Y, Z, fb.ptr, pb.ptr, ob.ptr))
+    return ob.get()
+
+
+def warp_bwd(f, pos, grad_out, dims):
+    """(dE/df, dE/dpos) of E with dE/d(warp_fwd) = grad_out."""
+    X, Y, Z = dims
+    n = X * Y * Z
+    fb, pb, gb = _dev(f, n), _dev(pos, (n, 3)), _dev(grad_out, n)
+    gf, gp = _dev(np.zeros(n), n), _dev(np.zeros((n, 3)), (n, 3))
+    check(lib().fr_warp_bwd(X, Y, Z, fb.ptr, pb.ptr, gb.ptr, gf.ptr, gp.ptr))
+    return gf.get(), gp.get()
+
+
+def gradient_fwd(f, dims):
+    X, Y, Z = dims
+    n = X * Y * Z
+    fb, gb = _dev(f, n), DeviceArray((n, 3), np.float64)
+    check(lib().fr_gradient_fwd(X, Y, Z, fb.ptr, gb.ptr))
+    return gb.get()
+
+
+def gradient_bwd(grad_grad, dims):
+    X, Y, Z = dims
+    n = X * Y * Z
+    gg, gf = _dev(grad_grad, (n, 3)), _dev(np.zeros(n), n)
+    check(lib().fr_gradient_bwd(X, Y, Z, gg.ptr, gf.ptr))
+    return gf.get()
+
+
+def laplacian_fwd(psi, dims):
+    X, Y, Z = dims
+    n = X * Y * Z
+    pb, lb = _dev(psi, (n, 3)), DeviceArray((n, 3), np.float64)
+    check(lib().fr_laplacian_fwd(X, Y, Z, pb.ptr, lb.ptr))
+    return lb.get()
+
+
+def laplacian_bwd(grad_L, dims):
+    X, Y, Z = dims
+    n = X * Y * Z
+    gl, gp = _dev(grad_L, (n, 3)), _dev(np.zeros((n, 3)), (n, 3))
+    check(lib().fr_laplacian_bwd(X, Y, Z, gl.ptr, gp.ptr))
+    return gp.get()

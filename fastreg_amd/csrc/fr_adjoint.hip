@@ -848,6 +848,84 @@ __global__ __launch_bounds__(256) void k_sum_final(const double *partials, int n
     if (threadIdx.x == 0) *out = sh[0];
 }
 
+// ---- single operators and their adjoints (the per-operator forward/backward pairs) ------------
+
+// out(p) = trilerp(f, pos(p)) over all voxels, positions clamped into the grid
+__global__ void op_warp_fwd(Dims d, const double *f, const double *pos, double *out) {
+    FOR_VOXELS(d, i) {
+        const D3 p = ld(pos, i);
+        const Corner k = corner(d, p.x, p.y, p.z);
+        double t[8];
+        gather1(f, k, t);
+        out[i] = tri(t, k.a, k.b, k.c);
+    }
+}
+// gf(corners of pos(p)) += weight x gout(p) (FP64 atomics); gpos(p) += gout(p) d trilerp / d pos
+__global__ void op_warp_bwd(Dims d, const double *f, const double *pos, const double *gout, double *gf,
+                            double *gpos) {
+    FOR_VOXELS(d, i) {
+        const double g = gout[i];
+        const D3 p = ld(pos, i);
+        const Corner k = corner(d, p.x, p.y, p.z);
+        double w[8], t[8];
+        tri_weights(k.a, k.b, k.c, w);
+        for (int j = 0; j < 8; ++j) aadd(&gf[k.g[j]], w[j] * g);
+        gather1(f, k, t);
+        const D3 dp = tri_dpos(t, k), o = ld(gpos, i);
+        st(gpos, i, d3(o.x + g * dp.x, o.y + g * dp.y, o.z + g * dp.z));
+    }
+}
+// grad(p) = (f(+1) - f(-1)) / 2 per axis on the interior, 0 elsewhere (vector_fields.cu:155-192)
+__global__ void op_grad_fwd(Dims d, const double *f, double *g) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        D3 o = d3(0, 0, 0);
+        if (interior(d, x, y, z)) {
+            const long sy = d.X, sz = (long)d.X * d.Y;
+            o = d3((f[i + 1] - f[i - 1]) / 2.0, (f[i + sy] - f[i - sy]) / 2.0, (f[i + sz] - f[i - sz]) / 2.0);
+        }
+        st(g, i, o);
+    }
+}
+// gf(i) += sum over the interior voxels j whose difference reads i: +gg(j).a / 2 for i = j + e_a,
+// -gg(j).a / 2 for i = j - e_a (a gather)
+__global__ void op_grad_bwd(Dims d, const double *gg, double *gf) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        const long sy = d.X, sz = (long)d.X * d.Y;
+        double a = 0;
+        if (interior(d, x - 1, y, z)) a += gg[3 * (i - 1)] / 2.0;
+        if (interior(d, x + 1, y, z)) a -= gg[3 * (i + 1)] / 2.0;
+        if (interior(d, x, y - 1, z)) a += gg[3 * (i - sy) + 1] / 2.0;
+        if (interior(d, x, y + 1, z)) a -= gg[3 * (i + sy) + 1] / 2.0;
+        if (interior(d, x, y, z - 1)) a += gg[3 * (i - sz) + 2] / 2.0;
+        if (interior(d, x, y, z + 1)) a -= gg[3 * (i + sz) + 2] / 2.0;
+        gf[i] += a;
+    }
+}
+// gpsi(i) += 6 gL(i) (i interior) - sum of gL over i's interior neighbours (the transpose of
+// L = 6 psi - sum of the neighbours, a gather)
+__global__ void op_laplacian_bwd(Dims d, const double *gL, double *gpsi) {
+    FOR_VOXELS(d, i) {
+        XYZ(d, i);
+        const long sy = d.X, sz = (long)d.X * d.Y;
+        D3 gp = ld(gpsi, i);
+        if (interior(d, x, y, z)) {
+            const D3 g = ld(gL, i);
+            gp = d3(gp.x + 6 * g.x, gp.y + 6 * g.y, gp.z + 6 * g.z);
+        }
+        const long nb[6] = {i + 1, i - 1, i + sy, i - sy, i + sz, i - sz};
+        const bool nin[6] = {interior(d, x + 1, y, z), interior(d, x - 1, y, z), interior(d, x, y + 1, z),
+                             interior(d, x, y - 1, z), interior(d, x, y, z + 1), interior(d, x, y, z - 1)};
+        for (int k = 0; k < 6; ++k)
+            if (nin[k]) {
+                const D3 g = ld(gL, nb[k]);
+                gp = d3(gp.x - g.x, gp.y - g.y, gp.z - g.z);
+            }
+        st(gpsi, i, gp);
+    }
+}
+
 }  // namespace frd
 
 // ---- host driver --------------------------------------------------------------------------------
@@ -969,6 +1047,19 @@ namespace {
 // t of iteration k (1-based) as the solver loop advances it (solver.cu:423-428): t_1 = alpha,
 // t_k = t_{k-1} + alpha, reset to alpha when k is a multiple of RESET_ITER; then
 // m1t = (p + 1) / t and m2 = C p^2 t^(p - 2) (solver.cu:198-207), all in double.
+void scalars_of(const fr_adjoint *h, double t, double *m1t, double *m2) {
+    const double m1 = h->p + 1.0;
+    *m1t = m1 / t;
+    *m2 = h->C * pow(h->p, 2.0) * pow(t, h->p - 2.0);
+}
+double t_of(const fr_adjoint *h, int k) {
+    double t = h->alpha;
+    for (int q = 2; q <= k; ++q) {
+        t += h->alpha;
+        if (h->reset_iter > 0 && q % h->reset_iter == 0) t = h->alpha;
+    }
+    return t;
+}
 void schedule(fr_adjoint *h) {
     h->m1t.assign(h->K, 0.0);
     h->m2.assign(h->K, 0.0);
@@ -978,14 +1069,12 @@ void schedule(fr_adjoint *h) {
             t += h->alpha;
             if (h->reset_iter > 0 && k % h->reset_iter == 0) t = h->alpha;
         }
-        const double m1 = h->p + 1.0;
-        h->m1t[k - 1] = m1 / t;
-        h->m2[k - 1] = h->C * pow(h->p, 2.0) * pow(t, h->p - 2.0);
+        scalars_of(h, t, &h->m1t[k - 1], &h->m2[k - 1]);
     }
 }
 
-// one iteration k (1-based): state `in` -> state `out`, intermediates into `it`
-int forward_iteration(fr_adjoint *h, const State &in, const State &out, const Inter &it, int k) {
+// one iteration with velocity scalars m1t, m2: state `in` -> state `out`, intermediates into `it`
+int forward_iteration(fr_adjoint *h, const State &in, const State &out, const Inter &it, double m1t, double m2) {
     const Dims d = h->d;
     const int g = h->grid;
     hipStream_t s = h->stream;
@@ -996,16 +1085,17 @@ int forward_iteration(fr_adjoint *h, const State &in, const State &out, const In
     frd::f_interp<<<g, TPB, 0, s>>>(d, it.L, h->T, in.pinv, it.Lo, it.Tp);
     frd::f_potential<<<g, TPB, 0, s>>>(d, h->S, h->gS, it.Tp, it.Lo, it.J, it.gU, h->w_reg);
     frd::f_vel_terms<<<g, TPB, 0, s>>>(d, in.v, it.dvv, it.dlv);
-    frd::f_velocity<<<g, TPB, 0, s>>>(d, in.v, it.dvv, it.dlv, out.rho, it.gU, out.v, a, h->m1t[k - 1], h->tau,
-                                      h->m2[k - 1]);
+    frd::f_velocity<<<g, TPB, 0, s>>>(d, in.v, it.dvv, it.dlv, out.rho, it.gU, out.v, a, m1t, h->tau, m2);
     frd::f_psi<<<g, TPB, 0, s>>>(d, in.psi, out.v, out.psi, a);
     frd::f_psi_inv<<<g, TPB, 0, s>>>(d, in.pinv, it.J, out.v, out.pinv, a);
     ADJ_TRY(hipGetLastError());
     return FR_OK;
 }
 
-// adjoint of iteration k: the adjoint of state `nx` (grho[cur], gv, gpsi, gpinv) becomes that of `in`
-int backward_iteration(fr_adjoint *h, const State &in, const State &nx, const Inter &it, int k, int *cur) {
+// adjoint of one iteration (velocity scalars m1t, m2): the adjoint of state `nx` (grho[cur], gv,
+// gpsi, gpinv) becomes that of `in`
+int backward_iteration(fr_adjoint *h, const State &in, const State &nx, const Inter &it, double m1t, double m2,
+                       int *cur) {
     const Dims d = h->d;
     const int g = h->grid;
     hipStream_t s = h->stream;
@@ -1014,7 +1104,7 @@ int backward_iteration(fr_adjoint *h, const State &in, const State &nx, const In
     frd::b_step1<<<g, TPB, 0, s>>>(d, it.J, nx.v, in.psi, h->gpinv, h->gJ, h->gv, h->gpsi, h->code, h->abc, h->gw, a);
     frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.psi, h->gw, h->gv);
     frd::b_step3<<<g, TPB, 0, s>>>(d, h->code, h->abc, h->gw, nx.rho, it.dlv, it.gU, h->gv, h->gdvv, h->gdlv, h->ggU,
-                                   gr1, a, h->m1t[k - 1], h->tau, h->m2[k - 1], h->scal + 2);
+                                   gr1, a, m1t, h->tau, m2, h->scal + 2);
     frd::b_step4<<<g, TPB, 0, s>>>(d, in.v, h->gdvv, h->gdlv, h->gv);
     frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.Tp, it.Lo, it.L, it.J, in.pinv, h->ggU, h->gJ, h->gpinv,
                                    h->gL, h->code, h->abc, h->gw, h->w_reg, h->scal + 1);
@@ -1138,7 +1228,7 @@ extern "C" int fr_adjoint_forward(fr_adjoint *h, const double *d_v0, double w_re
     State in = h->ck[0];
     for (int k = 1; k <= h->K; ++k) {
         const State out = k == h->K ? h->fin : (k % h->ckpt == 0 ? h->ck[k / h->ckpt] : h->nxt[(k - 1) % 2]);
-        if (int rc = forward_iteration(h, in, out, h->inter[0], k)) return rc;
+        if (int rc = forward_iteration(h, in, out, h->inter[0], h->m1t[k - 1], h->m2[k - 1])) return rc;
         in = out;
     }
     frd::f_objective<<<h->grid, TPB, 0, s>>>(h->d, h->T, h->S, h->LT, h->LS, h->no_labels, h->lw, h->fin.psi,
@@ -1173,11 +1263,14 @@ extern "C" int fr_adjoint_backward(fr_adjoint *h, double *d_grad_v0, double *gra
         // recompute the segment's states and intermediates from its checkpoint
         for (int j = 0; j < len; ++j) {
             const State in = j == 0 ? h->ck[sg] : h->nxt[j - 1];
-            if (int rc = forward_iteration(h, in, h->nxt[j], h->inter[j], start + j + 1)) return rc;
+            const int k = start + j + 1;
+            if (int rc = forward_iteration(h, in, h->nxt[j], h->inter[j], h->m1t[k - 1], h->m2[k - 1])) return rc;
         }
         for (int j = len - 1; j >= 0; --j) {
             const State in = j == 0 ? h->ck[sg] : h->nxt[j - 1];
-            if (int rc = backward_iteration(h, in, h->nxt[j], h->inter[j], start + j + 1, &cur)) return rc;
+            const int k = start + j + 1;
+            if (int rc = backward_iteration(h, in, h->nxt[j], h->inter[j], h->m1t[k - 1], h->m2[k - 1], &cur))
+                return rc;
         }
     }
     ADJ_TRY(hipEventRecord(h->ev[1], s));
@@ -1206,5 +1299,145 @@ extern "C" int fr_adjoint_info(fr_adjoint *h, size_t *device_bytes, float *forwa
     if (device_bytes) *device_bytes = h->bytes;
     if (forward_ms) *forward_ms = h->fwd_ms;
     if (backward_ms) *backward_ms = h->bwd_ms;
+    return FR_OK;
+}
+
+// ---- per-operator forward/backward pairs ---------------------------------------------------------
+
+namespace {
+
+int op_dims(int X, int Y, int Z, Dims *d, long *n, int *grid) {
+    if (X < 3 || Y < 3 || Z < 3) return err(FR_ERR_INVALID, "dims must be >= 3 along every axis (got %d %d %d)", X, Y, Z);
+    *d = Dims{X, Y, Z};
+    *n = (long)X * Y * Z;
+    const long nb = (*n + TPB - 1) / TPB;
+    *grid = (int)(nb < 8192 ? nb : 8192);
+    return FR_OK;
+}
+
+}  // namespace
+
+#define OP_PROLOGUE(...)                                                      \
+    Dims d;                                                                   \
+    long n;                                                                   \
+    int g;                                                                    \
+    if (int rc = op_dims(X, Y, Z, &d, &n, &g)) return rc;                     \
+    if (!(__VA_ARGS__)) return err(FR_ERR_INVALID, "null device pointer");    \
+    (void)n
+
+extern "C" int fr_warp_fwd(int X, int Y, int Z, const double *d_f, const double *d_pos, double *d_out) {
+    OP_PROLOGUE(d_f && d_pos && d_out);
+    frd::op_warp_fwd<<<g, TPB>>>(d, d_f, d_pos, d_out);
+    ADJ_TRY(hipGetLastError());
+    ADJ_TRY(hipDeviceSynchronize());
+    return FR_OK;
+}
+
+extern "C" int fr_warp_bwd(int X, int Y, int Z, const double *d_f, const double *d_pos, const double *d_grad_out,
+                           double *d_grad_f, double *d_grad_pos) {
+    OP_PROLOGUE(d_f && d_pos && d_grad_out && d_grad_f && d_grad_pos);
+    frd::op_warp_bwd<<<g, TPB>>>(d, d_f, d_pos, d_grad_out, d_grad_f, d_grad_pos);
+    ADJ_TRY(hipGetLastError());
+    ADJ_TRY(hipDeviceSynchronize());
+    return FR_OK;
+}
+
+extern "C" int fr_gradient_fwd(int X, int Y, int Z, const double *d_f, double *d_grad) {
+    OP_PROLOGUE(d_f && d_grad);
+    frd::op_grad_fwd<<<g, TPB>>>(d, d_f, d_grad);
+    ADJ_TRY(hipGetLastError());
+    ADJ_TRY(hipDeviceSynchronize());
+    return FR_OK;
+}
+
+extern "C" int fr_gradient_bwd(int X, int Y, int Z, const double *d_grad_grad, double *d_grad_f) {
+    OP_PROLOGUE(d_grad_grad && d_grad_f);
+    frd::op_grad_bwd<<<g, TPB>>>(d, d_grad_grad, d_grad_f);
+    ADJ_TRY(hipGetLastError());
+    ADJ_TRY(hipDeviceSynchronize());
+    return FR_OK;
+}
+
+extern "C" int fr_laplacian_fwd(int X, int Y, int Z, const double *d_psi, double *d_L) {
+    OP_PROLOGUE(d_psi && d_L);
+    frd::f_laplacian<<<g, TPB>>>(d, d_psi, d_L);
+    ADJ_TRY(hipGetLastError());
+    ADJ_TRY(hipDeviceSynchronize());
+    return FR_OK;
+}
+
+extern "C" int fr_laplacian_bwd(int X, int Y, int Z, const double *d_grad_L, double *d_grad_psi) {
+    OP_PROLOGUE(d_grad_L && d_grad_psi);
+    frd::op_laplacian_bwd<<<g, TPB>>>(d, d_grad_L, d_grad_psi);
+    ADJ_TRY(hipGetLastError());
+    ADJ_TRY(hipDeviceSynchronize());
+    return FR_OK;
+}
+
+namespace {
+
+int step_check(fr_adjoint *h, int k, double w_reg, double tau) {
+    if (!h) return err(FR_ERR_INVALID, "null handle");
+    if (!h->have_inputs) return err(FR_ERR_STATE, "step before fr_adjoint_set_images");
+    if (k < 1) return err(FR_ERR_INVALID, "iteration number must be >= 1 (got %d)", k);
+    if (!std::isfinite(w_reg) || !std::isfinite(tau)) return err(FR_ERR_INVALID, "w_reg and tau must be finite");
+    ADJ_TRY(hipDeviceSynchronize());  // the caller's arrays may have been written on any stream
+    return FR_OK;
+}
+
+}  // namespace
+
+extern "C" int fr_step_fwd(fr_adjoint *h, int k, double w_reg, double tau, const double *d_rho, const double *d_v,
+                           const double *d_psi, const double *d_psi_inv, double *d_rho1, double *d_v1, double *d_psi1,
+                           double *d_psi_inv1) {
+    if (int rc = step_check(h, k, w_reg, tau)) return rc;
+    if (!d_rho || !d_v || !d_psi || !d_psi_inv || !d_rho1 || !d_v1 || !d_psi1 || !d_psi_inv1)
+        return err(FR_ERR_INVALID, "null device pointer");
+    h->w_reg = w_reg;
+    h->tau = tau;
+    h->have_forward = false;  // the work buffers are reused
+    double m1t, m2;
+    scalars_of(h, t_of(h, k), &m1t, &m2);
+    const State in{const_cast<double *>(d_rho), const_cast<double *>(d_v), const_cast<double *>(d_psi),
+                   const_cast<double *>(d_psi_inv)};
+    const State out{d_rho1, d_v1, d_psi1, d_psi_inv1};
+    if (int rc = forward_iteration(h, in, out, h->inter[0], m1t, m2)) return rc;
+    ADJ_TRY(hipStreamSynchronize(h->stream));
+    return FR_OK;
+}
+
+extern "C" int fr_step_bwd(fr_adjoint *h, int k, double w_reg, double tau, const double *d_rho, const double *d_v,
+                           const double *d_psi, const double *d_psi_inv, double *d_g_rho, double *d_g_v,
+                           double *d_g_psi, double *d_g_psi_inv, double *grad_w_reg, double *grad_tau) {
+    if (int rc = step_check(h, k, w_reg, tau)) return rc;
+    if (!d_rho || !d_v || !d_psi || !d_psi_inv || !d_g_rho || !d_g_v || !d_g_psi || !d_g_psi_inv)
+        return err(FR_ERR_INVALID, "null device pointer");
+    h->w_reg = w_reg;
+    h->tau = tau;
+    h->have_forward = false;  // the work buffers are reused
+    double m1t, m2;
+    scalars_of(h, t_of(h, k), &m1t, &m2);
+    hipStream_t s = h->stream;
+    const size_t n = (size_t)h->n;
+    const State in{const_cast<double *>(d_rho), const_cast<double *>(d_v), const_cast<double *>(d_psi),
+                   const_cast<double *>(d_psi_inv)};
+    const State nx = h->nxt[0];
+    if (int rc = forward_iteration(h, in, nx, h->inter[0], m1t, m2)) return rc;
+    int cur = 0;
+    ADJ_TRY(hipMemcpyAsync(h->grho[cur], d_g_rho, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    ADJ_TRY(hipMemcpyAsync(h->gv, d_g_v, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    ADJ_TRY(hipMemcpyAsync(h->gpsi, d_g_psi, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    ADJ_TRY(hipMemcpyAsync(h->gpinv, d_g_psi_inv, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    ADJ_TRY(hipMemsetAsync(h->scal + 1, 0, 2 * sizeof(double), s));
+    if (int rc = backward_iteration(h, in, nx, h->inter[0], m1t, m2, &cur)) return rc;
+    ADJ_TRY(hipMemcpyAsync(d_g_rho, h->grho[cur], n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    ADJ_TRY(hipMemcpyAsync(d_g_v, h->gv, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    ADJ_TRY(hipMemcpyAsync(d_g_psi, h->gpsi, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    ADJ_TRY(hipMemcpyAsync(d_g_psi_inv, h->gpinv, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    double sc[2] = {0, 0};
+    ADJ_TRY(hipMemcpyAsync(sc, h->scal + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    ADJ_TRY(hipStreamSynchronize(s));
+    if (grad_w_reg) *grad_w_reg = sc[0];
+    if (grad_tau) *grad_tau = sc[1];
     return FR_OK;
 }

@@ -266,6 +266,38 @@ int fr_adjoint_get_state(fr_adjoint *h, int k, double *d_v, double *d_rho, doubl
 /* device bytes held by the context; device time of the last forward and backward calls */
 int fr_adjoint_info(fr_adjoint *h, size_t *device_bytes, float *forward_ms, float *backward_ms);
 
+/* One solver iteration as a forward/backward pair on caller states (device pointers; rho N
+ * doubles, v / psi / psi_inv N*3), with the images of h, the given W_REG and TAU and the t of
+ * iteration k (t_1 = ALPHA, RESET_ITER as the loop does).  fr_step_fwd writes the next state;
+ * fr_step_bwd takes the adjoint of the next state in d_g_* and leaves there the adjoint of the
+ * state, and returns dE/dW_REG and dE/dTAU of this iteration.  Both reuse h's work buffers, so
+ * fr_adjoint_backward needs a new fr_adjoint_forward afterwards.  No counterpart in the reference
+ * (one iteration of src/fastreg/cuda/solver.cu:335-407, differentiated). */
+int fr_step_fwd(fr_adjoint *h, int k, double w_reg, double tau, const double *d_rho, const double *d_v,
+                const double *d_psi, const double *d_psi_inv, double *d_rho1, double *d_v1, double *d_psi1,
+                double *d_psi_inv1);
+int fr_step_bwd(fr_adjoint *h, int k, double w_reg, double tau, const double *d_rho, const double *d_v,
+                const double *d_psi, const double *d_psi_inv, double *d_g_rho, double *d_g_v, double *d_g_psi,
+                double *d_g_psi_inv, double *grad_w_reg, double *grad_tau);
+
+/* Single operators of the iteration as FP64 forward/backward pairs over an X x Y x Z grid (device
+ * pointers; every call waits).  The backward functions add into their outputs (clear them first
+ * for a plain gradient).  No counterpart in the reference, which has no gradients; each cites the
+ * forward operator it differentiates.
+ * Trilinear sample at positions (N*3), clamped into the grid, all voxels (cuda_utils.hpp:60-136):
+ * out(p) = trilerp(f, pos(p)); backward: d_grad_f += the transpose (FP64 atomics), d_grad_pos +=
+ * d_grad_out * d trilerp / d pos (0 along clamped axes). */
+int fr_warp_fwd(int X, int Y, int Z, const double *d_f, const double *d_pos, double *d_out);
+int fr_warp_bwd(int X, int Y, int Z, const double *d_f, const double *d_pos, const double *d_grad_out,
+                double *d_grad_f, double *d_grad_pos);
+/* central-difference gradient of a scalar volume, (f(+1) - f(-1)) / 2 per axis on the interior,
+ * 0 elsewhere (N*3 out; vector_fields.cu:155-192) */
+int fr_gradient_fwd(int X, int Y, int Z, const double *d_f, double *d_grad);
+int fr_gradient_bwd(int X, int Y, int Z, const double *d_grad_grad, double *d_grad_f);
+/* -Laplacian of a vector field (N*3) on the interior, 0 elsewhere (vector_fields.cu:275-300) */
+int fr_laplacian_fwd(int X, int Y, int Z, const double *d_psi, double *d_L);
+int fr_laplacian_bwd(int X, int Y, int Z, const double *d_grad_L, double *d_grad_psi);
+
 /* ---- device memory plumbing (for callers without a HIP binding of their own) -------------- */
 
 int fr_dev_malloc(void **d_ptr, size_t bytes);
