@@ -60,11 +60,14 @@ struct Corner {
     float a, b, c;
 };
 
+// The clamp fminf(fmaxf(0, p), dim - 1) is one v_med3_f32: the median of (p, 0, dim - 1) is the
+// clamped value for every number, and for a NaN p med3 returns the minimum of the other two, 0,
+// which is what fmaxf(0, NaN) gives.
 __device__ __forceinline__ Corner corner(Dims d, float px, float py, float pz) {
     Corner k;
-    float cx = fminf(fmaxf(0.f, px), (float)(d.X - 1));
-    float cy = fminf(fmaxf(0.f, py), (float)(d.Y - 1));
-    float cz = fminf(fmaxf(0.f, pz), (float)(d.Z - 1));
+    float cx = __builtin_amdgcn_fmed3f(px, 0.f, (float)(d.X - 1));
+    float cy = __builtin_amdgcn_fmed3f(py, 0.f, (float)(d.Y - 1));
+    float cz = __builtin_amdgcn_fmed3f(pz, 0.f, (float)(d.Z - 1));
     k.gx = (int)floorf(cx);
     k.gy = (int)floorf(cy);
     k.gz = (int)floorf(cz);
@@ -75,6 +78,12 @@ __device__ __forceinline__ Corner corner(Dims d, float px, float py, float pz) {
     k.hy = min(k.gy + 1, d.Y - 1);
     k.hz = min(k.gz + 1, d.Z - 1);
     return k;
+}
+
+// q |q|, and 0 for a NaN q
+__device__ __forceinline__ float sq_signed(float q) {
+    const float s = q * fabsf(q);
+    return q == q ? s : 0.f;
 }
 
 __device__ __forceinline__ int lin(Dims d, int x, int y, int z) { return x + d.X * (y + d.Y * z); }
@@ -586,19 +595,19 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     // 6. (Dv)v (vector_fields.cu:352-404)
     const F3 xp = v_xp, xm = v_xm, yp = v_yp, ym = v_ym;
     const F3 zp = c.v_p1, zm = c.v_m1;
-#define MX(q) fmaxf((q), 0.f)
-#define MN(q) fminf((q), 0.f)
-    const float ax1 = ((MX(vc.x) * MX(vc.x) - MN(vc.x) * MN(vc.x)) + MN(xp.x) * MN(xp.x)) - MX(xp.x) * MX(xp.x);
+    // The reference's diagonal terms ((MX(u)^2 - MN(u)^2) + MN(w)^2) - MX(w)^2, MX = fmaxf(., 0),
+    // MN = fminf(., 0), are sq(u) - sq(w) with sq(q) = q |q| bit for bit: one of MX(q)^2, MN(q)^2 is
+    // the rounded q^2 and the other +0, q |q| is +-(rounded q^2), and a - (-b) == a + b.  A NaN q
+    // gives 0 in the reference's form (fmaxf and fminf drop it), and so in sq.
+    const float ax1 = sq_signed(vc.x) - sq_signed(xp.x);
     const float ax2 = vc.y * ((vc.y > 0.f) ? (vc.x - ym.x) : (yp.x - vc.x));
     const float ax3 = vc.z * ((vc.z > 0.f) ? (vc.x - zm.x) : (zp.x - vc.x));
     const float ay1 = vc.x * ((vc.x > 0.f) ? (vc.y - xm.y) : (xp.y - vc.y));
-    const float ay2 = ((MX(vc.y) * MX(vc.y) - MN(vc.y) * MN(vc.y)) + MN(yp.y) * MN(yp.y)) - MX(yp.y) * MX(yp.y);
+    const float ay2 = sq_signed(vc.y) - sq_signed(yp.y);
     const float ay3 = vc.z * ((vc.z > 0.f) ? (vc.y - zm.y) : (zp.y - vc.y));
     const float az1 = vc.x * ((vc.x > 0.f) ? (vc.z - xm.z) : (xp.z - vc.z));
     const float az2 = vc.y * ((vc.y > 0.f) ? (vc.z - ym.z) : (yp.z - vc.z));
-    const float az3 = ((MX(vc.z) * MX(vc.z) - MN(vc.z) * MN(vc.z)) + MN(zp.z) * MN(zp.z)) - MX(zp.z) * MX(zp.z);
-#undef MX
-#undef MN
+    const float az3 = sq_signed(vc.z) - sq_signed(zp.z);
     const F3 dvv = f3((0.5f * ax1 + ax2) + ax3, (ay1 + 0.5f * ay2) + ay3, (az1 + az2) + 0.5f * az3);
 
     // 7. upwind velocity Laplacian, the reference's indices and ranges (vector_fields.cu:307-346);

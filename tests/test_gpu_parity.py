@@ -54,6 +54,10 @@ def test_warp_bit_exact(oracle_mod, dims, seed):
     psi[::7, :3] = np.round(psi[::7, :3])          # exact integers hit the weight-0 neighbour
     psi[::11, 0] = X - 1                           # clamp edge
     psi[::13, 1] = -5.0                            # below the grid
+    psi[5::17, 0] = np.nan                         # non-finite coordinates clamp as fminf(fmaxf(0, p), X-1)
+    psi[6::19, 1] = np.inf
+    psi[7::23, 2] = -np.inf
+    psi[8::29, 0] = 3.0e30
     g_img, g_lab = fr.warp(img, lab, psi, dims)
     phi2 = np.stack([img, lab.astype(np.float32)], 1)
     o = O.apply(dims, phi2, psi)
