@@ -317,7 +317,9 @@ def adjoint_workload(args):
     objective) and of the backward pass (segment recompute + adjoint kernels) and their ratio;
     device memory of the context at checkpoint intervals 1, ckpt and K; central finite differences
     of the forward pass along `--adj-dirs` random directions at eps 1e-4 and at eps 1e-7 against the
-    adjoint's directional derivative, and for W_REG and TAU; and, in the CPU leg, the reference-path
+    adjoint's directional derivative, and for W_REG and TAU; the same at eps 1e-4 on the ramp pair
+    from a small positive v0, where no branch switches (`finite_differences_without_kinks`); and,
+    in the CPU leg, the reference-path
     gradient (torch autograd of the unrolled restatement, FP64, host cores) at 128^3 with
     `--cpu-adj-k` iterations against the adjoint at the same K."""
     import numpy as np
@@ -362,28 +364,46 @@ def adjoint_workload(args):
             a.close()
     a = main_ctx
 
-    def fd_check(eps, ndir, seed):
+    def fd_check(h, v, grad, eps, ndir, seed):
         r = np.random.default_rng(seed)
         rels = []
         for _ in range(ndir):
             d = r.normal(size=(n, 3))
-            fd = (a.forward(v0 + eps * d, w_reg, tau, lw) - a.forward(v0 - eps * d, w_reg, tau, lw)) / (2 * eps)
-            an = float((G * d).sum())
+            fd = (h.forward(v + eps * d, w_reg, tau, lw) - h.forward(v - eps * d, w_reg, tau, lw)) / (2 * eps)
+            an = float((grad * d).sum())
             rels.append(abs(fd - an) / abs(an))
         rels = np.asarray(rels)
         return {"eps": eps, "directions": ndir, "median_rel_err": float(np.median(rels)),
                 "max_rel_err": float(rels.max()), "frac_below_1e-3": float(np.mean(rels < 1e-3))}
 
+    def fd_scalars(h, v, eps, gw, gt):
+        fw = (h.forward(v, w_reg + eps, tau, lw) - h.forward(v, w_reg - eps, tau, lw)) / (2 * eps)
+        ft = (h.forward(v, w_reg, tau + eps, lw) - h.forward(v, w_reg, tau - eps, lw)) / (2 * eps)
+        return {"w_reg": {"fd": fw, "adjoint": gw, "rel_err": abs(fw - gw) / abs(gw)},
+                "tau": {"fd": ft, "adjoint": gt, "rel_err": abs(ft - gt) / abs(gt)}}
+
     fd = {}
     for eps in (1e-4, 1e-7):
-        fd[f"v0_eps_{eps:g}"] = fd_check(eps, args.adj_dirs, 55)
+        fd[f"v0_eps_{eps:g}"] = fd_check(a, v0, G, eps, args.adj_dirs, 55)
         print(f"[bench] FD {fd[f'v0_eps_{eps:g}']}", file=sys.stderr, flush=True)
     for eps in (1e-4, 1e-6):
-        fw = (a.forward(v0, w_reg + eps, tau, lw) - a.forward(v0, w_reg - eps, tau, lw)) / (2 * eps)
-        ft = (a.forward(v0, w_reg, tau + eps, lw) - a.forward(v0, w_reg, tau - eps, lw)) / (2 * eps)
-        fd[f"scalars_eps_{eps:g}"] = {"w_reg": {"fd": fw, "adjoint": GW, "rel_err": abs(fw - GW) / abs(GW)},
-                                      "tau": {"fd": ft, "adjoint": GT, "rel_err": abs(ft - GT) / abs(GT)}}
+        fd[f"scalars_eps_{eps:g}"] = fd_scalars(a, v0, eps, GW, GT)
     a.close()
+
+    # the same checks where no branch switches near the evaluation point: the ramp pair from a
+    # small positive v0 (every velocity component stays positive, psi ahead of and psi_inv behind
+    # the grid nodes), at the full step of 1e-4
+    ramp = phantoms.ramp((Z, Y, X))
+    rv0 = phantoms.ramp_velocity(dims, 5)
+    p = fr.default_params(vol_dims=dims, no_labels=ramp.no_labels, alpha=alpha)
+    r = fr.Adjoint(p, K, ckpt)
+    r.set_images(ramp.target, ramp.target_labels, ramp.source, ramp.source_labels)
+    RE = r.forward(rv0, w_reg, tau, lw)
+    RG, RGW, RGT = r.backward()
+    fd_ramp = {"objective": RE, "v0_eps_1e-4": fd_check(r, rv0, RG, 1e-4, args.adj_dirs, 56),
+               "scalars_eps_1e-4": fd_scalars(r, rv0, 1e-4, RGW, RGT)}
+    print(f"[bench] FD ramp {fd_ramp['v0_eps_1e-4']}", file=sys.stderr, flush=True)
+    r.close()
 
     cpu = None
     if not args.no_cpu:
@@ -411,6 +431,7 @@ def adjoint_workload(args):
         "times_by_ckpt": runs,
         "memory_saving_vs_keeping_every_iteration": round(no_ckpt_bytes / runs[ckpt]["device_bytes"], 2),
         "finite_differences": fd,
+        "finite_differences_without_kinks": fd_ramp,
         "vs_baseline": None,
     }
     if cpu:

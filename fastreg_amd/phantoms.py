@@ -15,6 +15,12 @@ The workloads the solver is measured and tested on:
   ellipsoid.  The source is made with 20 RBF centres (sigma 12, max displacement 6 voxels) and
   7 squaring steps.
 
+* ``ramp``: a pair on which the differentiated solver has no branch switch near the evaluation
+  point: source = slope * (x + y + z), target = source + shift, labels from the octant halves in
+  x and y.  The image force then has one sign everywhere, so from a small positive initial
+  velocity every component of v stays positive, psi moves forward and psi_inv backward of the
+  grid nodes, and no upwind branch, trilinear cell or clamp changes under a small perturbation.
+
 Arrays are returned in the solver's layout: float32 / int16 with shape (Z, Y, X), so that
 ``a.ravel()`` is x-fastest.
 """
@@ -199,3 +205,20 @@ def smooth_velocity(dims, seed, amp=1.0, offset=0.1, sigma=1.5):
     rng = np.random.default_rng(seed)
     f = np.stack([ndimage.gaussian_filter(rng.normal(size=(Z, Y, X)), sigma) for _ in range(3)], -1).reshape(-1, 3)
     return f * (amp / np.abs(f).max()) + offset
+
+
+def ramp(shape=(128, 128, 128), slope=1.0, shift=1.0):
+    """The ramp pair (see the module docstring); 3 labels (1..3)."""
+    z, y, x = np.meshgrid(*(np.arange(k) for k in shape), indexing="ij")
+    Z, Y, X = shape
+    src = (slope * (x + y + z)).astype(np.float32)
+    tgt = (src + np.float32(shift)).astype(np.float32)
+    lab = (1 + (x >= X // 2) + (y >= Y // 2)).astype(np.int16)
+    return Pair(tgt, lab, src, lab.copy(), 4)
+
+
+def ramp_velocity(dims, seed, base=5e-4, spread=2e-4):
+    """A small positive initial velocity for the ramp pair: base + spread * U(0, 1) per component,
+    (N, 3) float64, x-fastest."""
+    X, Y, Z = dims
+    return base + spread * np.random.default_rng(seed).random((X * Y * Z, 3))

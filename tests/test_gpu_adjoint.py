@@ -221,6 +221,44 @@ def test_gradient_matches_central_finite_differences():
         a.close()
 
 
+def test_gradient_matches_finite_differences_at_eps_1e4_on_the_ramp_pair():
+    """Central differences at the full step of 1e-4 where no branch switches: on the ramp pair,
+    from a small positive v0, every velocity component stays positive, psi stays ahead of and
+    psi_inv behind the grid nodes, so no upwind branch, trilinear cell or clamp changes between
+    the two evaluations, and FD agrees with the adjoint to 1e-6 (plus the rounding of E over the
+    step)."""
+    dims, K, ckpt, lw = (40, 36, 32), 32, 8, 1.0
+    say("adjoint vs FD on the ramp pair", dims, K, ckpt)
+    X, Y, Z = dims
+    n = X * Y * Z
+    pair = phantoms.ramp((Z, Y, X))
+    v0 = phantoms.ramp_velocity(dims, 3)
+    a = _adjoint(dims, pair, K, ckpt)
+    try:
+        e = a.forward(v0, W_REG, TAU, lw)
+        g, gw, gt = a.backward()
+        eps = 1e-4
+
+        def close(fd, an):
+            return abs(fd - an) <= 1e-6 * abs(an) + 8 * abs(e) * 2.2e-16 / eps
+
+        rng = np.random.default_rng(10)
+        for _ in range(4):
+            d = rng.normal(size=(n, 3))
+            fd = (a.forward(v0 + eps * d, W_REG, TAU, lw) - a.forward(v0 - eps * d, W_REG, TAU, lw)) / (2 * eps)
+            an = float((g * d).sum())
+            say("direction", fd, an)
+            assert close(fd, an), (fd, an)
+        fd = (a.forward(v0, W_REG + eps, TAU, lw) - a.forward(v0, W_REG - eps, TAU, lw)) / (2 * eps)
+        say("w_reg", fd, gw)
+        assert close(fd, gw), (fd, gw)
+        fd = (a.forward(v0, W_REG, TAU + eps, lw) - a.forward(v0, W_REG, TAU - eps, lw)) / (2 * eps)
+        say("tau", fd, gt)
+        assert close(fd, gt), (fd, gt)
+    finally:
+        a.close()
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     dims, K, lw = (22, 17, 15), 9, 0.25
     say("adjoint checkpoint intervals", dims, K)

@@ -123,3 +123,53 @@ def test_unrolled_gradient_matches_oracle_finite_differences(oracle_mod):
         lo[kw] -= eps_s
         fd = (E_oracle(v0, **hi) - E_oracle(v0, **lo)) / (2 * eps_s)
         assert abs(fd - grad) <= 1e-4 * abs(grad), (name, fd, grad)
+
+
+def test_unrolled_gradient_matches_finite_differences_at_eps_1e4_on_the_ramp_pair(oracle_mod):
+    """On the ramp pair from a small positive v0 no branch switches near the evaluation point
+    (every v component stays positive, psi ahead of and psi_inv behind the grid nodes), so central
+    differences at the full step of 1e-4 agree with the gradient to ~1e-8 (FD's own truncation).
+    The same check on the blob pair fails at this step: it crosses upwind and trilinear kinks."""
+    from oracle import unrolled as U
+    dims = (12, 11, 10)
+    n = 12 * 11 * 10
+    K, lw = 16, 1.0
+    X, Y, Z = dims
+    pair = phantoms.ramp((Z, Y, X))
+    v0 = phantoms.ramp_velocity(dims, 2)
+    g_ = U._Grid(dims)
+    T = torch.tensor(pair.target.reshape(n).astype(np.float64))
+    S = torch.tensor(pair.source.reshape(n).astype(np.float64))
+    LT = torch.tensor(pair.target_labels.reshape(n).astype(np.int64))
+    LS = torch.tensor(pair.source_labels.reshape(n).astype(np.int64))
+
+    def E_oracle(v, w=W_REG, tau=TAU):
+        psi = _oracle_states(oracle_mod, dims, pair, v, K, w, tau)[-1]["psi"][:, :3]
+        return U.objective(g_, torch.tensor(psi), T, S, LT, LS, pair.no_labels, lw).item()
+
+    v0t = torch.tensor(v0, requires_grad=True)
+    wt = torch.tensor(W_REG, dtype=torch.float64, requires_grad=True)
+    tt = torch.tensor(TAU, dtype=torch.float64, requires_grad=True)
+    E, states = U.unrolled(dims, pair.target, pair.target_labels, pair.source, pair.source_labels, v0t, wt, tt, K,
+                           alpha=ALPHA, no_labels=pair.no_labels, label_weight=lw, keep_states=True)
+    I = g_.I
+    assert min(float(st[1][I].min()) for st in states) > 0  # no velocity component reaches a kink
+    E.backward()
+    g = v0t.grad.numpy()
+    eps = 1e-4
+
+    def close(fd, an):  # 1e-6 relative, plus the rounding of E over the step
+        return abs(fd - an) <= 1e-6 * abs(an) + 8 * abs(E.item()) * 2.2e-16 / eps
+
+    rng = np.random.default_rng(9)
+    for _ in range(4):
+        d = rng.normal(size=(n, 3))
+        fd = (E_oracle(v0 + eps * d) - E_oracle(v0 - eps * d)) / (2 * eps)
+        an = float((g * d).sum())
+        assert close(fd, an), (fd, an)
+    for name, grad, kw in (("w_reg", wt.grad.item(), "w"), ("tau", tt.grad.item(), "tau")):
+        hi, lo = dict(w=W_REG, tau=TAU), dict(w=W_REG, tau=TAU)
+        hi[kw] += eps
+        lo[kw] -= eps
+        fd = (E_oracle(v0, **hi) - E_oracle(v0, **lo)) / (2 * eps)
+        assert close(fd, grad), (name, fd, grad)
