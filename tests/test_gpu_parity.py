@@ -89,7 +89,10 @@ def _assert_state_equal(gs, os_, what=""):
                                  f"gpu {a[bad[:3]]} oracle {b[bad[:3]]}")
 
 
-@pytest.mark.parametrize("dims,seed", [((24, 20, 18), 3), ((67, 9, 11), 4), ((12, 31, 8), 5)])
+@pytest.mark.parametrize("dims,seed", [((24, 20, 18), 3), ((67, 9, 11), 4), ((12, 31, 8), 5),
+                                       # the smallest volumes and single-voxel-thick interiors, more
+                                       # than two x tiles, more z planes than one chunk
+                                       ((3, 3, 3), 11), ((3, 17, 4), 12), ((131, 3, 5), 13), ((4, 5, 66), 14)])
 def test_step_teacher_forced_bit_exact(oracle_mod, dims, seed):
     """From an identical random (smooth, large-amplitude) state, one iteration on the GPU equals
     one oracle iteration in every field and in the max update norm."""
@@ -131,6 +134,28 @@ def test_run_bit_exact_with_log(oracle_mod):
     mt, mr = o.min_det()
     st = g.stats()
     assert st.min_det_true == mt and st.min_det_ref == mr
+
+
+@pytest.mark.parametrize("dims,seed", [((3, 3, 3), 21), ((5, 4, 3), 22), ((130, 11, 4), 23)])
+def test_degenerate_volume_run_bit_exact(oracle_mod, dims, seed):
+    """Verbose 12-iteration runs on volumes with one-voxel-thick interiors: state, max-norm
+    history (argmax 0 when every update is 0), energies, outputs and Dice equal the oracle."""
+    O = oracle_mod
+    say("degenerate run", dims, seed)
+    g, o, pair, rng = _both(dims, O, seed, max_iter=12, verbosity=2, alpha=0.4)
+    g.run()
+    r = o.run(O.SolverParams(max_iter=12, verbosity=2, no_labels=pair.no_labels, alpha=0.4))
+    st = g.stats()
+    assert st.iterations == r["iterations"]
+    mh, eh = g.history()
+    assert np.array_equal(mh, r["max_hist"])
+    np.testing.assert_allclose(eh, r["energy_hist"][:len(eh)], rtol=1e-12)
+    _assert_state_equal(g.get_state(), o.get_state(), "run")
+    for a, b, nm in zip(g.finalize(), o.outputs(), ("img1", "lab1", "img2", "lab2")):
+        assert np.array_equal(a, b), nm
+    dice, ov, vx = g.dice()
+    oov, ovx, _ = o.dice_counts(pair.no_labels)
+    assert np.array_equal(ov, oov) and np.array_equal(vx, ovx)
 
 
 def test_register_matches_oracle_c1_shape(oracle_mod):
