@@ -226,8 +226,9 @@ int fr_sum(const float *d_data, long long n, double *sum, void *hip_stream);
 /* Average device time per launch of the solver's two iteration kernels (pass A: -Laplacian of psi
  * and its warp, density, velocity and psi_inv updates; pass B: psi update and max norm), from HIP
  * events between consecutive launches of `reps` iterations run in sequence on the context stream,
- * as the iteration loop runs them (the state is restored afterwards).  Any output pointer may be
- * NULL.  No counterpart in the reference: it is the per-kernel measurement of this solver. */
+ * as the iteration loop runs them, after 150 untimed iterations that let the GPU clock settle (the
+ * state is restored afterwards).  Any output pointer may be NULL.  No counterpart in the
+ * reference: it is the per-kernel measurement of this solver. */
 int fr_time_kernels(fr_ctx *ctx, int reps, float *ms_pass_a, float *ms_pass_b);
 
 /* ---- the differentiated solver: K unrolled iterations in FP64 and their adjoint ---------------
