@@ -2,8 +2,10 @@
 // for gradients of a final objective with respect to the initial velocity and the regularisation
 // weights (w_reg, tau).
 //
-// Forward: the reference's iteration as separate steps (solver.cu:295-444), in FP64 and in the
-// reference's operation order, one thread per voxel:
+// Forward: the reference's iteration (solver.cu:295-444) in FP64 and in the reference's operation
+// order, one thread per voxel, as three kernels: F3 (f_laplacian), F1 F2 F4-F7 F9 (f_step, which
+// stores the intermediates the adjoint reads only when the backward pass recomputes a segment) and
+// F8 (f_psi):
 //   F1 rho1 = rho + alpha * div(rho, v)                           (solver.cu:101-173)
 //   F2 J = D psi_inv                                              (vector_fields.cu:503-535)
 //   F3 L = -Laplacian(psi)                                        (vector_fields.cu:275-300)
@@ -164,48 +166,43 @@ __device__ __forceinline__ void det_ref_grad(const double m[3][3], double g[3][3
 
 // ---- forward steps ----------------------------------------------------------------------------
 
-// F1 (+ copy of the boundary): rho1 = rho + alpha * div on the interior
-__global__ void f_density(Dims d, const double *rho, const double *v, double *rho1, double alpha) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        double r = rho[i];
-        if (interior(d, x, y, z)) {
-            const D3 vel = ld(v, i);
-            const double p = rho[i];
-            const double t11 = vel.x > 0 ? -vel.x * p : -vel.x * rho[lin(d, x + 1, y, z)];
-            const double vxm = v[3 * lin(d, x - 1, y, z)];
-            const double t12 = vxm > 0 ? vxm * rho[lin(d, x - 1, y, z)] : vxm * p;
-            const double t21 = vel.y > 0 ? -vel.y * p : -vel.y * rho[lin(d, x, y + 1, z)];
-            const double vym = v[3 * lin(d, x, y - 1, z) + 1];
-            const double t22 = vym > 0 ? vym * rho[lin(d, x, y - 1, z)] : vym * p;
-            const double t31 = vel.z > 0 ? -vel.z * p : -vel.z * rho[lin(d, x, y, z + 1)];
-            const double vzm = v[3 * lin(d, x, y, z - 1) + 2];
-            const double t32 = vzm > 0 ? vzm * rho[lin(d, x, y, z - 1)] : vzm * p;
-            const double div = ((((t11 + t12) + t21) + t22) + t31) + t32;
-            r = fma(alpha, div, p);
-        }
-        rho1[i] = r;
-    }
+// The forward steps of one voxel.  F3 (L) needs psi at the neighbours and F8 (psi1) v1 at the
+// taps, so they are kernels of their own; the other steps of a voxel read only the old state, L
+// and the images, and run in one kernel (f_step) that keeps their results in registers.
+
+// F1: rho1 = rho + alpha * div on the interior; rho elsewhere
+__device__ __forceinline__ double fv_density(Dims d, int x, int y, int z, long i, const double *rho, const double *v,
+                                             double alpha) {
+    const double p = rho[i];
+    if (!interior(d, x, y, z)) return p;
+    const D3 vel = ld(v, i);
+    const double t11 = vel.x > 0 ? -vel.x * p : -vel.x * rho[lin(d, x + 1, y, z)];
+    const double vxm = v[3 * lin(d, x - 1, y, z)];
+    const double t12 = vxm > 0 ? vxm * rho[lin(d, x - 1, y, z)] : vxm * p;
+    const double t21 = vel.y > 0 ? -vel.y * p : -vel.y * rho[lin(d, x, y + 1, z)];
+    const double vym = v[3 * lin(d, x, y - 1, z) + 1];
+    const double t22 = vym > 0 ? vym * rho[lin(d, x, y - 1, z)] : vym * p;
+    const double t31 = vel.z > 0 ? -vel.z * p : -vel.z * rho[lin(d, x, y, z + 1)];
+    const double vzm = v[3 * lin(d, x, y, z - 1) + 2];
+    const double t32 = vzm > 0 ? vzm * rho[lin(d, x, y, z - 1)] : vzm * p;
+    const double div = ((((t11 + t12) + t21) + t22) + t31) + t32;
+    return fma(alpha, div, p);
 }
 
-// F2: J = D psi_inv (rows = components, columns = d/dx, d/dy, d/dz), interior; 0 elsewhere
-__global__ void f_jacobian(Dims d, const double *pinv, double *J) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        if (interior(d, x, y, z)) {
-            const D3 xp = ld(pinv, lin(d, x + 1, y, z)), xm = ld(pinv, lin(d, x - 1, y, z));
-            const D3 yp = ld(pinv, lin(d, x, y + 1, z)), ym = ld(pinv, lin(d, x, y - 1, z));
-            const D3 zp = ld(pinv, lin(d, x, y, z + 1)), zm = ld(pinv, lin(d, x, y, z - 1));
-            const D3 jx = d3((xp.x - xm.x) * 0.5, (xp.y - xm.y) * 0.5, (xp.z - xm.z) * 0.5);
-            const D3 jy = d3((yp.x - ym.x) * 0.5, (yp.y - ym.y) * 0.5, (yp.z - ym.z) * 0.5);
-            const D3 jz = d3((zp.x - zm.x) * 0.5, (zp.y - zm.y) * 0.5, (zp.z - zm.z) * 0.5);
-            m[0] = jx.x; m[1] = jy.x; m[2] = jz.x;
-            m[3] = jx.y; m[4] = jy.y; m[5] = jz.y;
-            m[6] = jx.z; m[7] = jy.z; m[8] = jz.z;
-        }
-        for (int k = 0; k < 9; ++k) J[9 * i + k] = m[k];
-    }
+// F2: J = D psi_inv (m[3 r + c]: row r = component, column c = d/dx, d/dy, d/dz), interior; 0
+// elsewhere
+__device__ __forceinline__ void fv_jacobian(Dims d, int x, int y, int z, const double *pinv, double m[9]) {
+    for (int k = 0; k < 9; ++k) m[k] = 0.0;
+    if (!interior(d, x, y, z)) return;
+    const D3 xp = ld(pinv, lin(d, x + 1, y, z)), xm = ld(pinv, lin(d, x - 1, y, z));
+    const D3 yp = ld(pinv, lin(d, x, y + 1, z)), ym = ld(pinv, lin(d, x, y - 1, z));
+    const D3 zp = ld(pinv, lin(d, x, y, z + 1)), zm = ld(pinv, lin(d, x, y, z - 1));
+    const D3 jx = d3((xp.x - xm.x) * 0.5, (xp.y - xm.y) * 0.5, (xp.z - xm.z) * 0.5);
+    const D3 jy = d3((yp.x - ym.x) * 0.5, (yp.y - ym.y) * 0.5, (yp.z - ym.z) * 0.5);
+    const D3 jz = d3((zp.x - zm.x) * 0.5, (zp.y - zm.y) * 0.5, (zp.z - zm.z) * 0.5);
+    m[0] = jx.x; m[1] = jy.x; m[2] = jz.x;
+    m[3] = jx.y; m[4] = jy.y; m[5] = jz.y;
+    m[6] = jx.z; m[7] = jy.z; m[8] = jz.z;
 }
 
 // F3: L = -((((((-6 psi + psi(x+1)) + psi(x-1)) + psi(y+1)) + psi(y-1)) + psi(z+1)) + psi(z-1)), interior
@@ -228,105 +225,129 @@ __global__ void f_laplacian(Dims d, const double *psi, double *L) {
     }
 }
 
-// F4: Lo = trilerp(L, psi_inv), Tp = trilerp(T, psi_inv), all voxels
-__global__ void f_interp(Dims d, const double *L, const double *T, const double *pinv, double *Lo, double *Tp) {
-    FOR_VOXELS(d, i) {
-        const D3 p = ld(pinv, i);
-        const Corner k = corner(d, p.x, p.y, p.z);
-        double f[8];
-        D3 o;
-        gather3(L, k, 0, f);
-        o.x = tri(f, k.a, k.b, k.c);
-        gather3(L, k, 1, f);
-        o.y = tri(f, k.a, k.b, k.c);
-        gather3(L, k, 2, f);
-        o.z = tri(f, k.a, k.b, k.c);
-        st(Lo, i, o);
-        gather1(T, k, f);
-        Tp[i] = tri(f, k.a, k.b, k.c);
-    }
+// F4: Lo = trilerp(L, psi_inv), Tp = trilerp(T, psi_inv), every voxel
+__device__ __forceinline__ void fv_interp(Dims d, long i, const double *L, const double *T, const double *pinv, D3 *Lo,
+                                          double *Tp) {
+    const D3 p = ld(pinv, i);
+    const Corner k = corner(d, p.x, p.y, p.z);
+    double f[8];
+    gather3(L, k, 0, f);
+    Lo->x = tri(f, k.a, k.b, k.c);
+    gather3(L, k, 1, f);
+    Lo->y = tri(f, k.a, k.b, k.c);
+    gather3(L, k, 2, f);
+    Lo->z = tri(f, k.a, k.b, k.c);
+    gather1(T, k, f);
+    *Tp = tri(f, k.a, k.b, k.c);
 }
 
-// F5: gU = ((S - Tp) grad S + w_reg Lo) det_ref(J), interior
-__global__ void f_potential(Dims d, const double *S, const double *gS, const double *Tp, const double *Lo,
-                            const double *J, double *gU, double w_reg) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        D3 o = d3(0, 0, 0);
-        if (interior(d, x, y, z)) {
-            double m[3][3];
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) m[r][c] = J[9 * i + 3 * r + c];
-            const double det = det_ref(m), diff = S[i] - Tp[i];
-            const D3 g = ld(gS, i), lo = ld(Lo, i);
-            o = d3((g.x * diff + lo.x * w_reg) * det, (g.y * diff + lo.y * w_reg) * det, (g.z * diff + lo.z * w_reg) * det);
-        }
-        st(gU, i, o);
-    }
+// F5: gU = ((S - Tp) grad S + w_reg Lo) det_ref(J), interior; 0 elsewhere
+__device__ __forceinline__ D3 fv_potential(Dims d, int x, int y, int z, long i, const double *S, const double *gS,
+                                           double Tp, D3 lo, const double J[9], double w_reg) {
+    if (!interior(d, x, y, z)) return d3(0, 0, 0);
+    double m[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) m[r][c] = J[3 * r + c];
+    const double det = det_ref(m), diff = S[i] - Tp;
+    const D3 g = ld(gS, i);
+    return d3((g.x * diff + lo.x * w_reg) * det, (g.y * diff + lo.y * w_reg) * det, (g.z * diff + lo.z * w_reg) * det);
 }
 
 __device__ __forceinline__ double mx(double q) { return fmax(q, 0.0); }
 __device__ __forceinline__ double mn(double q) { return fmin(q, 0.0); }
 
-// F6: dvv = (Dv)v on the interior, dlv = the reference's upwind Laplacian of v on its range
-__global__ void f_vel_terms(Dims d, const double *v, double *dvv, double *dlv) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        D3 o = d3(0, 0, 0), l = d3(0, 0, 0);
-        if (interior(d, x, y, z)) {
-            const D3 c = ld(v, i), xp = ld(v, lin(d, x + 1, y, z)), xm = ld(v, lin(d, x - 1, y, z)),
-                     yp = ld(v, lin(d, x, y + 1, z)), ym = ld(v, lin(d, x, y - 1, z)), zp = ld(v, lin(d, x, y, z + 1)),
-                     zm = ld(v, lin(d, x, y, z - 1));
-            const double ax1 = ((mx(c.x) * mx(c.x) - mn(c.x) * mn(c.x)) + mn(xp.x) * mn(xp.x)) - mx(xp.x) * mx(xp.x);
-            const double ax2 = c.y > 0 ? c.y * (c.x - ym.x) : c.y * (yp.x - c.x);
-            const double ax3 = c.z > 0 ? c.z * (c.x - zm.x) : c.z * (zp.x - c.x);
-            const double ay1 = c.x > 0 ? c.x * (c.y - xm.y) : c.x * (xp.y - c.y);
-            const double ay2 = ((mx(c.y) * mx(c.y) - mn(c.y) * mn(c.y)) + mn(yp.y) * mn(yp.y)) - mx(yp.y) * mx(yp.y);
-            const double ay3 = c.z > 0 ? c.z * (c.y - zm.y) : c.z * (zp.y - c.y);
-            const double az1 = c.x > 0 ? c.x * (c.z - xm.z) : c.x * (xp.z - c.z);
-            const double az2 = c.y > 0 ? c.y * (c.z - ym.z) : c.y * (yp.z - c.z);
-            const double az3 = ((mx(c.z) * mx(c.z) - mn(c.z) * mn(c.z)) + mn(zp.z) * mn(zp.z)) - mx(zp.z) * mx(zp.z);
-            o = d3((0.5 * ax1 + ax2) + ax3, (ay1 + 0.5 * ay2) + ay3, (az1 + az2) + 0.5 * az3);
-        }
-        if (x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2 && z >= 2 && z <= d.Z - 3) {
-            const D3 c = ld(v, i);
-            const long sx = 1, sy = d.X, sz = (long)d.X * d.Y;
-            // one-sided second differences with the reference's y-1 / z-1 where +1 is meant
-            const long ax = c.x < 0 ? i + 2 * sx : i - 2 * sx, bx = c.x < 0 ? i + sx : i - sx;
-            const long ay = c.y < 0 ? i + 2 * sy : i - 2 * sy, by = i - sy;
-            const long az = c.z < 0 ? i + 2 * sz : i - 2 * sz, bz = i - sz;
-            D3 s = d3(0, 0, 0);
-            for (int q = 0; q < 3; ++q) {
-                const double cv = comp(c, q);
-                const double vxx = c.x < 0 ? (v[3 * ax + q] - 2.0 * v[3 * bx + q]) + cv : (cv - 2.0 * v[3 * bx + q]) + v[3 * ax + q];
-                const double vyy = c.y < 0 ? (v[3 * ay + q] - 2.0 * v[3 * by + q]) + cv : (cv - 2.0 * v[3 * by + q]) + v[3 * ay + q];
-                const double vzz = c.z < 0 ? (v[3 * az + q] - 2.0 * v[3 * bz + q]) + cv : (cv - 2.0 * v[3 * bz + q]) + v[3 * az + q];
-                const double r = -1.0 * ((vxx + vyy) + vzz);
-                if (q == 0) s.x = r; else if (q == 1) s.y = r; else s.z = r;
-            }
-            l = s;
-        }
-        st(dvv, i, o);
-        st(dlv, i, l);
+// F6: dvv = (Dv)v on the interior (0 elsewhere), dlv = the reference's upwind Laplacian of v on
+// its range (0 elsewhere)
+__device__ __forceinline__ void fv_vel_terms(Dims d, int x, int y, int z, long i, const double *v, D3 *dvv, D3 *dlv) {
+    D3 o = d3(0, 0, 0), l = d3(0, 0, 0);
+    if (interior(d, x, y, z)) {
+        const D3 c = ld(v, i), xp = ld(v, lin(d, x + 1, y, z)), xm = ld(v, lin(d, x - 1, y, z)),
+                 yp = ld(v, lin(d, x, y + 1, z)), ym = ld(v, lin(d, x, y - 1, z)), zp = ld(v, lin(d, x, y, z + 1)),
+                 zm = ld(v, lin(d, x, y, z - 1));
+        const double ax1 = ((mx(c.x) * mx(c.x) - mn(c.x) * mn(c.x)) + mn(xp.x) * mn(xp.x)) - mx(xp.x) * mx(xp.x);
+        const double ax2 = c.y > 0 ? c.y * (c.x - ym.x) : c.y * (yp.x - c.x);
+        const double ax3 = c.z > 0 ? c.z * (c.x - zm.x) : c.z * (zp.x - c.x);
+        const double ay1 = c.x > 0 ? c.x * (c.y - xm.y) : c.x * (xp.y - c.y);
+        const double ay2 = ((mx(c.y) * mx(c.y) - mn(c.y) * mn(c.y)) + mn(yp.y) * mn(yp.y)) - mx(yp.y) * mx(yp.y);
+        const double ay3 = c.z > 0 ? c.z * (c.y - zm.y) : c.z * (zp.y - c.y);
+        const double az1 = c.x > 0 ? c.x * (c.z - xm.z) : c.x * (xp.z - c.z);
+        const double az2 = c.y > 0 ? c.y * (c.z - ym.z) : c.y * (yp.z - c.z);
+        const double az3 = ((mx(c.z) * mx(c.z) - mn(c.z) * mn(c.z)) + mn(zp.z) * mn(zp.z)) - mx(zp.z) * mx(zp.z);
+        o = d3((0.5 * ax1 + ax2) + ax3, (ay1 + 0.5 * ay2) + ay3, (az1 + az2) + 0.5 * az3);
     }
+    if (x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2 && z >= 2 && z <= d.Z - 3) {
+        const D3 c = ld(v, i);
+        const long sx = 1, sy = d.X, sz = (long)d.X * d.Y;
+        // one-sided second differences with the reference's y-1 / z-1 where +1 is meant
+        const long ax = c.x < 0 ? i + 2 * sx : i - 2 * sx, bx = c.x < 0 ? i + sx : i - sx;
+        const long ay = c.y < 0 ? i + 2 * sy : i - 2 * sy, by = i - sy;
+        const long az = c.z < 0 ? i + 2 * sz : i - 2 * sz, bz = i - sz;
+        D3 s = d3(0, 0, 0);
+        for (int q = 0; q < 3; ++q) {
+            const double cv = comp(c, q);
+            const double vxx = c.x < 0 ? (v[3 * ax + q] - 2.0 * v[3 * bx + q]) + cv : (cv - 2.0 * v[3 * bx + q]) + v[3 * ax + q];
+            const double vyy = c.y < 0 ? (v[3 * ay + q] - 2.0 * v[3 * by + q]) + cv : (cv - 2.0 * v[3 * by + q]) + v[3 * ay + q];
+            const double vzz = c.z < 0 ? (v[3 * az + q] - 2.0 * v[3 * bz + q]) + cv : (cv - 2.0 * v[3 * bz + q]) + v[3 * az + q];
+            const double r = -1.0 * ((vxx + vyy) + vzz);
+            if (q == 0) s.x = r; else if (q == 1) s.y = r; else s.z = r;
+        }
+        l = s;
+    }
+    *dvv = o;
+    *dlv = l;
 }
 
-// F7: v1 = v - alpha (((m1t v + dvv) - tau dlv) + q gU), q = m2 / (|rho1| + 1e-5f), interior
-__global__ void f_velocity(Dims d, const double *v, const double *dvv, const double *dlv, const double *rho1,
-                           const double *gU, double *v1, double alpha, double m1t, double tau, double m2) {
+// F7: v1 = v - alpha (((m1t v + dvv) - tau dlv) + q gU), q = m2 / (|rho1| + 1e-5f), interior; v
+// elsewhere
+__device__ __forceinline__ D3 fv_velocity(Dims d, int x, int y, int z, D3 vv, D3 a, D3 l, double rho1, D3 g,
+                                          double alpha, double m1t, double tau, double m2) {
+    if (!interior(d, x, y, z)) return vv;
+    const double q = m2 / (fabs(rho1) + (double)1e-5f);
+    const D3 fr = d3(vv.x * m1t, vv.y * m1t, vv.z * m1t), df = d3(l.x * tau, l.y * tau, l.z * tau),
+             po = d3(g.x * q, g.y * q, g.z * q);
+    const D3 up = d3((((fr.x + a.x) - df.x) + po.x) * alpha, (((fr.y + a.y) - df.y) + po.y) * alpha,
+                     (((fr.z + a.z) - df.z) + po.z) * alpha);
+    return d3(vv.x - up.x, vv.y - up.y, vv.z - up.z);
+}
+
+// F9: psi_inv1 = psi_inv - alpha J v1, interior; psi_inv elsewhere
+__device__ __forceinline__ D3 fv_psi_inv(Dims d, int x, int y, int z, D3 o, const double m[9], D3 w, double alpha) {
+    if (!interior(d, x, y, z)) return o;
+    const D3 u = d3(((m[0] * w.x + m[1] * w.y) + m[2] * w.z) * alpha, ((m[3] * w.x + m[4] * w.y) + m[5] * w.z) * alpha,
+                    ((m[6] * w.x + m[7] * w.y) + m[8] * w.z) * alpha);
+    return d3(o.x - u.x, o.y - u.y, o.z - u.z);
+}
+
+// F1, F2, F4, F5, F6, F7, F9 of every voxel: the new rho, v and psi_inv from the old state, L and
+// the images; with `record`, also the intermediates the adjoint reads (J, Lo, Tp, gU, dlv)
+__global__ __launch_bounds__(256) void f_step(Dims d, const double *rho, const double *v, const double *pinv,
+                                              const double *L, const double *T, const double *S, const double *gS,
+                                              double *rho1, double *v1, double *pinv1, double *J_out, double *Lo_out,
+                                              double *Tp_out, double *gU_out, double *dlv_out, int record,
+                                              double alpha, double w_reg, double m1t, double tau, double m2) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
-        D3 o = ld(v, i);
-        if (interior(d, x, y, z)) {
-            const D3 vv = ld(v, i), a = ld(dvv, i), l = ld(dlv, i), g = ld(gU, i);
-            const double q = m2 / (fabs(rho1[i]) + (double)1e-5f);
-            const D3 fr = d3(vv.x * m1t, vv.y * m1t, vv.z * m1t), df = d3(l.x * tau, l.y * tau, l.z * tau),
-                     po = d3(g.x * q, g.y * q, g.z * q);
-            const D3 up = d3((((fr.x + a.x) - df.x) + po.x) * alpha, (((fr.y + a.y) - df.y) + po.y) * alpha,
-                             (((fr.z + a.z) - df.z) + po.z) * alpha);
-            o = d3(vv.x - up.x, vv.y - up.y, vv.z - up.z);
+        const double r1 = fv_density(d, x, y, z, i, rho, v, alpha);
+        double J[9];
+        fv_jacobian(d, x, y, z, pinv, J);
+        D3 lo;
+        double tp;
+        fv_interp(d, i, L, T, pinv, &lo, &tp);
+        const D3 gu = fv_potential(d, x, y, z, i, S, gS, tp, lo, J, w_reg);
+        D3 dvv, dlv;
+        fv_vel_terms(d, x, y, z, i, v, &dvv, &dlv);
+        const D3 w = fv_velocity(d, x, y, z, ld(v, i), dvv, dlv, r1, gu, alpha, m1t, tau, m2);
+        const D3 pi = fv_psi_inv(d, x, y, z, ld(pinv, i), J, w, alpha);
+        rho1[i] = r1;
+        st(v1, i, w);
+        st(pinv1, i, pi);
+        if (record) {
+            for (int k = 0; k < 9; ++k) J_out[9 * i + k] = J[k];
+            st(Lo_out, i, lo);
+            Tp_out[i] = tp;
+            st(gU_out, i, gu);
+            st(dlv_out, i, dlv);
         }
-        st(v1, i, o);
     }
 }
 
@@ -347,22 +368,6 @@ __global__ void f_psi(Dims d, const double *psi, const double *v1, double *psi1,
             o = d3(o.x + ux, o.y + uy, o.z + uz);
         }
         st(psi1, i, o);
-    }
-}
-
-// F9: psi_inv1 = psi_inv - alpha J v1, interior
-__global__ void f_psi_inv(Dims d, const double *pinv, const double *J, const double *v1, double *pinv1, double alpha) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        D3 o = ld(pinv, i);
-        if (interior(d, x, y, z)) {
-            const D3 w = ld(v1, i);
-            const double *m = J + 9 * i;
-            const D3 u = d3(((m[0] * w.x + m[1] * w.y) + m[2] * w.z) * alpha, ((m[3] * w.x + m[4] * w.y) + m[5] * w.z) * alpha,
-                            ((m[6] * w.x + m[7] * w.y) + m[8] * w.z) * alpha);
-            o = d3(o.x - u.x, o.y - u.y, o.z - u.z);
-        }
-        st(pinv1, i, o);
     }
 }
 
@@ -942,7 +947,7 @@ struct State {
 };
 // The intermediates of one iteration that its adjoint reads.
 struct Inter {
-    double *J, *L, *Lo, *Tp, *gU, *dvv, *dlv;
+    double *J, *L, *Lo, *Tp, *gU, *dlv;
 };
 
 int err(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -1024,14 +1029,13 @@ struct fr_adjoint {
     }
     int alloc_inter(Inter *t) {
         double *b = nullptr;
-        if (int rc = alloc(&b, 25 * (size_t)n)) return rc;
+        if (int rc = alloc(&b, 22 * (size_t)n)) return rc;
         t->J = b;
         t->L = b + 9 * n;
         t->Lo = b + 12 * n;
         t->Tp = b + 15 * n;
         t->gU = b + 16 * n;
-        t->dvv = b + 19 * n;
-        t->dlv = b + 22 * n;
+        t->dlv = b + 19 * n;
         return FR_OK;
     }
     ~fr_adjoint() {
@@ -1074,20 +1078,16 @@ void schedule(fr_adjoint *h) {
 }
 
 // one iteration with velocity scalars m1t, m2: state `in` -> state `out`, intermediates into `it`
-int forward_iteration(fr_adjoint *h, const State &in, const State &out, const Inter &it, double m1t, double m2) {
+int forward_iteration(fr_adjoint *h, const State &in, const State &out, const Inter &it, double m1t, double m2,
+                      bool record) {
     const Dims d = h->d;
     const int g = h->grid;
     hipStream_t s = h->stream;
     const double a = h->alpha;
-    frd::f_density<<<g, TPB, 0, s>>>(d, in.rho, in.v, out.rho, a);
-    frd::f_jacobian<<<g, TPB, 0, s>>>(d, in.pinv, it.J);
     frd::f_laplacian<<<g, TPB, 0, s>>>(d, in.psi, it.L);
-    frd::f_interp<<<g, TPB, 0, s>>>(d, it.L, h->T, in.pinv, it.Lo, it.Tp);
-    frd::f_potential<<<g, TPB, 0, s>>>(d, h->S, h->gS, it.Tp, it.Lo, it.J, it.gU, h->w_reg);
-    frd::f_vel_terms<<<g, TPB, 0, s>>>(d, in.v, it.dvv, it.dlv);
-    frd::f_velocity<<<g, TPB, 0, s>>>(d, in.v, it.dvv, it.dlv, out.rho, it.gU, out.v, a, m1t, h->tau, m2);
+    frd::f_step<<<g, TPB, 0, s>>>(d, in.rho, in.v, in.pinv, it.L, h->T, h->S, h->gS, out.rho, out.v, out.pinv, it.J,
+                                  it.Lo, it.Tp, it.gU, it.dlv, record ? 1 : 0, a, h->w_reg, m1t, h->tau, m2);
     frd::f_psi<<<g, TPB, 0, s>>>(d, in.psi, out.v, out.psi, a);
-    frd::f_psi_inv<<<g, TPB, 0, s>>>(d, in.pinv, it.J, out.v, out.pinv, a);
     ADJ_TRY(hipGetLastError());
     return FR_OK;
 }
@@ -1228,7 +1228,7 @@ extern "C" int fr_adjoint_forward(fr_adjoint *h, const double *d_v0, double w_re
     State in = h->ck[0];
     for (int k = 1; k <= h->K; ++k) {
         const State out = k == h->K ? h->fin : (k % h->ckpt == 0 ? h->ck[k / h->ckpt] : h->nxt[(k - 1) % 2]);
-        if (int rc = forward_iteration(h, in, out, h->inter[0], h->m1t[k - 1], h->m2[k - 1])) return rc;
+        if (int rc = forward_iteration(h, in, out, h->inter[0], h->m1t[k - 1], h->m2[k - 1], false)) return rc;
         in = out;
     }
     frd::f_objective<<<h->grid, TPB, 0, s>>>(h->d, h->T, h->S, h->LT, h->LS, h->no_labels, h->lw, h->fin.psi,
@@ -1264,7 +1264,8 @@ extern "C" int fr_adjoint_backward(fr_adjoint *h, double *d_grad_v0, double *gra
         for (int j = 0; j < len; ++j) {
             const State in = j == 0 ? h->ck[sg] : h->nxt[j - 1];
             const int k = start + j + 1;
-            if (int rc = forward_iteration(h, in, h->nxt[j], h->inter[j], h->m1t[k - 1], h->m2[k - 1])) return rc;
+            if (int rc = forward_iteration(h, in, h->nxt[j], h->inter[j], h->m1t[k - 1], h->m2[k - 1], true))
+                return rc;
         }
         for (int j = len - 1; j >= 0; --j) {
             const State in = j == 0 ? h->ck[sg] : h->nxt[j - 1];
@@ -1401,7 +1402,7 @@ extern "C" int fr_step_fwd(fr_adjoint *h, int k, double w_reg, double tau, const
     const State in{const_cast<double *>(d_rho), const_cast<double *>(d_v), const_cast<double *>(d_psi),
                    const_cast<double *>(d_psi_inv)};
     const State out{d_rho1, d_v1, d_psi1, d_psi_inv1};
-    if (int rc = forward_iteration(h, in, out, h->inter[0], m1t, m2)) return rc;
+    if (int rc = forward_iteration(h, in, out, h->inter[0], m1t, m2, false)) return rc;
     ADJ_TRY(hipStreamSynchronize(h->stream));
     return FR_OK;
 }
@@ -1422,7 +1423,7 @@ extern "C" int fr_step_bwd(fr_adjoint *h, int k, double w_reg, double tau, const
     const State in{const_cast<double *>(d_rho), const_cast<double *>(d_v), const_cast<double *>(d_psi),
                    const_cast<double *>(d_psi_inv)};
     const State nx = h->nxt[0];
-    if (int rc = forward_iteration(h, in, nx, h->inter[0], m1t, m2)) return rc;
+    if (int rc = forward_iteration(h, in, nx, h->inter[0], m1t, m2, true)) return rc;
     int cur = 0;
     ADJ_TRY(hipMemcpyAsync(h->grho[cur], d_g_rho, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     ADJ_TRY(hipMemcpyAsync(h->gv, d_g_v, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
