@@ -45,15 +45,15 @@ struct D3 {
 };
 
 __device__ __forceinline__ D3 d3(double x, double y, double z) { return D3{x, y, z}; }
-__device__ __forceinline__ D3 ld(const double *p, long i) { return d3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
-__device__ __forceinline__ void st(double *p, long i, D3 v) {
+__device__ __forceinline__ D3 ld(const double *p, int i) { return d3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+__device__ __forceinline__ void st(double *p, int i, D3 v) {
     p[3 * i] = v.x;
     p[3 * i + 1] = v.y;
     p[3 * i + 2] = v.z;
 }
 // FP64 atomic add in hardware (global_atomic_add_f64; every buffer here is ordinary device memory)
 __device__ __forceinline__ void aadd(double *p, double v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void add_at(double *p, long i, D3 v) {
+__device__ __forceinline__ void add_at(double *p, int i, D3 v) {
     aadd(&p[3 * i], v.x);
     aadd(&p[3 * i + 1], v.y);
     aadd(&p[3 * i + 2], v.z);
@@ -63,17 +63,18 @@ __device__ __forceinline__ double comp(D3 v, int c) { return c == 0 ? v.x : c ==
 struct Dims {
     int X, Y, Z;
 };
-__device__ __forceinline__ long lin(Dims d, int x, int y, int z) { return x + (long)d.X * (y + (long)d.Y * z); }
+// voxel indices are 32-bit: the library accepts volumes with 9 N < 2^31 (the Jacobian buffers)
+__device__ __forceinline__ int lin(Dims d, int x, int y, int z) { return x + d.X * (y + d.Y * z); }
 __device__ __forceinline__ bool interior(Dims d, int x, int y, int z) {
     return x >= 1 && x <= d.X - 2 && y >= 1 && y <= d.Y - 2 && z >= 1 && z <= d.Z - 2;
 }
 #define FOR_VOXELS(d, i)                                                                                    \
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x, n_ = (long)(d).X * (d).Y * (d).Z; i < n_; \
-         i += (long)gridDim.x * blockDim.x)
-#define XYZ(d, i)                                     \
-    const int x = (int)((i) % (d).X);                 \
-    const int y = (int)(((i) / (d).X) % (d).Y);       \
-    const int z = (int)((i) / ((long)(d).X * (d).Y))
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x, n_ = (d).X * (d).Y * (d).Z; i < n_; \
+         i += gridDim.x * blockDim.x)
+#define XYZ(d, i)                                                                 \
+    const int x = (int)((unsigned)(i) % (unsigned)(d).X);                         \
+    const int y = (int)(((unsigned)(i) / (unsigned)(d).X) % (unsigned)(d).Y);     \
+    const int z = (int)((unsigned)(i) / ((unsigned)(d).X * (unsigned)(d).Y))
 
 // lerp(v0, v1, t) = fma(t, v0, fma(-t, v1, v1)) (cuda_utils.hpp:43-46), in FP64
 __device__ __forceinline__ double lerp(double v0, double v1, double t) { return fma(t, v0, fma(-t, v1, v1)); }
@@ -81,7 +82,7 @@ __device__ __forceinline__ double lerp(double v0, double v1, double t) { return 
 // clamped corner and weights (cuda_utils.hpp:60-136); `in` marks the axes along which the
 // position is inside the grid (the derivative with respect to a clamped coordinate is 0)
 struct Corner {
-    long g[8];  // linear indices of the 8 corners, order 111, 110, 101, 100, 011, 010, 001, 000
+    int g[8];  // linear indices of the 8 corners, order 111, 110, 101, 100, 011, 010, 001, 000
     double a, b, c;
     int gx, gy, gz;  // the floor corner
     bool inx, iny, inz;
@@ -171,7 +172,7 @@ __device__ __forceinline__ void det_ref_grad(const double m[3][3], double g[3][3
 // and the images, and run in one kernel (f_step) that keeps their results in registers.
 
 // F1: rho1 = rho + alpha * div on the interior; rho elsewhere
-__device__ __forceinline__ double fv_density(Dims d, int x, int y, int z, long i, const double *rho, const double *v,
+__device__ __forceinline__ double fv_density(Dims d, int x, int y, int z, int i, const double *rho, const double *v,
                                              double alpha) {
     const double p = rho[i];
     if (!interior(d, x, y, z)) return p;
@@ -211,7 +212,7 @@ __global__ void f_laplacian(Dims d, const double *psi, double *L) {
         XYZ(d, i);
         D3 o = d3(0, 0, 0);
         if (interior(d, x, y, z)) {
-            const long nb[6] = {lin(d, x + 1, y, z), lin(d, x - 1, y, z), lin(d, x, y + 1, z),
+            const int nb[6] = {lin(d, x + 1, y, z), lin(d, x - 1, y, z), lin(d, x, y + 1, z),
                                 lin(d, x, y - 1, z), lin(d, x, y, z + 1), lin(d, x, y, z - 1)};
             const D3 c = ld(psi, i);
             D3 s = d3(c.x * -6.0, c.y * -6.0, c.z * -6.0);
@@ -226,7 +227,7 @@ __global__ void f_laplacian(Dims d, const double *psi, double *L) {
 }
 
 // F4: Lo = trilerp(L, psi_inv), Tp = trilerp(T, psi_inv), every voxel
-__device__ __forceinline__ void fv_interp(Dims d, long i, const double *L, const double *T, const double *pinv, D3 *Lo,
+__device__ __forceinline__ void fv_interp(Dims d, int i, const double *L, const double *T, const double *pinv, D3 *Lo,
                                           double *Tp) {
     const D3 p = ld(pinv, i);
     const Corner k = corner(d, p.x, p.y, p.z);
@@ -242,7 +243,7 @@ __device__ __forceinline__ void fv_interp(Dims d, long i, const double *L, const
 }
 
 // F5: gU = ((S - Tp) grad S + w_reg Lo) det_ref(J), interior; 0 elsewhere
-__device__ __forceinline__ D3 fv_potential(Dims d, int x, int y, int z, long i, const double *S, const double *gS,
+__device__ __forceinline__ D3 fv_potential(Dims d, int x, int y, int z, int i, const double *S, const double *gS,
                                            double Tp, D3 lo, const double J[9], double w_reg) {
     if (!interior(d, x, y, z)) return d3(0, 0, 0);
     double m[3][3];
@@ -258,7 +259,7 @@ __device__ __forceinline__ double mn(double q) { return fmin(q, 0.0); }
 
 // F6: dvv = (Dv)v on the interior (0 elsewhere), dlv = the reference's upwind Laplacian of v on
 // its range (0 elsewhere)
-__device__ __forceinline__ void fv_vel_terms(Dims d, int x, int y, int z, long i, const double *v, D3 *dvv, D3 *dlv) {
+__device__ __forceinline__ void fv_vel_terms(Dims d, int x, int y, int z, int i, const double *v, D3 *dvv, D3 *dlv) {
     D3 o = d3(0, 0, 0), l = d3(0, 0, 0);
     if (interior(d, x, y, z)) {
         const D3 c = ld(v, i), xp = ld(v, lin(d, x + 1, y, z)), xm = ld(v, lin(d, x - 1, y, z)),
@@ -277,11 +278,11 @@ __device__ __forceinline__ void fv_vel_terms(Dims d, int x, int y, int z, long i
     }
     if (x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2 && z >= 2 && z <= d.Z - 3) {
         const D3 c = ld(v, i);
-        const long sx = 1, sy = d.X, sz = (long)d.X * d.Y;
+        const int sx = 1, sy = d.X, sz = d.X * d.Y;
         // one-sided second differences with the reference's y-1 / z-1 where +1 is meant
-        const long ax = c.x < 0 ? i + 2 * sx : i - 2 * sx, bx = c.x < 0 ? i + sx : i - sx;
-        const long ay = c.y < 0 ? i + 2 * sy : i - 2 * sy, by = i - sy;
-        const long az = c.z < 0 ? i + 2 * sz : i - 2 * sz, bz = i - sz;
+        const int ax = c.x < 0 ? i + 2 * sx : i - 2 * sx, bx = c.x < 0 ? i + sx : i - sx;
+        const int ay = c.y < 0 ? i + 2 * sy : i - 2 * sy, by = i - sy;
+        const int az = c.z < 0 ? i + 2 * sz : i - 2 * sz, bz = i - sz;
         D3 s = d3(0, 0, 0);
         for (int q = 0; q < 3; ++q) {
             const double cv = comp(c, q);
@@ -454,7 +455,7 @@ __device__ __forceinline__ int rec_code(const Corner &k, int x, int y, int z) {
     if (ox < -1 || ox > 0 || oy < -1 || oy > 0 || oz < -1 || oz > 0) return RC_VALID | RC_IRREG;
     return RC_VALID | ((ox + 1) << 2) | ((oy + 1) << 3) | ((oz + 1) << 4);
 }
-__device__ __forceinline__ void put_rec(int *code, double *abc, double *gw, long i, int c, const Corner &k, D3 g) {
+__device__ __forceinline__ void put_rec(int *code, double *abc, double *gw, int i, int c, const Corner &k, D3 g) {
     code[i] = c;
     st(abc, i, d3(k.a, k.b, k.c));
     st(gw, i, g);
@@ -473,7 +474,7 @@ __device__ __forceinline__ D3 gather_cells(Dims d, int x, int y, int z, const in
             for (int sx = -1; sx <= 1; ++sx) {
                 const int xx = x + sx;
                 if (xx < 0 || xx >= d.X) continue;
-                const long i = lin(d, xx, yy, zz);
+                const int i = lin(d, xx, yy, zz);
                 const int c = code[i];
                 if ((c & (RC_VALID | RC_IRREG)) != RC_VALID) continue;
                 // the target is the source's (x, y, z) + t; corner sides: t == o (floor), t == o + 1
@@ -587,7 +588,7 @@ __device__ __forceinline__ bool in_lap(Dims d, int x, int y, int z) {
 __global__ void b_step4(Dims d, const double *v, const double *gdvv, const double *gdlv, double *gv) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
-        const long sy = d.X, sz = (long)d.X * d.Y;
+        const int sy = d.X, sz = d.X * d.Y;
         const D3 vi = ld(v, i);
         D3 a = d3(0, 0, 0);
         // (Dv)v, voxel i as the centre
@@ -612,40 +613,40 @@ __global__ void b_step4(Dims d, const double *v, const double *gdvv, const doubl
         }
         // voxel i as the x+1 / x-1 / y+1 / y-1 / z+1 / z-1 neighbour of an interior voxel j
         if (interior(d, x - 1, y, z)) {
-            const long j = i - 1;
+            const int j = i - 1;
             const D3 cj = ld(v, j), G = ld(gdvv, j);
             a.x += 0.5 * G.x * (2 * mn(vi.x) - 2 * mx(vi.x));
             if (!(cj.x > 0)) { a.y += G.y * cj.x; a.z += G.z * cj.x; }
         }
         if (interior(d, x + 1, y, z)) {
-            const long j = i + 1;
+            const int j = i + 1;
             const D3 cj = ld(v, j), G = ld(gdvv, j);
             if (cj.x > 0) { a.y -= G.y * cj.x; a.z -= G.z * cj.x; }
         }
         if (interior(d, x, y - 1, z)) {
-            const long j = i - sy;
+            const int j = i - sy;
             const D3 cj = ld(v, j), G = ld(gdvv, j);
             if (!(cj.y > 0)) { a.x += G.x * cj.y; a.z += G.z * cj.y; }
             a.y += 0.5 * G.y * (2 * mn(vi.y) - 2 * mx(vi.y));
         }
         if (interior(d, x, y + 1, z)) {
-            const long j = i + sy;
+            const int j = i + sy;
             const D3 cj = ld(v, j), G = ld(gdvv, j);
             if (cj.y > 0) { a.x -= G.x * cj.y; a.z -= G.z * cj.y; }
         }
         if (interior(d, x, y, z - 1)) {
-            const long j = i - sz;
+            const int j = i - sz;
             const D3 cj = ld(v, j), G = ld(gdvv, j);
             if (!(cj.z > 0)) { a.x += G.x * cj.z; a.y += G.y * cj.z; }
             a.z += 0.5 * G.z * (2 * mn(vi.z) - 2 * mx(vi.z));
         }
         if (interior(d, x, y, z + 1)) {
-            const long j = i + sz;
+            const int j = i + sz;
             const D3 cj = ld(v, j), G = ld(gdvv, j);
             if (cj.z > 0) { a.x -= G.x * cj.z; a.y -= G.y * cj.z; }
         }
         // the velocity Laplacian: dlv(j) = -(sum over axes of far - 2 near + centre)
-        const auto add = [&](long j, double f) {
+        const auto add = [&](int j, double f) {
             const D3 g = ld(gdlv, j);
             a = d3(a.x + f * g.x, a.y + f * g.y, a.z + f * g.z);
         };
@@ -738,8 +739,8 @@ __global__ void b_step8(Dims d, const double *gL, const double *gJ, const double
                         const double *grho1, double *gpsi, double *gpinv, double *grho, double *gv, double alpha) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
-        const long sy = d.X, sz = (long)d.X * d.Y;
-        const long nb[6] = {i + 1, i - 1, i + sy, i - sy, i + sz, i - sz};
+        const int sy = d.X, sz = d.X * d.Y;
+        const int nb[6] = {i + 1, i - 1, i + sy, i - sy, i + sz, i - sz};
         const bool nin[6] = {interior(d, x + 1, y, z), interior(d, x - 1, y, z), interior(d, x, y + 1, z),
                              interior(d, x, y - 1, z), interior(d, x, y, z + 1), interior(d, x, y, z - 1)};
         const bool in = interior(d, x, y, z);
@@ -774,7 +775,7 @@ __global__ void b_step8(Dims d, const double *gL, const double *gJ, const double
         D3 ga = d3(0, 0, 0);
         const D3 vi = ld(v, i);
         const double ri = rho[i];
-        const long step[3] = {1, sy, sz};
+        const int step[3] = {1, sy, sz};
         for (int ax = 0; ax < 3; ++ax) {
             const double via = comp(vi, ax);
             double gax = 0;
@@ -790,12 +791,12 @@ __global__ void b_step8(Dims d, const double *gL, const double *gJ, const double
                 if (!(w > 0)) gr += g * w;
             }
             if (nin[2 * ax + 1]) {  // j = i - e_a: i is j + e_a in t_a1(j)
-                const long j = i - step[ax];
+                const int j = i - step[ax];
                 const double g = alpha * grho1[j], vj = v[3 * j + ax];
                 if (!(vj > 0)) gr += -g * vj;
             }
             if (nin[2 * ax]) {  // j = i + e_a: i is j - e_a in t_a2(j), w = v(i).a
-                const long j = i + step[ax];
+                const int j = i + step[ax];
                 const double g = alpha * grho1[j];
                 if (via > 0) {
                     gr += g * via;
@@ -886,7 +887,7 @@ __global__ void op_grad_fwd(Dims d, const double *f, double *g) {
         XYZ(d, i);
         D3 o = d3(0, 0, 0);
         if (interior(d, x, y, z)) {
-            const long sy = d.X, sz = (long)d.X * d.Y;
+            const int sy = d.X, sz = d.X * d.Y;
             o = d3((f[i + 1] - f[i - 1]) / 2.0, (f[i + sy] - f[i - sy]) / 2.0, (f[i + sz] - f[i - sz]) / 2.0);
         }
         st(g, i, o);
@@ -897,7 +898,7 @@ __global__ void op_grad_fwd(Dims d, const double *f, double *g) {
 __global__ void op_grad_bwd(Dims d, const double *gg, double *gf) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
-        const long sy = d.X, sz = (long)d.X * d.Y;
+        const int sy = d.X, sz = d.X * d.Y;
         double a = 0;
         if (interior(d, x - 1, y, z)) a += gg[3 * (i - 1)] / 2.0;
         if (interior(d, x + 1, y, z)) a -= gg[3 * (i + 1)] / 2.0;
@@ -913,13 +914,13 @@ __global__ void op_grad_bwd(Dims d, const double *gg, double *gf) {
 __global__ void op_laplacian_bwd(Dims d, const double *gL, double *gpsi) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
-        const long sy = d.X, sz = (long)d.X * d.Y;
+        const int sy = d.X, sz = d.X * d.Y;
         D3 gp = ld(gpsi, i);
         if (interior(d, x, y, z)) {
             const D3 g = ld(gL, i);
             gp = d3(gp.x + 6 * g.x, gp.y + 6 * g.y, gp.z + 6 * g.z);
         }
-        const long nb[6] = {i + 1, i - 1, i + sy, i - sy, i + sz, i - sz};
+        const int nb[6] = {i + 1, i - 1, i + sy, i - sy, i + sz, i - sz};
         const bool nin[6] = {interior(d, x + 1, y, z), interior(d, x - 1, y, z), interior(d, x, y + 1, z),
                              interior(d, x, y - 1, z), interior(d, x, y, z + 1), interior(d, x, y, z - 1)};
         for (int k = 0; k < 6; ++k)
@@ -1138,6 +1139,9 @@ extern "C" int fr_adjoint_create(const fr_params *p, int K, int ckpt, fr_adjoint
     if (K < 1) return err(FR_ERR_INVALID, "K (unrolled iterations) must be >= 1 (got %d)", K);
     if (ckpt < 1 || ckpt > K) return err(FR_ERR_INVALID, "checkpoint interval must be in [1, K] (got %d, K = %d)", ckpt, K);
     if (p->no_labels < 1 || p->no_labels > 32768) return err(FR_ERR_INVALID, "NO_LABELS must be in [1, 32768]");
+    if (9LL * p->vol_dims[0] * p->vol_dims[1] * p->vol_dims[2] >= (1LL << 31))
+        return err(FR_ERR_INVALID, "volume of %lld voxels is too large for the differentiated solver (9*N must be < 2^31)",
+                   (long long)p->vol_dims[0] * p->vol_dims[1] * p->vol_dims[2]);
     fr_adjoint *h = new (std::nothrow) fr_adjoint;
     if (!h) return err(FR_ERR_ALLOC, "out of host memory");
     h->d = Dims{p->vol_dims[0], p->vol_dims[1], p->vol_dims[2]};
@@ -1309,6 +1313,8 @@ namespace {
 
 int op_dims(int X, int Y, int Z, Dims *d, long *n, int *grid) {
     if (X < 3 || Y < 3 || Z < 3) return err(FR_ERR_INVALID, "dims must be >= 3 along every axis (got %d %d %d)", X, Y, Z);
+    if (9LL * X * Y * Z >= (1LL << 31))
+        return err(FR_ERR_INVALID, "volume of %lld voxels is too large (9*N must be < 2^31)", (long long)X * Y * Z);
     *d = Dims{X, Y, Z};
     *n = (long)X * Y * Z;
     const long nb = (*n + TPB - 1) / TPB;
