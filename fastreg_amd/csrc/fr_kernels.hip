@@ -495,6 +495,10 @@ __device__ __forceinline__ void lt_compute(const PassAView &w, int zz, int r, in
     s = add3(s, ym);
     s = add3(s, zp);
     s = add3(s, zm);
+    // The empty asm keeps the sum out of the exec-masked block that selects the interior lanes
+    // (the compiler would sink it there), so that the lane shifts fold into the adds as DPP
+    // operands: 6 fewer VALU instructions per row.
+    asm volatile("" : "+v"(s.x), "+v"(s.y), "+v"(s.z));
     const F3 L = mul3(-1.f, s);
     const int x = w.xb + lane, y = w.yb + r;
     const Dims d = w.d;
