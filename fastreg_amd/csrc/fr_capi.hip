@@ -267,9 +267,11 @@ extern "C" int fr_set_storage(fr_ctx *c, int storage) {
     if (storage != FR_STORAGE_F32 && storage != FR_STORAGE_BF16 && storage != FR_STORAGE_F16)
         return fail(FR_ERR_INVALID, "storage must be FR_STORAGE_F32, FR_STORAGE_BF16 or FR_STORAGE_F16");
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (storage != FR_STORAGE_F32 && !c->vr[0]) {
+    if (storage != FR_STORAGE_F32) {
+        // allocated on first use; a failed second allocation leaves the first for the next call
         int rc;
-        if ((rc = dalloc(&c->vr[0], 2 * (size_t)c->n)) || (rc = dalloc(&c->vr[1], 2 * (size_t)c->n))) return rc;
+        if (!c->vr[0] && (rc = dalloc(&c->vr[0], 2 * (size_t)c->n))) return rc;
+        if (!c->vr[1] && (rc = dalloc(&c->vr[1], 2 * (size_t)c->n))) return rc;
     }
     c->st = storage == FR_STORAGE_BF16 ? frk::ST_BF16 : storage == FR_STORAGE_F16 ? frk::ST_F16 : frk::ST_F32;
     c->initialised = false;  // the state must be initialised in the new storage
