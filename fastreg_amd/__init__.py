@@ -191,7 +191,8 @@ class DeviceArray:
 
     def upload(self, a):
         a = np.ascontiguousarray(a, dtype=self.dtype)
-        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"upload of {a.nbytes} bytes into a device array of {self.nbytes} bytes")
         check(lib().fr_memcpy_htod(self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes))
 
     def get(self):
