@@ -69,7 +69,8 @@ struct fr_ctx {
     unsigned *vr[2] = {nullptr, nullptr};  // 16-bit (v, rho) records, 2 unsigned per voxel
     int cur = 0;
     int zc[3] = {0, 8, 2};       // interior z-slices per block of pass A ([1]) and pass B ([2])
-    int zc_auto[3] = {0, 8, 2};  // the choice made at create
+    int zc_auto[3] = {0, 8, 2};  // the automatic choice for the current volume and storage
+    bool zc_user[3] = {false, false, false};  // set by fr_set_chunking (else automatic)
     bool initialised = false;
     // loop control and history
     frk::Ctl *ctl = nullptr;
@@ -182,8 +183,11 @@ static int chunk_for(Dims d, int xs, int ty, long long slots) {
 // at 182x218x182 (-1.5 us against filling every slot with 20) and 23 at 160x192x160 (-0.5 us);
 // smaller volumes, served largely from the 256 MB Infinity Cache, run faster with every slot
 // filled (128^3: 8 slices, 1.9 us faster than the 13 this rule would give), as pass A does.
-static int chunk_for_b(Dims d, long long n_cu, long long occ) {
-    if (24LL * d.X * d.Y * d.Z < 100LL * 1000 * 1000) return chunk_for(d, frk::XS_B, frk::TY_B, n_cu * occ);
+static int chunk_for_b(Dims d, long long n_cu, long long occ, int storage) {
+    // 16-bit storage reads v_new as 8-byte records and unpacks it: less bound by memory, and
+    // faster with every slot filled (at 182x218x182, 1.8 us faster with 20 slices than with 30)
+    if (storage != frk::ST_F32 || 24LL * d.X * d.Y * d.Z < 100LL * 1000 * 1000)
+        return chunk_for(d, frk::XS_B, frk::TY_B, n_cu * occ);
     const long long tiles = (long long)((d.X - 2 + frk::XS_B - 1) / frk::XS_B) * ((d.Y - 2 + frk::TY_B - 1) / frk::TY_B);
     const int interior = d.Z - 2;
     long long best_cost = -1;
@@ -211,8 +215,9 @@ static int auto_chunking(fr_ctx *c) {
     const long long n_cu = cus > 0 ? cus : 1;
     c->zc_auto[0] = 0;
     c->zc_auto[1] = chunk_for(c->d, frk::XS_A, frk::TY_A, n_cu * (occ_a > 0 ? occ_a : 1));
-    c->zc_auto[2] = chunk_for_b(c->d, n_cu, occ_b > 0 ? occ_b : 1);
-    for (int k = 0; k < 3; ++k) c->zc[k] = c->zc_auto[k];
+    c->zc_auto[2] = chunk_for_b(c->d, n_cu, occ_b > 0 ? occ_b : 1, c->st);
+    for (int k = 0; k < 3; ++k)
+        if (!c->zc_user[k]) c->zc[k] = c->zc_auto[k];
     return FR_OK;
 }
 
@@ -280,6 +285,8 @@ extern "C" int fr_set_chunking(fr_ctx *c, int zc_a, int zc_b) {
     if (zc_a < 0 || zc_b < 0) return fail(FR_ERR_INVALID, "z-chunk sizes must be >= 0 (0 = automatic)");
     c->zc[1] = zc_a ? zc_a : c->zc_auto[1];
     c->zc[2] = zc_b ? zc_b : c->zc_auto[2];
+    c->zc_user[1] = zc_a != 0;
+    c->zc_user[2] = zc_b != 0;
     return FR_OK;
 }
 
@@ -302,6 +309,7 @@ extern "C" int fr_set_storage(fr_ctx *c, int storage) {
         if (!c->vr[1] && (rc = dalloc(&c->vr[1], 2 * (size_t)c->n))) return rc;
     }
     c->st = storage == FR_STORAGE_BF16 ? frk::ST_BF16 : storage == FR_STORAGE_F16 ? frk::ST_F16 : frk::ST_F32;
+    if (int rc = auto_chunking(c)) return rc;  // the automatic chunking depends on the storage
     c->initialised = false;  // the state must be initialised in the new storage
     c->have_run = c->have_final = false;
     return FR_OK;
