@@ -105,6 +105,14 @@ bool read(const std::string &path, Image &img, std::string &err) {
         return false;
     }
     size_t n = (size_t)img.dims[0] * img.dims[1] * img.dims[2];
+    if (n > ((size_t)1 << 31)) {  // beyond what the solver accepts; refuse before allocating
+        err = path + ": image of " + std::to_string(n) + " voxels is too large";
+        return false;
+    }
+    if (!std::isfinite(h.vox_offset) || h.vox_offset < 348.f || h.vox_offset > 1e12f) {
+        err = path + ": bad vox_offset";
+        return false;
+    }
     std::vector<unsigned char> raw(n * bpv);
     f.seekg((std::streamoff)h.vox_offset);
     if (!f.read(reinterpret_cast<char *>(raw.data()), (std::streamsize)raw.size())) {

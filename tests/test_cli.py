@@ -103,3 +103,33 @@ def test_app_registration_matches_library_and_oracle(tmp_path, oracle_mod):
     log = o.run(O.SolverParams(max_iter=15, alpha=0.2, no_labels=pair.no_labels, verbosity=1))
     for a, b in zip(outs, o.outputs()):
         assert np.array_equal(a.ravel(), b)
+
+
+def _header_file(path, dims, vox_offset=352.0, datatype=16):
+    """A NIfTI-1 header (348 bytes + 4-byte extension flag) with the given dims, and no data."""
+    import struct
+    h = bytearray(352)
+    struct.pack_into("<i", h, 0, 348)
+    struct.pack_into("<8h", h, 40, 3, *dims, 1, 1, 1, 1)
+    struct.pack_into("<h", h, 70, datatype)
+    struct.pack_into("<f", h, 108, vox_offset)
+    h[344:348] = b"n+1\0"
+    open(path, "wb").write(bytes(h))
+
+
+def test_app_rejects_malformed_nifti_headers(tmp_path):
+    """Huge dimensions, a non-finite or out-of-range vox_offset and missing voxel data end in an
+    error message and a nonzero status, not an allocation failure or a crash."""
+    good = tmp_path / "good.ini"
+    good.write_text("VOL_DIMS_X=4\nVOL_DIMS_Y=4\nVOL_DIMS_Z=4\n")
+    cases = {"huge.nii": dict(dims=(32000, 32000, 32000)), "nan_off.nii": dict(dims=(4, 4, 4), vox_offset=float("nan")),
+             "neg_off.nii": dict(dims=(4, 4, 4), vox_offset=-8.0), "short.nii": dict(dims=(4, 4, 4))}
+    expect = {"huge.nii": "too large", "nan_off.nii": "bad vox_offset", "neg_off.nii": "bad vox_offset",
+              "short.nii": "truncated voxel data"}
+    for name, kw in cases.items():
+        path = str(tmp_path / name)
+        _header_file(path, **kw)
+        r = subprocess.run([APP, str(good), path, "tl", "s", "sl", str(tmp_path)], capture_output=True, text=True,
+                           timeout=60)
+        assert r.returncode not in (0, -6, -11, 134, 139), (name, r.returncode, r.stderr)
+        assert expect[name] in r.stderr, (name, r.stderr)
