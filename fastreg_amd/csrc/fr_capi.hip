@@ -113,6 +113,11 @@ static int check_params(const fr_params *p) {
     if (p->max_iter < 0) return fail(FR_ERR_INVALID, "MAX_ITER must be >= 0");
     if (p->no_labels < 1 || p->no_labels > 32768) return fail(FR_ERR_INVALID, "NO_LABELS must be in [1, 32768]");
     if (p->verbosity < 0 || p->verbosity > 2) return fail(FR_ERR_INVALID, "verbosity must be 0, 1 or 2");
+    const float fv[6] = {p->rho_0, p->tau, p->alpha, p->w_reg, p->p, p->C};
+    const char *fn[6] = {"RHO_0", "TAU", "ALPHA", "W_REG", "p", "C"};
+    for (int k = 0; k < 6; ++k)
+        if (!std::isfinite(fv[k])) return fail(FR_ERR_INVALID, "%s must be finite (got %g)", fn[k], (double)fv[k]);
+    if (std::isnan(p->max_update_norm)) return fail(FR_ERR_INVALID, "MAX_UPDATE_NORM must not be NaN");
     return FR_OK;
 }
 

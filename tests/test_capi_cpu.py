@@ -87,7 +87,9 @@ def test_load_ini_missing_file():
 
 
 @pytest.mark.parametrize("kw", [dict(vol_dims=(2, 10, 10)), dict(no_labels=0), dict(max_iter=-1),
-                                dict(verbosity=3), dict(vol_dims=(2000, 2000, 2000))])
+                                dict(verbosity=3), dict(vol_dims=(2000, 2000, 2000)),
+                                dict(alpha=float("nan")), dict(tau=float("inf")), dict(w_reg=float("-inf")),
+                                dict(rho_0=float("nan")), dict(max_update_norm=float("nan"))])
 def test_create_rejects_invalid_params_before_touching_the_gpu(kw):
     p = fr.default_params(**kw)
     h = C.c_void_p()
