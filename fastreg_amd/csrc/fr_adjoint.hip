@@ -1139,6 +1139,8 @@ extern "C" int fr_adjoint_create(const fr_params *p, int K, int ckpt, fr_adjoint
     if (K < 1) return err(FR_ERR_INVALID, "K (unrolled iterations) must be >= 1 (got %d)", K);
     if (ckpt < 1 || ckpt > K) return err(FR_ERR_INVALID, "checkpoint interval must be in [1, K] (got %d, K = %d)", ckpt, K);
     if (p->no_labels < 1 || p->no_labels > 32768) return err(FR_ERR_INVALID, "NO_LABELS must be in [1, 32768]");
+    if (!std::isfinite(p->alpha) || !std::isfinite(p->rho_0) || !std::isfinite(p->p) || !std::isfinite(p->C))
+        return err(FR_ERR_INVALID, "ALPHA, RHO_0, p and C must be finite");
     if (9LL * p->vol_dims[0] * p->vol_dims[1] * p->vol_dims[2] >= (1LL << 31))
         return err(FR_ERR_INVALID, "volume of %lld voxels is too large for the differentiated solver (9*N must be < 2^31)",
                    (long long)p->vol_dims[0] * p->vol_dims[1] * p->vol_dims[2]);

@@ -107,7 +107,8 @@ def test_null_arguments_are_errors_not_crashes():
 
 
 @pytest.mark.parametrize("K,ckpt,kw", [(0, 1, {}), (8, 0, {}), (8, 9, {}), (4, 2, dict(vol_dims=(2, 8, 8))),
-                                       (4, 2, dict(no_labels=0))])
+                                       (4, 2, dict(no_labels=0)), (4, 2, dict(alpha=float("nan"))),
+                                       (4, 2, dict(vol_dims=(700, 700, 700)))])
 def test_adjoint_create_rejects_invalid_arguments_before_touching_the_gpu(K, ckpt, kw):
     p = fr.default_params(**kw)
     h = C.c_void_p()
