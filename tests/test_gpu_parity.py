@@ -109,6 +109,23 @@ def test_step_teacher_forced_bit_exact(oracle_mod, dims, seed):
         assert mg == mo and ag == ao, (mg, mo, ag, ao)
 
 
+@pytest.mark.parametrize("dims,seed,amp", [((70, 40, 36), 31, 4.0), ((131, 19, 25), 32, 9.0)])
+def test_step_with_large_displacements_bit_exact(oracle_mod, dims, seed, amp):
+    """States displaced by several voxels: most voxels' psi_inv and psi leave the LDS stages, so
+    pass A's and pass B's gather fallbacks carry the step; bit-exact against the oracle."""
+    say("large-displacement step", dims, seed, amp)
+    O = oracle_mod
+    g, o, pair, rng = _both(dims, O, seed)
+    st = phantoms.random_state(dims, rng, amp=amp, smooth=3.0)
+    g.set_state(**st)
+    o.set_state(**st)
+    for k, t in enumerate([0.004, 0.3]):
+        mg, ag = g.step(t)
+        mo, ao = o.step(t)
+        _assert_state_equal(g.get_state(), o.get_state(), f"step {k}")
+        assert mg == mo and ag == ao, (mg, mo, ag, ao)
+
+
 def test_run_bit_exact_with_log(oracle_mod):
     """A 25-iteration verbose run: state, max-norm history, log lines and Dice equal the oracle."""
     O = oracle_mod
