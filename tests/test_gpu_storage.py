@@ -43,13 +43,13 @@ def _same_state(g, o, what):
 
 
 @pytest.mark.parametrize("storage", STORAGES)
-@pytest.mark.parametrize("dims,seed", [((24, 20, 18), 3), ((67, 9, 11), 4)])
-def test_step_teacher_forced(oracle_mod, storage, dims, seed):
+@pytest.mark.parametrize("dims,seed,amp", [((24, 20, 18), 3, 1.5), ((67, 9, 11), 4, 1.5), ((70, 40, 36), 31, 4.0)])
+def test_step_teacher_forced(oracle_mod, storage, dims, seed, amp):
     O = oracle_mod
-    say("storage step", storage, dims)
+    say("storage step", storage, dims, amp)
     g, o, pair = _both(O, dims, seed, storage)
     rng = np.random.default_rng(seed)
-    st = phantoms.random_state(dims, rng, amp=1.5)
+    st = phantoms.random_state(dims, rng, amp=amp)
     g.set_state(**st)
     o.set_state(**st)
     _same_state(g, o, "set_state")  # both round v and rho the same way
