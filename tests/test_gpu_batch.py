@@ -119,3 +119,27 @@ def test_autotuned_batch_matches_oracle(oracle_mod):
         _check(vols, "tuned")
     finally:
         b.close()
+
+
+def test_batch_of_degenerate_volumes_matches_oracle(oracle_mod):
+    """A batch of the smallest volumes and one-voxel-thick interiors next to a 3-tile-wide one."""
+    O = oracle_mod
+    dims_list = [(3, 3, 3), (5, 4, 3), (3, 17, 4), (131, 11, 5), (4, 5, 40)]
+    vols = []
+    for k, dims in enumerate(dims_list):
+        X, Y, Z = dims
+        pair = phantoms.blobs(shape=(Z, Y, X), seed=200 + k)
+        p = fr.default_params(vol_dims=dims, no_labels=pair.no_labels, max_iter=7, alpha=0.3)
+        g = fr.Solver(p)
+        g.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+        o = O.Oracle(dims, "f32")
+        o.init(pair.target, pair.target_labels, pair.source, pair.source_labels, p.rho_0)
+        r = o.run(O.SolverParams(max_iter=7, alpha=0.3, no_labels=pair.no_labels))
+        vols.append((g, o, r))
+    say("degenerate batch", dims_list)
+    b = fr.Batch([g for g, _, _ in vols])
+    try:
+        b.run()
+        _check(vols, "degenerate batch")
+    finally:
+        b.close()
