@@ -320,10 +320,11 @@ __device__ __forceinline__ D3 fv_psi_inv(Dims d, int x, int y, int z, D3 o, cons
 }
 
 // F1, F2, F4, F5, F6, F7, F9 of every voxel: the new rho, v and psi_inv from the old state, L and
-// the images; with `record`, also the intermediates the adjoint reads (J, Lo, Tp, gU, dlv)
+// the images; with `record`, also the intermediates the adjoint reads (Lo, Tp, gU, dlv; the
+// adjoint recomputes J from psi_inv, 72 bytes per voxel it would otherwise store and read twice)
 __global__ __launch_bounds__(256) void f_step(Dims d, const double *rho, const double *v, const double *pinv,
                                               const double *L, const double *T, const double *S, const double *gS,
-                                              double *rho1, double *v1, double *pinv1, double *J_out, double *Lo_out,
+                                              double *rho1, double *v1, double *pinv1, double *Lo_out,
                                               double *Tp_out, double *gU_out, double *dlv_out, int record,
                                               double alpha, double w_reg, double m1t, double tau, double m2) {
     FOR_VOXELS(d, i) {
@@ -343,7 +344,6 @@ __global__ __launch_bounds__(256) void f_step(Dims d, const double *rho, const d
         st(v1, i, w);
         st(pinv1, i, pi);
         if (record) {
-            for (int k = 0; k < 9; ++k) J_out[9 * i + k] = J[k];
             st(Lo_out, i, lo);
             Tp_out[i] = tp;
             st(gU_out, i, gu);
@@ -508,7 +508,8 @@ __global__ void b_scatter_irregular(Dims d, const int *code, const double *pos, 
 //   psi1 = psi + alpha trilerp(v1, psi):      gpsi += alpha (d trilerp / d pos)^T gpsi1,
 //                                              record (cell of psi, alpha gpsi1) for the corners of v1
 // gpinv, gpsi, gv1 are updated in place (off the interior they pass through); gJ is written.
-__global__ void b_step1(Dims d, const double *J, const double *v1, const double *psi, double *gpinv, double *gJ,
+// J is recomputed from psi_inv (F2's own function: the same values) rather than stored.
+__global__ void b_step1(Dims d, const double *pinv, const double *v1, const double *psi, double *gpinv, double *gJ,
                         double *gv1, double *gpsi, int *code, double *abc, double *gw, double alpha) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
@@ -518,7 +519,8 @@ __global__ void b_step1(Dims d, const double *J, const double *v1, const double 
             continue;
         }
         const D3 g9 = ld(gpinv, i), w = ld(v1, i);
-        const double *m = J + 9 * i;
+        double m[9];
+        fv_jacobian(d, x, y, z, pinv, m);  // J of the forward step, recomputed from psi_inv
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) gJ[9 * i + 3 * r + c] = -alpha * comp(g9, r) * comp(w, c);
         const D3 gv = ld(gv1, i);
@@ -675,7 +677,7 @@ __global__ void b_step4(Dims d, const double *v, const double *gdvv, const doubl
 // (d trilerp(T) / d pos)^T gTp, and the record (cell of psi_inv, gLo) for the corners of L.
 // gL is cleared here (the irregular scatter and the gather fill it).
 __global__ __launch_bounds__(256) void b_step5(Dims d, const double *S, const double *gS, const double *T,
-                                               const double *Tp, const double *Lo, const double *L, const double *J,
+                                               const double *Tp, const double *Lo, const double *L,
                                                const double *pinv, const double *ggU, double *gJ, double *gpinv,
                                                double *gL, int *code, double *abc, double *gw, double w_reg,
                                                double *gwsum) {
@@ -688,9 +690,10 @@ __global__ __launch_bounds__(256) void b_step5(Dims d, const double *S, const do
             code[i] = 0;
             continue;
         }
-        double m[3][3], dg[3][3];
+        double m[3][3], dg[3][3], J[9];
+        fv_jacobian(d, x, y, z, pinv, J);  // recomputed from psi_inv, as in step 1
         for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 3; ++c) m[r][c] = J[9 * i + 3 * r + c];
+            for (int c = 0; c < 3; ++c) m[r][c] = J[3 * r + c];
         const double det = det_ref(m), diff = S[i] - Tp[i];
         const D3 g = ld(ggU, i), s = ld(gS, i), lo = ld(Lo, i);
         const double gdet = g.x * (diff * s.x + w_reg * lo.x) + g.y * (diff * s.y + w_reg * lo.y) +
@@ -948,7 +951,7 @@ struct State {
 };
 // The intermediates of one iteration that its adjoint reads.
 struct Inter {
-    double *J, *L, *Lo, *Tp, *gU, *dlv;
+    double *L, *Lo, *Tp, *gU, *dlv;
 };
 
 int err(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -1030,13 +1033,12 @@ struct fr_adjoint {
     }
     int alloc_inter(Inter *t) {
         double *b = nullptr;
-        if (int rc = alloc(&b, 22 * (size_t)n)) return rc;
-        t->J = b;
-        t->L = b + 9 * n;
-        t->Lo = b + 12 * n;
-        t->Tp = b + 15 * n;
-        t->gU = b + 16 * n;
-        t->dlv = b + 19 * n;
+        if (int rc = alloc(&b, 13 * (size_t)n)) return rc;
+        t->L = b;
+        t->Lo = b + 3 * n;
+        t->Tp = b + 6 * n;
+        t->gU = b + 7 * n;
+        t->dlv = b + 10 * n;
         return FR_OK;
     }
     ~fr_adjoint() {
@@ -1086,7 +1088,7 @@ int forward_iteration(fr_adjoint *h, const State &in, const State &out, const In
     hipStream_t s = h->stream;
     const double a = h->alpha;
     frd::f_laplacian<<<g, TPB, 0, s>>>(d, in.psi, it.L);
-    frd::f_step<<<g, TPB, 0, s>>>(d, in.rho, in.v, in.pinv, it.L, h->T, h->S, h->gS, out.rho, out.v, out.pinv, it.J,
+    frd::f_step<<<g, TPB, 0, s>>>(d, in.rho, in.v, in.pinv, it.L, h->T, h->S, h->gS, out.rho, out.v, out.pinv,
                                   it.Lo, it.Tp, it.gU, it.dlv, record ? 1 : 0, a, h->w_reg, m1t, h->tau, m2);
     frd::f_psi<<<g, TPB, 0, s>>>(d, in.psi, out.v, out.psi, a);
     ADJ_TRY(hipGetLastError());
@@ -1102,12 +1104,13 @@ int backward_iteration(fr_adjoint *h, const State &in, const State &nx, const In
     hipStream_t s = h->stream;
     const double a = h->alpha;
     double *gr1 = h->grho[*cur], *gr0 = h->grho[1 - *cur];
-    frd::b_step1<<<g, TPB, 0, s>>>(d, it.J, nx.v, in.psi, h->gpinv, h->gJ, h->gv, h->gpsi, h->code, h->abc, h->gw, a);
+    frd::b_step1<<<g, TPB, 0, s>>>(d, in.pinv, nx.v, in.psi, h->gpinv, h->gJ, h->gv, h->gpsi, h->code, h->abc, h->gw,
+                                   a);
     frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.psi, h->gw, h->gv);
     frd::b_step3<<<g, TPB, 0, s>>>(d, h->code, h->abc, h->gw, nx.rho, it.dlv, it.gU, h->gv, h->gdvv, h->gdlv, h->ggU,
                                    gr1, a, m1t, h->tau, m2, h->scal + 2);
     frd::b_step4<<<g, TPB, 0, s>>>(d, in.v, h->gdvv, h->gdlv, h->gv);
-    frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.Tp, it.Lo, it.L, it.J, in.pinv, h->ggU, h->gJ, h->gpinv,
+    frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.Tp, it.Lo, it.L, in.pinv, h->ggU, h->gJ, h->gpinv,
                                    h->gL, h->code, h->abc, h->gw, h->w_reg, h->scal + 1);
     frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.pinv, h->gw, h->gL);
     frd::b_step7<<<g, TPB, 0, s>>>(d, h->code, h->abc, h->gw, h->gL);
