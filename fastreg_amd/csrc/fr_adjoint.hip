@@ -320,12 +320,12 @@ __device__ __forceinline__ D3 fv_psi_inv(Dims d, int x, int y, int z, D3 o, cons
 }
 
 // F1, F2, F4, F5, F6, F7, F9 of every voxel: the new rho, v and psi_inv from the old state, L and
-// the images; with `record`, also the intermediates the adjoint reads (Lo, Tp, gU, dlv; the
-// adjoint recomputes J from psi_inv, 72 bytes per voxel it would otherwise store and read twice)
+// the images; with `record`, also the intermediates the adjoint reads (gU, dlv; the adjoint
+// recomputes J from psi_inv and Lo, Tp from L and T at psi_inv, which it gathers anyway)
 __global__ __launch_bounds__(256) void f_step(Dims d, const double *rho, const double *v, const double *pinv,
                                               const double *L, const double *T, const double *S, const double *gS,
-                                              double *rho1, double *v1, double *pinv1, double *Lo_out,
-                                              double *Tp_out, double *gU_out, double *dlv_out, int record,
+                                              double *rho1, double *v1, double *pinv1, double *gU_out,
+                                              double *dlv_out, int record,
                                               double alpha, double w_reg, double m1t, double tau, double m2) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
@@ -344,8 +344,6 @@ __global__ __launch_bounds__(256) void f_step(Dims d, const double *rho, const d
         st(v1, i, w);
         st(pinv1, i, pi);
         if (record) {
-            st(Lo_out, i, lo);
-            Tp_out[i] = tp;
             st(gU_out, i, gu);
             st(dlv_out, i, dlv);
         }
@@ -677,8 +675,8 @@ __global__ void b_step4(Dims d, const double *v, const double *gdvv, const doubl
 // (d trilerp(T) / d pos)^T gTp, and the record (cell of psi_inv, gLo) for the corners of L.
 // gL is cleared here (the irregular scatter and the gather fill it).
 __global__ __launch_bounds__(256) void b_step5(Dims d, const double *S, const double *gS, const double *T,
-                                               const double *Tp, const double *Lo, const double *L,
-                                               const double *pinv, const double *ggU, double *gJ, double *gpinv,
+                                               const double *L, const double *pinv, const double *ggU, double *gJ,
+                                               double *gpinv,
                                                double *gL, int *code, double *abc, double *gw, double w_reg,
                                                double *gwsum) {
     __shared__ double sh[4];
@@ -694,8 +692,11 @@ __global__ __launch_bounds__(256) void b_step5(Dims d, const double *S, const do
         fv_jacobian(d, x, y, z, pinv, J);  // recomputed from psi_inv, as in step 1
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) m[r][c] = J[3 * r + c];
-        const double det = det_ref(m), diff = S[i] - Tp[i];
-        const D3 g = ld(ggU, i), s = ld(gS, i), lo = ld(Lo, i);
+        D3 lo;
+        double tp;
+        fv_interp(d, i, L, T, pinv, &lo, &tp);  // Lo and Tp of the forward step, recomputed (F4)
+        const double det = det_ref(m), diff = S[i] - tp;
+        const D3 g = ld(ggU, i), s = ld(gS, i);
         const double gdet = g.x * (diff * s.x + w_reg * lo.x) + g.y * (diff * s.y + w_reg * lo.y) +
                             g.z * (diff * s.z + w_reg * lo.z);
         const D3 glo = d3(w_reg * det * g.x, w_reg * det * g.y, w_reg * det * g.z);
@@ -951,7 +952,7 @@ struct State {
 };
 // The intermediates of one iteration that its adjoint reads.
 struct Inter {
-    double *L, *Lo, *Tp, *gU, *dlv;
+    double *L, *gU, *dlv;
 };
 
 int err(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -1033,12 +1034,10 @@ struct fr_adjoint {
     }
     int alloc_inter(Inter *t) {
         double *b = nullptr;
-        if (int rc = alloc(&b, 13 * (size_t)n)) return rc;
+        if (int rc = alloc(&b, 9 * (size_t)n)) return rc;
         t->L = b;
-        t->Lo = b + 3 * n;
-        t->Tp = b + 6 * n;
-        t->gU = b + 7 * n;
-        t->dlv = b + 10 * n;
+        t->gU = b + 3 * n;
+        t->dlv = b + 6 * n;
         return FR_OK;
     }
     ~fr_adjoint() {
@@ -1089,7 +1088,7 @@ int forward_iteration(fr_adjoint *h, const State &in, const State &out, const In
     const double a = h->alpha;
     frd::f_laplacian<<<g, TPB, 0, s>>>(d, in.psi, it.L);
     frd::f_step<<<g, TPB, 0, s>>>(d, in.rho, in.v, in.pinv, it.L, h->T, h->S, h->gS, out.rho, out.v, out.pinv,
-                                  it.Lo, it.Tp, it.gU, it.dlv, record ? 1 : 0, a, h->w_reg, m1t, h->tau, m2);
+                                  it.gU, it.dlv, record ? 1 : 0, a, h->w_reg, m1t, h->tau, m2);
     frd::f_psi<<<g, TPB, 0, s>>>(d, in.psi, out.v, out.psi, a);
     ADJ_TRY(hipGetLastError());
     return FR_OK;
@@ -1110,7 +1109,7 @@ int backward_iteration(fr_adjoint *h, const State &in, const State &nx, const In
     frd::b_step3<<<g, TPB, 0, s>>>(d, h->code, h->abc, h->gw, nx.rho, it.dlv, it.gU, h->gv, h->gdvv, h->gdlv, h->ggU,
                                    gr1, a, m1t, h->tau, m2, h->scal + 2);
     frd::b_step4<<<g, TPB, 0, s>>>(d, in.v, h->gdvv, h->gdlv, h->gv);
-    frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.Tp, it.Lo, it.L, in.pinv, h->ggU, h->gJ, h->gpinv,
+    frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.L, in.pinv, h->ggU, h->gJ, h->gpinv,
                                    h->gL, h->code, h->abc, h->gw, h->w_reg, h->scal + 1);
     frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.pinv, h->gw, h->gL);
     frd::b_step7<<<g, TPB, 0, s>>>(d, h->code, h->abc, h->gw, h->gL);
