@@ -322,7 +322,7 @@ __device__ __forceinline__ D3 fv_psi_inv(Dims d, int x, int y, int z, D3 o, cons
 // F1, F2, F4, F5, F6, F7, F9 of every voxel: the new rho, v and psi_inv from the old state, L and
 // the images; with `record`, also the intermediates the adjoint reads (gU, dlv; the adjoint
 // recomputes J from psi_inv and Lo, Tp from L and T at psi_inv, which it gathers anyway)
-__global__ __launch_bounds__(256) void f_step(Dims d, const double *rho, const double *v, const double *pinv,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void f_step(Dims d, const double *rho, const double *v, const double *pinv,
                                               const double *L, const double *T, const double *S, const double *gS,
                                               double *rho1, double *v1, double *pinv1, double *gU_out,
                                               double *dlv_out, int record,
