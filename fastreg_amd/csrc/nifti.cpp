@@ -60,6 +60,13 @@ static int bytes_of(int16_t dt) {
     }
 }
 
+// a value of type V from raw bytes (no alignment or aliasing assumptions)
+template <typename V> static V load(const unsigned char *p) {
+    V v;
+    std::memcpy(&v, p, sizeof v);
+    return v;
+}
+
 bool read(const std::string &path, Image &img, std::string &err) {
     std::ifstream f(path, std::ios::binary);
     if (!f) {
@@ -125,14 +132,14 @@ bool read(const std::string &path, Image &img, std::string &err) {
         if (swapped) swap_bytes(p, bpv);
         double v = 0;
         switch (h.datatype) {
-            case DT_UINT8: v = *reinterpret_cast<uint8_t *>(p); break;
-            case DT_INT8: v = *reinterpret_cast<int8_t *>(p); break;
-            case DT_INT16: v = *reinterpret_cast<int16_t *>(p); break;
-            case DT_UINT16: v = *reinterpret_cast<uint16_t *>(p); break;
-            case DT_INT32: v = *reinterpret_cast<int32_t *>(p); break;
-            case DT_UINT32: v = *reinterpret_cast<uint32_t *>(p); break;
-            case DT_FLOAT32: v = *reinterpret_cast<float *>(p); break;
-            case DT_FLOAT64: v = *reinterpret_cast<double *>(p); break;
+            case DT_UINT8: v = load<uint8_t>(p); break;
+            case DT_INT8: v = load<int8_t>(p); break;
+            case DT_INT16: v = load<int16_t>(p); break;
+            case DT_UINT16: v = load<uint16_t>(p); break;
+            case DT_INT32: v = load<int32_t>(p); break;
+            case DT_UINT32: v = load<uint32_t>(p); break;
+            case DT_FLOAT32: v = load<float>(p); break;
+            case DT_FLOAT64: v = load<double>(p); break;
         }
         if (h.scl_slope != 0.f && std::isfinite(h.scl_slope) && !(h.scl_slope == 1.f && h.scl_inter == 0.f))
             v = v * h.scl_slope + h.scl_inter;
