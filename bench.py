@@ -66,6 +66,10 @@ def main():
     ap.add_argument("--batch-iters", type=int, default=150)
     ap.add_argument("--batch-sizes", choices=("mixed", "small"), default="mixed",
                     help="mixed: uniform over the five sizes; small: uniform over 48^3 and 96^3")
+    ap.add_argument("--batch-storage", choices=("fp32", "bf16", "fp16"), default="fp32",
+                    help="storage of v and rho in the batched leg (the one-at-a-time leg is the plain FP32 form); "
+                         "with bf16 / fp16 the Dice of every pair is compared with the FP32 form's and, for the "
+                         "pairs of up to 96^3 voxels, with an FP64 oracle run")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="before timing, run one device-to-device copy of this many MiB (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -494,7 +498,10 @@ def batch_workload(args):
     twice on the same contexts: one pair at a time (fr_solver_run per pair, HIP events per pair)
     and as one batch (fr_batch_run: one launch per pass per iteration for all pairs, replayed
     from a hipGraph, z-chunks from the autotuner).  Both must give the same max-norm history for
-    every pair."""
+    every pair.  With --batch-storage bf16 / fp16 the one-at-a-time leg stays the plain FP32 form
+    and the batch stores v and rho in 16 bits; then the history check covers the first four pairs
+    (run one at a time in the same storage), and the batched Dice of every pair is compared with
+    the FP32 form's and, for the pairs of up to 96^3, with an FP64 oracle run's."""
     import multiprocessing as mp
     import numpy as np
 
@@ -532,6 +539,22 @@ def batch_workload(args):
         hist.append(s.history()[0][: st.iterations].copy())
     plain_wall = (time.perf_counter() - tw) * 1e3
     plain_ms = float(sum(per_pair))
+    storage = {"fp32": fr.STORAGE_F32, "bf16": fr.STORAGE_BF16, "fp16": fr.STORAGE_F16}[args.batch_storage]
+    dice32 = []
+    if storage != fr.STORAGE_F32:
+        # Dice of the plain FP32 form, then the batched leg's storage; the first pairs are run one at a
+        # time in that storage too, for the history check below
+        for s in solvers:
+            s.finalize(want_outputs=False)
+            dice32.append(s.stats().dice_all)
+            s.set_storage(storage)
+        hist = hist[: min(4, len(solvers))]
+        for k in range(len(hist)):
+            s = solvers[k]
+            s.reinit()
+            s.run()
+            st = s.stats()
+            hist[k] = s.history()[0][: st.iterations].copy()
     for s in solvers:
         s.reinit()
 
@@ -546,12 +569,24 @@ def batch_workload(args):
     b.run()
     batch_ms = b.time_ms()
     batch_wall = (time.perf_counter() - tw) * 1e3
-    for k, s in enumerate(solvers):
+    for k in range(len(hist)):
+        s = solvers[k]
         st = s.stats()
         assert np.array_equal(s.history()[0][: st.iterations], hist[k]), f"pair {k}: batch and single runs differ"
     b.close()
 
     n = len(solvers)
+    dice = None
+    if dice32:
+        dice16 = []
+        for s in solvers:
+            s.finalize(want_outputs=False)
+            dice16.append(s.stats().dice_all)
+        dev = np.abs(np.asarray(dice16) - np.asarray(dice32))
+        dice = {"batched_storage_vs_plain_fp32": {"max_abs_diff": round(float(dev.max()), 6),
+                                                  "mean_abs_diff": round(float(dev.mean()), 6)}}
+        dice["vs_fp64_oracle"] = _batch_dice_vs_fp64(jobs, pairs, args.batch_iters, dice16, dice32)
+        print(f"[bench] Dice: {dice}", file=sys.stderr, flush=True)
     out = {
         "metric": f"pairs/s, {n} pairs of uneven sizes x {args.batch_iters} iterations",
         "value": round(n / (batch_ms / 1e3), 2),
@@ -575,7 +610,44 @@ def batch_workload(args):
         "speedup": round(plain_ms / batch_ms, 3),
         "per_pair_latency_ms_batched": round(batch_ms, 2),
     }
+    if dice is not None:
+        out["dtype"] = f"{args.batch_storage} storage of v and rho, fp32 arithmetic (one-at-a-time leg: fp32)"
+        out["dice"] = dice
     print(json.dumps(out), flush=True)
+
+
+def _batch_dice_vs_fp64(jobs, pairs, iters, dice_batched, dice_fp32, max_voxels=96 ** 3):
+    """Dice of the pairs of up to `max_voxels` voxels from the FP64 oracle (the reference's
+    algorithm in double, same iterations and weights), against the batched leg's and the FP32
+    form's Dice: per size, the largest absolute difference of each."""
+    import numpy as np
+    from oracle import oracle as O
+
+    O.build()
+    per = {}
+    t0 = time.time()
+    for k, (i, dims) in enumerate(jobs):
+        if dims[0] * dims[1] * dims[2] > max_voxels:
+            continue
+        pair = pairs[i]
+        o = O.Oracle(dims, "f64")
+        o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+        o.run(O.SolverParams(max_iter=iters, no_labels=pair.no_labels))
+        o.outputs()
+        ov, vx, _ = o.dice_counts(pair.no_labels)
+        o.close()
+        d64 = 2.0 * ov[1:].sum() / vx[1:].sum()
+        e = per.setdefault("x".join(map(str, dims)), {"pairs": 0, "batched_max_abs_diff": 0.0,
+                                                      "fp32_max_abs_diff": 0.0})
+        e["pairs"] += 1
+        if sum(v["pairs"] for v in per.values()) % 8 == 0:
+            print(f"[bench] FP64 oracle: {sum(v['pairs'] for v in per.values())} pairs done", file=sys.stderr,
+                  flush=True)
+        e["batched_max_abs_diff"] = round(float(max(e["batched_max_abs_diff"], abs(dice_batched[k] - d64))), 6)
+        e["fp32_max_abs_diff"] = round(float(max(e["fp32_max_abs_diff"], abs(dice_fp32[k] - d64))), 6)
+    print(f"[bench] FP64 oracle Dice of {sum(e['pairs'] for e in per.values())} pairs: {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    return per
 
 
 if __name__ == "__main__":
