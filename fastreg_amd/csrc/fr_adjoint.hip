@@ -459,27 +459,77 @@ __device__ __forceinline__ void put_rec(int *code, double *abc, double *gw, int 
     st(gw, i, g);
 }
 
-// sum over the regular sources within one voxel of target (x, y, z) of weight x adjoint
-__device__ __forceinline__ D3 gather_cells(Dims d, int x, int y, int z, const int *code, const double *abc,
-                                           const double *gw) {
+// The gathers of the records run over tiles of 64 (x) x 4 (y) targets, each block marching a chunk
+// of z-planes.  The records of the tile's neighbourhood (x0-1 .. x0+64, y0-1 .. y0+4) for the three
+// planes z-1, z, z+1 sit in LDS (a ring of 3 planes; records outside the grid, and the cell data of
+// sources that are not regular, are stored as code 0 / zeros), so a target reads its 27 sources
+// from LDS and every record is read from memory about 1.6 times instead of 27.
+constexpr int GT_X = 64, GT_Y = 4, GT_THREADS = GT_X * GT_Y;
+constexpr int GS_X = GT_X + 2, GS_Y = GT_Y + 2, GS_N = GS_X * GS_Y;
+constexpr int GS_PER_THREAD = (GS_N + GT_THREADS - 1) / GT_THREADS;
+struct RecStage {
+    int code[3][GS_N];
+    double v[6][3][GS_N];  // cell fractions a, b, c; adjoint g.x, g.y, g.z
+};
+struct RecRegs {
+    int code[GS_PER_THREAD];
+    double v[6][GS_PER_THREAD];
+};
+
+// this thread's share of plane zz of the tile's neighbourhood, from memory into registers
+__device__ __forceinline__ void rec_fetch(Dims d, int x0, int y0, int zz, const int *code, const double *abc,
+                                          const double *gw, RecRegs &r) {
+    for (int q = 0; q < GS_PER_THREAD; ++q) {
+        const int e = threadIdx.x + q * GT_THREADS;
+        const int gx = x0 - 1 + e % GS_X, gy = y0 - 1 + e / GS_X;
+        int c = 0;
+        D3 f = d3(0, 0, 0), g = d3(0, 0, 0);
+        if (e < GS_N && gx >= 0 && gx < d.X && gy >= 0 && gy < d.Y && zz >= 0 && zz < d.Z) {
+            const int i = lin(d, gx, gy, zz);
+            c = code[i];
+            if ((c & (RC_VALID | RC_IRREG)) == RC_VALID) {
+                f = ld(abc, i);
+                g = ld(gw, i);
+            } else {
+                c = 0;
+            }
+        }
+        r.code[q] = c;
+        r.v[0][q] = f.x;
+        r.v[1][q] = f.y;
+        r.v[2][q] = f.z;
+        r.v[3][q] = g.x;
+        r.v[4][q] = g.y;
+        r.v[5][q] = g.z;
+    }
+}
+__device__ __forceinline__ void rec_put(RecStage &st, int slot, const RecRegs &r) {
+    for (int q = 0; q < GS_PER_THREAD; ++q) {
+        const int e = threadIdx.x + q * GT_THREADS;
+        if (e < GS_N) {
+            st.code[slot][e] = r.code[q];
+            for (int k = 0; k < 6; ++k) st.v[k][slot][e] = r.v[k][q];
+        }
+    }
+}
+
+// sum over the regular sources within one voxel of target (x0 + lx, y0 + ly, z) of weight x
+// adjoint, in the order sz, sy, sx = -1, 0, 1 (sources outside the grid have code 0)
+__device__ __forceinline__ D3 rec_gather(const RecStage &st, int lx, int ly, int z) {
     D3 acc = d3(0, 0, 0);
     for (int sz = -1; sz <= 1; ++sz) {
-        const int zz = z + sz;
-        if (zz < 0 || zz >= d.Z) continue;
+        const int slot = (z + sz + 3) % 3;
         for (int sy = -1; sy <= 1; ++sy) {
-            const int yy = y + sy;
-            if (yy < 0 || yy >= d.Y) continue;
             for (int sx = -1; sx <= 1; ++sx) {
-                const int xx = x + sx;
-                if (xx < 0 || xx >= d.X) continue;
-                const int i = lin(d, xx, yy, zz);
-                const int c = code[i];
-                if ((c & (RC_VALID | RC_IRREG)) != RC_VALID) continue;
+                const int e = (ly + 1 + sy) * GS_X + (lx + 1 + sx);
+                const int c = st.code[slot][e];
+                if (c == 0) continue;
                 // the target is the source's (x, y, z) + t; corner sides: t == o (floor), t == o + 1
                 const int ox = ((c >> 2) & 1) - 1, oy = ((c >> 3) & 1) - 1, oz = ((c >> 4) & 1) - 1;
                 const int tx = -sx, ty = -sy, tz = -sz;
                 if ((tx != ox && tx != ox + 1) || (ty != oy && ty != oy + 1) || (tz != oz && tz != oz + 1)) continue;
-                const D3 f = ld(abc, i), g = ld(gw, i);
+                const D3 f = d3(st.v[0][slot][e], st.v[1][slot][e], st.v[2][slot][e]),
+                         g = d3(st.v[3][slot][e], st.v[4][slot][e], st.v[5][slot][e]);
                 const double wa = tx == ox ? 1 - f.x : f.x, wb = ty == oy ? 1 - f.y : f.y, wc = tz == oz ? 1 - f.z : f.z;
                 const double w = wa * wb * wc;
                 acc = d3(acc.x + w * g.x, acc.y + w * g.y, acc.z + w * g.z);
@@ -487,6 +537,36 @@ __device__ __forceinline__ D3 gather_cells(Dims d, int x, int y, int z, const in
         }
     }
     return acc;
+}
+
+// Block b of a gather launch: tile (b % tiles_x, (b / tiles_x) % tiles_y), z-planes
+// [zc * (b / (tiles_x tiles_y)), + zc).  body(i, x, y, z, sum) runs for every target of the tile in
+// the grid; every thread of the block runs the same number of loop steps (block-wide barriers).
+template <typename Body>
+__device__ __forceinline__ void rec_gather_tile(Dims d, int tiles_x, int tiles_y, int zc, const int *code,
+                                                const double *abc, const double *gw, Body body) {
+    __shared__ RecStage st;
+    const int b = blockIdx.x;
+    const int x0 = (b % tiles_x) * GT_X, y0 = ((b / tiles_x) % tiles_y) * GT_Y;
+    const int z0 = (b / (tiles_x * tiles_y)) * zc, z1 = min(z0 + zc, d.Z);
+    const int lx = threadIdx.x % GT_X, ly = threadIdx.x / GT_X;
+    const int x = x0 + lx, y = y0 + ly;
+    RecRegs r;
+    rec_fetch(d, x0, y0, z0 - 1, code, abc, gw, r);
+    rec_put(st, (z0 + 2) % 3, r);
+    rec_fetch(d, x0, y0, z0, code, abc, gw, r);
+    rec_put(st, z0 % 3, r);
+    rec_fetch(d, x0, y0, z0 + 1, code, abc, gw, r);
+    for (int z = z0; z < z1; ++z) {
+        rec_put(st, (z + 1) % 3, r);  // plane z + 1 (its slot held plane z - 2, no longer read)
+        __syncthreads();
+        if (z + 2 <= z1) rec_fetch(d, x0, y0, z + 2, code, abc, gw, r);  // in flight during the gather
+        if (x < d.X && y < d.Y) {
+            const D3 sc = rec_gather(st, lx, ly, z);
+            body(lin(d, x, y, z), x, y, z, sc);
+        }
+        __syncthreads();
+    }
 }
 
 // the irregular sources' scatter: out(corners of pos(i)) += weights x gw(i)
@@ -545,15 +625,15 @@ __global__ void b_step1(Dims d, const double *pinv, const double *v1, const doub
 //   gv = (1 - alpha m1t) gv1, gdvv = -alpha gv1, gdlv = alpha tau gv1, ggU = -alpha q gv1,
 //   grho1 += -alpha (gv1 . gU) dq/drho1, dE/dtau += alpha gv1 . dlv (block sums, one atomic per block).
 // gv holds gv1 on entry and gv on exit.
-__global__ __launch_bounds__(256) void b_step3(Dims d, const int *code, const double *abc, const double *gw,
-                                               const double *rho1, const double *dlv, const double *gU, double *gv,
-                                               double *gdvv, double *gdlv, double *ggU, double *grho1, double alpha,
-                                               double m1t, double tau, double m2, double *gtau) {
+__global__ __launch_bounds__(GT_THREADS) void b_step3(Dims d, int tiles_x, int tiles_y, int zc, const int *code,
+                                                      const double *abc, const double *gw, const double *rho1,
+                                                      const double *dlv, const double *gU, double *gv, double *gdvv,
+                                                      double *gdlv, double *ggU, double *grho1, double alpha,
+                                                      double m1t, double tau, double m2, double *gtau) {
     __shared__ double sh[4];
     double acc = 0;
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        const D3 own = ld(gv, i), sc = gather_cells(d, x, y, z, code, abc, gw);
+    rec_gather_tile(d, tiles_x, tiles_y, zc, code, abc, gw, [&](int i, int x, int y, int z, D3 sc) {
+        const D3 own = ld(gv, i);
         const D3 g = d3(own.x + sc.x, own.y + sc.y, own.z + sc.z);
         if (interior(d, x, y, z)) {
             const double den = fabs(rho1[i]) + (double)1e-5f, q = m2 / den;
@@ -572,7 +652,7 @@ __global__ __launch_bounds__(256) void b_step3(Dims d, const int *code, const do
             st(gdlv, i, d3(0, 0, 0));
             st(ggU, i, d3(0, 0, 0));
         }
-    }
+    });
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) aadd(gtau, s);
 }
@@ -727,12 +807,12 @@ __global__ __launch_bounds__(256) void b_step5(Dims d, const double *S, const do
 }
 
 // Step 7: gL += the records' gather (the transpose of Lo = trilerp(L, psi_inv))
-__global__ void b_step7(Dims d, const int *code, const double *abc, const double *gw, double *gL) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        const D3 o = ld(gL, i), g = gather_cells(d, x, y, z, code, abc, gw);
+__global__ __launch_bounds__(GT_THREADS) void b_step7(Dims d, int tiles_x, int tiles_y, int zc, const int *code,
+                                                      const double *abc, const double *gw, double *gL) {
+    rec_gather_tile(d, tiles_x, tiles_y, zc, code, abc, gw, [&](int i, int, int, int, D3 g) {
+        const D3 o = ld(gL, i);
         st(gL, i, d3(o.x + g.x, o.y + g.y, o.z + g.z));
-    }
+    });
 }
 
 // Step 8: transposes of F3, F2 and F1 gathered into voxel i:
@@ -1003,6 +1083,8 @@ struct fr_adjoint {
     double *scal = nullptr;  // [0] objective, [1] dE/dw_reg, [2] dE/dtau
     double *partials = nullptr;
     int grid = 0;
+    // launch shape of the record gathers: 64 x 4 tiles, zc z-planes per block
+    int gt_tiles_x = 0, gt_tiles_y = 0, gt_zc = 0, gt_blocks = 0;
     // per-iteration scalars of the last forward (index k-1 for iteration k)
     std::vector<double> m1t, m2;
     double w_reg = 0, tau = 0, lw = 0, energy = 0;
@@ -1106,13 +1188,15 @@ int backward_iteration(fr_adjoint *h, const State &in, const State &nx, const In
     frd::b_step1<<<g, TPB, 0, s>>>(d, in.pinv, nx.v, in.psi, h->gpinv, h->gJ, h->gv, h->gpsi, h->code, h->abc, h->gw,
                                    a);
     frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.psi, h->gw, h->gv);
-    frd::b_step3<<<g, TPB, 0, s>>>(d, h->code, h->abc, h->gw, nx.rho, it.dlv, it.gU, h->gv, h->gdvv, h->gdlv, h->ggU,
-                                   gr1, a, m1t, h->tau, m2, h->scal + 2);
+    frd::b_step3<<<h->gt_blocks, frd::GT_THREADS, 0, s>>>(d, h->gt_tiles_x, h->gt_tiles_y, h->gt_zc, h->code, h->abc,
+                                                        h->gw, nx.rho, it.dlv, it.gU, h->gv, h->gdvv, h->gdlv,
+                                                        h->ggU, gr1, a, m1t, h->tau, m2, h->scal + 2);
     frd::b_step4<<<g, TPB, 0, s>>>(d, in.v, h->gdvv, h->gdlv, h->gv);
     frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.L, in.pinv, h->ggU, h->gJ, h->gpinv,
                                    h->gL, h->code, h->abc, h->gw, h->w_reg, h->scal + 1);
     frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.pinv, h->gw, h->gL);
-    frd::b_step7<<<g, TPB, 0, s>>>(d, h->code, h->abc, h->gw, h->gL);
+    frd::b_step7<<<h->gt_blocks, frd::GT_THREADS, 0, s>>>(d, h->gt_tiles_x, h->gt_tiles_y, h->gt_zc, h->code, h->abc,
+                                                        h->gw, h->gL);
     frd::b_step8<<<g, TPB, 0, s>>>(d, h->gL, h->gJ, in.rho, in.v, gr1, h->gpsi, h->gpinv, gr0, h->gv, a);
     ADJ_TRY(hipGetLastError());
     *cur = 1 - *cur;
@@ -1161,6 +1245,20 @@ extern "C" int fr_adjoint_create(const fr_params *p, int K, int ckpt, fr_adjoint
     h->no_labels = p->no_labels;
     const long nb = (h->n + TPB - 1) / TPB;
     h->grid = (int)(nb < 8192 ? nb : 8192);
+    {
+        // about two gather blocks per CU (each holds a 62 KB stage), every block a run of z-planes
+        int dev = 0, n_cu = 256;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu < 1)
+            n_cu = 256;
+        h->gt_tiles_x = (h->d.X + frd::GT_X - 1) / frd::GT_X;
+        h->gt_tiles_y = (h->d.Y + frd::GT_Y - 1) / frd::GT_Y;
+        const long tiles = (long)h->gt_tiles_x * h->gt_tiles_y;
+        long chunks = 2L * n_cu / tiles;
+        chunks = chunks < 1 ? 1 : chunks > h->d.Z ? h->d.Z : chunks;
+        h->gt_zc = (int)((h->d.Z + chunks - 1) / chunks);
+        h->gt_blocks = (int)(tiles * ((h->d.Z + h->gt_zc - 1) / h->gt_zc));
+    }
     const size_t n = (size_t)h->n;
     int rc = FR_OK;
     auto fail = [&](int code) {
