@@ -772,9 +772,24 @@ __global__ __launch_bounds__(256) void b_step5(Dims d, const double *S, const do
         fv_jacobian(d, x, y, z, pinv, J);  // recomputed from psi_inv, as in step 1
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) m[r][c] = J[3 * r + c];
-        D3 lo;
-        double tp;
-        fv_interp(d, i, L, T, pinv, &lo, &tp);  // Lo and Tp of the forward step, recomputed (F4)
+        // the taps of L and T at psi_inv, read once: Lo and Tp of the forward step (F4, the
+        // operations of fv_interp) and the position derivatives of both samples
+        const D3 p = ld(pinv, i);
+        const Corner k = corner(d, p.x, p.y, p.z);
+        double f[8];
+        D3 lo, dpl[3];
+        gather3(L, k, 0, f);
+        lo.x = tri(f, k.a, k.b, k.c);
+        dpl[0] = tri_dpos(f, k);
+        gather3(L, k, 1, f);
+        lo.y = tri(f, k.a, k.b, k.c);
+        dpl[1] = tri_dpos(f, k);
+        gather3(L, k, 2, f);
+        lo.z = tri(f, k.a, k.b, k.c);
+        dpl[2] = tri_dpos(f, k);
+        gather1(T, k, f);
+        const double tp = tri(f, k.a, k.b, k.c);
+        const D3 dt = tri_dpos(f, k);
         const double det = det_ref(m), diff = S[i] - tp;
         const D3 g = ld(ggU, i), s = ld(gS, i);
         const double gdet = g.x * (diff * s.x + w_reg * lo.x) + g.y * (diff * s.y + w_reg * lo.y) +
@@ -785,18 +800,11 @@ __global__ __launch_bounds__(256) void b_step5(Dims d, const double *S, const do
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) gJ[9 * i + 3 * r + c] += gdet * dg[r][c];
         acc += det * (g.x * lo.x + g.y * lo.y + g.z * lo.z);
-        const D3 p = ld(pinv, i);
-        const Corner k = corner(d, p.x, p.y, p.z);
-        double f[8];
         D3 o = d3(0, 0, 0);
         for (int q = 0; q < 3; ++q) {
-            gather3(L, k, q, f);
-            const D3 dp = tri_dpos(f, k);
             const double gq = comp(glo, q);
-            o = d3(o.x + gq * dp.x, o.y + gq * dp.y, o.z + gq * dp.z);
+            o = d3(o.x + gq * dpl[q].x, o.y + gq * dpl[q].y, o.z + gq * dpl[q].z);
         }
-        gather1(T, k, f);
-        const D3 dt = tri_dpos(f, k);
         o = d3(o.x + gtp * dt.x, o.y + gtp * dt.y, o.z + gtp * dt.z);
         const D3 gp = ld(gpinv, i);
         st(gpinv, i, d3(gp.x + o.x, gp.y + o.y, gp.z + o.z));
