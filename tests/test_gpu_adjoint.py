@@ -136,6 +136,23 @@ def test_gradient_matches_autograd_at_depth_64():
     _compare((e, g, gw, gt), _autograd(dims, pair, v0, K, lw), 1e-9)
 
 
+def test_gradient_with_uneven_gather_tiles():
+    """Shapes whose record gathers (64 x 4 tiles marching z-chunks) end in partial tiles and a
+    partial last z-chunk.  On a 256-CU GPU, 70 x 66 x 71 has 2 x 17 tiles and 5-plane chunks, the
+    last of one plane; 130 x 7 x 9 has a 2-column last x-tile and a 3-row last y-tile."""
+    for dims, K, ckpt in (((70, 66, 71), 4, 2), ((130, 7, 9), 5, 5)):
+        say("adjoint vs autograd, uneven gather tiles", dims, K, ckpt)
+        pair = _pair(dims, 14)
+        v0 = _v0(dims, 14)
+        a = _adjoint(dims, pair, K, ckpt)
+        try:
+            e = a.forward(v0, W_REG, TAU, 0.5)
+            g, gw, gt = a.backward()
+        finally:
+            a.close()
+        _compare((e, g, gw, gt), _autograd(dims, pair, v0, K, 0.5), 1e-10)
+
+
 def test_gradient_with_displacements_of_several_voxels():
     """A large v0 moves psi and psi_inv by more than a voxel from the first iteration on, so that the
     transposes of the trilinear samples take their atomic path for the displaced sources."""
