@@ -587,8 +587,10 @@ __global__ void b_scatter_irregular(Dims d, const int *code, const double *pos, 
 //                                              record (cell of psi, alpha gpsi1) for the corners of v1
 // gpinv, gpsi, gv1 are updated in place (off the interior they pass through); gJ is written.
 // J is recomputed from psi_inv (F2's own function: the same values) rather than stored.
-__global__ void b_step1(Dims d, const double *pinv, const double *v1, const double *psi, double *gpinv, double *gJ,
-                        double *gv1, double *gpsi, int *code, double *abc, double *gw, double alpha) {
+__global__ void b_step1(Dims d, const double *__restrict__ pinv, const double *__restrict__ v1,
+                        const double *__restrict__ psi, double *__restrict__ gpinv, double *__restrict__ gJ,
+                        double *__restrict__ gv1, double *__restrict__ gpsi, int *__restrict__ code,
+                        double *__restrict__ abc, double *__restrict__ gw, double alpha) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
         if (!interior(d, x, y, z)) {
@@ -665,7 +667,8 @@ __device__ __forceinline__ bool in_lap(Dims d, int x, int y, int z) {
 // Step 4: transposes of F6 gathered into gv: dvv = (Dv)v (interior) and dlv, the velocity
 // Laplacian (its range; centre 3 times, "far" neighbours (+-2) once, "near" ones (+-1) -2 times,
 // chosen by the sign of the centre's component, with y-1 / z-1 as the near neighbour always).
-__global__ void b_step4(Dims d, const double *v, const double *gdvv, const double *gdlv, double *gv) {
+__global__ void b_step4(Dims d, const double *__restrict__ v, const double *__restrict__ gdvv,
+                        const double *__restrict__ gdlv, double *__restrict__ gv) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
         const int sy = d.X, sz = d.X * d.Y;
@@ -754,11 +757,12 @@ __global__ void b_step4(Dims d, const double *v, const double *gdvv, const doubl
 // then of F4 at this voxel's position psi_inv: gpinv += (d trilerp(L) / d pos)^T gLo +
 // (d trilerp(T) / d pos)^T gTp, and the record (cell of psi_inv, gLo) for the corners of L.
 // gL is cleared here (the irregular scatter and the gather fill it).
-__global__ __launch_bounds__(256) void b_step5(Dims d, const double *S, const double *gS, const double *T,
-                                               const double *L, const double *pinv, const double *ggU, double *gJ,
-                                               double *gpinv,
-                                               double *gL, int *code, double *abc, double *gw, double w_reg,
-                                               double *gwsum) {
+__global__ __launch_bounds__(256) void b_step5(Dims d, const double *__restrict__ S, const double *__restrict__ gS,
+                             const double *__restrict__ T, const double *__restrict__ L,
+                             const double *__restrict__ pinv, const double *__restrict__ ggU, double *__restrict__ gJ,
+                             double *__restrict__ gpinv, double *__restrict__ gL, int *__restrict__ code,
+                             double *__restrict__ abc, double *__restrict__ gw, double w_reg,
+                             double *__restrict__ gwsum) {
     __shared__ double sh[4];
     double acc = 0;
     FOR_VOXELS(d, i) {
@@ -827,8 +831,10 @@ __global__ __launch_bounds__(GT_THREADS) void b_step7(Dims d, int tiles_x, int t
 //   L(j) = 6 psi(j) - (sum of its 6 neighbours), interior j:       gpsi(i)
 //   J(j)[r][a] = (psi_inv(j + e_a) - psi_inv(j - e_a))[r] / 2:      gpinv(i)
 //   rho1(j) = rho(j) + alpha div(rho, v)(j) (upwind fluxes):        grho(i) = grho1(i) + ..., gv(i)
-__global__ void b_step8(Dims d, const double *gL, const double *gJ, const double *rho, const double *v,
-                        const double *grho1, double *gpsi, double *gpinv, double *grho, double *gv, double alpha) {
+__global__ void b_step8(Dims d, const double *__restrict__ gL, const double *__restrict__ gJ,
+                        const double *__restrict__ rho, const double *__restrict__ v,
+                        const double *__restrict__ grho1, double *__restrict__ gpsi, double *__restrict__ gpinv,
+                        double *__restrict__ grho, double *__restrict__ gv, double alpha) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
         const int sy = d.X, sz = d.X * d.Y;
