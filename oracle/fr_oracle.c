@@ -26,6 +26,8 @@
  *   src/fastreg/cuda/reductor.cu:426-539, src/fastreg/reductor.cpp:107-120   k_reduce_max
  *   src/fastreg/cuda/reductor.cu:11-112,224-324, src/fastreg/reductor.cpp:45-62  k_energy
  *   src/fastreg/fastreg.cpp:3-22,58-75          or_init / or_outputs
+ *   (not in the reference)                      k_min_det: min over the interior of the true det and
+ *                                               of det_ref of the central-difference D psi
  *
  * How each function is pinned (tests/test_oracle_kat.py; the reference ships no tests, golden
  * vectors or fixtures, and cannot be compiled here: it is CUDA with Boost/OpenCV/ITK):
@@ -47,6 +49,16 @@
  *   solver             iteration-1 closed form v1 = -alpha*m2/(rho0+1e-5)*(S-T)*grad S,
  *                      psi1 = id + alpha v1, psi_inv1 = id - alpha v1; S = T is a fixed point;
  *                      FP32 vs FP64 runs agree to rounding; t schedule with RESET_ITER
+ *   k_update_density   one donor-cell step of a single moving voxel along each axis, both signs:
+ *                      exactly alpha u rho0 moves between p and p+e in the upwind direction, mass
+ *                      conserved; the FP32 result is alpha*div + rho rounded once (a velocity for
+ *                      which rounding the product first gives another float)
+ *   k_energy           S = T at psi = id -> 0; ramp S = T = x/8 with psi = id + (1,0,0) ->
+ *                      e_data = 0.5 (X-1) Y Z / 64; psi = id + A x -> e_reg = 0.5 sum_a |A col a|^2
+ *                      (dim_a - 2) N / dim_a (mirrored faces); random images at psi = id vs numpy;
+ *                      the logged energy of iteration 1
+ *   k_min_det          psi = A x -> det(A) exactly (and det_ref(A) != det(A)); psi_x = x + b x^2 ->
+ *                      1 + 2b(X-2), the minimum over the interior only
  *   k_dice_counts      identical labels -> 1; last slice per axis ignored (brute force)
  *   k_reduce_max       brute force over norms with first-index ties
  */

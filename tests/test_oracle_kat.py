@@ -640,3 +640,208 @@ def test_voxel_max_energy_brute_force(oracle_mod):
     # equal energies everywhere: the lowest voxel
     e, arg = O.voxel_max_energy(dims, np.zeros((n, 2)), np.zeros((n, 2)), np.zeros((n, 4, 4)), w, "f64")
     assert (e, arg) == (0.0, 0)
+
+
+# ---------------------------------------------------------------------------------------------
+# energy, min det J and the density update, pinned by closed forms
+# ---------------------------------------------------------------------------------------------
+
+def _const_pair(dims, value=0.5):
+    n = dims[0] * dims[1] * dims[2]
+    img = np.full(n, value, np.float32)
+    lab = np.ones(n, np.int16)
+    return img, lab
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_energy_data_term_closed_forms(oracle_mod, prec):
+    """e_data = 0.5 sum_p (T(p) - S o psi(p))^2 over all voxels (reductor.cpp:45-62): 0 for S = T
+    at psi = id; for the ramp S = T = x/8 and psi = id + (1, 0, 0), S o psi(p) = S(min(x+1, X-1)),
+    so every voxel off the last x-slice differs by -1/8 and e_data = 0.5 (X-1) Y Z / 64."""
+    O = oracle_mod
+    dims = (9, 7, 6)
+    X, Y, Z = dims
+    x, y, z = grid(dims)
+    ramp = (x / 8).astype(np.float32)
+    lab = np.ones(x.size, np.int16)
+    o = O.Oracle(dims, prec)
+    o.init(ramp, lab, ramp, lab)
+    assert o.energy() == (0.0, 0.0)
+    psi = identity4(dims, np.float64)
+    psi[:, 0] += 1.0
+    o.set_state(psi=psi)
+    e_data, e_reg = o.energy()
+    assert e_data == 0.5 * (X - 1) * Y * Z / 64.0
+    assert e_reg == 0.0      # psi - id is constant: D(psi - id) = 0, faces included
+    # random images at psi = id: S o id = S exactly, so e_data is numpy's 0.5 sum (T - S)^2
+    rng = np.random.default_rng(11)
+    T = rng.random(x.size).astype(np.float32)
+    S = rng.random(x.size).astype(np.float32)
+    o.init(T, lab, S, lab)
+    e_data, e_reg = o.energy()
+    np.testing.assert_allclose(e_data, 0.5 * np.sum((T.astype(np.float64) - S) ** 2), rtol=1e-13)
+    assert e_reg == 0.0
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_energy_reg_term_affine_closed_form(oracle_mod, prec):
+    """e_reg = 0.5 sum_p ||D(psi - id)(p)||_F^2 with mirrored neighbours at the faces
+    (vector_fields.cu:537-585): for psi = id + A x the central difference along axis a is column a
+    of A, except on the two faces of that axis, where the mirrored neighbour makes it 0.  So
+    e_reg = 0.5 sum_a |A[:, a]|^2 (dim_a - 2) (N / dim_a).  A is not symmetric and its entries are
+    dyadic, so the sum is exact in double.  A constant image keeps e_data at 0."""
+    O = oracle_mod
+    dims = (8, 7, 9)
+    x, y, z = grid(dims)
+    A = np.array([[0.25, -0.125, 0.5], [0.0, 0.375, -0.25], [0.125, 0.0, -0.5]])
+    img, lab = _const_pair(dims)
+    o = O.Oracle(dims, prec)
+    o.init(img, lab, img, lab)
+    psi = identity4(dims, np.float64)
+    psi[:, :3] += (A @ np.stack([x, y, z])).T
+    o.set_state(psi=psi)
+    e_data, e_reg = o.energy()
+    n = x.size
+    expect = 0.5 * sum((A[:, a] ** 2).sum() * (dims[a] - 2) * (n // dims[a]) for a in range(3))
+    assert e_data == 0.0
+    assert e_reg == expect
+
+
+def test_energy_in_the_run_log_is_data_plus_weighted_reg(oracle_mod):
+    """The logged energy of iteration 1 (psi = id) is e_data + W_REG * e_reg with e_reg = 0, i.e.
+    0.5 sum (T - S)^2 (solver.cu:357-363)."""
+    O = oracle_mod
+    dims = (10, 9, 8)
+    rng = np.random.default_rng(12)
+    n = dims[0] * dims[1] * dims[2]
+    T = rng.random(n).astype(np.float32)
+    S = rng.random(n).astype(np.float32)
+    lab = np.ones(n, np.int16)
+    o = O.Oracle(dims, "f64")
+    o.init(T, lab, S, lab)
+    r = o.run(O.SolverParams(max_iter=1, verbosity=2, no_labels=2))
+    np.testing.assert_allclose(r["energy_hist"][0], 0.5 * np.sum((T.astype(np.float64) - S) ** 2), rtol=1e-13)
+
+
+def _det3(A):
+    from fractions import Fraction as F
+    a = [[F(v) for v in row] for row in A]
+    return (a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0])
+            + a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]))
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_min_det_of_an_affine_map_is_det_of_its_matrix(oracle_mod, prec):
+    """For psi = A x the central-difference Jacobian is A at every interior voxel, so min det J
+    (the true determinant) is det(A) exactly (dyadic entries), while the reference's det() formula
+    gives a different number for this non-symmetric A (its misparenthesised second-order term)."""
+    O = oracle_mod
+    dims = (7, 6, 8)
+    x, y, z = grid(dims)
+    A = np.array([[1.25, 0.5, -0.25], [0.25, 0.75, 0.5], [0.5, -0.25, 1.5]])
+    img, lab = _const_pair(dims)
+    o = O.Oracle(dims, prec)
+    o.init(img, lab, img, lab)
+    psi = np.zeros((x.size, 4))
+    psi[:, :3] = (A @ np.stack([x, y, z])).T
+    o.set_state(psi=psi)
+    mt, mr = o.min_det()
+    exact = float(_det3(A))
+    assert mt == exact
+    ref_formula = (A[0, 0] * A[1, 1] * A[2, 2] + A[1, 0] * A[2, 1] * A[0, 2] + A[2, 0] * A[0, 1] * A[1, 2]
+                   - ((A[0, 2] * A[1, 1] + A[2, 0]) * (A[1, 2] * A[2, 1] * A[0, 0]) + A[2, 2] * A[0, 1] * A[1, 0]))
+    assert mr == ref_formula and mr != mt
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_min_det_is_the_minimum_over_the_interior(oracle_mod, prec):
+    """psi_x = x + b x^2 (b < 0), psi_y = y, psi_z = z: the central difference of the quadratic is
+    exact, det J(x) = 1 + 2 b x, and its minimum over the interior x in [1, X-2] is 1 + 2 b (X-2).
+    A fold placed on the x = X-1 face (outside the interior) does not count."""
+    O = oracle_mod
+    dims = (11, 6, 5)
+    X = dims[0]
+    x, y, z = grid(dims)
+    b = -1.0 / 32
+    img, lab = _const_pair(dims)
+    o = O.Oracle(dims, prec)
+    o.init(img, lab, img, lab)
+    psi = identity4(dims, np.float64)
+    psi[:, 0] = x + b * x * x
+    o.set_state(psi=psi)
+    mt, mr = o.min_det()
+    assert mt == 1 + 2 * b * (X - 2)
+    assert mr == mt           # diagonal Jacobian: det() and the determinant agree
+    # Fold psi_y over on the x = X-1 face (d psi_y / dy = -3 there): a determinant taken on that
+    # face would be negative.  The interior voxels at x = X-2 see the change only in d psi_y / dx,
+    # a below-diagonal entry of their (otherwise diagonal) Jacobian, which leaves det unchanged.
+    psi2 = psi.copy()
+    face = x == X - 1
+    psi2[face, 1] = y[face] * -3.0
+    o.set_state(psi=psi2)
+    mt2, _ = o.min_det()
+    assert mt2 == mt
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_density_update_is_a_donor_cell_step(oracle_mod, prec, axis):
+    """One iteration from a uniform density rho0 with a single moving voxel p of velocity u along
+    `axis` (solver.cu:101-173).  v(p) is the velocity through the face between p and p+e, with the
+    upwind (donor) density: for u > 0 mass alpha u rho0 leaves p for p+e, for u < 0 it comes back
+    from p+e to p.  Every other voxel keeps rho0, and the total mass is conserved."""
+    O = oracle_mod
+    dims = (9, 8, 7)
+    X, Y, Z = dims
+    n = X * Y * Z
+    e = [1, X, X * Y][axis]
+    p = 4 + X * (4 + Y * 3)
+    img, lab = _const_pair(dims)
+    alpha, rho0 = 0.25, 1.0
+    for u in (2.0, -2.0):
+        o = O.Oracle(dims, prec)
+        o.init(img, lab, img, lab, rho0)
+        v = np.zeros((n, 4))
+        v[p, axis] = u
+        o.set_state(v=v)
+        o.step(alpha, alpha=alpha)
+        rho = o.get_state()["rho"].astype(np.float64)
+        moved = alpha * abs(u) * rho0
+        exp = np.full(n, rho0)
+        exp[p] = rho0 - moved if u > 0 else rho0 + moved
+        exp[p + e] = rho0 + moved if u > 0 else rho0 - moved
+        assert np.array_equal(rho, exp), (u, np.nonzero(rho != exp))
+        assert rho.sum() == n * rho0
+        o.close()
+
+
+def test_density_update_rounds_once(oracle_mod):
+    """rho += alpha * div is one fused multiply-add (nvcc contracts the statement, solver.cu:171):
+    the FP32 result is alpha*div + rho rounded once.  The velocity below is chosen so that rounding
+    the product first would give a different float."""
+    from fractions import Fraction as F
+    O = oracle_mod
+    dims = (9, 8, 7)
+    X, Y, Z = dims
+    n = X * Y * Z
+    p = 4 + X * (4 + Y * 3)
+    alpha = np.float32(0.004)
+    rho0 = np.float32(1.0)
+    rng = np.random.default_rng(13)
+    for _ in range(1000):           # a u whose single and double roundings differ
+        u = np.float32(rng.uniform(0.5, 3.0))
+        div = np.float32(-u * rho0)
+        once = np.float32(F(float(alpha)) * F(float(div)) + F(float(rho0)))
+        twice = np.float32(np.float32(alpha * div) + rho0)
+        if once != twice:
+            break
+    else:
+        pytest.fail("no distinguishing velocity found")
+    img, lab = _const_pair(dims)
+    o = O.Oracle(dims, "f32")
+    o.init(img, lab, img, lab, float(rho0))
+    v = np.zeros((n, 4), np.float32)
+    v[p, 0] = u
+    o.set_state(v=v)
+    o.step(float(alpha), alpha=float(alpha))
+    assert o.get_state()["rho"][p] == once
