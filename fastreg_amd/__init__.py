@@ -110,6 +110,7 @@ def lib():
     sig("fr_solver_get_state", vp, vp, vp, vp, vp)
     sig("fr_solver_energy", vp, P(d), P(d))
     sig("fr_warp", i, i, i, vp, vp, vp, vp, vp, vp)
+    sig("fr_field_identity", i, i, i, vp, vp)
     sig("fr_time_kernels", vp, i, P(f), P(f))
     sig("fr_get_inverse", i, i, i, vp, vp, vp)
     sig("fr_interpolate_gradient", i, i, i, vp, vp, vp, vp)
@@ -122,6 +123,8 @@ def lib():
     sig("fr_adjoint_backward", vp, vp, P(d), P(d))
     sig("fr_adjoint_get_state", vp, i, vp, vp, vp, vp)
     sig("fr_adjoint_info", vp, P(sz), P(f), P(f))
+    sig("fr_adjoint_set_gather_chunk", vp, i)
+    sig("fr_adjoint_get_gather_chunk", vp, P(i), P(i), P(i))
     sig("fr_step_fwd", vp, i, d, d, vp, vp, vp, vp, vp, vp, vp, vp)
     sig("fr_step_bwd", vp, i, d, d, vp, vp, vp, vp, vp, vp, vp, vp, P(d), P(d))
     sig("fr_warp_fwd", i, i, i, vp, vp, vp)
@@ -547,6 +550,16 @@ class Adjoint:
                                 C.byref(gw), C.byref(gt)))
         self.forward_calls += 1  # the handle's last forward pass is no longer differentiable
         return dict(zip(("rho", "v", "psi", "psi_inv"), (b.get() for b in gs))), gw.value, gt.value
+
+    def set_gather_chunk(self, zc):
+        """z-planes per block of the backward pass's record gathers (0 = automatic)."""
+        check(lib().fr_adjoint_set_gather_chunk(self.h, int(zc)))
+
+    def gather_chunk(self):
+        """(tile width, tile height, z-planes per block) of the backward pass's record gathers."""
+        tx, ty, zc = C.c_int(), C.c_int(), C.c_int()
+        check(lib().fr_adjoint_get_gather_chunk(self.h, C.byref(tx), C.byref(ty), C.byref(zc)))
+        return tx.value, ty.value, zc.value
 
     def info(self):
         """(device bytes held, device ms of the last forward, of the last backward)."""

@@ -3,7 +3,9 @@
 ``python -m fastreg_amd.build`` (or ``build()``) compiles, in-tree:
 
 * ``fastreg_amd/libfastreg_hip.so`` — kernels + C-ABI (include/fastreg_c.h), hipcc for gfx950;
-* ``fastreg_amd/fastreg_app`` — the params.ini-driven CLI, g++ against the C-ABI library.
+* ``fastreg_amd/fastreg_app`` — the params.ini-driven CLI, g++ against the C-ABI library;
+* ``tests/cpp/reference_caller`` — a test program written against the reference's C++ classes
+  (include/fastreg.hpp, include/solver.hpp), g++ against the same library.
 
 HIP flags: -ffp-contract=off (multiply-adds round twice unless written fmaf, matching the
 oracle's operation order), correctly rounded f32 division/sqrt, denormals kept, and packed-FP32
@@ -25,6 +27,8 @@ APP = os.path.join(PKG, "fastreg_app")
 
 HIP_SOURCES = ["fr_kernels.hip", "fr_capi.hip", "fr_adjoint.hip"]
 APP_SOURCES = ["app.cpp", "nifti.cpp"]
+CALLER_SRC = os.path.join(ROOT, "tests", "cpp", "reference_caller.cpp")
+CALLER = os.path.join(ROOT, "tests", "cpp", "reference_caller")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -71,9 +75,21 @@ def build_app() -> str | None:
     return APP
 
 
+def build_caller() -> str | None:
+    if not os.path.exists(CALLER_SRC):
+        return None
+    cxx = shutil.which("g++") or "g++"
+    tmp = CALLER + ".tmp"
+    _run([cxx, "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, CALLER_SRC, "-o", tmp, "-L", PKG, "-lfastreg_hip",
+          "-Wl,-rpath," + os.path.relpath(PKG, os.path.dirname(CALLER)).join(["$ORIGIN/", ""])])
+    os.replace(tmp, CALLER)
+    return CALLER
+
+
 def build(verbose: bool = False) -> None:
     build_lib(verbose)
     build_app()
+    build_caller()
 
 
 if __name__ == "__main__":

@@ -121,8 +121,8 @@ int main(int argc, char *argv[]) {
 
     try {
         FastReg fastreg(params);
-        fastreg::hip::DeviceBuffer<float> d_img1(n), d_img2(n);
-        fastreg::hip::DeviceBuffer<int16_t> d_lab1(n), d_lab2(n);
+        fastreg::cuda::DeviceBuffer<float> d_img1(n), d_img2(n);
+        fastreg::cuda::DeviceBuffer<int16_t> d_lab1(n), d_lab2(n);
         d_img1.upload(img1.data.data());
         d_img2.upload(img2.data.data());
         d_lab1.upload(hl1.data());

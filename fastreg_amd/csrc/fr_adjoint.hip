@@ -625,7 +625,8 @@ __global__ void b_step1(Dims d, const double *__restrict__ pinv, const double *_
 // Step 3: gv1 completed by the records' gather, then the adjoint of F7
 //   v1 = v - alpha (m1t v + dvv - tau dlv + q gU), q = m2 / (|rho1| + 1e-5f)   (interior; v1 = v elsewhere):
 //   gv = (1 - alpha m1t) gv1, gdvv = -alpha gv1, gdlv = alpha tau gv1, ggU = -alpha q gv1,
-//   grho1 += -alpha (gv1 . gU) dq/drho1, dE/dtau += alpha gv1 . dlv (block sums, one atomic per block).
+//   grho1 += -alpha (gv1 . gU) dq/drho1, dE/dtau += alpha gv1 . dlv (one partial sum per block into
+//   gtau[blockIdx.x], added up in block order by k_sum_accumulate).
 // gv holds gv1 on entry and gv on exit.
 __global__ __launch_bounds__(GT_THREADS) void b_step3(Dims d, int tiles_x, int tiles_y, int zc, const int *code,
                                                       const double *abc, const double *gw, const double *rho1,
@@ -656,7 +657,7 @@ __global__ __launch_bounds__(GT_THREADS) void b_step3(Dims d, int tiles_x, int t
         }
     });
     const double s = block_sum(acc, sh);
-    if (threadIdx.x == 0) aadd(gtau, s);
+    if (threadIdx.x == 0) gtau[blockIdx.x] = s;
 }
 
 // voxel (x, y, z) in the range of the velocity Laplacian (x in [2, X-3], y in [2, Y-2], z in [2, Z-3])
@@ -753,7 +754,8 @@ __global__ void b_step4(Dims d, const double *__restrict__ v, const double *__re
 
 // Step 5 (own voxel, interior): the adjoint of F5, gU = ((S - Tp) gS + w Lo) det_ref(J):
 //   gLo = w det ggU, gTp = -det (ggU . gS), gJ += (ggU . ((S - Tp) gS + w Lo)) d det_ref / dJ,
-//   dE/dw_reg += det (ggU . Lo) (block sums, one atomic per block);
+//   dE/dw_reg += det (ggU . Lo) (one partial sum per block into gwsum[blockIdx.x], added up in block
+//   order by k_sum_accumulate);
 // then of F4 at this voxel's position psi_inv: gpinv += (d trilerp(L) / d pos)^T gLo +
 // (d trilerp(T) / d pos)^T gTp, and the record (cell of psi_inv, gLo) for the corners of L.
 // gL is cleared here (the irregular scatter and the gather fill it).
@@ -815,7 +817,7 @@ __global__ __launch_bounds__(256) void b_step5(Dims d, const double *__restrict_
         put_rec(code, abc, gw, i, rec_code(k, x, y, z), k, glo);
     }
     const double sm = block_sum(acc, sh);
-    if (threadIdx.x == 0) aadd(gwsum, sm);
+    if (threadIdx.x == 0) gwsum[blockIdx.x] = sm;
 }
 
 // Step 7: gL += the records' gather (the transpose of Lo = trilerp(L, psi_inv))
@@ -950,6 +952,20 @@ __global__ __launch_bounds__(256) void k_sum_final(const double *partials, int n
         __syncthreads();
     }
     if (threadIdx.x == 0) *out = sh[0];
+}
+
+// *out += the sum of np partials, in a fixed order (the same bits on every run, unlike atomics)
+__global__ __launch_bounds__(256) void k_sum_accumulate(const double *partials, int np, double *out) {
+    __shared__ double sh[256];
+    double s = 0;
+    for (int j = threadIdx.x; j < np; j += 256) s += partials[j];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out += sh[0];
 }
 
 // ---- single operators and their adjoints (the per-operator forward/backward pairs) ------------
@@ -1096,9 +1112,11 @@ struct fr_adjoint {
     double *abc = nullptr, *gw = nullptr;
     double *scal = nullptr;  // [0] objective, [1] dE/dw_reg, [2] dE/dtau
     double *partials = nullptr;
+    double *part_tau = nullptr, *part_w = nullptr;  // per-block partials of dE/dtau, dE/dw_reg of one iteration
     int grid = 0;
-    // launch shape of the record gathers: 64 x 4 tiles, zc z-planes per block
-    int gt_tiles_x = 0, gt_tiles_y = 0, gt_zc = 0, gt_blocks = 0;
+    // launch shape of the record gathers: 64 x 4 tiles, zc z-planes per block (gt_zc_auto: the
+    // choice made at creation, about two blocks per CU)
+    int gt_tiles_x = 0, gt_tiles_y = 0, gt_zc = 0, gt_blocks = 0, gt_zc_auto = 0;
     // per-iteration scalars of the last forward (index k-1 for iteration k)
     std::vector<double> m1t, m2;
     double w_reg = 0, tau = 0, lw = 0, energy = 0;
@@ -1204,10 +1222,12 @@ int backward_iteration(fr_adjoint *h, const State &in, const State &nx, const In
     frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.psi, h->gw, h->gv);
     frd::b_step3<<<h->gt_blocks, frd::GT_THREADS, 0, s>>>(d, h->gt_tiles_x, h->gt_tiles_y, h->gt_zc, h->code, h->abc,
                                                         h->gw, nx.rho, it.dlv, it.gU, h->gv, h->gdvv, h->gdlv,
-                                                        h->ggU, gr1, a, m1t, h->tau, m2, h->scal + 2);
+                                                        h->ggU, gr1, a, m1t, h->tau, m2, h->part_tau);
+    frd::k_sum_accumulate<<<1, 256, 0, s>>>(h->part_tau, h->gt_blocks, h->scal + 2);
     frd::b_step4<<<g, TPB, 0, s>>>(d, in.v, h->gdvv, h->gdlv, h->gv);
     frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.L, in.pinv, h->ggU, h->gJ, h->gpinv,
-                                   h->gL, h->code, h->abc, h->gw, h->w_reg, h->scal + 1);
+                                   h->gL, h->code, h->abc, h->gw, h->w_reg, h->part_w);
+    frd::k_sum_accumulate<<<1, 256, 0, s>>>(h->part_w, g, h->scal + 1);
     frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.pinv, h->gw, h->gL);
     frd::b_step7<<<h->gt_blocks, frd::GT_THREADS, 0, s>>>(d, h->gt_tiles_x, h->gt_tiles_y, h->gt_zc, h->code, h->abc,
                                                         h->gw, h->gL);
@@ -1270,7 +1290,7 @@ extern "C" int fr_adjoint_create(const fr_params *p, int K, int ckpt, fr_adjoint
         const long tiles = (long)h->gt_tiles_x * h->gt_tiles_y;
         long chunks = 2L * n_cu / tiles;
         chunks = chunks < 1 ? 1 : chunks > h->d.Z ? h->d.Z : chunks;
-        h->gt_zc = (int)((h->d.Z + chunks - 1) / chunks);
+        h->gt_zc = h->gt_zc_auto = (int)((h->d.Z + chunks - 1) / chunks);
         h->gt_blocks = (int)(tiles * ((h->d.Z + h->gt_zc - 1) / h->gt_zc));
     }
     const size_t n = (size_t)h->n;
@@ -1300,7 +1320,8 @@ extern "C" int fr_adjoint_create(const fr_params *p, int K, int ckpt, fr_adjoint
         (rc = h->alloc(&h->gL, 3 * n)) || (rc = h->alloc(&h->code, n)) || (rc = h->alloc(&h->abc, 3 * n)) ||
         (rc = h->alloc(&h->gw, 3 * n)) ||
         (rc = h->alloc(&h->ggU, 3 * n)) || (rc = h->alloc(&h->gdvv, 3 * n)) || (rc = h->alloc(&h->gdlv, 3 * n)) ||
-        (rc = h->alloc(&h->scal, 4)) || (rc = h->alloc(&h->partials, (size_t)h->grid)))
+        (rc = h->alloc(&h->scal, 4)) || (rc = h->alloc(&h->partials, (size_t)h->grid)) ||
+        (rc = h->alloc(&h->part_tau, (size_t)h->gt_tiles_x * h->gt_tiles_y * h->d.Z)) || (rc = h->alloc(&h->part_w, (size_t)h->grid)))
         return fail(rc);
     *out = h;
     return FR_OK;
@@ -1423,6 +1444,23 @@ extern "C" int fr_adjoint_info(fr_adjoint *h, size_t *device_bytes, float *forwa
     return FR_OK;
 }
 
+extern "C" int fr_adjoint_set_gather_chunk(fr_adjoint *h, int zc) {
+    if (!h) return err(FR_ERR_INVALID, "null handle");
+    if (zc < 0) return err(FR_ERR_INVALID, "z-planes per gather block must be >= 0 (0 = automatic), got %d", zc);
+    ADJ_TRY(hipStreamSynchronize(h->stream));
+    h->gt_zc = zc == 0 ? h->gt_zc_auto : (zc > h->d.Z ? h->d.Z : zc);
+    h->gt_blocks = h->gt_tiles_x * h->gt_tiles_y * ((h->d.Z + h->gt_zc - 1) / h->gt_zc);
+    return FR_OK;
+}
+
+extern "C" int fr_adjoint_get_gather_chunk(const fr_adjoint *h, int *tile_x, int *tile_y, int *zc) {
+    if (!h) return err(FR_ERR_INVALID, "null handle");
+    if (tile_x) *tile_x = frd::GT_X;
+    if (tile_y) *tile_y = frd::GT_Y;
+    if (zc) *zc = h->gt_zc;
+    return FR_OK;
+}
+
 // ---- per-operator forward/backward pairs ---------------------------------------------------------
 
 namespace {
@@ -1508,6 +1546,20 @@ int step_check(fr_adjoint *h, int k, double w_reg, double tau) {
     return FR_OK;
 }
 
+// Whether arrays a and b of wa and wb doubles per voxel (rho: 1; v, psi, psi_inv: 3) share memory.
+bool overlap(const void *a, int wa, const void *b, int wb, long n) {
+    const char *pa = static_cast<const char *>(a), *pb = static_cast<const char *>(b);
+    return pa < pb + (size_t)wb * n * sizeof(double) && pb < pa + (size_t)wa * n * sizeof(double);
+}
+
+// whether any two of the m arrays share memory
+bool any_overlap(const void *const *ptrs, const int *widths, int m, long n) {
+    for (int a = 0; a < m; ++a)
+        for (int b = a + 1; b < m; ++b)
+            if (overlap(ptrs[a], widths[a], ptrs[b], widths[b], n)) return true;
+    return false;
+}
+
 }  // namespace
 
 extern "C" int fr_step_fwd(fr_adjoint *h, int k, double w_reg, double tau, const double *d_rho, const double *d_v,
@@ -1516,6 +1568,16 @@ extern "C" int fr_step_fwd(fr_adjoint *h, int k, double w_reg, double tau, const
     if (int rc = step_check(h, k, w_reg, tau)) return rc;
     if (!d_rho || !d_v || !d_psi || !d_psi_inv || !d_rho1 || !d_v1 || !d_psi1 || !d_psi_inv1)
         return err(FR_ERR_INVALID, "null device pointer");
+    {
+        // the kernels read neighbourhoods of the inputs while writing the outputs
+        const void *p[8] = {d_rho, d_v, d_psi, d_psi_inv, d_rho1, d_v1, d_psi1, d_psi_inv1};
+        const int w[8] = {1, 3, 3, 3, 1, 3, 3, 3};
+        for (int o = 4; o < 8; ++o)
+            for (int a = 0; a < 4; ++a)
+                if (overlap(p[o], w[o], p[a], w[a], h->n))
+                    return err(FR_ERR_INVALID, "fr_step_fwd: output array %d overlaps input array %d", o - 4, a);
+        if (any_overlap(p + 4, w + 4, 4, h->n)) return err(FR_ERR_INVALID, "fr_step_fwd: the output arrays overlap");
+    }
     h->w_reg = w_reg;
     h->tau = tau;
     h->have_forward = false;  // the work buffers are reused
@@ -1535,6 +1597,12 @@ extern "C" int fr_step_bwd(fr_adjoint *h, int k, double w_reg, double tau, const
     if (int rc = step_check(h, k, w_reg, tau)) return rc;
     if (!d_rho || !d_v || !d_psi || !d_psi_inv || !d_g_rho || !d_g_v || !d_g_psi || !d_g_psi_inv)
         return err(FR_ERR_INVALID, "null device pointer");
+    {
+        // the adjoint arrays are read in, then written back one after another
+        const void *p[4] = {d_g_rho, d_g_v, d_g_psi, d_g_psi_inv};
+        const int w[4] = {1, 3, 3, 3};
+        if (any_overlap(p, w, 4, h->n)) return err(FR_ERR_INVALID, "fr_step_bwd: the adjoint arrays overlap");
+    }
     h->w_reg = w_reg;
     h->tau = tau;
     h->have_forward = false;  // the work buffers are reused

@@ -66,6 +66,7 @@ struct fr_ctx {
     float *pinv[2] = {nullptr, nullptr};
     float *psi = nullptr;
     int st = frk::ST_F32;                  // storage of v and rho
+    unsigned storage_gen = 0;              // bumped by every fr_set_storage (batches check it)
     unsigned *vr[2] = {nullptr, nullptr};  // 16-bit (v, rho) records, 2 unsigned per voxel
     int cur = 0;
     int zc[3] = {0, 8, 2};       // interior z-slices per block of pass A ([1]) and pass B ([2])
@@ -314,6 +315,7 @@ extern "C" int fr_set_storage(fr_ctx *c, int storage) {
         if (!c->vr[1] && (rc = dalloc(&c->vr[1], 2 * (size_t)c->n))) return rc;
     }
     c->st = storage == FR_STORAGE_BF16 ? frk::ST_BF16 : storage == FR_STORAGE_F16 ? frk::ST_F16 : frk::ST_F32;
+    ++c->storage_gen;  // a batch laid out before this call holds the old buffers and kernels
     if (int rc = auto_chunking(c)) return rc;  // the automatic chunking depends on the storage
     c->initialised = false;  // the state must be initialised in the new storage
     c->have_run = c->have_final = false;
@@ -750,12 +752,23 @@ extern "C" int fr_warp(int X, int Y, int Z, const float *img, const int16_t *lab
     return FR_OK;
 }
 
+extern "C" int fr_field_identity(int X, int Y, int Z, float *field4, void *stream) {
+    if (X < 1 || Y < 1 || Z < 1 || !field4) return fail(FR_ERR_INVALID, "bad fr_field_identity arguments");
+    if (16LL * X * Y * Z >= (1LL << 31)) return fail(FR_ERR_INVALID, "volume too large (16*N must be < 2^31)");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(frk::launch_identity4(Dims{X, Y, Z}, field4, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return FR_OK;
+}
+
 // ---------------------------------------------------------------------------------------------
 // batches: many volumes advanced together, one launch per pass per iteration, run as a hipGraph
 // ---------------------------------------------------------------------------------------------
 
 struct fr_batch {
     std::vector<fr_ctx *> ctx;
+    std::vector<unsigned> storage_gen;  // each context's storage generation when the layout was made
+    int st = frk::ST_F32;               // the storage of every volume at that time
     frk::IterArgs *d_base = nullptr;  // 3 per volume: (unused), pass A, pass B arguments at buffer parity 0
     int *d_starts = nullptr;          // 3 x (n + 1): first block of each volume in each pass's grid
     int total[3] = {0, 0, 0};
@@ -779,9 +792,30 @@ static void free_batch(fr_batch *b) {
     delete b;
 }
 
+// every volume must have one storage (the batched kernels are instantiated per storage)
+static int batch_same_storage(fr_ctx *const *ctxs, int n) {
+    for (int q = 1; q < n; ++q)
+        if (ctxs[q]->st != ctxs[0]->st)
+            return fail(FR_ERR_INVALID, "context %d: the storage of v and rho must be the same for every volume of a "
+                                        "batch", q);
+    return FR_OK;
+}
+
+// A run uses the buffers and kernels of the storage each volume had when the layout was made: a
+// later fr_set_storage on a volume (new buffers, other kernels) makes the batch unusable until
+// fr_batch_relayout.
+static int batch_check_layout(const fr_batch *b) {
+    for (size_t q = 0; q < b->ctx.size(); ++q)
+        if (b->ctx[q]->storage_gen != b->storage_gen[q] || b->ctx[q]->st != b->st)
+            return fail(FR_ERR_STATE, "batch volume %d: its storage was set after the batch was laid out; call "
+                                      "fr_batch_relayout", (int)q);
+    return FR_OK;
+}
+
 // the launch-time arguments of every volume and the block ranges of each pass
 static int batch_layout(fr_batch *b) {
     const int n = (int)b->ctx.size();
+    if (int rc = batch_same_storage(b->ctx.data(), n)) return rc;
     std::vector<frk::IterArgs> base(3 * (size_t)n);
     std::vector<int> starts(3 * (size_t)(n + 1));
     starts[(size_t)n] = 0;  // slot 0 (no pass) is empty
@@ -806,6 +840,9 @@ static int batch_layout(fr_batch *b) {
     }
     HIP_TRY(hipMemcpy(b->d_base, base.data(), sizeof(frk::IterArgs) * base.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->d_starts, starts.data(), sizeof(int) * starts.size(), hipMemcpyHostToDevice));
+    b->st = b->ctx[0]->st;
+    b->storage_gen.resize((size_t)n);
+    for (int q = 0; q < n; ++q) b->storage_gen[(size_t)q] = b->ctx[q]->storage_gen;
     for (int k = 0; k < 2; ++k) {  // captured runs refer to the old layout
         if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
         if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
@@ -824,10 +861,11 @@ extern "C" int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
             if (ctxs[r] == ctxs[q]) return fail(FR_ERR_INVALID, "context %d appears twice", q);
         const fr_params &p = ctxs[q]->prm, &p0 = ctxs[0]->prm;
         if (p.max_iter != p0.max_iter || p.alpha != p0.alpha || p.reset_iter != p0.reset_iter || p.p != p0.p ||
-            p.C != p0.C || p.verbosity != p0.verbosity || ctxs[q]->st != ctxs[0]->st)
-            return fail(FR_ERR_INVALID, "context %d: MAX_ITER, ALPHA, RESET_ITER, p, C, verbosity and the storage of v "
-                                        "and rho must be the same for every volume of a batch", q);
+            p.C != p0.C || p.verbosity != p0.verbosity)
+            return fail(FR_ERR_INVALID, "context %d: MAX_ITER, ALPHA, RESET_ITER, p, C and verbosity must be the same for "
+                                        "every volume of a batch", q);
     }
+    if (int rc = batch_same_storage(ctxs, n)) return rc;
     fr_batch *b = new fr_batch();
     b->ctx.assign(ctxs, ctxs + n);
     int rc;
@@ -889,7 +927,7 @@ static int batch_launches(fr_batch *b, int par) {
         }
         for (int pass = 1; pass < 3; ++pass)
             HIP_TRY(frk::launch_pass_batch(pass, b->d_base, b->d_starts + (size_t)pass * (n + 1), n, b->total[pass], it,
-                                           b->ctx[0]->st, b->stream));
+                                           b->st, b->stream));
         t += p.alpha;
         if (p.reset_iter > 0 && (iter + 1) % p.reset_iter == 0) t = p.alpha;
     }
@@ -900,6 +938,7 @@ extern "C" int fr_batch_run(fr_batch *b) {
     if (!b) return fail(FR_ERR_INVALID, "null batch");
     const int n = (int)b->ctx.size();
     int rc;
+    if ((rc = batch_check_layout(b))) return rc;
     const int par = b->ctx[0]->cur;
     for (int q = 0; q < n; ++q) {
         fr_ctx *c = b->ctx[q];
@@ -950,6 +989,7 @@ extern "C" int fr_batch_time_iterations(fr_batch *b, int iters, float *ms) {
     const int mi = b->ctx[0]->prm.max_iter;
     if (iters > mi) return fail(FR_ERR_INVALID, "iters must be <= MAX_ITER");
     int rc;
+    if ((rc = batch_check_layout(b))) return rc;
     for (int q = 0; q < n; ++q) {
         fr_ctx *c = b->ctx[q];
         if (!c->initialised) return fail(FR_ERR_STATE, "batch volume %d not initialised", q);
@@ -972,7 +1012,7 @@ extern "C" int fr_batch_time_iterations(fr_batch *b, int iters, float *ms) {
         vel_scalars(p, t, &it.m1t, &it.m2);
         for (int pass = 1; pass < 3; ++pass)
             HIP_TRY(frk::launch_pass_batch(pass, b->d_base, b->d_starts + (size_t)pass * (n + 1), n, b->total[pass], it,
-                                           b->ctx[0]->st, b->stream));
+                                           b->st, b->stream));
         t += p.alpha;
     }
     HIP_TRY(hipEventRecord(b->ev1, b->stream));

@@ -122,5 +122,7 @@ hipError_t launch_vr_init(int st, unsigned *r0, unsigned *r1, int n, float rho_0
 hipError_t launch_vr_set(int st, unsigned *r0, unsigned *r1, const float *v4, const float *rho, int n, hipStream_t s);
 hipError_t launch_vr_get(int st, const unsigned *r, float *v4, float *rho, int n, hipStream_t s);
 hipError_t launch_unpack3to4(const float *src3, float *dst4, int n, hipStream_t s);
+// (x, y, z, 0) at every voxel of a float4 field
+hipError_t launch_identity4(Dims d, float *dst4, hipStream_t s);
 
 }  // namespace frk

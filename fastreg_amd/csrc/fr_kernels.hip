@@ -81,10 +81,13 @@ __device__ __forceinline__ Corner corner(Dims d, float px, float py, float pz) {
     return k;
 }
 
-// q |q|, and 0 for a NaN q
+// MX(q)^2 - MN(q)^2 with MX = fmaxf(., 0), MN = fminf(., 0), bit for bit: one of the two squares
+// is +0 and the other the rounded q^2, so the difference is q |q| when that is nonzero, and +0
+// when it is zero (q = +-0, or |q| so small that q^2 underflows: q |q| would be -0 for a negative
+// q) or NaN (fmaxf and fminf drop the NaN).  One ordered not-equal compare covers both cases.
 __device__ __forceinline__ float sq_signed(float q) {
     const float s = q * fabsf(q);
-    return q == q ? s : 0.f;
+    return (s < 0.f || s > 0.f) ? s : 0.f;
 }
 
 __device__ __forceinline__ int lin(Dims d, int x, int y, int z) { return x + d.X * (y + d.Y * z); }
@@ -613,9 +616,9 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const F3 xp = v_xp, xm = v_xm, yp = v_yp, ym = v_ym;
     const F3 zp = c.v_p1, zm = c.v_m1;
     // The reference's diagonal terms ((MX(u)^2 - MN(u)^2) + MN(w)^2) - MX(w)^2, MX = fmaxf(., 0),
-    // MN = fminf(., 0), are sq(u) - sq(w) with sq(q) = q |q| bit for bit: one of MX(q)^2, MN(q)^2 is
-    // the rounded q^2 and the other +0, q |q| is +-(rounded q^2), and a - (-b) == a + b.  A NaN q
-    // gives 0 in the reference's form (fmaxf and fminf drop it), and so in sq.
+    // MN = fminf(., 0), are sq(u) - sq(w) bit for bit, zero signs included: sq(u) is the first
+    // difference exactly (+0 when it is zero), adding +0 = MN(w)^2 or subtracting +0 = MX(w)^2
+    // changes nothing, and a - (-b) == a + b.
     const float ax1 = sq_signed(vc.x) - sq_signed(xp.x);
     const float ax2 = vc.y * ((vc.y > 0.f) ? (vc.x - ym.x) : (yp.x - vc.x));
     const float ax3 = vc.z * ((vc.z > 0.f) ? (vc.x - zm.x) : (zp.x - vc.x));
@@ -1230,6 +1233,18 @@ __global__ void k_pack4to3(const float *__restrict__ src4, float *dst3a, float *
     }
 }
 
+// init_identity_kernel (vector_fields.cu:64-79): (x, y, z, 0) at voxel (x, y, z)
+__global__ void k_identity4(Dims d, float *__restrict__ dst4) {
+    const int n = d.X * d.Y * d.Z;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int x = i % d.X, r = i / d.X, y = r % d.Y, z = r / d.Y;
+        dst4[4 * i] = (float)x;
+        dst4[4 * i + 1] = (float)y;
+        dst4[4 * i + 2] = (float)z;
+        dst4[4 * i + 3] = 0.f;
+    }
+}
+
 __global__ void k_unpack3to4(const float *__restrict__ src3, float *__restrict__ dst4, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         F3 v = ld3(src3, i);
@@ -1555,6 +1570,12 @@ hipError_t launch_vr_get(int st, const unsigned *r, float *v4, float *rho, int n
 
 hipError_t launch_pack4to3(const float *src4, float *dst3a, float *dst3b, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_pack4to3, dim3(grid_for(n, 256)), dim3(256), 0, s, src4, dst3a, dst3b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_identity4(Dims d, float *dst4, hipStream_t s) {
+    const int n = d.X * d.Y * d.Z;
+    hipLaunchKernelGGL(k_identity4, dim3(grid_for(n, 256)), dim3(256), 0, s, d, dst4);
     return hipGetLastError();
 }
 

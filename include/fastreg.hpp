@@ -18,7 +18,7 @@ public:
         fr_ctx *c = nullptr;
         fastreg::check(fr_create(&q, &c));
         ctx_ = c;
-        psi_ = std::make_shared<fastreg::hip::DeformationField>(params.volume_dims);
+        psi_ = std::make_shared<fastreg::cuda::DeformationField>(params.volume_dims);
     }
     ~FastReg() { fr_destroy(ctx_); }
     FastReg(const FastReg &) = delete;
@@ -27,7 +27,7 @@ public:
     Params &getParams() { return params_; }
 
     // psi of the last registration (float4 per voxel, absolute voxel coordinates)
-    std::shared_ptr<fastreg::hip::DeformationField> getDeformationField() {
+    std::shared_ptr<fastreg::cuda::DeformationField> getDeformationField() {
         if (ran_) fastreg::check(fr_get_deformation(ctx_, psi_->data()));
         return psi_;
     }
@@ -63,5 +63,5 @@ private:
     bool ran_ = false;
     Params params_;
     fr_ctx *ctx_ = nullptr;
-    std::shared_ptr<fastreg::hip::DeformationField> psi_;
+    std::shared_ptr<fastreg::cuda::DeformationField> psi_;
 };

@@ -182,7 +182,10 @@ int fr_solver_energy(fr_ctx *ctx, double *e_data, double *e_reg);
 typedef struct fr_batch fr_batch;
 int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out);
 int fr_batch_destroy(fr_batch *b);
-/* re-read each context's z-chunking (fr_set_chunking) into the batch's launch layout */
+/* Re-read each context's z-chunking (fr_set_chunking) and storage into the batch's launch layout.
+ * A batch runs the buffers and kernels of the storage its volumes had when the layout was made:
+ * after fr_set_storage on any of them, fr_batch_run and fr_batch_time_iterations fail with
+ * FR_ERR_STATE until this is called (and it fails while the volumes' storages differ). */
 int fr_batch_relayout(fr_batch *b);
 /* MAX_ITER iterations of every volume from its current state (all volumes must have been
  * initialised and be at the same point: fresh, or after the same number of runs); asynchronous */
@@ -201,6 +204,11 @@ int fr_batch_time_iterations(fr_batch *b, int iters, float *ms);
  * pointer may be NULL.  hip_stream may be NULL (default stream). */
 int fr_warp(int X, int Y, int Z, const float *d_img, const int16_t *d_lab, const float *d_psi4, float *d_out_img,
             int16_t *d_out_lab, void *hip_stream);
+
+/* init_identity_kernel (src/fastreg/cuda/vector_fields.cu:64-79, DeformationField::clear,
+ * src/fastreg/vector_fields.cpp:87-92): field(p) = (x, y, z, 0) at every voxel p = (x, y, z) of a
+ * float4 field.  hip_stream may be NULL (default stream).  Waits. */
+int fr_field_identity(int X, int Y, int Z, float *d_field4, void *hip_stream);
 
 /* ---- the reference's further public functions (not used by its registration loop) -------- */
 
@@ -266,13 +274,21 @@ int fr_adjoint_backward(fr_adjoint *h, double *d_grad_v0, double *grad_w_reg, do
 int fr_adjoint_get_state(fr_adjoint *h, int k, double *d_v, double *d_rho, double *d_psi, double *d_psi_inv);
 /* device bytes held by the context; device time of the last forward and backward calls */
 int fr_adjoint_info(fr_adjoint *h, size_t *device_bytes, float *forward_ms, float *backward_ms);
+/* z-planes per block of the backward pass's two record gathers (the transposes of the trilinear
+ * samples), which march tiles of tile_x x tile_y voxels; 0 restores the automatic choice (about
+ * two blocks per CU).  Results do not depend on it.  No counterpart in the reference: a launch
+ * shape of this implementation, for tests and tuning. */
+int fr_adjoint_set_gather_chunk(fr_adjoint *h, int zc);
+int fr_adjoint_get_gather_chunk(const fr_adjoint *h, int *tile_x, int *tile_y, int *zc);
 
 /* One solver iteration as a forward/backward pair on caller states (device pointers; rho N
  * doubles, v / psi / psi_inv N*3), with the images of h, the given W_REG and TAU and the t of
  * iteration k (t_1 = ALPHA, RESET_ITER as the loop does).  fr_step_fwd writes the next state;
  * fr_step_bwd takes the adjoint of the next state in d_g_* and leaves there the adjoint of the
  * state, and returns dE/dW_REG and dE/dTAU of this iteration.  Both reuse h's work buffers, so
- * fr_adjoint_backward needs a new fr_adjoint_forward afterwards.  No counterpart in the reference
+ * fr_adjoint_backward needs a new fr_adjoint_forward afterwards.  fr_step_fwd's outputs must not
+ * overlap its inputs or each other, and fr_step_bwd's four adjoint arrays must not overlap each
+ * other (FR_ERR_INVALID otherwise: the kernels read the inputs' neighbourhoods while writing).  No counterpart in the reference
  * (one iteration of src/fastreg/cuda/solver.cu:335-407, differentiated). */
 int fr_step_fwd(fr_adjoint *h, int k, double w_reg, double tau, const double *d_rho, const double *d_v,
                 const double *d_psi, const double *d_psi_inv, double *d_rho1, double *d_v1, double *d_psi1,

@@ -1,15 +1,23 @@
-// fastreg::hip::Solver and the device field containers it works on — the reference's
-// fastreg::cuda::Solver (include/solver.hpp:64-113), Volume (include/cuda/volume.hpp:19-46) and
-// DeformationField (include/cuda/vector_fields.hpp:53-67), as thin C++ wrappers over the C-ABI
-// in fastreg_c.h.  Header-only; link libfastreg_hip.so.
+// fastreg::cuda::Solver and the device containers it works on, under the reference's names and
+// signatures: Solver (include/solver.hpp:60-113), Volume (include/cuda/volume.hpp:19-46),
+// VectorField and DeformationField (include/cuda/vector_fields.hpp:21-67), ScalarField
+// (include/cuda/scalar_fields.hpp:20-65).  The namespace keeps the reference's name `cuda` so that
+// callers written against the reference compile unchanged; the code behind it is HIP on gfx950,
+// reached through the C-ABI of fastreg_c.h (header-only; link libfastreg_hip.so).
+//
+// Layouts differ from the reference's where that saves HBM traffic: a Volume holds float
+// intensities and int16 labels as two arrays (the reference packs a float2 per voxel); vector
+// fields keep the reference's float4 (x, y, z, 0) per voxel.
 #pragma once
 
-#include <array>
+#include <cmath>
 #include <cstdint>
+#include <iostream>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "fastreg_c.h"
 #include "params.hpp"
@@ -21,7 +29,7 @@ inline void check(int rc) {
     if (rc != FR_OK) throw std::runtime_error(std::string("fastreg: ") + fr_last_error());
 }
 
-namespace hip {
+namespace cuda {
 
 // device buffer of `count` elements of T (RAII over fr_dev_malloc / fr_dev_free)
 template <typename T>
@@ -42,64 +50,147 @@ public:
     size_t size() const { return n_; }
     void upload(const T *host) { check(fr_memcpy_htod(p_, host, n_ * sizeof(T))); }
     void download(T *host) const { check(fr_memcpy_dtoh(host, p_, n_ * sizeof(T))); }
+    void zero() { check(fr_dev_memset(p_, 0, n_ * sizeof(T))); }
 
 private:
     T *p_ = nullptr;
     size_t n_ = 0;
 };
 
-// An image (float intensities) and its int16 labels on the device.  The reference packs both
-// into one float2 volume; the solver here keeps them as separate arrays.
+inline size_t voxels_of(const Vec3i &d) { return (size_t)d[0] * d[1] * d[2]; }
+
+// An image (float intensities) and its int16 labels on the device (cuda::Volume,
+// volume.hpp:19-46).  A new volume is cleared, as the reference's constructor does.
 class Volume {
 public:
-    explicit Volume(const std::array<int, 3> &dims)
-        : dims_(dims), img_((size_t)dims[0] * dims[1] * dims[2]), lab_((size_t)dims[0] * dims[1] * dims[2]) {}
-    // copy intensities and labels from device arrays (Volume::initialise, volume.cpp:30-36)
-    void initialise(const float *d_img, const int16_t *d_lab) {
-        check(fr_memcpy_dtod(img_.data(), d_img, img_.size() * sizeof(float)));
-        check(fr_memcpy_dtod(lab_.data(), d_lab, lab_.size() * sizeof(int16_t)));
+    explicit Volume(const Vec3i &dims) : dims_(dims), img_(voxels_of(dims)), lab_(voxels_of(dims)) { clear(); }
+
+    Vec3i get_dims() const { return dims_; }
+
+    // copy intensities and labels from device arrays of no_voxels elements (volume.cpp:30-35)
+    void initialise(float *image_data, short *labels_data) {
+        check(fr_memcpy_dtod(img_.data(), image_data, img_.size() * sizeof(float)));
+        check(fr_memcpy_dtod(lab_.data(), labels_data, lab_.size() * sizeof(int16_t)));
     }
+    // intensities and labels to 0 (volume.cpp:37-42)
+    void clear() {
+        img_.zero();
+        lab_.zero();
+    }
+    // every voxel's intensity and label, x outermost, in the reference's format (volume.cpp:44-61)
+    void print() const {
+        std::vector<float> im(img_.size());
+        std::vector<int16_t> lb(lab_.size());
+        img_.download(im.data());
+        lab_.download(lb.data());
+        std::cout << "--- IMAGE ---" << std::endl;
+        for (int i = 0; i < dims_[0]; i++)
+            for (int j = 0; j < dims_[1]; j++)
+                for (int k = 0; k < dims_[2]; k++) {
+                    const size_t v = i + (size_t)dims_[0] * (j + (size_t)dims_[1] * k);
+                    std::cout << "(x,y,z)=(" << i << ", " << j << ", " << k << "), intensity: " << im[v]
+                              << ", label: " << (float)lb[v] << std::endl;
+                }
+    }
+
     float *image() { return img_.data(); }
+    const float *image() const { return img_.data(); }
     int16_t *labels() { return lab_.data(); }
-    const std::array<int, 3> &get_dims() const { return dims_; }
+    const int16_t *labels() const { return lab_.data(); }
 
 private:
-    std::array<int, 3> dims_;
+    Vec3i dims_;
     DeviceBuffer<float> img_;
     DeviceBuffer<int16_t> lab_;
 };
 
-// psi or psi_inv: float4 (x, y, z, 0) per voxel in absolute voxel coordinates
-class DeformationField {
+// a float4 (x, y, z, 0) field on the device (cuda::VectorField, vector_fields.hpp:21-48); a new
+// field is cleared to 0
+class VectorField {
 public:
-    explicit DeformationField(const std::array<int, 3> &dims)
-        : dims_(dims), buf_(4 * (size_t)dims[0] * dims[1] * dims[2]) {}
+    explicit VectorField(const Vec3i &dims) : dims_(dims), buf_(4 * voxels_of(dims)) { buf_.zero(); }
+    virtual ~VectorField() = default;
+
+    Vec3i get_dims() const { return dims_; }
     float *data() { return buf_.data(); }
     const float *data() const { return buf_.data(); }
-    const std::array<int, 3> &get_dims() const { return dims_; }
+    void upload(const float *host_xyzw) { buf_.upload(host_xyzw); }
     void download(float *host_xyzw) const { buf_.download(host_xyzw); }
 
-    // DeformationField::get_inverse (vector_fields.cpp:96-103): 48 fixed-point sweeps of
-    // psi_inv = -trilerp(*this, psi_inv), starting from psi_inv's current values
+    // every component to 0 (vector_fields.cpp:24-29)
+    virtual void clear() { buf_.zero(); }
+    // the voxels with a component above 1e-5 in magnitude, x outermost (vector_fields.cpp:31-55)
+    void print() const {
+        std::vector<float> h(buf_.size());
+        download(h.data());
+        std::cout << "--- FIELD ---" << std::endl;
+        for (int i = 0; i < dims_[0]; i++)
+            for (int j = 0; j < dims_[1]; j++)
+                for (int k = 0; k < dims_[2]; k++) {
+                    const size_t v = 4 * (i + (size_t)dims_[0] * (j + (size_t)dims_[1] * k));
+                    const float u = h[v], w1 = h[v + 1], w2 = h[v + 2];
+                    if (std::fabs(u) > 1e-5f || std::fabs(w1) > 1e-5f || std::fabs(w2) > 1e-5f)
+                        std::cout << "(x,y,z)=(" << i << ", " << j << ", " << k << "), (u,v,w)=(" << u << ", " << w1
+                                  << "," << w2 << ")" << std::endl;
+                }
+    }
+    // voxels with a NaN among x, y, z (vector_fields.cpp:57-79)
+    int get_no_nans() const {
+        std::vector<float> h(buf_.size());
+        download(h.data());
+        int n = 0;
+        for (size_t v = 0; v < h.size(); v += 4)
+            if (std::isnan(h[v]) || std::isnan(h[v + 1]) || std::isnan(h[v + 2])) ++n;
+        return n;
+    }
+
+protected:
+    Vec3i dims_;
+    DeviceBuffer<float> buf_;
+};
+
+typedef VectorField VolumeGradient;
+typedef VectorField Laplacian;
+typedef VectorField PotentialGradient;
+typedef VectorField Velocity;
+typedef VectorField DVV;
+
+// psi or psi_inv in absolute voxel coordinates (cuda::DeformationField, vector_fields.hpp:53-67);
+// a new field is the identity
+class DeformationField : public VectorField {
+public:
+    explicit DeformationField(const Vec3i &dims) : VectorField(dims) { clear(); }
+
+    // the identity, (x, y, z, 0) at voxel (x, y, z) (vector_fields.cpp:87-92)
+    void clear() override { check(fr_field_identity(dims_[0], dims_[1], dims_[2], data(), nullptr)); }
+
+    // phi_psi = phi o *this: trilinear intensities and floor-corner labels at every voxel
+    // (vector_fields.cpp:103-113)
+    void apply(std::shared_ptr<Volume> phi, std::shared_ptr<Volume> phi_psi) const {
+        const Vec3i d = phi->get_dims();
+        check(fr_warp(d[0], d[1], d[2], phi->image(), phi->labels(), data(), phi_psi->image(), phi_psi->labels(),
+                      nullptr));
+    }
+
+    // 48 fixed-point sweeps of psi_inv = -trilerp(*this, psi_inv), starting from psi_inv's current
+    // values (vector_fields.cpp:94-101)
     void get_inverse(DeformationField &psi_inv) const {
         check(fr_get_inverse(dims_[0], dims_[1], dims_[2], data(), psi_inv.data(), nullptr));
     }
 
-    // interpolate_gradient (vector_fields.cu:194-225): out = trilerp(field, *this) per voxel
-    void interpolate(const DeformationField &field, DeformationField &out) const {
+    // out = trilerp(field, *this) per voxel (interpolate_gradient, vector_fields.cu:194-225)
+    void interpolate(const VectorField &field, VectorField &out) const {
         check(fr_interpolate_gradient(dims_[0], dims_[1], dims_[2], field.data(), out.data(), data(), nullptr));
     }
-
-private:
-    std::array<int, 3> dims_;
-    DeviceBuffer<float> buf_;
 };
 
-// a scalar field on the device (ScalarField, scalar_fields.hpp:20-65)
+// a float scalar field on the device (cuda::ScalarField, scalar_fields.hpp:20-65), cleared to 0
 class ScalarField {
 public:
-    explicit ScalarField(const std::array<int, 3> &dims) : dims_(dims), buf_((size_t)dims[0] * dims[1] * dims[2]) {}
+    explicit ScalarField(const Vec3i &dims) : dims_(dims), buf_(voxels_of(dims)) { buf_.zero(); }
+    Vec3i get_dims() const { return dims_; }
     float *data() { return buf_.data(); }
+    void clear() { buf_.zero(); }
     // ScalarField::sum (scalar_fields.cpp:29-33), accumulated in FP64
     double sum() const {
         double s = 0.0;
@@ -108,9 +199,12 @@ public:
     }
 
 private:
-    std::array<int, 3> dims_;
+    Vec3i dims_;
     DeviceBuffer<float> buf_;
 };
+
+typedef ScalarField Density;
+typedef ScalarField Divergence;
 
 class Solver {
 public:
@@ -124,14 +218,15 @@ public:
     Solver(const Solver &) = delete;
     Solver &operator=(const Solver &) = delete;
 
-    // every estimate_psi starts from a fresh state; kept for interface parity (solver.cpp:69-81)
+    // The reference clears its derivative fields here (solver.cpp:69-81); every estimate_psi here
+    // starts from a fresh state, so there is nothing to clear.
     void clear() {}
 
     // estimate_psi (solver.cpp:83-115): register phi_n to phi_global; on return
-    // phi_global_psi_inv = phi_global o psi_inv, phi_n_psi = phi_n o psi, and psi / psi_inv hold
-    // the deformation and its inverse
-    void estimate_psi(const std::shared_ptr<Volume> &phi_global, std::shared_ptr<Volume> phi_global_psi_inv,
-                      const std::shared_ptr<Volume> &phi_n, std::shared_ptr<Volume> phi_n_psi,
+    // phi_global_psi_inv = phi_global o psi_inv, phi_n_psi = phi_n o psi, and psi / psi_inv hold the
+    // deformation and its inverse (either may be null)
+    void estimate_psi(std::shared_ptr<Volume> phi_global, std::shared_ptr<Volume> phi_global_psi_inv,
+                      std::shared_ptr<Volume> phi_n, std::shared_ptr<Volume> phi_n_psi,
                       std::shared_ptr<DeformationField> psi, std::shared_ptr<DeformationField> psi_inv) {
         check(fr_solver_init(ctx_, phi_global->image(), phi_global->labels(), phi_n->image(), phi_n->labels()));
         check(fr_solver_run(ctx_));
@@ -149,6 +244,12 @@ public:
         return {e, v};
     }
 
+    fr_stats stats() const {
+        fr_stats st{};
+        check(fr_get_stats(ctx_, &st));
+        return st;
+    }
+
     fr_ctx *handle() { return ctx_; }
     const Params &params() const { return params_; }
 
@@ -157,5 +258,9 @@ private:
     fr_ctx *ctx_ = nullptr;
 };
 
-}  // namespace hip
+}  // namespace cuda
+
+// the same classes under the name of the platform they run on
+namespace hip = cuda;
+
 }  // namespace fastreg

@@ -2,6 +2,7 @@
 FP64 oracle iteration by iteration, its gradient (hand-written adjoint kernels, checkpointed)
 against torch autograd of the unrolled restatement (oracle/unrolled.py) and against central
 finite differences of its own forward pass, independent of the checkpoint interval."""
+import ctypes as C
 import sys
 
 import numpy as np
@@ -138,19 +139,79 @@ def test_gradient_matches_autograd_at_depth_64():
 
 def test_gradient_with_uneven_gather_tiles():
     """Shapes whose record gathers (64 x 4 tiles marching z-chunks) end in partial tiles and a
-    partial last z-chunk.  On a 256-CU GPU, 70 x 66 x 71 has 2 x 17 tiles and 5-plane chunks, the
-    last of one plane; 130 x 7 x 9 has a 2-column last x-tile and a 3-row last y-tile."""
-    for dims, K, ckpt in (((70, 66, 71), 4, 2), ((130, 7, 9), 5, 5)):
-        say("adjoint vs autograd, uneven gather tiles", dims, K, ckpt)
+    partial last z-chunk, whatever the GPU's CU count: 70 x 66 x 71 with 5-plane chunks (2 x 17
+    tiles, the last x-tile 6 columns wide, the last y-tile 2 rows high, the last chunk one plane);
+    130 x 7 x 9 with 4-plane chunks (a 2-column last x-tile, a 3-row last y-tile, a one-plane last
+    chunk)."""
+    for dims, K, ckpt, zc in (((70, 66, 71), 4, 2, 5), ((130, 7, 9), 5, 5, 4)):
+        say("adjoint vs autograd, uneven gather tiles", dims, K, ckpt, zc)
         pair = _pair(dims, 14)
         v0 = _v0(dims, 14)
         a = _adjoint(dims, pair, K, ckpt)
         try:
+            a.set_gather_chunk(zc)
+            tx, ty, got = a.gather_chunk()
+            assert got == zc
+            assert dims[0] % tx != 0 and dims[1] % ty != 0 and dims[2] % zc == 1, (tx, ty, zc)
             e = a.forward(v0, W_REG, TAU, 0.5)
             g, gw, gt = a.backward()
         finally:
             a.close()
         _compare((e, g, gw, gt), _autograd(dims, pair, v0, K, 0.5), 1e-10)
+
+
+def test_scalar_gradients_are_the_same_bits_on_every_run():
+    """dE/dW_REG and dE/dTAU are sums over blocks added in a fixed order (no atomics): two backward
+    passes of the same forward pass return the same bits, and so does the whole gradient when no
+    source is displaced by a voxel or more (no atomic scatter runs)."""
+    dims, K, ckpt = (40, 36, 30), 6, 3
+    say("adjoint determinism", dims, K)
+    pair = _pair(dims, 15)
+    v0 = _v0(dims, 15, amp=0.5, off=0.05)
+    a = _adjoint(dims, pair, K, ckpt)
+    try:
+        a.forward(v0, W_REG, TAU, 1.0)
+        g1, gw1, gt1 = a.backward()
+        a.forward(v0, W_REG, TAU, 1.0)
+        g2, gw2, gt2 = a.backward()
+    finally:
+        a.close()
+    assert gw1 == gw2 and gt1 == gt2
+    assert np.array_equal(g1, g2)
+
+
+def test_step_pairs_reject_overlapping_arrays():
+    """fr_step_fwd's outputs may not overlap its inputs or each other, and fr_step_bwd's adjoint
+    arrays may not overlap each other: both return FR_ERR_INVALID before running anything."""
+    dims = (12, 10, 9)
+    n = dims[0] * dims[1] * dims[2]
+    pair = _pair(dims, 16)
+    a = _adjoint(dims, pair, 2, 1)
+    idx = np.arange(n)
+    ident = np.stack([idx % dims[0], (idx // dims[0]) % dims[1], idx // (dims[0] * dims[1])], 1).astype(np.float64)
+    state = {"rho": np.ones(n), "v": _v0(dims, 16), "psi": ident, "psi_inv": ident}
+    bufs = a._state_bufs(state) + [fr.DeviceArray.from_host(np.zeros(w * n)) for w in (1, 3, 3, 3)]
+    L = fr.lib()
+    try:
+        ok = [b.ptr for b in bufs]
+        assert L.fr_step_fwd(a.h, 1, W_REG, TAU, *ok) == 0
+        for o, i in ((5, 1), (4, 0), (7, 3), (6, 2)):
+            bad = list(ok)
+            bad[o] = ok[i]          # an output in place of its own input
+            assert L.fr_step_fwd(a.h, 1, W_REG, TAU, *bad) == fr.FR_ERR_INVALID
+            assert b"overlaps" in L.fr_last_error()
+        bad = list(ok)
+        bad[7] = ok[6]              # two outputs on the same memory
+        assert L.fr_step_fwd(a.h, 1, W_REG, TAU, *bad) == fr.FR_ERR_INVALID
+        gw, gt = C.c_double(), C.c_double()
+        assert L.fr_step_bwd(a.h, 1, W_REG, TAU, *ok, C.byref(gw), C.byref(gt)) == 0
+        bad = list(ok)
+        bad[6] = ok[5]              # two adjoint arrays on the same memory
+        assert L.fr_step_bwd(a.h, 1, W_REG, TAU, *bad, C.byref(gw), C.byref(gt)) == fr.FR_ERR_INVALID
+    finally:
+        for b in bufs:
+            b.free()
+        a.close()
 
 
 def test_gradient_with_displacements_of_several_voxels():

@@ -143,3 +143,45 @@ def test_batch_of_degenerate_volumes_matches_oracle(oracle_mod):
         _check(vols, "degenerate batch")
     finally:
         b.close()
+
+
+def test_batch_refuses_to_run_after_a_storage_change(oracle_mod):
+    """A batch holds its volumes' buffers and kernels of the storage they had when it was laid out:
+    after fr_set_storage on one of them, running or timing the batch fails with FR_ERR_STATE (not a
+    run on stale buffers), relayout fails while the storages differ, and once every volume has the
+    new storage and the batch is relaid out it runs and matches the oracle in that storage."""
+    O = oracle_mod
+    vols = []
+    for k, dims in enumerate([(20, 18, 16), (35, 12, 22)]):
+        X, Y, Z = dims
+        pair = phantoms.blobs(shape=(Z, Y, X), seed=60 + k)
+        p = fr.default_params(vol_dims=dims, no_labels=pair.no_labels, max_iter=5, alpha=0.3)
+        g = fr.Solver(p)
+        g.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+        vols.append((g, pair))
+    b = fr.Batch([g for g, _ in vols])
+    try:
+        vols[1][0].set_storage(fr.STORAGE_BF16)
+        for call in (b.run, lambda: b.time_iterations(1)):
+            with pytest.raises(fr.FastRegError) as e:
+                call()
+            assert e.value.code == fr.FR_ERR_STATE
+        with pytest.raises(fr.FastRegError) as e:
+            b.relayout()
+        assert e.value.code == fr.FR_ERR_INVALID
+        vols[0][0].set_storage(fr.STORAGE_BF16)
+        b.relayout()
+        refs = []
+        for g, pair in vols:
+            g.reinit()
+            o = O.Oracle(g.dims, "f32", storage=fr.STORAGE_BF16)
+            o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+            refs.append((o, o.run(O.SolverParams(max_iter=5, alpha=0.3, no_labels=pair.no_labels))))
+        b.run()
+        for (g, _), (o, r) in zip(vols, refs):
+            assert g.stats().iterations == r["iterations"] == 5
+            gs, os_ = g.get_state(), o.get_state()
+            for f in ("v", "rho", "psi", "psi_inv"):
+                assert np.array_equal(gs[f], os_[f]), f
+    finally:
+        b.close()

@@ -80,9 +80,15 @@ def _both(dims, O, seed, src_pair=None, **kw):
     return g, o, pair, rng
 
 
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint32 if a.dtype == np.float32 else np.uint64)
+
+
 def _assert_state_equal(gs, os_, what=""):
+    """Every field equal bit for bit (raw bits: -0 differs from +0, a NaN equals only itself)."""
     for k in ("v", "rho", "psi", "psi_inv"):
-        a, b = gs[k], os_[k]
+        a, b = _bits(gs[k]), _bits(os_[k])
         if not np.array_equal(a, b):
             bad = np.flatnonzero((a != b).reshape(len(a), -1).any(-1) if a.ndim > 1 else (a != b))
             raise AssertionError(f"{what} field {k}: {len(bad)} voxels differ, first {bad[:5]}: "
@@ -103,6 +109,33 @@ def test_step_teacher_forced_bit_exact(oracle_mod, dims, seed):
     g.set_state(**st)
     o.set_state(**st)
     for k, t in enumerate([0.004, 0.008, 0.5]):
+        mg, ag = g.step(t)
+        mo, ao = o.step(t)
+        _assert_state_equal(g.get_state(), o.get_state(), f"step {k}")
+        assert mg == mo and ag == ao, (mg, mo, ag, ao)
+
+
+def test_step_with_signed_zeros_and_underflowing_velocities(oracle_mod):
+    """A state whose velocity components include -0, +0 and values whose squares underflow
+    (+-1e-30), next to ordinary ones: the (Dv)v diagonal terms must give +0 where the reference's
+    fmaxf/fminf form does, so every field matches the oracle in its raw bits, zero signs included."""
+    O = oracle_mod
+    dims = (40, 22, 18)
+    say("signed zeros", dims)
+    g, o, pair, rng = _both(dims, O, 33)
+    st = phantoms.random_state(dims, rng, amp=1.5)
+    v = st["v"].copy()
+    comp = v[:, :3]
+    pick = rng.random(comp.shape)
+    comp[pick < 0.2] = -0.0
+    comp[(pick >= 0.2) & (pick < 0.3)] = 0.0
+    comp[(pick >= 0.3) & (pick < 0.4)] = -1e-30
+    comp[(pick >= 0.4) & (pick < 0.5)] = 1e-30
+    st["v"] = v
+    g.set_state(**st)
+    o.set_state(**st)
+    _assert_state_equal(g.get_state(), o.get_state(), "set_state")
+    for k, t in enumerate([0.004, 0.5]):
         mg, ag = g.step(t)
         mo, ao = o.step(t)
         _assert_state_equal(g.get_state(), o.get_state(), f"step {k}")

@@ -36,7 +36,7 @@ def _both(O, dims, seed, storage, **kw):
 def _same_state(g, o, what):
     gs, os_ = g.get_state(), o.get_state()
     for k in ("v", "rho", "psi", "psi_inv"):
-        a, b = gs[k], os_[k]
+        a, b = gs[k].view(np.uint32), os_[k].view(np.uint32)   # raw bits: zero signs count
         if not np.array_equal(a, b):
             bad = np.flatnonzero((a != b).reshape(len(a), -1).any(-1) if a.ndim > 1 else (a != b))
             raise AssertionError(f"{what} {k}: {len(bad)} voxels differ, first {bad[:4]}")
