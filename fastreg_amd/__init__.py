@@ -116,6 +116,7 @@ def lib():
     sig("fr_interpolate_gradient", i, i, i, vp, vp, vp, vp)
     sig("fr_voxel_max_energy", vp, P(f), P(ll))
     sig("fr_sum", vp, ll, P(d), vp)
+    sig("fr_sum_noalloc", vp, ll, P(d), vp)
     sig("fr_adjoint_create", P(FrParams), i, i, P(vp))
     sig("fr_adjoint_destroy", vp)
     sig("fr_adjoint_set_images", vp, vp, vp, vp, vp)

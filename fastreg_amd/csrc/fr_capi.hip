@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1087,6 +1088,54 @@ extern "C" int fr_sum(const float *d, long long n, double *sum, void *stream) {
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(FR_ERR_HIP, "fr_sum: %s", hipGetErrorString(e));
+    *sum = h;
+    return FR_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// fr_sum_noalloc: the sum's scratch in a static device array instead of a per-call allocation
+// ---------------------------------------------------------------------------------------------
+
+// The partial sums of up to SUM_SCRATCH_PARTIALS blocks and the total.  A __device__ array exists
+// once on every GPU the library runs on; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (the array's name is a host-side stand-in).
+constexpr int SUM_SCRATCH_PARTIALS = 1024;
+static __device__ __align__(256) double scratch[SUM_SCRATCH_PARTIALS + 1];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static double *g_scratch_ptr[SCRATCH_MAX_DEVICES];  // per device, looked up at its first call
+static std::mutex g_scratch_mu;
+
+// the device address of `scratch` on the current device (looked up once, with that device current)
+static int scratch_on_current_device(double **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+        return fail(FR_ERR_INVALID, "device %d is beyond the %d devices fr_sum_noalloc supports", dev,
+                    SCRATCH_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return FR_OK;
+}
+
+extern "C" int fr_sum_noalloc(const float *d, long long n, double *sum, void *stream) {
+    if (!d || n < 0 || !sum) return fail(FR_ERR_INVALID, "bad fr_sum_noalloc arguments");
+    const int np = frk::sum_partials_count(n);
+    if (np > SUM_SCRATCH_PARTIALS)
+        return fail(FR_ERR_INVALID, "fr_sum_noalloc: %d partial sums exceed its scratch of %d", np,
+                    SUM_SCRATCH_PARTIALS);
+    double *ptr = nullptr;
+    if (int rc = scratch_on_current_device(&ptr)) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(frk::launch_sum(d, n, ptr, ptr + np, s));  // every partial is written: nothing to clear
+    double h = 0.0;
+    HIP_TRY(hipMemcpyAsync(&h, ptr + np, sizeof h, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     *sum = h;
     return FR_OK;
 }

@@ -231,6 +231,16 @@ int fr_voxel_max_energy(fr_ctx *ctx, float *max_energy, long long *voxel);
  * sum of n device floats, accumulated in FP64 (the reference accumulates float block sums).  Waits. */
 int fr_sum(const float *d_data, long long n, double *sum, void *hip_stream);
 
+/* fr_sum without any device allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same kernels and the same result bits as fr_sum; the partial sums of its at most
+ * 1024 blocks and the total live in a static device array of 1025 doubles (8.2 KB on each GPU the
+ * library runs on, held until the process ends), so every n >= 0 fits (a partial count above the
+ * array would be refused with FR_ERR_INVALID).  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other threads, other streams) race and must be serialised by
+ * the caller.  Waits. */
+int fr_sum_noalloc(const float *d_data, long long n, double *sum, void *hip_stream);
+
 /* Average device time per launch of the solver's two iteration kernels (pass A: -Laplacian of psi
  * and its warp, density, velocity and psi_inv updates; pass B: psi update and max norm), from HIP
  * events between consecutive launches of `reps` iterations run in sequence on the context stream,
