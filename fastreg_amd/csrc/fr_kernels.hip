@@ -407,20 +407,28 @@ struct YRow {
 };
 
 // L = -Laplacian(psi) and T on the planes around the one being computed, staged in LDS for the
-// whole block: a ring of 4 planes (z-1, z, z+1 are read while z+2 is written), 10 rows (the
-// block's 8 rows and one on either side), 64 columns (the lanes of a wave), (L.x, L.y, L.z, T)
-// per voxel.  L is not a field in memory: the block computes it in the stage, from psi staged the
-// same way (a ring of 4 planes of the 12 rows y-2 .. y+9 around the block), in the reference's
-// operation order (vector_fields.cu:275-300), 0 off the interior.  The 8 taps of a voxel whose
-// psi_inv lies within one voxel of it are all in the stage, so the trilinear taps are LDS reads
-// and the block loads each plane of psi and T once, in whole rows.
-constexpr int LT_ROWS = TY_A + 2, LT_PLANES = 4;
-constexpr int PS_ROWS = TY_A + 4, PS_PLANES = 4;
+// whole block: a ring of 4 planes (z-1, z, z+1 are read while z+2 is written) of the 12 rows
+// y-2 .. y+9 around the block's 8 rows (row j holds y = yb - 1 + j; rows 1..10, the block's rows
+// and one on either side, hold L; rows 0 and 11 are written and never read), 64 columns (the
+// lanes of a wave), (L.x, L.y, L.z, T) per voxel.  L is not a field in memory: the block computes
+// it in the stage in the reference's operation order (vector_fields.cu:275-300), 0 off the
+// interior.  Each wave holds two psi rows of the stage's 12 and their planes z+1..z+3 in
+// registers: the z-neighbours of L's 7-point stencil come from there, the x-neighbours from the
+// adjacent lanes, and only the y-neighbours (rows of other waves) from a 2-plane psi ring in LDS.
+// The 8 taps of a voxel whose psi_inv lies within one voxel of it are all in the L/T stage, so
+// the trilinear taps are LDS reads and the block loads each plane of psi and T once, in whole
+// rows.
+constexpr int LT_ROWS = TY_A + 2, LT_PLANES = 4;  // rows holding L
+constexpr int ST_ROWS = TY_A + 4, PS_PLANES = 2;  // rows of both stages
 struct LTStage {
-    float4 v[LT_PLANES][LT_ROWS][WAVE];
+    float4 v[LT_PLANES][ST_ROWS][WAVE];
 };
 struct PsiStage {
-    float4 v[PS_PLANES][PS_ROWS][WAVE];  // (x, y, z, unused): 16-byte entries for aligned LDS access
+    float4 v[PS_PLANES][ST_ROWS][WAVE];  // (x, y, z, unused): 16-byte entries for aligned LDS access
+};
+// one psi row of the stage at the planes around the L plane being computed: zz-1, zz, zz+1
+struct PsiWin {
+    F3 m, c, p;
 };
 
 struct PassAView {
@@ -432,7 +440,7 @@ struct PassAView {
     unsigned X12, XY12, X4, XY4;
     int xb, yb;           // the L/T stage's first column (x of lane 0) and first row (the block's y - 1)
     LTStage *lt;
-    PsiStage *ps;         // row j holds y = yb - 1 + j
+    PsiStage *ps;         // row j holds y = yb - 1 + j (as the L/T stage)
     YStage *ys;           // row j holds y = yb - 1 + j
 };
 
@@ -480,17 +488,20 @@ __device__ __forceinline__ float t_row_load(const PassAView &w, int x, int y, in
 __device__ __forceinline__ void psi_store(const PassAView &w, int zz, int row, int lane, F3 p) {
     w.ps->v[zz & (PS_PLANES - 1)][row][lane] = make_float4(p.x, p.y, p.z, 0.f);
 }
+__device__ __forceinline__ PsiWin psi_win(F3 m, F3 c, F3 p) { return PsiWin{m, c, p}; }
 __device__ __forceinline__ F3 ps_at(const PassAView &w, int zz, int row, int lane) {
     const float4 q = w.ps->v[zz & (PS_PLANES - 1)][row][lane];
     return f3(q.x, q.y, q.z);
 }
 
-// L of row r (y = yb + r) of plane zz from the psi stage (planes zz-1..zz+1, rows r..r+2) and the
-// x-neighbours of the adjacent lanes, with T of that row, into the L/T stage.  Every lane of the
-// wave runs it; the end lanes' L (x-neighbours missing) is never used.
-__device__ __forceinline__ void lt_compute(const PassAView &w, int zz, int r, int lane, float t) {
-    const F3 c = ps_at(w, zz, r + 1, lane), ym = ps_at(w, zz, r, lane), yp = ps_at(w, zz, r + 2, lane),
-             zm = ps_at(w, zz - 1, r + 1, lane), zp = ps_at(w, zz + 1, r + 1, lane);
+// L of stage row j (y = yb - 1 + j) of plane zz, with T of that row, into the L/T stage: psi of the
+// row at planes zz-1..zz+1 from the wave's registers (pw), its x-neighbours from the adjacent
+// lanes, its y-neighbours (rows j-1, j+1, clamped into the stage for the rows that are never
+// read) from the psi ring.  Every lane of the wave runs it; the end lanes' L (x-neighbours
+// missing) is never used.
+__device__ __forceinline__ void lt_compute(const PassAView &w, int zz, int j, int lane, float t, const PsiWin &pw) {
+    const F3 c = pw.c, ym = ps_at(w, zz, max(j - 1, 0), lane), yp = ps_at(w, zz, min(j + 1, ST_ROWS - 1), lane),
+             zm = pw.m, zp = pw.p;
     F3 s = mul3(-6.f, c);
     s = add3(s, lane_next3(c));
     s = add3(s, lane_prev3(c));
@@ -503,10 +514,10 @@ __device__ __forceinline__ void lt_compute(const PassAView &w, int zz, int r, in
     // operands: 6 fewer VALU instructions per row.
     asm volatile("" : "+v"(s.x), "+v"(s.y), "+v"(s.z));
     const F3 L = mul3(-1.f, s);
-    const int x = w.xb + lane, y = w.yb + r;
+    const int x = w.xb + lane, y = w.yb - 1 + j;
     const Dims d = w.d;
     const bool in = x >= 1 && x <= d.X - 2 && y >= 1 && y <= d.Y - 2 && zz >= 1 && zz <= d.Z - 2;
-    w.lt->v[zz & (LT_PLANES - 1)][r][lane] = in ? make_float4(L.x, L.y, L.z, t) : make_float4(0.f, 0.f, 0.f, t);
+    w.lt->v[zz & (LT_PLANES - 1)][j][lane] = in ? make_float4(L.x, L.y, L.z, t) : make_float4(0.f, 0.f, 0.f, t);
 }
 
 // L at grid point (x, y, z) from psi in memory (0 off the interior and outside the grid)
@@ -530,10 +541,11 @@ __device__ __forceinline__ F3 l_global(const PassAView &w, int x, int y, int z) 
 __device__ __forceinline__ void lt_at(const PassAView &w, const Corner &k, int z, bool store, F3 &Lo, float &Tv) {
     int cx = k.gx - w.xb, ry = k.gy - w.yb;
     const int dz = k.gz - z;
-    // columns 1..62 of the stage hold valid L (the end lanes lack an x-neighbour)
+    // columns 1..62 of the stage hold valid L (the end lanes lack an x-neighbour), and stage rows
+    // 1..10 (the corner's row y = yb + ry and the one above are stage rows ry + 1 and ry + 2)
     const bool in = cx >= 1 && cx <= WAVE - 3 && ry >= 0 && ry <= LT_ROWS - 2 && (dz == -1 || dz == 0);
     cx = min(max(cx, 1), WAVE - 3);
-    ry = min(max(ry, 0), LT_ROWS - 2);
+    ry = min(max(ry, 0), LT_ROWS - 2) + 1;
     const int zg = dz == -1 ? z - 1 : z;
     const int s0 = zg & (LT_PLANES - 1), s1 = (zg + 1) & (LT_PLANES - 1);
     const LTStage &st = *w.lt;
@@ -743,10 +755,10 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     w.X4 = 4u * d.X;
     w.XY4 = 4u * XY;
     const bool lap_xy = x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2;
-    // The stages.  This wave loads psi rows `wave` and `wave + 8` of each plane (rows 8..11 come
-    // from waves 0..3; the other waves load row 11 as well) and computes L/T rows `wave` and
-    // `wave + 8` (rows 8 and 9 from waves 0 and 1; the others compute row 9 as well), writing the
-    // same values, so that no wave branches.
+    // The stages.  This wave holds psi rows j0 = wave + 1 (the block's rows) and j1 of the stages
+    // (rows 9, 10, 11, 0 from waves 0..3; waves 4..7 hold rows 9 and 10 as well) and computes L/T
+    // for both of them, writing the same values where two waves hold the same row, so that no wave
+    // branches.
     __shared__ LTStage lt_stage;
     __shared__ PsiStage psi_stage;
     __shared__ YStage y_stage;
@@ -756,36 +768,47 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     w.ys = &y_stage;
     w.xb = x - lane;
     w.yb = y - wave - 1;
-    const int row0 = wave, row1 = min(wave + TY_A, LT_ROWS - 1);
-    const int prow0 = wave, prow1 = min(wave + TY_A, PS_ROWS - 1);
-    const int ty0 = w.yb + row0, ty1 = w.yb + row1, py0 = w.yb - 1 + prow0, py1 = w.yb - 1 + prow1;
+    const int j0 = wave + 1;
+    const int j1 = wave < 3 ? TY_A + 1 + wave : wave == 3 ? 0 : TY_A + 1 + (wave & 1);
+    const int py0 = w.yb - 1 + j0, py1 = w.yb - 1 + j1;  // the y of both rows (psi and T)
     const int zs = tc.zs;
-    // prologue: psi planes zs-2..zs+1, L/T planes zs-1 and zs; psi planes zs+2 and zs+3 (into the
-    // slots of zs-2 and zs-1), L/T plane zs+1; T rows of plane zs+2 for the first step
-    for (int zz = zs - 2; zz <= zs + 1; ++zz) {
-        const F3 p0 = psi_row_load(w, x, py0, zz), p1 = psi_row_load(w, x, py1, zz);
-        psi_store(w, zz, prow0, lane, p0);
-        psi_store(w, zz, prow1, lane, p1);
-    }
-    __syncthreads();
-    for (int zz = zs - 1; zz <= zs; ++zz) {
-        const float t0 = t_row_load(w, x, ty0, zz), t1 = t_row_load(w, x, ty1, zz);
-        lt_compute(w, zz, row0, lane, t0);
-        lt_compute(w, zz, row1, lane, t1);
-    }
-    __syncthreads();
-    for (int zz = zs + 2; zz <= zs + 3; ++zz) {
-        const F3 p0 = psi_row_load(w, x, py0, zz), p1 = psi_row_load(w, x, py1, zz);
-        psi_store(w, zz, prow0, lane, p0);
-        psi_store(w, zz, prow1, lane, p1);
-    }
-    __syncthreads();
+    // prologue: L/T planes zs-1, zs, zs+1 from psi planes zs-2..zs+2, each psi plane written to the
+    // ring before the L plane whose y-neighbours it holds; psi planes zs+1..zs+3 of both rows stay
+    // in registers for the first step, with T of plane zs+2
+    PsiWin w0, w1;
     {
-        const float t0 = t_row_load(w, x, ty0, zs + 1), t1 = t_row_load(w, x, ty1, zs + 1);
-        lt_compute(w, zs + 1, row0, lane, t0);
-        lt_compute(w, zs + 1, row1, lane, t1);
+        const F3 a0 = psi_row_load(w, x, py0, zs - 2), a1 = psi_row_load(w, x, py1, zs - 2);
+        const F3 b0 = psi_row_load(w, x, py0, zs - 1), b1 = psi_row_load(w, x, py1, zs - 1);
+        const F3 c0 = psi_row_load(w, x, py0, zs), c1 = psi_row_load(w, x, py1, zs);
+        psi_store(w, zs - 1, j0, lane, b0);
+        psi_store(w, zs - 1, j1, lane, b1);
+        __syncthreads();
+        float t0 = t_row_load(w, x, py0, zs - 1), t1 = t_row_load(w, x, py1, zs - 1);
+        lt_compute(w, zs - 1, j0, lane, t0, psi_win(a0, b0, c0));
+        lt_compute(w, zs - 1, j1, lane, t1, psi_win(a1, b1, c1));
+        const F3 d0 = psi_row_load(w, x, py0, zs + 1), d1 = psi_row_load(w, x, py1, zs + 1);
+        psi_store(w, zs, j0, lane, c0);
+        psi_store(w, zs, j1, lane, c1);
+        __syncthreads();
+        t0 = t_row_load(w, x, py0, zs);
+        t1 = t_row_load(w, x, py1, zs);
+        lt_compute(w, zs, j0, lane, t0, psi_win(b0, c0, d0));
+        lt_compute(w, zs, j1, lane, t1, psi_win(b1, c1, d1));
+        const F3 e0 = psi_row_load(w, x, py0, zs + 2), e1 = psi_row_load(w, x, py1, zs + 2);
+        psi_store(w, zs + 1, j0, lane, d0);
+        psi_store(w, zs + 1, j1, lane, d1);
+        __syncthreads();
+        t0 = t_row_load(w, x, py0, zs + 1);
+        t1 = t_row_load(w, x, py1, zs + 1);
+        lt_compute(w, zs + 1, j0, lane, t0, psi_win(c0, d0, e0));
+        lt_compute(w, zs + 1, j1, lane, t1, psi_win(c1, d1, e1));
+        const F3 f0 = psi_row_load(w, x, py0, zs + 3), f1 = psi_row_load(w, x, py1, zs + 3);
+        psi_store(w, zs + 2, j0, lane, e0);
+        psi_store(w, zs + 2, j1, lane, e1);
+        w0 = psi_win(d0, e0, f0);
+        w1 = psi_win(d1, e1, f1);
     }
-    float tn0 = t_row_load(w, x, ty0, zs + 2), tn1 = t_row_load(w, x, ty1, zs + 2);
+    float tn0 = t_row_load(w, x, py0, zs + 2), tn1 = t_row_load(w, x, py1, zs + 2);
 
     int i = x + d.X * y + XY * tc.zs;
     ColumnWindow c;
@@ -834,40 +857,45 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     for (int s = 0, z = tc.zs; s < nz; s += 2, z += 2, i += 2 * XY) {
         // (each half-step interleaves the next plane's loads with the current plane's arithmetic,
         // so that a wave keeps computing while its loads queue for the memory pipeline.  It first
-        // computes L/T of plane z+2 from psi planes z+1..z+3, then loads psi of plane z+4 (written
-        // into the ring at its end) and T of plane z+3 (used by the next half-step); the LDS writes
-        // complete before the barrier that precedes their readers.)
+        // computes L/T of plane z+2 from psi planes z+1..z+3 (registers; the y-neighbours of plane
+        // z+2 from the ring), then loads psi of plane z+4 (it enters the registers at the end) and
+        // T of plane z+3 (used by the next half-step), and writes psi of plane z+3 into the ring;
+        // the LDS writes complete before the barrier that precedes their readers.)
         const PlaneNb n0 = ys_read(w, z, jo, lane, c.v_0.y < 0.f);
-        lt_compute(w, z + 2, row0, lane, tn0);
-        lt_compute(w, z + 2, row1, lane, tn1);
+        lt_compute(w, z + 2, j0, lane, tn0, w0);
+        lt_compute(w, z + 2, j1, lane, tn1, w1);
         const F3 qa0 = psi_row_load(w, x, py0, z + 4), qa1 = psi_row_load(w, x, py1, z + 4);
-        tn0 = t_row_load(w, x, ty0, z + 3);
-        tn1 = t_row_load(w, x, ty1, z + 3);
+        tn0 = t_row_load(w, x, py0, z + 3);
+        tn1 = t_row_load(w, x, py1, z + 3);
         if (has_h) hB = yrow_load<ST>(w, ih + XY * (z + 2));
         const WindowIn<ST> in0 = window_in<ST>(w, i, XY);
         voxel_a<ST>(a, w, n0, c, i, z, lap_xy && z >= 2 && z <= d.Z - 3, own);
-        psi_store(w, z + 4, prow0, lane, qa0);
-        psi_store(w, z + 4, prow1, lane, qa1);
+        psi_store(w, z + 3, j0, lane, w0.p);
+        psi_store(w, z + 3, j1, lane, w1.p);
         ys_store(w, z + 1, jo, lane, c.v_p1, c.r_p1, c.p_p1, c.s_p1);
         if (has_h) ys_store(w, z + 1, jh, lane, hA.v, hA.r, hA.p, hA.s);
+        w0 = psi_win(w0.c, w0.p, qa0);
+        w1 = psi_win(w1.c, w1.p, qa1);
         slide<ST>(c, in0);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
         const PlaneNb n1 = ys_read(w, z + 1, jo, lane, c.v_0.y < 0.f);
-        lt_compute(w, z + 3, row0, lane, tn0);
-        lt_compute(w, z + 3, row1, lane, tn1);
+        lt_compute(w, z + 3, j0, lane, tn0, w0);
+        lt_compute(w, z + 3, j1, lane, tn1, w1);
         const F3 qb0 = psi_row_load(w, x, py0, z + 5), qb1 = psi_row_load(w, x, py1, z + 5);
-        tn0 = t_row_load(w, x, ty0, z + 4);
-        tn1 = t_row_load(w, x, ty1, z + 4);
+        tn0 = t_row_load(w, x, py0, z + 4);
+        tn1 = t_row_load(w, x, py1, z + 4);
         if (has_h) hA = yrow_load<ST>(w, ih + XY * (z + 3));
         const WindowIn<ST> in1 = window_in<ST>(w, i + XY, XY);
         voxel_a<ST>(a, w, n1, c, i + XY, z + 1, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, own && s + 1 < nz);
-        psi_store(w, z + 5, prow0, lane, qb0);
-        psi_store(w, z + 5, prow1, lane, qb1);
+        psi_store(w, z + 4, j0, lane, w0.p);
+        psi_store(w, z + 4, j1, lane, w1.p);
         ys_store(w, z + 2, jo, lane, c.v_p1, c.r_p1, c.p_p1, c.s_p1);
         if (has_h) ys_store(w, z + 2, jh, lane, hB.v, hB.r, hB.p, hB.s);
+        w0 = psi_win(w0.c, w0.p, qb0);
+        w1 = psi_win(w1.c, w1.p, qb1);
         slide<ST>(c, in1);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
