@@ -756,9 +756,10 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     w.XY4 = 4u * XY;
     const bool lap_xy = x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2;
     // The stages.  This wave holds psi rows j0 = wave + 1 (the block's rows) and j1 of the stages
-    // (rows 9, 10, 11, 0 from waves 0..3; waves 4..7 hold rows 9 and 10 as well) and computes L/T
-    // for both of them, writing the same values where two waves hold the same row, so that no wave
-    // branches.
+    // (rows 9, 10, 11, 0 from waves 0..3; waves 4..7 load rows 9 and 10 as well, so that the loads
+    // need no branch) and computes L/T for its own row and, on waves 0 and 1, for rows 9 and 10 (the
+    // prologue computes every row it loads, writing the same values where two waves hold one row).
+    // Waves 4..7 carry the y-stage's halo rows instead.
     __shared__ LTStage lt_stage;
     __shared__ PsiStage psi_stage;
     __shared__ YStage y_stage;
@@ -771,6 +772,8 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     const int j0 = wave + 1;
     const int j1 = wave < 3 ? TY_A + 1 + wave : wave == 3 ? 0 : TY_A + 1 + (wave & 1);
     const int py0 = w.yb - 1 + j0, py1 = w.yb - 1 + j1;  // the y of both rows (psi and T)
+    const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+    const bool has_l1 = wave_u < 2, has_p1 = wave_u < 4;
     const int zs = tc.zs;
     // prologue: L/T planes zs-1, zs, zs+1 from psi planes zs-2..zs+2, each psi plane written to the
     // ring before the L plane whose y-neighbours it holds; psi planes zs+1..zs+3 of both rows stay
@@ -863,7 +866,7 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         // the LDS writes complete before the barrier that precedes their readers.)
         const PlaneNb n0 = ys_read(w, z, jo, lane, c.v_0.y < 0.f);
         lt_compute(w, z + 2, j0, lane, tn0, w0);
-        lt_compute(w, z + 2, j1, lane, tn1, w1);
+        if (has_l1) lt_compute(w, z + 2, j1, lane, tn1, w1);
         const F3 qa0 = psi_row_load(w, x, py0, z + 4), qa1 = psi_row_load(w, x, py1, z + 4);
         tn0 = t_row_load(w, x, py0, z + 3);
         tn1 = t_row_load(w, x, py1, z + 3);
@@ -871,7 +874,7 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         const WindowIn<ST> in0 = window_in<ST>(w, i, XY);
         voxel_a<ST>(a, w, n0, c, i, z, lap_xy && z >= 2 && z <= d.Z - 3, own);
         psi_store(w, z + 3, j0, lane, w0.p);
-        psi_store(w, z + 3, j1, lane, w1.p);
+        if (has_p1) psi_store(w, z + 3, j1, lane, w1.p);
         ys_store(w, z + 1, jo, lane, c.v_p1, c.r_p1, c.p_p1, c.s_p1);
         if (has_h) ys_store(w, z + 1, jh, lane, hA.v, hA.r, hA.p, hA.s);
         w0 = psi_win(w0.c, w0.p, qa0);
@@ -883,7 +886,7 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         __builtin_amdgcn_s_barrier();
         const PlaneNb n1 = ys_read(w, z + 1, jo, lane, c.v_0.y < 0.f);
         lt_compute(w, z + 3, j0, lane, tn0, w0);
-        lt_compute(w, z + 3, j1, lane, tn1, w1);
+        if (has_l1) lt_compute(w, z + 3, j1, lane, tn1, w1);
         const F3 qb0 = psi_row_load(w, x, py0, z + 5), qb1 = psi_row_load(w, x, py1, z + 5);
         tn0 = t_row_load(w, x, py0, z + 4);
         tn1 = t_row_load(w, x, py1, z + 4);
@@ -891,7 +894,7 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         const WindowIn<ST> in1 = window_in<ST>(w, i + XY, XY);
         voxel_a<ST>(a, w, n1, c, i + XY, z + 1, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, own && s + 1 < nz);
         psi_store(w, z + 4, j0, lane, w0.p);
-        psi_store(w, z + 4, j1, lane, w1.p);
+        if (has_p1) psi_store(w, z + 4, j1, lane, w1.p);
         ys_store(w, z + 2, jo, lane, c.v_p1, c.r_p1, c.p_p1, c.s_p1);
         if (has_h) ys_store(w, z + 2, jh, lane, hB.v, hB.r, hB.p, hB.s);
         w0 = psi_win(w0.c, w0.p, qb0);
