@@ -535,17 +535,20 @@ __device__ __forceinline__ F3 l_global(const PassAView &w, int x, int y, int z) 
     return mul3(-1.f, s);
 }
 
-// L and T at psi_inv(p) (corner k) of the voxel at plane z.  The taps come from the stage; a lane
-// whose cell is not in it (a displacement of a voxel or more) and that stores its result computes
-// them from memory instead, in a branch that waits for its own loads only.
-__device__ __forceinline__ void lt_at(const PassAView &w, const Corner &k, int z, bool store, F3 &Lo, float &Tv) {
-    int cx = k.gx - w.xb, ry = k.gy - w.yb;
+// Whether the 8 taps of corner k (of the voxel at plane z) are all in the L/T stage: columns 1..62
+// (the end lanes lack an x-neighbour), rows y = yb .. yb + 9 of the corner and its +1 row (stage
+// rows 1..10), planes z-1..z+1.  A voxel displaced by a voxel or more fails it.
+__device__ __forceinline__ bool taps_in_stage(const PassAView &w, const Corner &k, int z) {
+    const int cx = k.gx - w.xb, ry = k.gy - w.yb, dz = k.gz - z;
+    return cx >= 1 && cx <= WAVE - 3 && ry >= 0 && ry <= LT_ROWS - 2 && (dz == -1 || dz == 0);
+}
+
+// L and T at psi_inv(p) (corner k) of the voxel at plane z from the stage, the indices clamped
+// into it (the values are those of the reference when taps_in_stage holds)
+__device__ __forceinline__ void lt_stage_taps(const PassAView &w, const Corner &k, int z, F3 &Lo, float &Tv) {
     const int dz = k.gz - z;
-    // columns 1..62 of the stage hold valid L (the end lanes lack an x-neighbour), and stage rows
-    // 1..10 (the corner's row y = yb + ry and the one above are stage rows ry + 1 and ry + 2)
-    const bool in = cx >= 1 && cx <= WAVE - 3 && ry >= 0 && ry <= LT_ROWS - 2 && (dz == -1 || dz == 0);
-    cx = min(max(cx, 1), WAVE - 3);
-    ry = min(max(ry, 0), LT_ROWS - 2) + 1;
+    const int cx = min(max(k.gx - w.xb, 1), WAVE - 3);
+    const int ry = min(max(k.gy - w.yb, 0), LT_ROWS - 2) + 1;  // the corner's stage row
     const int zg = dz == -1 ? z - 1 : z;
     const int s0 = zg & (LT_PLANES - 1), s1 = (zg + 1) & (LT_PLANES - 1);
     const LTStage &st = *w.lt;
@@ -556,19 +559,21 @@ __device__ __forceinline__ void lt_at(const PassAView &w, const Corner &k, int z
             tri(f111.y, f110.y, f101.y, f100.y, f011.y, f010.y, f001.y, f000.y, k.a, k.b, k.c),
             tri(f111.z, f110.z, f101.z, f100.z, f011.z, f010.z, f001.z, f000.z, k.a, k.b, k.c));
     Tv = tri(f111.w, f110.w, f101.w, f100.w, f011.w, f010.w, f001.w, f000.w, k.a, k.b, k.c);
-    if (store && !in) {
-        // the +1 corners are not clamped: where the reference clamps one its weight is exactly 0,
-        // and its L here is 0 (outside the interior), a finite value
-        const F3 l111 = l_global(w, k.gx + 1, k.gy + 1, k.gz + 1), l110 = l_global(w, k.gx + 1, k.gy + 1, k.gz),
-                 l101 = l_global(w, k.gx + 1, k.gy, k.gz + 1), l100 = l_global(w, k.gx + 1, k.gy, k.gz),
-                 l011 = l_global(w, k.gx, k.gy + 1, k.gz + 1), l010 = l_global(w, k.gx, k.gy + 1, k.gz),
-                 l001 = l_global(w, k.gx, k.gy, k.gz + 1), l000 = l_global(w, k.gx, k.gy, k.gz);
-        Lo = f3(tri(l111.x, l110.x, l101.x, l100.x, l011.x, l010.x, l001.x, l000.x, k.a, k.b, k.c),
-                tri(l111.y, l110.y, l101.y, l100.y, l011.y, l010.y, l001.y, l000.y, k.a, k.b, k.c),
-                tri(l111.z, l110.z, l101.z, l100.z, l011.z, l010.z, l001.z, l000.z, k.a, k.b, k.c));
-        const unsigned g = corner_index(w.d, k);
-        Tv = tri_at1(w.rt, 4u * g, 4u, w.X4, w.XY4, k);
-    }
+}
+
+// L and T at psi_inv(p) (corner k) from memory: L at each corner from psi by l_global.  The +1
+// corners are not clamped: where the reference clamps one its weight is exactly 0, and its L here
+// is 0 (outside the interior), a finite value.
+__device__ __forceinline__ void lt_memory_taps(const PassAView &w, const Corner &k, F3 &Lo, float &Tv) {
+    const F3 l111 = l_global(w, k.gx + 1, k.gy + 1, k.gz + 1), l110 = l_global(w, k.gx + 1, k.gy + 1, k.gz),
+             l101 = l_global(w, k.gx + 1, k.gy, k.gz + 1), l100 = l_global(w, k.gx + 1, k.gy, k.gz),
+             l011 = l_global(w, k.gx, k.gy + 1, k.gz + 1), l010 = l_global(w, k.gx, k.gy + 1, k.gz),
+             l001 = l_global(w, k.gx, k.gy, k.gz + 1), l000 = l_global(w, k.gx, k.gy, k.gz);
+    Lo = f3(tri(l111.x, l110.x, l101.x, l100.x, l011.x, l010.x, l001.x, l000.x, k.a, k.b, k.c),
+            tri(l111.y, l110.y, l101.y, l100.y, l011.y, l010.y, l001.y, l000.y, k.a, k.b, k.c),
+            tri(l111.z, l110.z, l101.z, l100.z, l011.z, l010.z, l001.z, l000.z, k.a, k.b, k.c));
+    const unsigned g = corner_index(w.d, k);
+    Tv = tri_at1(w.rt, 4u * g, 4u, w.X4, w.XY4, k);
 }
 
 // own-column z-windows; rho and psi_inv run one plane further ahead than they are used, so that
@@ -578,55 +583,53 @@ struct ColumnWindow {
     float r_m1, r_0, r_p1, r_p2, s_m1, s_0, s_p1;
 };
 
-// every lane of the wave runs this (the x-neighbours are exchanged between lanes); `store` is
-// false on halo lanes and on the voxel past the chunk
-template <int ST>
-__device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, const PlaneNb &n,
-                                        const ColumnWindow &c, int i, int z, bool lap, bool store) {
-    // x-neighbours of rho, v, psi_inv, S from the adjacent lanes
-    const float r_xp = lane_next(c.r_0), r_xm = lane_prev(c.r_0);
-    const F3 v_xp = lane_next3(c.v_0), v_xm = lane_prev3(c.v_0);
-    const F3 v_xpp = lane_next3(v_xp), v_xmm = lane_prev3(v_xm);
-    const F3 p_xp = lane_next3(c.p_0), p_xm = lane_prev3(c.p_0);
-    const float s_xp = lane_next(c.s_0), s_xm = lane_prev(c.s_0);
-    // y-neighbours of rho and v
-    const float r_yp = n.r_yp, r_ym = n.r_ym;
-    const F3 v_yp = n.v_yp, v_ym = n.v_ym, v_y2 = n.v_y2;
-
-    // 1. density (solver.cu:101-173)
-    const F3 vc = c.v_0;
-    const float rc = c.r_0;
-    const float t11 = (vc.x > 0.f) ? -vc.x * rc : -vc.x * r_xp;
-    const float vxm = v_xm.x;
-    const float t12 = (vxm > 0.f) ? vxm * r_xm : vxm * rc;
-    const float t21 = (vc.y > 0.f) ? -vc.y * rc : -vc.y * r_yp;
-    const float vym = v_ym.y;
-    const float t22 = (vym > 0.f) ? vym * r_ym : vym * rc;
-    const float t31 = (vc.z > 0.f) ? -vc.z * rc : -vc.z * c.r_p1;
-    const float vzm = c.v_m1.z;
-    const float t32 = (vzm > 0.f) ? vzm * c.r_m1 : vzm * rc;
-    const float div = ((((t11 + t12) + t21) + t22) + t31) + t32;
-    const float rho_n = q16<ST>(fmaf(a.alpha, div, rc));
-
-    // 2. J_inv and det() (vector_fields.cu:503-535)
-    const F3 jx = mul3(0.5f, sub3(p_xp, p_xm));
-    const F3 jy = mul3(0.5f, sub3(n.p_yp, n.p_ym));
-    const F3 jz = mul3(0.5f, sub3(c.p_p1, c.p_m1));
-    const float det = det_ref(jx.x, jy.x, jz.x, jx.y, jy.y, jz.y, jx.z, jy.z, jz.z);
-
-    // 3-4. L o psi_inv and T o psi_inv (vector_fields.cu:231-247, 81-99)
+// Everything one voxel's update reads: rho, v, psi_inv and S at the voxel and its stencil
+// neighbours (v at +-2 along x and z, and at y+-2 on the upwind side in y, chosen by v.y < 0), and
+// L and T sampled at psi_inv.
+struct VoxelIn {
+    float rc, r_xp, r_xm, r_yp, r_ym, r_zp, r_zm;
+    F3 vc, v_xp, v_xm, v_xpp, v_xmm, v_yp, v_ym, v_y2, v_zp, v_zm, v_zpp, v_zmm;
+    F3 pc, p_xp, p_xm, p_yp, p_ym, p_zp, p_zm;
+    float sc, s_xp, s_xm, s_yp, s_ym, s_zp, s_zm;
     F3 Lo;
     float Tv;
-    lt_at(w, corner(w.d, c.p_0.x, c.p_0.y, c.p_0.z), z, store, Lo, Tv);
+};
 
-    // 5. grad S and the potential gradient (vector_fields.cu:155-192, solver.cu:56-80)
-    const F3 gS = f3((s_xp - s_xm) * 0.5f, (n.s_yp - n.s_ym) * 0.5f, (c.s_p1 - c.s_m1) * 0.5f);
-    const float diff = c.s_0 - Tv;
-    const F3 nU = mul3(det, add3(mul3(diff, gS), mul3(a.w_reg, Lo)));
+// steps 1-8 and 10 of one iteration for one interior voxel (lap: inside the range of the upwind
+// velocity Laplacian), the values rounded to the storage format as they are computed
+template <int ST>
+__device__ __forceinline__ void voxel_math(const IterArgs &a, const VoxelIn &q, bool lap, float &rho_n, F3 &vn,
+                                           F3 &pinv_n) {
+    // 1. density (solver.cu:101-173)
+    const F3 vc = q.vc;
+    const float rc = q.rc;
+    const float t11 = (vc.x > 0.f) ? -vc.x * rc : -vc.x * q.r_xp;
+    const float vxm = q.v_xm.x;
+    const float t12 = (vxm > 0.f) ? vxm * q.r_xm : vxm * rc;
+    const float t21 = (vc.y > 0.f) ? -vc.y * rc : -vc.y * q.r_yp;
+    const float vym = q.v_ym.y;
+    const float t22 = (vym > 0.f) ? vym * q.r_ym : vym * rc;
+    const float t31 = (vc.z > 0.f) ? -vc.z * rc : -vc.z * q.r_zp;
+    const float vzm = q.v_zm.z;
+    const float t32 = (vzm > 0.f) ? vzm * q.r_zm : vzm * rc;
+    const float div = ((((t11 + t12) + t21) + t22) + t31) + t32;
+    rho_n = q16<ST>(fmaf(a.alpha, div, rc));
+
+    // 2. J_inv and det() (vector_fields.cu:503-535)
+    const F3 jx = mul3(0.5f, sub3(q.p_xp, q.p_xm));
+    const F3 jy = mul3(0.5f, sub3(q.p_yp, q.p_ym));
+    const F3 jz = mul3(0.5f, sub3(q.p_zp, q.p_zm));
+    const float det = det_ref(jx.x, jy.x, jz.x, jx.y, jy.y, jz.y, jx.z, jy.z, jz.z);
+
+    // 3-5. L o psi_inv and T o psi_inv (vector_fields.cu:231-247, 81-99; in q), grad S and the
+    // potential gradient (vector_fields.cu:155-192, solver.cu:56-80)
+    const F3 gS = f3((q.s_xp - q.s_xm) * 0.5f, (q.s_yp - q.s_ym) * 0.5f, (q.s_zp - q.s_zm) * 0.5f);
+    const float diff = q.sc - q.Tv;
+    const F3 nU = mul3(det, add3(mul3(diff, gS), mul3(a.w_reg, q.Lo)));
 
     // 6. (Dv)v (vector_fields.cu:352-404)
-    const F3 xp = v_xp, xm = v_xm, yp = v_yp, ym = v_ym;
-    const F3 zp = c.v_p1, zm = c.v_m1;
+    const F3 xp = q.v_xp, xm = q.v_xm, yp = q.v_yp, ym = q.v_ym;
+    const F3 zp = q.v_zp, zm = q.v_zm;
     // The reference's diagonal terms ((MX(u)^2 - MN(u)^2) + MN(w)^2) - MX(w)^2, MX = fmaxf(., 0),
     // MN = fminf(., 0), are sq(u) - sq(w) bit for bit, zero signs included: sq(u) is the first
     // difference exactly (+0 when it is zero), adding +0 = MN(w)^2 or subtracting +0 = MX(w)^2
@@ -647,10 +650,10 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     // (x, 0, z+1), inside the buffer.
     // evaluated on every lane and selected, so that the z-march has no branch
     const bool nx = vc.x < 0.f, ny = vc.y < 0.f, nz = vc.z < 0.f;
-    const F3 v_x2 = nx ? v_xpp : v_xmm;
+    const F3 v_x2 = nx ? q.v_xpp : q.v_xmm;
     const F3 Ax = nx ? v_x2 : vc, Bx = nx ? xp : xm, Cx = nx ? vc : v_x2;
-    const F3 Ay = ny ? v_y2 : vc, Cy = ny ? vc : v_y2;
-    const F3 Az = nz ? c.v_p2 : vc, Cz = nz ? vc : c.v_m2;
+    const F3 Ay = ny ? q.v_y2 : vc, Cy = ny ? vc : q.v_y2;
+    const F3 Az = nz ? q.v_zpp : vc, Cz = nz ? vc : q.v_zmm;
     const F3 vxx = add3(sub3(Ax, mul3(2.f, Bx)), Cx);
     const F3 vyy = add3(sub3(Ay, mul3(2.f, ym)), Cy);
     const F3 vzz = add3(sub3(Az, mul3(2.f, zm)), Cz);
@@ -663,17 +666,123 @@ __device__ __forceinline__ void voxel_a(const IterArgs &a, const PassAView &w, c
     const float qv = a.m2 / (fabsf(rho_n) + 1e-5f);
     const F3 po = mul3(qv, nU);
     const F3 up = mul3(a.alpha, add3(sub3(add3(fr, dvv), df), po));
-    const F3 vn = f3(q16<ST>(vc.x - up.x), q16<ST>(vc.y - up.y), q16<ST>(vc.z - up.z));
+    vn = f3(q16<ST>(vc.x - up.x), q16<ST>(vc.y - up.y), q16<ST>(vc.z - up.z));
 
     // 10. psi_inv -= alpha * (J_inv v_new) (solver.cu:263-279)
-    const F3 pc = c.p_0;
+    const F3 pc = q.pc;
     const F3 Jv = f3((jx.x * vn.x + jy.x * vn.y) + jz.x * vn.z, (jx.y * vn.x + jy.y * vn.y) + jz.y * vn.z,
                      (jx.z * vn.x + jy.z * vn.y) + jz.z * vn.z);
     const F3 ui = mul3(a.alpha, Jv);
+    pinv_n = f3(pc.x - ui.x, pc.y - ui.y, pc.z - ui.z);
+}
+
+// One voxel of the z-march: every lane of the wave runs it (the x-neighbours are exchanged between
+// lanes); `store` is false on halo lanes and on the voxel past the chunk.  The taps of L and T come
+// from the stage.  A voxel whose taps are not all there (a displacement of a voxel or more) is
+// reported (the return value) and its lane recomputes it from memory after the march, storing the
+// result over what the march stored for it (computed with clamped taps; the same thread writes the
+// same address later in program order).  The march itself thus has no branch and no register
+// pressure from that rare path, and its stores need no per-voxel predicate.
+template <int ST>
+__device__ __forceinline__ bool voxel_a(const IterArgs &a, const PassAView &w, const PlaneNb &n,
+                                        const ColumnWindow &c, int i, int z, bool lap, bool store) {
+    VoxelIn q;
+    // x-neighbours of rho, v, psi_inv, S from the adjacent lanes, y-neighbours from the LDS ring,
+    // z-neighbours from the column window
+    q.rc = c.r_0;
+    q.r_xp = lane_next(c.r_0);
+    q.r_xm = lane_prev(c.r_0);
+    q.r_yp = n.r_yp;
+    q.r_ym = n.r_ym;
+    q.r_zp = c.r_p1;
+    q.r_zm = c.r_m1;
+    q.vc = c.v_0;
+    q.v_xp = lane_next3(c.v_0);
+    q.v_xm = lane_prev3(c.v_0);
+    q.v_xpp = lane_next3(q.v_xp);
+    q.v_xmm = lane_prev3(q.v_xm);
+    q.v_yp = n.v_yp;
+    q.v_ym = n.v_ym;
+    q.v_y2 = n.v_y2;
+    q.v_zp = c.v_p1;
+    q.v_zm = c.v_m1;
+    q.v_zpp = c.v_p2;
+    q.v_zmm = c.v_m2;
+    q.pc = c.p_0;
+    q.p_xp = lane_next3(c.p_0);
+    q.p_xm = lane_prev3(c.p_0);
+    q.p_yp = n.p_yp;
+    q.p_ym = n.p_ym;
+    q.p_zp = c.p_p1;
+    q.p_zm = c.p_m1;
+    q.sc = c.s_0;
+    q.s_xp = lane_next(c.s_0);
+    q.s_xm = lane_prev(c.s_0);
+    q.s_yp = n.s_yp;
+    q.s_ym = n.s_ym;
+    q.s_zp = c.s_p1;
+    q.s_zm = c.s_m1;
+    const Corner k = corner(w.d, c.p_0.x, c.p_0.y, c.p_0.z);
+    const bool in = taps_in_stage(w, k, z);
+    lt_stage_taps(w, k, z, q.Lo, q.Tv);
+    float rho_n;
+    F3 vn, pn;
+    voxel_math<ST>(a, q, lap, rho_n, vn, pn);
     // a voxel outside the chunk (the odd step of the last trip) stores at an offset past every
     // buffer's range, which the buffer unit drops
     vr_store<ST>(w.wvr, i, store, vn, rho_n);
-    bst3_nt(w.wp, store ? 12u * i : NO_STORE, f3(pc.x - ui.x, pc.y - ui.y, pc.z - ui.z));
+    bst3_nt(w.wp, store ? 12u * i : NO_STORE, pn);
+    return store && !in;
+}
+
+// The same voxel from memory alone (the recovery after the march): the old state's stencil
+// neighbours, L at the 8 corners of psi_inv from psi, T gathered; the same arithmetic.
+template <int ST>
+__device__ __forceinline__ void voxel_a_memory(const IterArgs &a, const PassAView &w, int i, int z, bool lap) {
+    const int X = w.X, XY = w.XY;
+    VoxelIn q;
+    const VR<ST> r0 = vr_load<ST>(w.rvr, i);
+    q.vc = vr_v<ST>(r0);
+    q.rc = vr_r<ST>(r0);
+    const VR<ST> rxp = vr_load<ST>(w.rvr, i + 1), rxm = vr_load<ST>(w.rvr, i - 1), ryp = vr_load<ST>(w.rvr, i + X),
+                 rym = vr_load<ST>(w.rvr, i - X), rzp = vr_load<ST>(w.rvr, i + XY), rzm = vr_load<ST>(w.rvr, i - XY);
+    q.v_xp = vr_v<ST>(rxp);
+    q.r_xp = vr_r<ST>(rxp);
+    q.v_xm = vr_v<ST>(rxm);
+    q.r_xm = vr_r<ST>(rxm);
+    q.v_yp = vr_v<ST>(ryp);
+    q.r_yp = vr_r<ST>(ryp);
+    q.v_ym = vr_v<ST>(rym);
+    q.r_ym = vr_r<ST>(rym);
+    q.v_zp = vr_v<ST>(rzp);
+    q.r_zp = vr_r<ST>(rzp);
+    q.v_zm = vr_v<ST>(rzm);
+    q.r_zm = vr_r<ST>(rzm);
+    q.v_xpp = vr_v<ST>(v_load<ST>(w.rvr, i + 2));
+    q.v_xmm = vr_v<ST>(v_load<ST>(w.rvr, i - 2));
+    q.v_y2 = vr_v<ST>(v_load<ST>(w.rvr, q.vc.y < 0.f ? i + 2 * X : i - 2 * X));
+    q.v_zpp = vr_v<ST>(v_load<ST>(w.rvr, i + 2 * XY));
+    q.v_zmm = vr_v<ST>(v_load<ST>(w.rvr, i - 2 * XY));
+    q.pc = bld3(w.rp, 12u * i);
+    q.p_xp = bld3(w.rp, 12u * (i + 1));
+    q.p_xm = bld3(w.rp, 12u * (i - 1));
+    q.p_yp = bld3(w.rp, 12u * (i + X));
+    q.p_ym = bld3(w.rp, 12u * (i - X));
+    q.p_zp = bld3(w.rp, 12u * (i + XY));
+    q.p_zm = bld3(w.rp, 12u * (i - XY));
+    q.sc = bld1(w.rs, 4u * i);
+    q.s_xp = bld1(w.rs, 4u * (i + 1));
+    q.s_xm = bld1(w.rs, 4u * (i - 1));
+    q.s_yp = bld1(w.rs, 4u * (i + X));
+    q.s_ym = bld1(w.rs, 4u * (i - X));
+    q.s_zp = bld1(w.rs, 4u * (i + XY));
+    q.s_zm = bld1(w.rs, 4u * (i - XY));
+    lt_memory_taps(w, corner(w.d, q.pc.x, q.pc.y, q.pc.z), q.Lo, q.Tv);
+    float rho_n;
+    F3 vn, pn;
+    voxel_math<ST>(a, q, lap, rho_n, vn, pn);
+    vr_store<ST>(w.wvr, i, true, vn, rho_n);
+    bst3_nt(w.wp, 12u * i, pn);
 }
 
 // the window entries entering at the next step, loaded at the start of a step (before its stores,
@@ -857,6 +966,7 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     // values are never stored) and the odd step of the last trip drops its stores, so the memory
     // counter waits in the loop are the exact distances of one step of pipelining.
     const int nz = tc.ze - tc.zs;
+    int fb_lo = INT_MAX, fb_hi = -1;  // the first and last plane of this lane's voxels left to the recovery
     for (int s = 0, z = tc.zs; s < nz; s += 2, z += 2, i += 2 * XY) {
         // (each half-step interleaves the next plane's loads with the current plane's arithmetic,
         // so that a wave keeps computing while its loads queue for the memory pipeline.  It first
@@ -872,7 +982,9 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         tn1 = t_row_load(w, x, py1, z + 3);
         if (has_h) hB = yrow_load<ST>(w, ih + XY * (z + 2));
         const WindowIn<ST> in0 = window_in<ST>(w, i, XY);
-        voxel_a<ST>(a, w, n0, c, i, z, lap_xy && z >= 2 && z <= d.Z - 3, own);
+        const bool f0 = voxel_a<ST>(a, w, n0, c, i, z, lap_xy && z >= 2 && z <= d.Z - 3, own);
+        fb_lo = min(fb_lo, f0 ? z : INT_MAX);
+        fb_hi = f0 ? z : fb_hi;
         psi_store(w, z + 3, j0, lane, w0.p);
         if (has_p1) psi_store(w, z + 3, j1, lane, w1.p);
         ys_store(w, z + 1, jo, lane, c.v_p1, c.r_p1, c.p_p1, c.s_p1);
@@ -892,7 +1004,10 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         tn1 = t_row_load(w, x, py1, z + 4);
         if (has_h) hA = yrow_load<ST>(w, ih + XY * (z + 3));
         const WindowIn<ST> in1 = window_in<ST>(w, i + XY, XY);
-        voxel_a<ST>(a, w, n1, c, i + XY, z + 1, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3, own && s + 1 < nz);
+        const bool f1 = voxel_a<ST>(a, w, n1, c, i + XY, z + 1, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3,
+                                    own && s + 1 < nz);
+        fb_lo = min(fb_lo, f1 ? z + 1 : INT_MAX);
+        fb_hi = f1 ? z + 1 : fb_hi;
         psi_store(w, z + 4, j0, lane, w0.p);
         if (has_p1) psi_store(w, z + 4, j1, lane, w1.p);
         ys_store(w, z + 2, jo, lane, c.v_p1, c.r_p1, c.p_p1, c.s_p1);
@@ -904,6 +1019,15 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
+    }
+    // The recovery: this lane's voxels whose taps left the stage, recomputed from memory (the old
+    // state, which this pass does not write) and stored over the march's values for them.  They are
+    // rare (displacements of a voxel or more).
+    for (int zz = fb_lo; zz <= fb_hi; ++zz) {
+        const int ii = x + d.X * y + XY * zz;
+        const F3 p = bld3(w.rp, 12u * ii);
+        if (!taps_in_stage(w, corner(d, p.x, p.y, p.z), zz))
+            voxel_a_memory<ST>(a, w, ii, zz, lap_xy && zz >= 2 && zz <= d.Z - 3);
     }
 }
 
