@@ -86,6 +86,19 @@ struct fr_ctx {
     int ndet = 0;
     int16_t *lab_tmp = nullptr;
     unsigned long long *max_slot = nullptr;  // fr_voxel_max_energy
+    // the iteration loop of a run as a hipGraph, per buffer parity at its start (captured at the
+    // first run from that parity; dropped when the storage or the chunking changes)
+    hipGraph_t run_graph[2] = {nullptr, nullptr};
+    hipGraphExec_t run_exec[2] = {nullptr, nullptr};
+    unsigned long long *maxrun = nullptr;  // max_iter per-iteration maxima (the slots reduced)
+    // pinned host copies of the control word, the per-iteration maxima and the energies, written
+    // by the copies each run queues after its loop
+    struct RunHost {
+        frk::Ctl ctl;
+        unsigned long long *mp;
+        double *e;
+    };
+    RunHost *h_run = nullptr;  // hipHostMalloc'd: the struct, then max_iter maxima, 3 max_iter energies
     // host copies of the last run
     std::vector<float> h_max;
     std::vector<long long> h_argmax;
@@ -144,11 +157,22 @@ extern "C" int fr_params_default(fr_params *p) {
     return FR_OK;
 }
 
+static void drop_run_graphs(fr_ctx *c) {
+    for (int k = 0; k < 2; ++k) {
+        if (c->run_exec[k]) (void)hipGraphExecDestroy(c->run_exec[k]);
+        if (c->run_graph[k]) (void)hipGraphDestroy(c->run_graph[k]);
+        c->run_exec[k] = nullptr;
+        c->run_graph[k] = nullptr;
+    }
+}
+
 static void free_ctx(fr_ctx *c) {
     if (!c) return;
+    drop_run_graphs(c);
+    if (c->h_run) (void)hipHostFree(c->h_run);
     void *bufs[] = {c->T, c->S, c->LT, c->LS, c->rho[0], c->rho[1], c->v[0], c->v[1], c->pinv[0], c->pinv[1], c->psi,
                     c->ctl, c->maxpack, c->energy, c->partials, c->dice_counts, c->det_parts, c->lab_tmp,
-                    c->vr[0], c->vr[1], c->max_slot};
+                    c->vr[0], c->vr[1], c->max_slot, c->maxrun};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -247,9 +271,20 @@ extern "C" int fr_create(const fr_params *p, fr_ctx **out) {
         (rc = dalloc(&c->psi, 3 * n)) || (rc = dalloc(&c->ctl, 1)) || (rc = dalloc(&c->maxpack, (size_t)mi * frk::NSLOT)) ||
         (rc = dalloc(&c->energy, 3 * (size_t)mi)) || (rc = dalloc(&c->partials, 2 * (size_t)c->nparts)) ||
         (rc = dalloc(&c->dice_counts, 2 * (size_t)p->no_labels + 1)) || (rc = dalloc(&c->det_parts, 2 * (size_t)c->ndet)) ||
-        (rc = dalloc(&c->lab_tmp, n)) || (rc = dalloc(&c->max_slot, 1))) {
+        (rc = dalloc(&c->lab_tmp, n)) || (rc = dalloc(&c->max_slot, 1)) || (rc = dalloc(&c->maxrun, (size_t)mi))) {
         free_ctx(c);
         return rc;
+    }
+    {
+        void *h = nullptr;
+        const size_t hb = sizeof(fr_ctx::RunHost) + sizeof(unsigned long long) * mi + sizeof(double) * 3 * (size_t)mi;
+        if (hipHostMalloc(&h, hb, hipHostMallocDefault) != hipSuccess) {
+            free_ctx(c);
+            return fail(FR_ERR_ALLOC, "hipHostMalloc(%zu bytes) failed", hb);
+        }
+        c->h_run = static_cast<fr_ctx::RunHost *>(h);
+        c->h_run->mp = reinterpret_cast<unsigned long long *>(c->h_run + 1);
+        c->h_run->e = reinterpret_cast<double *>(c->h_run->mp + mi);
     }
     hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
@@ -290,10 +325,12 @@ extern "C" int fr_set_stream(fr_ctx *c, void *s) {
 extern "C" int fr_set_chunking(fr_ctx *c, int zc_a, int zc_b) {
     if (!c) return fail(FR_ERR_INVALID, "null context");
     if (zc_a < 0 || zc_b < 0) return fail(FR_ERR_INVALID, "z-chunk sizes must be >= 0 (0 = automatic)");
+    HIP_TRY(hipStreamSynchronize(c->stream));
     c->zc[1] = zc_a ? zc_a : c->zc_auto[1];
     c->zc[2] = zc_b ? zc_b : c->zc_auto[2];
     c->zc_user[1] = zc_a != 0;
     c->zc_user[2] = zc_b != 0;
+    drop_run_graphs(c);  // the captured launches have the old tile grids
     return FR_OK;
 }
 
@@ -317,6 +354,7 @@ extern "C" int fr_set_storage(fr_ctx *c, int storage) {
     }
     c->st = storage == FR_STORAGE_BF16 ? frk::ST_BF16 : storage == FR_STORAGE_F16 ? frk::ST_F16 : frk::ST_F32;
     ++c->storage_gen;  // a batch laid out before this call holds the old buffers and kernels
+    drop_run_graphs(c);  // and so do the context's own captured runs
     if (int rc = auto_chunking(c)) return rc;  // the automatic chunking depends on the storage
     c->initialised = false;  // the state must be initialised in the new storage
     c->have_run = c->have_final = false;
@@ -458,21 +496,58 @@ static void logf_(std::string &s, const char *fmt, ...) {
     s += buf;
 }
 
-extern "C" int fr_solver_run(fr_ctx *c) {
-    if (!c) return fail(FR_ERR_INVALID, "null context");
-    if (!c->initialised) return fail(FR_ERR_STATE, "fr_solver_run before fr_solver_init");
-    int rc = reset_control(c);
-    if (rc) return rc;
+// the launches of a run's iteration loop from the current buffer parity
+static int run_launches(fr_ctx *c) {
     const fr_params &p = c->prm;
-    c->run_start_buf = c->cur;  // ping-pong index at the start of the run
-    HIP_TRY(hipEventRecord(c->ev0, c->stream));
     float alpha = p.alpha;
     float t = alpha;
     for (int iter = 1; iter <= p.max_iter; ++iter) {
-        if ((rc = launch_iteration(c, iter, t, verbose_now(p, iter)))) return rc;
+        if (int rc = launch_iteration(c, iter, t, verbose_now(p, iter))) return rc;
         // the t schedule of solver.cu:428-432
         t += alpha;
         if (p.reset_iter > 0 && (iter + 1) % p.reset_iter == 0) t = p.alpha;
+    }
+    return FR_OK;
+}
+
+extern "C" int fr_solver_run(fr_ctx *c) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (!c->initialised) return fail(FR_ERR_STATE, "fr_solver_run before fr_solver_init");
+    int rc = reset_control(c);  // eager memsets: no memset nodes in the captured loop
+    if (rc) return rc;
+    const fr_params &p = c->prm;
+    const int par = c->cur;
+    c->run_start_buf = par;  // ping-pong index at the start of the run
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(c->stream, &cs));
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    if (cs != hipStreamCaptureStatusNone) {
+        // the caller is capturing this stream: the launches go into the caller's graph
+        if ((rc = run_launches(c))) return rc;
+    } else {
+        // The loop is the same sequence of launches on every run from one buffer parity: it is
+        // captured once into a hipGraph and replayed (one launch instead of 2 per iteration).
+        if (!c->run_exec[par]) {
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            rc = run_launches(c);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(c->stream, &g);
+            c->cur = par;
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(FR_ERR_HIP, "capture of the iteration loop: %s", hipGetErrorString(e));
+            const hipError_t ei = hipGraphInstantiate(&c->run_exec[par], g, nullptr, nullptr, 0);
+            if (ei != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                c->run_exec[par] = nullptr;
+                return fail(FR_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+            }
+            c->run_graph[par] = g;
+        }
+        HIP_TRY(hipGraphLaunch(c->run_exec[par], c->stream));
+        c->cur = par ^ (p.max_iter & 1);  // as the launches leave it
     }
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     c->have_run = true;
@@ -481,21 +556,25 @@ extern "C" int fr_solver_run(fr_ctx *c) {
     return FR_OK;
 }
 
-// collect the run's history and replay the reference's messages (solver.cu:330-426)
+// collect the run's history and replay the reference's messages (solver.cu:330-426): the slots
+// of each iteration reduced on the device, then the control word, maxima and energies copied into
+// pinned host memory
 static int collect_run(fr_ctx *c) {
     const fr_params &p = c->prm;
     int mi = p.max_iter > 0 ? p.max_iter : 1;
-    frk::Ctl ctl{};
-    std::vector<unsigned long long> mps((size_t)mi * frk::NSLOT), mp(mi);
-    std::vector<double> e(3 * (size_t)mi);
-    HIP_TRY(hipMemcpyAsync(&ctl, c->ctl, sizeof ctl, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(mps.data(), c->maxpack, sizeof(unsigned long long) * mps.size(), hipMemcpyDeviceToHost,
-                           c->stream));
-    HIP_TRY(hipMemcpyAsync(e.data(), c->energy, sizeof(double) * 3 * mi, hipMemcpyDeviceToHost, c->stream));
+    if (p.max_iter > 0) {
+        HIP_TRY(frk::launch_slots_reduce(c->maxpack, p.max_iter, c->maxrun, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->h_run->mp, c->maxrun, sizeof(unsigned long long) * mi, hipMemcpyDeviceToHost,
+                               c->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(&c->h_run->ctl, c->ctl, sizeof(frk::Ctl), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->h_run->e, c->energy, sizeof(double) * 3 * mi, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    for (int k = 0; k < mi; ++k) mp[k] = slots_max(&mps[(size_t)k * frk::NSLOT]);
+    const frk::Ctl ctl = c->h_run->ctl;
+    const unsigned long long *mp = c->h_run->mp;
+    const double *e = c->h_run->e;
 
     int done = p.max_iter, reason = 0;
     if (ctl.stopped) {

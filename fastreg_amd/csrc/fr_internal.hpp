@@ -124,5 +124,7 @@ hipError_t launch_vr_get(int st, const unsigned *r, float *v4, float *rho, int n
 hipError_t launch_unpack3to4(const float *src3, float *dst4, int n, hipStream_t s);
 // (x, y, z, 0) at every voxel of a float4 field
 hipError_t launch_identity4(Dims d, float *dst4, hipStream_t s);
+// out[k] = the maximum of iteration k's NSLOT slots, k < iters
+hipError_t launch_slots_reduce(const unsigned long long *slots, int iters, unsigned long long *out, hipStream_t s);
 
 }  // namespace frk

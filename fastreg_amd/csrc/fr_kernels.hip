@@ -1388,6 +1388,15 @@ __global__ void k_pack4to3(const float *__restrict__ src4, float *dst3a, float *
     }
 }
 
+// the maximum of each iteration's NSLOT slots: one wave per iteration
+__global__ __launch_bounds__(256) void k_slots_reduce(const unsigned long long *__restrict__ slots, int iters,
+                                                      unsigned long long *__restrict__ out) {
+    const int k = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (k >= iters) return;  // uniform over the wave
+    const unsigned long long m = slots_max(slots + (size_t)k * NSLOT);
+    if ((threadIdx.x & 63) == 0) out[k] = m;
+}
+
 // init_identity_kernel (vector_fields.cu:64-79): (x, y, z, 0) at voxel (x, y, z)
 __global__ void k_identity4(Dims d, float *__restrict__ dst4) {
     const int n = d.X * d.Y * d.Z;
@@ -1725,6 +1734,12 @@ hipError_t launch_vr_get(int st, const unsigned *r, float *v4, float *rho, int n
 
 hipError_t launch_pack4to3(const float *src4, float *dst3a, float *dst3b, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_pack4to3, dim3(grid_for(n, 256)), dim3(256), 0, s, src4, dst3a, dst3b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_slots_reduce(const unsigned long long *slots, int iters, unsigned long long *out, hipStream_t s) {
+    if (iters <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_slots_reduce, dim3((iters + 3) / 4), dim3(256), 0, s, slots, iters, out);
     return hipGetLastError();
 }
 
