@@ -1,17 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark: voxel-iterations/s of the registration solver on one MI355X.
 
-Workload: the plain FP32 method on the brain-sized phantom, 182 x 218 x 182 voxels, the
-reference's params.ini weights (alpha 0.004, tau 0.2, w_reg 0.2, rho_0 1, p 2, C 0.25), 32 labels.
-A "step" is one solver iteration over the whole volume; `--steps` iterations (default 300, the
-workload's iteration count) are timed with HIP events around the iteration loop, right after
-`--warmup` untimed iterations (default 150, ~28 ms) on a separate context: pass A runs ~15 %
-slower during the first ~25 ms of continuous work on an idle GPU.  Inputs are resident in HBM
-before timing.
+Workload (default, `--workload plain`): the plain FP32 method on the brain-sized phantom,
+182 x 218 x 182 voxels, 300 iterations, the reference's params.ini weights (alpha 0.004, tau 0.2,
+w_reg 0.2, rho_0 1, p 2, C 0.25), 32 labels.  A "step" is one whole registration of that pair:
+a fresh state from the inputs (already resident in HBM), the 300-iteration loop (a replayed
+hipGraph), and the finalisation (both warps into output buffers, Dice and min det J), i.e. what
+FastReg::operator() does on the device.  `--warmup` registrations run untimed, then `--steps`
+registrations are timed on the host clock between device synchronisations; `value` is voxels x
+300 x steps over that time.  The loop alone is also timed with HIP events (`loop_*`).
+
+`--workload c1`: the same on the 64^3 blob pair, 200 iterations per registration (9 labels).
 
 Printed: one JSON line with the metric, the roofline of the dominant kernel (algorithmic bytes
 per launch / its average launch time from HIP events) and a CPU baseline (the oracle, the
-reference's kernels restated in C and run on this host's cores over a bounded sample).
+reference's kernels restated in C and run on this host's cores: the median of 3 runs over a
+bounded sample).
 """
 from __future__ import annotations
 
@@ -40,19 +44,58 @@ def profiled_traffic():
     return None
 
 
+def host_cpu():
+    """The host's CPU model and logical CPU count (what the CPU baseline ran on)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
+def cpu_baseline_sample(pair, iters, runs=3):
+    """The oracle (FP32, the reference's operation order, OpenMP over z) on `iters` iterations of the
+    pair, `runs` times: voxel-iterations/s of the median run."""
+    import numpy as np
+    from oracle import oracle as O
+
+    O.build()
+    dims = pair.dims
+    n = dims[0] * dims[1] * dims[2]
+    times = []
+    for _ in range(runs):
+        o = O.Oracle(dims, "f32")
+        o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+        tc = time.perf_counter()
+        o.run(O.SolverParams(max_iter=iters, no_labels=pair.no_labels))
+        times.append(time.perf_counter() - tc)
+        o.close()
+    med = float(np.median(times))
+    return {"value": round(n * iters / med, 1), "unit": "voxel-iterations/s", "cores": O.num_threads(),
+            "kind": "port", "runs_s": [round(t, 3) for t in times], "host": host_cpu(),
+            "sample": f"oracle FP32 reference-order restatement, {iters} iterations of the same {dims[0]}x{dims[1]}x"
+                      f"{dims[2]} pair per run, median of {runs} runs, OpenMP over z"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=150)
+    ap.add_argument("--steps", type=int, default=10, help="timed registrations (plain, c1)")
+    ap.add_argument("--warmup", type=int, default=3, help="untimed registrations before them")
     ap.add_argument("--kernel-reps", type=int, default=50)
-    ap.add_argument("--cpu-iters", type=int, default=120,
-                    help="oracle iterations for the CPU baseline sample (about 10-20 s on 16 host cores)")
+    ap.add_argument("--cpu-iters", type=int, default=40,
+                    help="oracle iterations per CPU-baseline run at 182x218x182 (3 runs, about 4 s each on "
+                         "16 host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--zc", default="0,0",
                     help="interior z-slices per block of pass A, pass B (0 = automatic)")
-    ap.add_argument("--workload", choices=("plain", "batch", "precision", "adjoint"), default="plain",
-                    help="plain: the single-pair headline metric (default); batch: 64 pairs of uneven sizes, "
+    ap.add_argument("--workload", choices=("plain", "c1", "batch", "precision", "adjoint"), default="plain",
+                    help="plain: the single-pair headline metric at 182x218x182 (default); c1: the same on the "
+                         "64^3 blob pair, 200 iterations; batch: 64 pairs of uneven sizes, "
                          "150 iterations each, batched form vs one pair at a time; precision: FP32, BF16 and "
                          "FP16 storage of v and rho against an FP64 oracle run (64^3, 300 iterations) and their "
                          "throughput at 182x218x182; adjoint: the differentiated solver at 128^3, K unrolled "
@@ -85,13 +128,22 @@ def main():
         return precision_workload(args)
     if args.workload == "adjoint":
         return adjoint_workload(args)
-    zc = tuple(int(v) for v in args.zc.split(","))
+    return registration_workload(args)
+
+
+def registration_workload(args):
+    """The plain (182x218x182, 300 iterations) or C1 (64^3, 200 iterations) registration, `--steps`
+    registrations timed after `--warmup` untimed ones, inputs resident in HBM."""
+    import ctypes as C
     import numpy as np
     import fastreg_amd as fr
     from fastreg_amd import phantoms
 
+    zc = tuple(int(v) for v in args.zc.split(","))
+    c1 = args.workload == "c1"
+    iters = 200 if c1 else 300
     t0 = time.time()
-    pair = phantoms.brain()
+    pair = phantoms.blobs() if c1 else phantoms.brain()
     dims = pair.dims
     X, Y, Z = dims
     n = X * Y * Z
@@ -108,36 +160,42 @@ def main():
         src.free()
         dst.free()
 
-    # Two contexts, both initialised before any iteration runs: the warm-up iterations on one, then
-    # at once the timed run on the other, so that the GPU does not idle (and clock down) between
-    # them while a context is created and its inputs uploaded.
-    pw = fr.default_params(vol_dims=dims, no_labels=pair.no_labels, max_iter=max(args.warmup, 1))
-    w = fr.Solver(pw)
-    w.set_chunking(*zc)
-    w.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
-    p = fr.default_params(vol_dims=dims, no_labels=pair.no_labels, max_iter=args.steps)
+    L = fr.lib()
+    p = fr.default_params(vol_dims=dims, no_labels=pair.no_labels, max_iter=iters)
     s = fr.Solver(p)
     s.set_chunking(*zc)
     chunking = s.chunking()
-    s.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
-    fr.check(fr.lib().fr_dev_sync())
-    if args.warmup > 0:
-        w.run()
-        fr.check(fr.lib().fr_dev_sync())  # (its statistics are not read: that copy would idle the GPU)
+    s.init(pair.target, pair.target_labels, pair.source, pair.source_labels)  # inputs now resident
+    outs = [fr.DeviceArray(n, np.float32), fr.DeviceArray(n, np.int16), fr.DeviceArray(n, np.float32),
+            fr.DeviceArray(n, np.int16)]
+    st = fr.FrStats()
+
+    def registration():
+        """one registration from the resident inputs; returns its loop time from HIP events (ms)"""
+        s.reinit()
+        fr.check(L.fr_solver_run(s.h))
+        fr.check(L.fr_solver_finalize(s.h, *[b.ptr for b in outs]))  # waits for the outputs
+        fr.check(L.fr_get_stats(s.h, C.byref(st)))
+        assert st.iterations == iters, st.iterations
+        return st.solver_ms
+
+    for _ in range(max(args.warmup, 0)):
+        registration()
+    fr.check(L.fr_dev_sync())
     tw0 = time.perf_counter()
-    s.run()
-    st = s.stats()                      # synchronises
-    wall_ms = (time.perf_counter() - tw0) * 1e3
-    ms = st.solver_ms
-    assert st.iterations == args.steps, st
-    value = n * args.steps / (ms / 1e3)
-    print(f"[bench] {args.steps} iterations: {ms:.2f} ms (events), {wall_ms:.2f} ms (wall)", file=sys.stderr,
-          flush=True)
+    loop_ms = [registration() for _ in range(args.steps)]
+    fr.check(L.fr_dev_sync())
+    wall_s = time.perf_counter() - tw0
+    ms_step = 1e3 * wall_s / args.steps
+    value = n * iters * args.steps / wall_s
+    loop_med = float(np.median(loop_ms))
+    print(f"[bench] {args.steps} registrations of {iters} iterations: {ms_step:.2f} ms each (wall), loop "
+          f"{loop_med:.2f} ms (events, median)", file=sys.stderr, flush=True)
+    dice, min_det = st.dice_all, st.min_det_true
 
     # per-kernel durations: HIP events between the launches of iterations run in sequence on the
     # context stream (the caches as the iteration loop leaves them)
     ms_a, ms_b = s.time_kernels(args.kernel_reps)
-    w.close()
     # algorithmic bytes per interior voxel: pass A reads rho 4, v 12, psi_inv 12, S 4, psi 12 (for
     # -Laplacian(psi), formed in the kernel) and T 4 (sampled at psi_inv) and writes rho 4, v 12,
     # psi_inv 12; pass B reads psi 12 and v 12 (gathered at psi) and writes psi 12
@@ -147,50 +205,44 @@ def main():
     dom = max(kern, key=lambda k: kern[k][0])
     dms, dbytes = kern[dom]
     achieved = dbytes / (dms / 1e3) / 1e9
-    prof = profiled_traffic()
+    prof = None if c1 else profiled_traffic()
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": prof.get(dom) if prof else None,
                 "traffic_source": os.path.relpath(PROFILE, ROOT) if prof else None,
                 "kernel_ms": {"pass_a": round(ms_a, 4), "pass_b": round(ms_b, 4)},
                 "alg_bytes_per_launch": {"pass_a": bytes_a, "pass_b": bytes_b},
-                "iteration_frac_88B": round(value * 88 / (HBM_PEAK_GBS * 1e9), 4)}
-    fin = s.finalize(want_outputs=False)
-    st2 = s.stats()
+                "iteration_frac_88B": round(n * 1e3 / (loop_med / iters) * 88 / (HBM_PEAK_GBS * 1e9), 4)}
+    for b in outs:
+        b.free()
     s.close()
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        O.build()
-        o = O.Oracle(dims, "f32")
-        o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
-        tc = time.perf_counter()
-        o.run(O.SolverParams(max_iter=args.cpu_iters, no_labels=pair.no_labels))
-        cs = time.perf_counter() - tc
-        o.close()
-        cpu = {"value": round(n * args.cpu_iters / cs, 1), "unit": "voxel-iterations/s", "cores": O.num_threads(),
-               "kind": "port",
-               "sample": f"oracle FP32 reference-order restatement, {args.cpu_iters} iterations of the same "
-                         f"182x218x182 workload, OpenMP over z ({cs:.1f} s)"}
+        cpu = cpu_baseline_sample(pair, iters if c1 else args.cpu_iters)
 
+    name = f"{X}x{Y}x{Z}"
     out = {
-        "metric": "voxel-iterations/s at 182x218x182 fp32",
+        "metric": f"voxel-iterations/s at {name} fp32",
         "value": round(value, 1),
         "unit": "voxel-iterations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms / args.steps, 5),
+        "ms_per_step": round(ms_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded ellipsoid-shell phantom with 32 Voronoi labels; source warped by an RBF "
-                "velocity exponential)",
-        "config": {"workload": "plain solver, 182x218x182 fp32, params.ini weights", "dims": list(dims),
-                   "iterations": args.steps, "voxels": n, "dice_all": round(st2.dice_all, 6),
-                   "min_det_true": st2.min_det_true, "wall_ms": round(wall_ms, 2),
+        "data": ("synthetic (seeded blob phantom with 8 Voronoi labels; source warped by an RBF velocity exponential)"
+                 if c1 else "synthetic (seeded ellipsoid-shell phantom with 32 Voronoi labels; source warped by an "
+                            "RBF velocity exponential)"),
+        "config": {"workload": f"plain solver, {name} fp32, {iters} iterations, params.ini weights; a step is one "
+                               f"registration (fresh state, loop, warps, Dice, min det J)",
+                   "dims": list(dims), "iterations": iters, "voxels": n, "dice_all": round(dice, 6),
+                   "min_det_true": min_det, "loop_ms_median": round(loop_med, 3),
+                   "loop_us_per_iteration": round(1e3 * loop_med / iters, 2),
+                   "loop_voxel_iterations_per_s": round(n * iters / (loop_med / 1e3), 1),
                    "z_chunk": {"pass_a": chunking[0], "pass_b": chunking[1]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
