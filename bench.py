@@ -260,7 +260,8 @@ def precision_workload(args):
     the throughput of each storage at 182x218x182 (300 iterations, HIP events) and its kernel times.
     (With this phantom and step the displacements stay below 1e-3 voxel, a few FP32 ulps of the
     absolute coordinates psi holds, so FP32 itself is far from FP64 in psi and, through the floor
-    of the label warp, in Dice.)"""
+    of the label warp, in Dice; `label_flips_vs_fp64` counts the labels that differ from FP64's and
+    how many of them that floor explains.)"""
     import numpy as np
     import fastreg_amd as fr
     from fastreg_amd import phantoms
@@ -276,7 +277,7 @@ def precision_workload(args):
     o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
     r = o.run(O.SolverParams(max_iter=iters, verbosity=2, no_labels=pair.no_labels))
     ref = o.get_state()
-    o.outputs()
+    _, ref_lab1, _, ref_lab2 = o.outputs()
     ov, vx, _ = o.dice_counts(pair.no_labels)
     ref_dice = 2.0 * ov[1:].sum() / vx[1:].sum()
     ref_det = o.min_det()[0]
@@ -292,6 +293,23 @@ def precision_workload(args):
         b = np.asarray(b, np.float64)
         return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
+    def label_flips(p32, lab32, p64, lab64):
+        """Voxels whose warped label differs from the FP64 run's, and how many of them the floor of
+        the label warp explains: a component where the FP64 coordinate lies below the grid point
+        and this run's does not (the FP64 lookup takes the neighbour below), and among those, how
+        many hold no displacement at all in this run (psi == id exactly: every increment was below
+        half an ulp of the coordinate and was rounded away)."""
+        p32 = np.asarray(p32, np.float64)[:, :3]
+        p64 = np.asarray(p64, np.float64)[:, :3]
+        id3 = ident[:, :3]
+        diff = np.asarray(lab32).reshape(-1) != np.asarray(lab64).reshape(-1)
+        below = ((p64 < id3) & (p32 >= id3)).any(1)
+        exact = (p32 == id3).all(1)
+        return {"labels_differ": int(diff.sum()), "explained_by_floor_below_grid": int((diff & below).sum()),
+                "of_which_no_displacement_held": int((diff & below & exact).sum()),
+                "median_fp64_displacement_there": float(np.median(np.abs(p64 - id3).max(1)[diff])) if diff.any()
+                else 0.0}
+
     names = {fr.STORAGE_F32: "fp32", fr.STORAGE_BF16: "bf16", fr.STORAGE_F16: "fp16"}
     errors, states = {}, {}
     for stg, nm in names.items():
@@ -303,7 +321,7 @@ def precision_workload(args):
         st = s.stats()
         _, eh = s.history()
         g = s.get_state()
-        s.finalize(want_outputs=False)
+        _, lab1, _, lab2 = s.finalize()
         st = s.stats()
         eh = np.asarray(eh[:iters], np.float64)
         errors[nm] = {
@@ -314,6 +332,9 @@ def precision_workload(args):
             "energy_max_rel_dev": float(np.max(np.abs(eh - ref_e) / np.abs(ref_e))),
             "dice_all": round(st.dice_all, 6),
             "min_det_true": round(st.min_det_true, 6),
+            # the Dice gap to FP64, voxel by voxel: source labels o psi and target labels o psi_inv
+            "label_flips_vs_fp64": {"source_o_psi": label_flips(g["psi"], lab2, ref["psi"], ref_lab2),
+                                    "target_o_psi_inv": label_flips(g["psi_inv"], lab1, ref["psi_inv"], ref_lab1)},
         }
         states[nm] = g
         if nm != "fp32":  # what the storage alone changes: against the FP32 form

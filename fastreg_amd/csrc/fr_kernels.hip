@@ -24,6 +24,7 @@
 
 #include <climits>
 #include <cstddef>
+#include <type_traits>
 #include <math.h>
 
 namespace frk {
@@ -401,10 +402,18 @@ struct YStage {
     float4 vr[YS_PLANES][YS_ROWS][WAVE];
     float4 ps[YS_PLANES][YS_ROWS][WAVE];
 };
+// a halo row as loaded: FP32 values, or in the 16-bit forms the (v, rho) record, unpacked when the
+// row is written to the stage a step after its load, so that the unpacking does not wait for it
 struct YRow {
     F3 v, p;
     float r, s;
 };
+template <int ST> struct YRow16 {
+    VR<ST> vr;
+    F3 p;
+    float s;
+};
+template <int ST> using YRowOf = typename std::conditional<ST == ST_F32, YRow, YRow16<ST>>::type;
 
 // L = -Laplacian(psi) and T on the planes around the one being computed, staged in LDS for the
 // whole block: a ring of 4 planes (z-1, z, z+1 are read while z+2 is written) of the 12 rows
@@ -444,11 +453,15 @@ struct PassAView {
     YStage *ys;           // row j holds y = yb - 1 + j
 };
 
-template <int ST> __device__ __forceinline__ YRow yrow_load(const PassAView &w, int i) {
+template <int ST> __device__ __forceinline__ YRowOf<ST> yrow_load(const PassAView &w, int i) {
     const VR<ST> q = vr_load<ST>(w.rvr, i);
-    YRow h;
-    h.v = vr_v<ST>(q);
-    h.r = vr_r<ST>(q);
+    YRowOf<ST> h;
+    if constexpr (ST == ST_F32) {
+        h.v = vr_v<ST>(q);
+        h.r = vr_r<ST>(q);
+    } else {
+        h.vr = q;
+    }
     h.p = bld3(w.rp, 12u * i);
     h.s = bld1(w.rs, 4u * i);
     return h;
@@ -456,6 +469,11 @@ template <int ST> __device__ __forceinline__ YRow yrow_load(const PassAView &w, 
 __device__ __forceinline__ void ys_store(const PassAView &w, int zz, int row, int lane, F3 v, float r, F3 p, float s) {
     w.ys->vr[zz & (YS_PLANES - 1)][row][lane] = make_float4(v.x, v.y, v.z, r);
     w.ys->ps[zz & (YS_PLANES - 1)][row][lane] = make_float4(p.x, p.y, p.z, s);
+}
+template <int ST> __device__ __forceinline__ void ys_store_row(const PassAView &w, int zz, int row, int lane,
+                                                              const YRowOf<ST> &h) {
+    if constexpr (ST == ST_F32) ys_store(w, zz, row, lane, h.v, h.r, h.p, h.s);
+    else ys_store(w, zz, row, lane, vr_v<ST>(h.vr), vr_r<ST>(h.vr), h.p, h.s);
 }
 // the y-neighbours of plane zz for the wave whose row is j, the upwind side chosen by vneg (v.y < 0)
 __device__ __forceinline__ PlaneNb ys_read(const PassAView &w, int zz, int j, int lane, bool vneg) {
@@ -801,16 +819,20 @@ template <int ST> __device__ __forceinline__ WindowIn<ST> window_in(const PassAV
     return n;
 }
 
-template <int ST> __device__ __forceinline__ void slide(ColumnWindow &c, const WindowIn<ST> &n) {
+// The newest (v, rho) entry: FP32 values enter the window at once; a 16-bit record is kept as
+// loaded (pend) and unpacked at the start of the next step (take), a whole step after its load, so
+// that the unpacking at the end of a step does not wait for a load issued in that step.
+template <int ST> __device__ __forceinline__ void slide(ColumnWindow &c, const WindowIn<ST> &n, VR<ST> &pend) {
     c.v_m2 = c.v_m1;
     c.v_m1 = c.v_0;
     c.v_0 = c.v_p1;
     c.v_p1 = c.v_p2;
-    c.v_p2 = vr_v<ST>(n.vr);
+    if constexpr (ST == ST_F32) c.v_p2 = n.vr.v;
     c.r_m1 = c.r_0;
     c.r_0 = c.r_p1;
     c.r_p1 = c.r_p2;
-    c.r_p2 = vr_r<ST>(n.vr);
+    if constexpr (ST == ST_F32) c.r_p2 = n.vr.r;
+    else pend = n.vr;
     c.p_m1 = c.p_0;
     c.p_0 = c.p_p1;
     c.p_p1 = c.p_p2;
@@ -818,6 +840,13 @@ template <int ST> __device__ __forceinline__ void slide(ColumnWindow &c, const W
     c.s_m1 = c.s_0;
     c.s_0 = c.s_p1;
     c.s_p1 = n.s;
+}
+
+template <int ST> __device__ __forceinline__ void take(ColumnWindow &c, const VR<ST> &pend) {
+    if constexpr (ST != ST_F32) {
+        c.v_p2 = vr_v<ST>(pend);
+        c.r_p2 = vr_r<ST>(pend);
+    }
 }
 
 // Scheduling groups of one half-step: a few loads first (the window entries, whose moves close the
@@ -924,10 +953,12 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
 
     int i = x + d.X * y + XY * tc.zs;
     ColumnWindow c;
+    VR<ST> pend;  // the 16-bit forms' newest (v, rho) record, unpacked at the start of a step
     c.v_m2 = vr_v<ST>(v_load<ST>(w.rvr, i - 2 * XY));
     {
         const VR<ST> q_m1 = vr_load<ST>(w.rvr, i - XY), q_0 = vr_load<ST>(w.rvr, i),
                      q_p1 = vr_load<ST>(w.rvr, i + XY), q_p2 = vr_load<ST>(w.rvr, i + 2 * XY);
+        if constexpr (ST != ST_F32) pend = q_p2;
         c.v_m1 = vr_v<ST>(q_m1);
         c.v_0 = vr_v<ST>(q_0);
         c.v_p1 = vr_v<ST>(q_p1);
@@ -952,11 +983,11 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
     const bool has_h = __builtin_amdgcn_readfirstlane(wave) >= 4;
     const int ih = x + d.X * (w.yb - 1 + jh);
     {
-        const YRow h0 = yrow_load<ST>(w, ih + XY * zs);
+        const YRowOf<ST> h0 = yrow_load<ST>(w, ih + XY * zs);
         ys_store(w, zs, jo, lane, c.v_0, c.r_0, c.p_0, c.s_0);
-        ys_store(w, zs, jh, lane, h0.v, h0.r, h0.p, h0.s);
+        ys_store_row<ST>(w, zs, jh, lane, h0);
     }
-    YRow hA = yrow_load<ST>(w, ih + XY * (zs + 1)), hB = hA;
+    YRowOf<ST> hA = yrow_load<ST>(w, ih + XY * (zs + 1)), hB = hA;
     // drain the prologue's loads, so that the loop's memory-counter waits are set by the loop's
     // own steady state and not by the prologue's last loads; the stage's first planes are written
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
@@ -974,6 +1005,7 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         // z+2 from the ring), then loads psi of plane z+4 (it enters the registers at the end) and
         // T of plane z+3 (used by the next half-step), and writes psi of plane z+3 into the ring;
         // the LDS writes complete before the barrier that precedes their readers.)
+        take<ST>(c, pend);
         const PlaneNb n0 = ys_read(w, z, jo, lane, c.v_0.y < 0.f);
         lt_compute(w, z + 2, j0, lane, tn0, w0);
         if (has_l1) lt_compute(w, z + 2, j1, lane, tn1, w1);
@@ -988,14 +1020,15 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         psi_store(w, z + 3, j0, lane, w0.p);
         if (has_p1) psi_store(w, z + 3, j1, lane, w1.p);
         ys_store(w, z + 1, jo, lane, c.v_p1, c.r_p1, c.p_p1, c.s_p1);
-        if (has_h) ys_store(w, z + 1, jh, lane, hA.v, hA.r, hA.p, hA.s);
+        if (has_h) ys_store_row<ST>(w, z + 1, jh, lane, hA);
         w0 = psi_win(w0.c, w0.p, qa0);
         w1 = psi_win(w1.c, w1.p, qa1);
-        slide<ST>(c, in0);
+        slide<ST>(c, in0, pend);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
+        take<ST>(c, pend);
         const PlaneNb n1 = ys_read(w, z + 1, jo, lane, c.v_0.y < 0.f);
         lt_compute(w, z + 3, j0, lane, tn0, w0);
         if (has_l1) lt_compute(w, z + 3, j1, lane, tn1, w1);
@@ -1011,10 +1044,10 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         psi_store(w, z + 4, j0, lane, w0.p);
         if (has_p1) psi_store(w, z + 4, j1, lane, w1.p);
         ys_store(w, z + 2, jo, lane, c.v_p1, c.r_p1, c.p_p1, c.s_p1);
-        if (has_h) ys_store(w, z + 2, jh, lane, hB.v, hB.r, hB.p, hB.s);
+        if (has_h) ys_store_row<ST>(w, z + 2, jh, lane, hB);
         w0 = psi_win(w0.c, w0.p, qb0);
         w1 = psi_win(w1.c, w1.p, qb1);
-        slide<ST>(c, in1);
+        slide<ST>(c, in1, pend);
         INTERLEAVE_LOADS();
         __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)

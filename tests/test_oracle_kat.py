@@ -845,3 +845,36 @@ def test_density_update_rounds_once(oracle_mod):
     o.set_state(v=v)
     o.step(float(alpha), alpha=float(alpha))
     assert o.get_state()["rho"][p] == once
+
+
+def test_fp32_label_gap_to_fp64_is_the_floor_of_absorbed_increments(oracle_mod):
+    """The FP32 path's Dice differs from an FP64 run's by several hundredths on the 64^3 blob pair.
+    Voxel by voxel, every warped label that differs has a component where the FP64 coordinate lies
+    just below the grid point and the FP32 one does not, so the floor of the label warp (the
+    corner lookup, vector_fields.cu:81-107) takes the neighbour below in FP64 only; at most of
+    them FP32's psi is the identity exactly: each iteration's increment alpha * v o psi
+    (solver.cu:230-247) was below half an ulp of the absolute coordinate and was rounded away,
+    while FP64 accumulated it."""
+    O = oracle_mod
+    pair = phantoms.blobs()
+    dims = pair.dims
+    X, Y, Z = dims
+    runs = {}
+    for prec in ("f64", "f32"):
+        o = O.Oracle(dims, prec)
+        o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+        o.run(O.SolverParams(max_iter=150, no_labels=pair.no_labels))
+        ov, vx, _ = o.dice_counts(pair.no_labels)
+        runs[prec] = (o.get_state()["psi"][:, :3].astype(np.float64), o.outputs()[3], 2.0 * ov[1:].sum() / vx[1:].sum())
+        o.close()
+    zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    ident = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1).astype(np.float64)
+    p64, lab64, dice64 = runs["f64"]
+    p32, lab32, dice32 = runs["f32"]
+    diff = lab32 != lab64
+    below = ((p64 < ident) & (p32 >= ident)).any(1)
+    exact = (p32 == ident).all(1)
+    assert dice32 - dice64 > 0.01 and diff.sum() > 1000
+    assert not (diff & ~below).any()           # every differing label: FP64 below the grid, FP32 not
+    assert (diff & exact).sum() > 0.8 * diff.sum()
+    assert np.abs(p64 - ident).max(1)[diff].max() < 1e-3    # all at sub-millivoxel displacements
