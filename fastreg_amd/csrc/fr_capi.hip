@@ -520,9 +520,9 @@ extern "C" int fr_solver_run(fr_ctx *c) {
     c->run_start_buf = par;  // ping-pong index at the start of the run
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     HIP_TRY(hipStreamIsCapturing(c->stream, &cs));
-    HIP_TRY(hipEventRecord(c->ev0, c->stream));
     if (cs != hipStreamCaptureStatusNone) {
         // the caller is capturing this stream: the launches go into the caller's graph
+        HIP_TRY(hipEventRecord(c->ev0, c->stream));
         if ((rc = run_launches(c))) return rc;
     } else {
         // The loop is the same sequence of launches on every run from one buffer parity: it is
@@ -546,6 +546,8 @@ extern "C" int fr_solver_run(fr_ctx *c) {
             }
             c->run_graph[par] = g;
         }
+        // the run's time (ev0 -> ev1) is the replay's, without the one-time capture above
+        HIP_TRY(hipEventRecord(c->ev0, c->stream));
         HIP_TRY(hipGraphLaunch(c->run_exec[par], c->stream));
         c->cur = par ^ (p.max_iter & 1);  // as the launches leave it
     }
