@@ -465,6 +465,8 @@ __device__ __forceinline__ void put_rec(int *code, double *abc, double *gw, int 
 // sources that are not regular, are stored as code 0 / zeros), so a target reads its 27 sources
 // from LDS and every record is read from memory about 1.6 times instead of 27.
 constexpr int GT_X = 64, GT_Y = 4, GT_THREADS = GT_X * GT_Y;
+// tiles of the z-marching one-thread-per-column kernels (64 x 4 columns per block)
+constexpr int MT_X = 64, MT_Y = 4, MT_BLOCKS_PER_CU = 8;
 constexpr int GS_X = GT_X + 2, GS_Y = GT_Y + 2, GS_N = GS_X * GS_Y;
 constexpr int GS_PER_THREAD = (GS_N + GT_THREADS - 1) / GT_THREADS;
 struct RecStage {
@@ -668,87 +670,136 @@ __device__ __forceinline__ bool in_lap(Dims d, int x, int y, int z) {
 // Step 4: transposes of F6 gathered into gv: dvv = (Dv)v (interior) and dlv, the velocity
 // Laplacian (its range; centre 3 times, "far" neighbours (+-2) once, "near" ones (+-1) -2 times,
 // chosen by the sign of the centre's component, with y-1 / z-1 as the near neighbour always).
-__global__ void b_step4(Dims d, const double *__restrict__ v, const double *__restrict__ gdvv,
-                        const double *__restrict__ gdlv, double *__restrict__ gv) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        const int sy = d.X, sz = d.X * d.Y;
-        const D3 vi = ld(v, i);
-        D3 a = d3(0, 0, 0);
-        // (Dv)v, voxel i as the centre
-        if (interior(d, x, y, z)) {
-            const D3 c = vi, xp = ld(v, i + 1), xm = ld(v, i - 1), yp = ld(v, i + sy), ym = ld(v, i - sy),
-                     zp = ld(v, i + sz), zm = ld(v, i - sz), G = ld(gdvv, i);
-            a.x += 0.5 * G.x * (2 * mx(c.x) - 2 * mn(c.x));
-            if (c.y > 0) { a.y += G.x * (c.x - ym.x); a.x += G.x * c.y; }
-            else { a.y += G.x * (yp.x - c.x); a.x -= G.x * c.y; }
-            if (c.z > 0) { a.z += G.x * (c.x - zm.x); a.x += G.x * c.z; }
-            else { a.z += G.x * (zp.x - c.x); a.x -= G.x * c.z; }
-            if (c.x > 0) { a.x += G.y * (c.y - xm.y); a.y += G.y * c.x; }
-            else { a.x += G.y * (xp.y - c.y); a.y -= G.y * c.x; }
-            a.y += 0.5 * G.y * (2 * mx(c.y) - 2 * mn(c.y));
-            if (c.z > 0) { a.z += G.y * (c.y - zm.y); a.y += G.y * c.z; }
-            else { a.z += G.y * (zp.y - c.y); a.y -= G.y * c.z; }
-            if (c.x > 0) { a.x += G.z * (c.z - xm.z); a.z += G.z * c.x; }
-            else { a.x += G.z * (xp.z - c.z); a.z -= G.z * c.x; }
-            if (c.y > 0) { a.y += G.z * (c.z - ym.z); a.z += G.z * c.y; }
-            else { a.y += G.z * (yp.z - c.z); a.z -= G.z * c.y; }
-            a.z += 0.5 * G.z * (2 * mx(c.z) - 2 * mn(c.z));
-        }
-        // voxel i as the x+1 / x-1 / y+1 / y-1 / z+1 / z-1 neighbour of an interior voxel j
-        if (interior(d, x - 1, y, z)) {
-            const int j = i - 1;
-            const D3 cj = ld(v, j), G = ld(gdvv, j);
-            a.x += 0.5 * G.x * (2 * mn(vi.x) - 2 * mx(vi.x));
-            if (!(cj.x > 0)) { a.y += G.y * cj.x; a.z += G.z * cj.x; }
-        }
-        if (interior(d, x + 1, y, z)) {
-            const int j = i + 1;
-            const D3 cj = ld(v, j), G = ld(gdvv, j);
-            if (cj.x > 0) { a.y -= G.y * cj.x; a.z -= G.z * cj.x; }
-        }
-        if (interior(d, x, y - 1, z)) {
-            const int j = i - sy;
-            const D3 cj = ld(v, j), G = ld(gdvv, j);
-            if (!(cj.y > 0)) { a.x += G.x * cj.y; a.z += G.z * cj.y; }
-            a.y += 0.5 * G.y * (2 * mn(vi.y) - 2 * mx(vi.y));
-        }
-        if (interior(d, x, y + 1, z)) {
-            const int j = i + sy;
-            const D3 cj = ld(v, j), G = ld(gdvv, j);
-            if (cj.y > 0) { a.x -= G.x * cj.y; a.z -= G.z * cj.y; }
-        }
-        if (interior(d, x, y, z - 1)) {
-            const int j = i - sz;
-            const D3 cj = ld(v, j), G = ld(gdvv, j);
-            if (!(cj.z > 0)) { a.x += G.x * cj.z; a.y += G.y * cj.z; }
-            a.z += 0.5 * G.z * (2 * mn(vi.z) - 2 * mx(vi.z));
-        }
-        if (interior(d, x, y, z + 1)) {
-            const int j = i + sz;
-            const D3 cj = ld(v, j), G = ld(gdvv, j);
-            if (cj.z > 0) { a.x -= G.x * cj.z; a.y -= G.y * cj.z; }
-        }
-        // the velocity Laplacian: dlv(j) = -(sum over axes of far - 2 near + centre)
-        const auto add = [&](int j, double f) {
-            const D3 g = ld(gdlv, j);
-            a = d3(a.x + f * g.x, a.y + f * g.y, a.z + f * g.z);
-        };
-        if (in_lap(d, x, y, z)) add(i, -3.0);
-        if (in_lap(d, x - 2, y, z) && v[3 * (i - 2)] < 0) add(i - 2, -1.0);
-        if (in_lap(d, x + 2, y, z) && !(v[3 * (i + 2)] < 0)) add(i + 2, -1.0);
-        if (in_lap(d, x - 1, y, z) && v[3 * (i - 1)] < 0) add(i - 1, 2.0);
-        if (in_lap(d, x + 1, y, z) && !(v[3 * (i + 1)] < 0)) add(i + 1, 2.0);
-        // far y neighbour: j = i - 2 sy by linear address (for y = 0 that is (x, Y-2, z-1))
-        if (y >= 2 ? in_lap(d, x, y - 2, z) : (y == 0 && in_lap(d, x, d.Y - 2, z - 1)))
-            if (v[3 * (i - 2 * sy) + 1] < 0) add(i - 2 * sy, -1.0);
-        if (in_lap(d, x, y + 2, z) && !(v[3 * (i + 2 * sy) + 1] < 0)) add(i + 2 * sy, -1.0);
-        if (in_lap(d, x, y + 1, z)) add(i + sy, 2.0);
-        if (in_lap(d, x, y, z - 2) && v[3 * (i - 2 * sz) + 2] < 0) add(i - 2 * sz, -1.0);
-        if (in_lap(d, x, y, z + 2) && !(v[3 * (i + 2 * sz) + 2] < 0)) add(i + 2 * sz, -1.0);
-        if (in_lap(d, x, y, z + 1)) add(i + sz, 2.0);
+// The z-window of one column for step 4: v at z-2..z+2, gdvv at z-1..z+1, gdlv at z-2..z+2
+// (zeros outside the grid; no branch below reads them there)
+struct B4Win {
+    D3 v[5], g[3], l[5];
+};
+// the transposes of step 4 gathered for voxel (x, y, z): its z-neighbours from the window, the
+// x- and y-neighbours from memory (rows the block's other lanes and waves load at the same time)
+__device__ __forceinline__ D3 b4_voxel(Dims d, int x, int y, int z, int i, const double *__restrict__ v,
+                                       const double *__restrict__ gdvv, const double *__restrict__ gdlv,
+                                       const B4Win &w) {
+    const int sy = d.X;
+    const D3 vi = w.v[2];
+    D3 a = d3(0, 0, 0);
+    // (Dv)v, voxel i as the centre
+    if (interior(d, x, y, z)) {
+        const D3 c = vi, xp = ld(v, i + 1), xm = ld(v, i - 1), yp = ld(v, i + sy), ym = ld(v, i - sy),
+                 zp = w.v[3], zm = w.v[1], G = w.g[1];
+        a.x += 0.5 * G.x * (2 * mx(c.x) - 2 * mn(c.x));
+        if (c.y > 0) { a.y += G.x * (c.x - ym.x); a.x += G.x * c.y; }
+        else { a.y += G.x * (yp.x - c.x); a.x -= G.x * c.y; }
+        if (c.z > 0) { a.z += G.x * (c.x - zm.x); a.x += G.x * c.z; }
+        else { a.z += G.x * (zp.x - c.x); a.x -= G.x * c.z; }
+        if (c.x > 0) { a.x += G.y * (c.y - xm.y); a.y += G.y * c.x; }
+        else { a.x += G.y * (xp.y - c.y); a.y -= G.y * c.x; }
+        a.y += 0.5 * G.y * (2 * mx(c.y) - 2 * mn(c.y));
+        if (c.z > 0) { a.z += G.y * (c.y - zm.y); a.y += G.y * c.z; }
+        else { a.z += G.y * (zp.y - c.y); a.y -= G.y * c.z; }
+        if (c.x > 0) { a.x += G.z * (c.z - xm.z); a.z += G.z * c.x; }
+        else { a.x += G.z * (xp.z - c.z); a.z -= G.z * c.x; }
+        if (c.y > 0) { a.y += G.z * (c.z - ym.z); a.z += G.z * c.y; }
+        else { a.y += G.z * (yp.z - c.z); a.z -= G.z * c.y; }
+        a.z += 0.5 * G.z * (2 * mx(c.z) - 2 * mn(c.z));
+    }
+    // voxel i as the x+1 / x-1 / y+1 / y-1 / z+1 / z-1 neighbour of an interior voxel j
+    if (interior(d, x - 1, y, z)) {
+        const int j = i - 1;
+        const D3 cj = ld(v, j), G = ld(gdvv, j);
+        a.x += 0.5 * G.x * (2 * mn(vi.x) - 2 * mx(vi.x));
+        if (!(cj.x > 0)) { a.y += G.y * cj.x; a.z += G.z * cj.x; }
+    }
+    if (interior(d, x + 1, y, z)) {
+        const int j = i + 1;
+        const D3 cj = ld(v, j), G = ld(gdvv, j);
+        if (cj.x > 0) { a.y -= G.y * cj.x; a.z -= G.z * cj.x; }
+    }
+    if (interior(d, x, y - 1, z)) {
+        const int j = i - sy;
+        const D3 cj = ld(v, j), G = ld(gdvv, j);
+        if (!(cj.y > 0)) { a.x += G.x * cj.y; a.z += G.z * cj.y; }
+        a.y += 0.5 * G.y * (2 * mn(vi.y) - 2 * mx(vi.y));
+    }
+    if (interior(d, x, y + 1, z)) {
+        const int j = i + sy;
+        const D3 cj = ld(v, j), G = ld(gdvv, j);
+        if (cj.y > 0) { a.x -= G.x * cj.y; a.z -= G.z * cj.y; }
+    }
+    if (interior(d, x, y, z - 1)) {
+        const D3 cj = w.v[1], G = w.g[0];
+        if (!(cj.z > 0)) { a.x += G.x * cj.z; a.y += G.y * cj.z; }
+        a.z += 0.5 * G.z * (2 * mn(vi.z) - 2 * mx(vi.z));
+    }
+    if (interior(d, x, y, z + 1)) {
+        const D3 cj = w.v[3], G = w.g[2];
+        if (cj.z > 0) { a.x -= G.x * cj.z; a.y -= G.y * cj.z; }
+    }
+    // the velocity Laplacian: dlv(j) = -(sum over axes of far - 2 near + centre)
+    const auto add_v = [&](D3 g, double f) { a = d3(a.x + f * g.x, a.y + f * g.y, a.z + f * g.z); };
+    const auto add = [&](int j, double f) { add_v(ld(gdlv, j), f); };
+    if (in_lap(d, x, y, z)) add_v(w.l[2], -3.0);
+    if (in_lap(d, x - 2, y, z) && v[3 * (i - 2)] < 0) add(i - 2, -1.0);
+    if (in_lap(d, x + 2, y, z) && !(v[3 * (i + 2)] < 0)) add(i + 2, -1.0);
+    if (in_lap(d, x - 1, y, z) && v[3 * (i - 1)] < 0) add(i - 1, 2.0);
+    if (in_lap(d, x + 1, y, z) && !(v[3 * (i + 1)] < 0)) add(i + 1, 2.0);
+    // far y neighbour: j = i - 2 sy by linear address (for y = 0 that is (x, Y-2, z-1))
+    if (y >= 2 ? in_lap(d, x, y - 2, z) : (y == 0 && in_lap(d, x, d.Y - 2, z - 1)))
+        if (v[3 * (i - 2 * sy) + 1] < 0) add(i - 2 * sy, -1.0);
+    if (in_lap(d, x, y + 2, z) && !(v[3 * (i + 2 * sy) + 1] < 0)) add(i + 2 * sy, -1.0);
+    if (in_lap(d, x, y + 1, z)) add(i + sy, 2.0);
+    if (in_lap(d, x, y, z - 2) && w.v[0].z < 0) add_v(w.l[0], -1.0);
+    if (in_lap(d, x, y, z + 2) && !(w.v[4].z < 0)) add_v(w.l[4], -1.0);
+    if (in_lap(d, x, y, z + 1)) add_v(w.l[3], 2.0);
+    return a;
+}
+__device__ __forceinline__ D3 ld_or0(const double *__restrict__ p, int i, bool ok) {
+    return ok ? ld(p, i) : d3(0, 0, 0);
+}
+
+// Step 4: transposes of F6 gathered into gv: dvv = (Dv)v (interior) and dlv, the velocity
+// Laplacian (its range; centre 3 times, "far" neighbours (+-2) once, "near" ones (+-1) -2 times,
+// chosen by the sign of the centre's component, with y-1 / z-1 as the near neighbour always).
+// One thread per (x, y) column of a 64 x 4 tile marching a chunk of zc planes, its column's
+// z-neighbours in registers, so that each plane of v, gdvv and gdlv is read from memory once per
+// column (the one-thread-per-voxel form re-read them from beyond the L2 about 3 times).
+__global__ __launch_bounds__(256) void b_step4(Dims d, int tiles_x, int tiles_y, int zc,
+                                                const double *__restrict__ v, const double *__restrict__ gdvv,
+                                                const double *__restrict__ gdlv, double *__restrict__ gv) {
+    const int b = blockIdx.x;
+    const int x = (b % tiles_x) * MT_X + (int)(threadIdx.x % MT_X);
+    const int y = ((b / tiles_x) % tiles_y) * MT_Y + (int)(threadIdx.x / MT_X);
+    const int z0 = (b / (tiles_x * tiles_y)) * zc, z1 = min(z0 + zc, d.Z);
+    if (x >= d.X || y >= d.Y) return;
+    const int sz = d.X * d.Y, c0 = x + d.X * y;
+    B4Win w;
+    for (int k = 0; k < 5; ++k) {  // planes z0-2 .. z0+2
+        const int p = z0 - 2 + k;
+        const bool ok = p >= 0 && p < d.Z;
+        w.v[k] = ld_or0(v, c0 + sz * p, ok);
+        w.l[k] = ld_or0(gdlv, c0 + sz * p, ok);
+    }
+    for (int k = 0; k < 3; ++k) {  // planes z0-1 .. z0+1
+        const int p = z0 - 1 + k;
+        w.g[k] = ld_or0(gdvv, c0 + sz * p, p >= 0 && p < d.Z);
+    }
+    for (int z = z0; z < z1; ++z) {
+        // the entries of the next step, loaded a step ahead: plane z+3 of v and gdlv, z+2 of gdvv
+        const D3 nv = ld_or0(v, c0 + sz * (z + 3), z + 3 < d.Z), nl = ld_or0(gdlv, c0 + sz * (z + 3), z + 3 < d.Z),
+                 ng = ld_or0(gdvv, c0 + sz * (z + 2), z + 2 < d.Z);
+        const int i = c0 + sz * z;
+        const D3 a = b4_voxel(d, x, y, z, i, v, gdvv, gdlv, w);
         const D3 o = ld(gv, i);
         st(gv, i, d3(o.x + a.x, o.y + a.y, o.z + a.z));
+        for (int k = 0; k < 4; ++k) {
+            w.v[k] = w.v[k + 1];
+            w.l[k] = w.l[k + 1];
+        }
+        w.v[4] = nv;
+        w.l[4] = nl;
+        w.g[0] = w.g[1];
+        w.g[1] = w.g[2];
+        w.g[2] = ng;
     }
 }
 
@@ -1117,6 +1168,8 @@ struct fr_adjoint {
     // launch shape of the record gathers: 64 x 4 tiles, zc z-planes per block (gt_zc_auto: the
     // choice made at creation, about two blocks per CU)
     int gt_tiles_x = 0, gt_tiles_y = 0, gt_zc = 0, gt_blocks = 0, gt_zc_auto = 0;
+    // launch shape of the z-marching kernels (b_step4): the same 64 x 4 tiles, mt_zc planes per block
+    int mt_tiles_x = 0, mt_tiles_y = 0, mt_zc = 0, mt_blocks = 0;
     // per-iteration scalars of the last forward (index k-1 for iteration k)
     std::vector<double> m1t, m2;
     double w_reg = 0, tau = 0, lw = 0, energy = 0;
@@ -1224,7 +1277,7 @@ int backward_iteration(fr_adjoint *h, const State &in, const State &nx, const In
                                                         h->gw, nx.rho, it.dlv, it.gU, h->gv, h->gdvv, h->gdlv,
                                                         h->ggU, gr1, a, m1t, h->tau, m2, h->part_tau);
     frd::k_sum_accumulate<<<1, 256, 0, s>>>(h->part_tau, h->gt_blocks, h->scal + 2);
-    frd::b_step4<<<g, TPB, 0, s>>>(d, in.v, h->gdvv, h->gdlv, h->gv);
+    frd::b_step4<<<h->mt_blocks, 256, 0, s>>>(d, h->mt_tiles_x, h->mt_tiles_y, h->mt_zc, in.v, h->gdvv, h->gdlv, h->gv);
     frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.L, in.pinv, h->ggU, h->gJ, h->gpinv,
                                    h->gL, h->code, h->abc, h->gw, h->w_reg, h->part_w);
     frd::k_sum_accumulate<<<1, 256, 0, s>>>(h->part_w, g, h->scal + 1);
@@ -1292,6 +1345,14 @@ extern "C" int fr_adjoint_create(const fr_params *p, int K, int ckpt, fr_adjoint
         chunks = chunks < 1 ? 1 : chunks > h->d.Z ? h->d.Z : chunks;
         h->gt_zc = h->gt_zc_auto = (int)((h->d.Z + chunks - 1) / chunks);
         h->gt_blocks = (int)(tiles * ((h->d.Z + h->gt_zc - 1) / h->gt_zc));
+        // about MT_BLOCKS_PER_CU blocks per CU for the z-marching kernels
+        h->mt_tiles_x = (h->d.X + frd::MT_X - 1) / frd::MT_X;
+        h->mt_tiles_y = (h->d.Y + frd::MT_Y - 1) / frd::MT_Y;
+        const long mtiles = (long)h->mt_tiles_x * h->mt_tiles_y;
+        long mchunks = (long)frd::MT_BLOCKS_PER_CU * n_cu / mtiles;
+        mchunks = mchunks < 1 ? 1 : mchunks > h->d.Z ? h->d.Z : mchunks;
+        h->mt_zc = (int)((h->d.Z + mchunks - 1) / mchunks);
+        h->mt_blocks = (int)(mtiles * ((h->d.Z + h->mt_zc - 1) / h->mt_zc));
     }
     const size_t n = (size_t)h->n;
     int rc = FR_OK;
