@@ -110,6 +110,9 @@ struct fr_ctx {
     std::string log;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool have_run = false, have_final = false;
+    // the last fr_solver_run was recorded into a caller's graph capture: every replay of that
+    // graph reruns the loop, so the run's results are read from the device again at each request
+    bool captured = false;
 };
 
 static int collect_run(fr_ctx *c);
@@ -379,7 +382,8 @@ extern "C" int fr_solver_init(fr_ctx *c, const float *tgt, const int16_t *tgt_la
     if (c->st != frk::ST_F32) HIP_TRY(frk::launch_vr_init(c->st, c->vr[0], c->vr[1], c->n, c->prm.rho_0, c->stream));
     c->cur = 0;
     c->initialised = true;
-    c->have_run = c->have_final = false;
+    c->have_run = c->captured;  // a replay of the caller's graph follows
+    c->have_final = false;
     return FR_OK;
 }
 
@@ -447,9 +451,9 @@ static int launch_iteration(fr_ctx *c, int iter, float t, bool with_energy) {
 
 static int reset_control(fr_ctx *c) {
     int mi = c->prm.max_iter > 0 ? c->prm.max_iter : 1;
-    HIP_TRY(hipMemsetAsync(c->ctl, 0, sizeof(frk::Ctl), c->stream));
-    HIP_TRY(hipMemsetAsync(c->maxpack, 0, sizeof(unsigned long long) * mi * frk::NSLOT, c->stream));
-    HIP_TRY(hipMemsetAsync(c->energy, 0, sizeof(double) * 3 * mi, c->stream));
+    // one kernel, not memsets: a caller capturing fr_solver_run gets no memset nodes in its graph
+    HIP_TRY(frk::launch_zero3(c->ctl, sizeof(frk::Ctl), c->maxpack, sizeof(unsigned long long) * mi * frk::NSLOT,
+                              c->energy, sizeof(double) * 3 * mi, c->stream));
     return FR_OK;
 }
 
@@ -472,8 +476,8 @@ extern "C" int fr_solver_step(fr_ctx *c, float t, float *max_norm, long long *ma
     if (!c->initialised) return fail(FR_ERR_STATE, "fr_solver_step before fr_solver_init");
     int rc;
     if (c->have_run && (rc = collect_run(c))) return rc;
+    c->captured = false;
     if ((rc = reset_control(c))) return rc;
-    if (rc) return rc;
     if ((rc = launch_iteration(c, 1, t, false))) return rc;
     unsigned long long sl[frk::NSLOT];
     HIP_TRY(hipMemcpyAsync(sl, c->maxpack, sizeof sl, hipMemcpyDeviceToHost, c->stream));
@@ -513,8 +517,12 @@ static int run_launches(fr_ctx *c) {
 extern "C" int fr_solver_run(fr_ctx *c) {
     if (!c) return fail(FR_ERR_INVALID, "null context");
     if (!c->initialised) return fail(FR_ERR_STATE, "fr_solver_run before fr_solver_init");
-    int rc = reset_control(c);  // eager memsets: no memset nodes in the captured loop
-    if (rc) return rc;
+    int rc;
+    // an earlier run not yet collected (it may have stopped early) settles the buffer parity first
+    if (c->have_run && (rc = collect_run(c))) return rc;
+    // the control words are cleared by a kernel: eagerly before the context's own captured loop,
+    // as a kernel node of a caller's capture
+    if ((rc = reset_control(c))) return rc;
     const fr_params &p = c->prm;
     const int par = c->cur;
     c->run_start_buf = par;  // ping-pong index at the start of the run
@@ -524,7 +532,9 @@ extern "C" int fr_solver_run(fr_ctx *c) {
         // the caller is capturing this stream: the launches go into the caller's graph
         HIP_TRY(hipEventRecord(c->ev0, c->stream));
         if ((rc = run_launches(c))) return rc;
+        c->captured = true;
     } else {
+        c->captured = false;
         // The loop is the same sequence of launches on every run from one buffer parity: it is
         // captured once into a hipGraph and replayed (one launch instead of 2 per iteration).
         if (!c->run_exec[par]) {
@@ -573,7 +583,12 @@ static int collect_run(fr_ctx *c) {
     HIP_TRY(hipMemcpyAsync(c->h_run->e, c->energy, sizeof(double) * 3 * mi, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
+        // events recorded inside a caller's graph may not time: the run's time is then unknown
+        if (!c->captured) return fail(FR_ERR_HIP, "hipEventElapsedTime of the run failed");
+        (void)hipGetLastError();
+        ms = 0.f;
+    }
     const frk::Ctl ctl = c->h_run->ctl;
     const unsigned long long *mp = c->h_run->mp;
     const double *e = c->h_run->e;

@@ -122,6 +122,8 @@ hipError_t launch_vr_init(int st, unsigned *r0, unsigned *r1, int n, float rho_0
 hipError_t launch_vr_set(int st, unsigned *r0, unsigned *r1, const float *v4, const float *rho, int n, hipStream_t s);
 hipError_t launch_vr_get(int st, const unsigned *r, float *v4, float *rho, int n, hipStream_t s);
 hipError_t launch_unpack3to4(const float *src3, float *dst4, int n, hipStream_t s);
+// zeros into three buffers of b0, b1, b2 bytes (multiples of 4) in one kernel
+hipError_t launch_zero3(void *p0, size_t b0, void *p1, size_t b1, void *p2, size_t b2, hipStream_t s);
 // (x, y, z, 0) at every voxel of a float4 field
 hipError_t launch_identity4(Dims d, float *dst4, hipStream_t s);
 // out[k] = the maximum of iteration k's NSLOT slots, k < iters

@@ -1431,6 +1431,18 @@ __global__ __launch_bounds__(256) void k_slots_reduce(const unsigned long long *
 }
 
 // init_identity_kernel (vector_fields.cu:64-79): (x, y, z, 0) at voxel (x, y, z)
+// zeros into three word arrays (n0, n1, n2 32-bit words): clears memory without memset nodes, so
+// that a caller's graph capture of the loop holds kernels only
+__global__ void k_zero3(unsigned *__restrict__ p0, long long n0, unsigned *__restrict__ p1, long long n1,
+                        unsigned *__restrict__ p2, long long n2) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n0 + n1 + n2; k += stride) {
+        if (k < n0) p0[k] = 0u;
+        else if (k < n0 + n1) p1[k - n0] = 0u;
+        else p2[k - n0 - n1] = 0u;
+    }
+}
+
 __global__ void k_identity4(Dims d, float *__restrict__ dst4) {
     const int n = d.X * d.Y * d.Z;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1773,6 +1785,16 @@ hipError_t launch_pack4to3(const float *src4, float *dst3a, float *dst3b, int n,
 hipError_t launch_slots_reduce(const unsigned long long *slots, int iters, unsigned long long *out, hipStream_t s) {
     if (iters <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_slots_reduce, dim3((iters + 3) / 4), dim3(256), 0, s, slots, iters, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero3(void *p0, size_t b0, void *p1, size_t b1, void *p2, size_t b2, hipStream_t s) {
+    if ((b0 | b1 | b2) & 3u) return hipErrorInvalidValue;
+    const long long n = (long long)((b0 + b1 + b2) / 4);
+    const int grid = (int)(n < 256LL * 1024 ? (n + 255) / 256 : 1024);
+    hipLaunchKernelGGL(k_zero3, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, static_cast<unsigned *>(p0),
+                       (long long)(b0 / 4), static_cast<unsigned *>(p1), (long long)(b1 / 4),
+                       static_cast<unsigned *>(p2), (long long)(b2 / 4));
     return hipGetLastError();
 }
 

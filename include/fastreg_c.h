@@ -151,7 +151,14 @@ int fr_solver_step(fr_ctx *ctx, float t, float *max_norm, long long *max_voxel);
 
 /* The iteration loop (solver.cu:327-433) for MAX_ITER iterations from the current state with the
  * reference's t schedule, stopping rules and messages; asynchronous on the context stream
- * (fr_get_stats / fr_get_history / fr_get_log synchronise). */
+ * (fr_get_stats / fr_get_history / fr_get_log synchronise).  The loop is captured into a hipGraph
+ * on the first run from each ping-pong parity and replayed afterwards.
+ * When the caller is capturing the context stream (fr_set_stream, then hipStreamBeginCapture),
+ * the loop's kernels are recorded into the caller's graph instead (kernel nodes only, no memset
+ * nodes) and nothing runs yet.  Each replay of that graph reruns the loop on this context's
+ * buffers; call fr_solver_init before every replay (outside the capture), and the statistics,
+ * history, log, state and fr_solver_finalize then give that replay's results (solver_ms may be
+ * 0).  The next fr_solver_run or fr_solver_step outside a capture ends this mode. */
 int fr_solver_run(fr_ctx *ctx);
 
 /* Final warps and figures (solver.cu:435-443, fastreg.cpp:71-72): img1/lab1 <- target o psi_inv,
