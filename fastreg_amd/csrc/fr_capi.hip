@@ -1044,10 +1044,9 @@ extern "C" int fr_batch_run(fr_batch *b) {
         if (c->cur != par)
             return fail(FR_ERR_STATE, "batch volumes are at different buffer parities (volume %d); re-initialise them", q);
     }
-    for (int q = 0; q < n; ++q) {  // control words cleared eagerly (no memset nodes in the graph)
+    for (int q = 0; q < n; ++q)  // control words cleared eagerly, before the graph's launch
         if ((rc = reset_control(b->ctx[q]))) return rc;
-        HIP_TRY(hipStreamSynchronize(b->ctx[q]->stream));
-    }
+    for (int q = 0; q < n; ++q) HIP_TRY(hipStreamSynchronize(b->ctx[q]->stream));
     if (!b->exec[par]) {
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         rc = batch_launches(b, par);
@@ -1072,6 +1071,7 @@ extern "C" int fr_batch_run(fr_batch *b) {
         fr_ctx *c = b->ctx[q];
         HIP_TRY(hipStreamWaitEvent(c->stream, b->done, 0));  // later calls on the context see the run
         c->run_start_buf = par;
+        c->captured = false;
         c->cur = par ^ (p.max_iter & 1);
         c->have_run = true;
         c->have_final = false;
