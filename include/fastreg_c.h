@@ -158,7 +158,8 @@ int fr_solver_step(fr_ctx *ctx, float t, float *max_norm, long long *max_voxel);
  * nodes) and nothing runs yet.  Each replay of that graph reruns the loop on this context's
  * buffers; call fr_solver_init before every replay (outside the capture), and the statistics,
  * history, log, state and fr_solver_finalize then give that replay's results (solver_ms may be
- * 0).  The next fr_solver_run or fr_solver_step outside a capture ends this mode. */
+ * 0).  The next fr_solver_run or fr_solver_step outside a capture ends this mode.  The caller's
+ * graph refers to this context's buffers and events: destroy it before the context. */
 int fr_solver_run(fr_ctx *ctx);
 
 /* Final warps and figures (solver.cu:435-443, fastreg.cpp:71-72): img1/lab1 <- target o psi_inv,
