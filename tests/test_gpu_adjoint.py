@@ -74,6 +74,43 @@ def test_forward_is_bit_exact_against_the_fp64_oracle(oracle_mod, dims, seed, K,
         a.close()
 
 
+def test_forward_at_the_workload_size_is_bit_exact_against_the_fp64_oracle(oracle_mod):
+    """The differentiated workload's own forward pass: the 128^3 blob pair, K = 64, checkpoints
+    every 8, from its smooth v0; every kept state (iterations 8, 16, .., 64) equals the FP64
+    oracle's, run on the host the same way.  With the gradient equal to autograd of the unrolled
+    restatement at this size (the full-size test below), the gradient is that of the oracle's
+    function."""
+    O = oracle_mod
+    dims, K, ckpt = (128, 128, 128), 64, 8
+    X, Y, Z = dims
+    n = X * Y * Z
+    say("adjoint forward vs FP64 oracle, full size", dims, K)
+    pair = phantoms.blobs(shape=(Z, Y, X), seed=5, scale=2.0)
+    v0 = phantoms.smooth_velocity(dims, 5, amp=0.5, offset=0.05, sigma=4.0)
+    a = _adjoint(dims, pair, K, ckpt)
+    o = O.Oracle(dims, "f64")
+    try:
+        a.forward(v0, W_REG, TAU, 1.0)
+        o.init(pair.target, pair.target_labels, pair.source, pair.source_labels, 1.0)
+        v4 = np.zeros((n, 4))
+        v4[:, :3] = v0
+        o.set_state(v=v4)
+        t = ALPHA
+        for k in range(1, K + 1):
+            if k >= 2:
+                t += ALPHA
+            o.step(t, ALPHA, TAU, W_REG, 2.0, 0.25)
+            if k % ckpt == 0:
+                ref, got = o.get_state(), a.state(k)
+                for key in ("v", "psi", "psi_inv"):
+                    assert np.array_equal(got[key], ref[key][:, :3]), (k, key)
+                assert np.array_equal(got["rho"], ref["rho"]), (k, "rho")
+                say("  iteration", k, "equal")
+    finally:
+        o.close()
+        a.close()
+
+
 def test_gradient_matches_autograd_of_the_unrolled_restatement():
     from oracle import unrolled as U
     dims, K, ckpt, lw = (20, 18, 16), 7, 3, 0.5
