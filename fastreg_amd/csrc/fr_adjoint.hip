@@ -65,6 +65,9 @@ struct Dims {
 };
 // voxel indices are 32-bit: the library accepts volumes with 9 N < 2^31 (the Jacobian buffers)
 __device__ __forceinline__ int lin(Dims d, int x, int y, int z) { return x + d.X * (y + d.Y * z); }
+// the adjoint of the Jacobian of psi_inv, entry k = 3 r + c of voxel i: one plane of N entries per
+// k (the neighbours' entries that step 8 gathers are then contiguous across a wave)
+__device__ __forceinline__ int gj_at(Dims d, int i, int k) { return k * (d.X * d.Y * d.Z) + i; }
 __device__ __forceinline__ bool interior(Dims d, int x, int y, int z) {
     return x >= 1 && x <= d.X - 2 && y >= 1 && y <= d.Y - 2 && z >= 1 && z <= d.Z - 2;
 }
@@ -584,27 +587,25 @@ __global__ void b_scatter_irregular(Dims d, const int *code, const double *pos, 
 }
 
 // Step 1 (adjoints of F9 and F8, own voxel), interior:
-//   psi_inv1 = psi_inv - alpha J v1:         gJ = -alpha gpinv1 (x) v1, gv1 += -alpha J^T gpinv1
+//   psi_inv1 = psi_inv - alpha J v1:         gv1 += -alpha J^T gpinv1 (and gJ = -alpha gpinv1 (x) v1,
+//                                              formed in step 5 with the rest of gJ)
 //   psi1 = psi + alpha trilerp(v1, psi):      gpsi += alpha (d trilerp / d pos)^T gpsi1,
 //                                              record (cell of psi, alpha gpsi1) for the corners of v1
-// gpinv, gpsi, gv1 are updated in place (off the interior they pass through); gJ is written.
+// gpsi, gv1 are updated in place (off the interior they pass through); gpinv is read only.
 // J is recomputed from psi_inv (F2's own function: the same values) rather than stored.
 __global__ void b_step1(Dims d, const double *__restrict__ pinv, const double *__restrict__ v1,
-                        const double *__restrict__ psi, double *__restrict__ gpinv, double *__restrict__ gJ,
+                        const double *__restrict__ psi, const double *__restrict__ gpinv,
                         double *__restrict__ gv1, double *__restrict__ gpsi, int *__restrict__ code,
                         double *__restrict__ abc, double *__restrict__ gw, double alpha) {
     FOR_VOXELS(d, i) {
         XYZ(d, i);
         if (!interior(d, x, y, z)) {
-            for (int k = 0; k < 9; ++k) gJ[9 * i + k] = 0.0;
             code[i] = 0;
             continue;
         }
-        const D3 g9 = ld(gpinv, i), w = ld(v1, i);
+        const D3 g9 = ld(gpinv, i);
         double m[9];
         fv_jacobian(d, x, y, z, pinv, m);  // J of the forward step, recomputed from psi_inv
-        for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 3; ++c) gJ[9 * i + 3 * r + c] = -alpha * comp(g9, r) * comp(w, c);
         const D3 gv = ld(gv1, i);
         st(gv1, i, d3(gv.x - alpha * (m[0] * g9.x + m[3] * g9.y + m[6] * g9.z),
                       gv.y - alpha * (m[1] * g9.x + m[4] * g9.y + m[7] * g9.z),
@@ -804,7 +805,8 @@ __global__ __launch_bounds__(256) void b_step4(Dims d, int tiles_x, int tiles_y,
 }
 
 // Step 5 (own voxel, interior): the adjoint of F5, gU = ((S - Tp) gS + w Lo) det_ref(J):
-//   gLo = w det ggU, gTp = -det (ggU . gS), gJ += (ggU . ((S - Tp) gS + w Lo)) d det_ref / dJ,
+//   gLo = w det ggU, gTp = -det (ggU . gS), gJ = -alpha gpinv1 (x) v1 (F9's part, step 1's
+//   transpose) + (ggU . ((S - Tp) gS + w Lo)) d det_ref / dJ, written once (0 off the interior),
 //   dE/dw_reg += det (ggU . Lo) (one partial sum per block into gwsum[blockIdx.x], added up in block
 //   order by k_sum_accumulate);
 // then of F4 at this voxel's position psi_inv: gpinv += (d trilerp(L) / d pos)^T gLo +
@@ -812,9 +814,10 @@ __global__ __launch_bounds__(256) void b_step4(Dims d, int tiles_x, int tiles_y,
 // gL is cleared here (the irregular scatter and the gather fill it).
 __global__ __launch_bounds__(256) void b_step5(Dims d, const double *__restrict__ S, const double *__restrict__ gS,
                              const double *__restrict__ T, const double *__restrict__ L,
-                             const double *__restrict__ pinv, const double *__restrict__ ggU, double *__restrict__ gJ,
+                             const double *__restrict__ pinv, const double *__restrict__ v1,
+                             const double *__restrict__ ggU, double *__restrict__ gJ,
                              double *__restrict__ gpinv, double *__restrict__ gL, int *__restrict__ code,
-                             double *__restrict__ abc, double *__restrict__ gw, double w_reg,
+                             double *__restrict__ abc, double *__restrict__ gw, double alpha, double w_reg,
                              double *__restrict__ gwsum) {
     __shared__ double sh[4];
     double acc = 0;
@@ -822,9 +825,11 @@ __global__ __launch_bounds__(256) void b_step5(Dims d, const double *__restrict_
         XYZ(d, i);
         st(gL, i, d3(0, 0, 0));
         if (!interior(d, x, y, z)) {
+            for (int k = 0; k < 9; ++k) gJ[gj_at(d, i, k)] = 0.0;
             code[i] = 0;
             continue;
         }
+        const D3 g9 = ld(gpinv, i), w1 = ld(v1, i);  // gpinv still holds gpinv1: steps 1-4 read it only
         double m[3][3], dg[3][3], J[9];
         fv_jacobian(d, x, y, z, pinv, J);  // recomputed from psi_inv, as in step 1
         for (int r = 0; r < 3; ++r)
@@ -855,7 +860,8 @@ __global__ __launch_bounds__(256) void b_step5(Dims d, const double *__restrict_
         const double gtp = -det * (g.x * s.x + g.y * s.y + g.z * s.z);
         det_ref_grad(m, dg);
         for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 3; ++c) gJ[9 * i + 3 * r + c] += gdet * dg[r][c];
+            for (int c = 0; c < 3; ++c)
+                gJ[gj_at(d, i, 3 * r + c)] = -alpha * comp(g9, r) * comp(w1, c) + gdet * dg[r][c];
         acc += det * (g.x * lo.x + g.y * lo.y + g.z * lo.z);
         D3 o = d3(0, 0, 0);
         for (int q = 0; q < 3; ++q) {
@@ -880,17 +886,53 @@ __global__ __launch_bounds__(GT_THREADS) void b_step7(Dims d, int tiles_x, int t
     });
 }
 
+// The z-window of one column for step 8: gL, the z-column (entries 2, 5, 8) of gJ and grho1 at
+// z-1, z, z+1, rho at z and z+1, v.z at z-1 (zeros outside the grid; no branch reads them there)
+struct B8Win {
+    D3 gl[3], gj[3];
+    double gr[3], r0, rp, vzm;
+};
+__device__ __forceinline__ D3 ld_jz(Dims d, const double *__restrict__ gJ, int i, bool ok) {
+    return ok ? d3(gJ[gj_at(d, i, 2)], gJ[gj_at(d, i, 5)], gJ[gj_at(d, i, 8)]) : d3(0, 0, 0);
+}
+__device__ __forceinline__ double ld1_or0(const double *__restrict__ p, int i, bool ok) { return ok ? p[i] : 0.0; }
+
 // Step 8: transposes of F3, F2 and F1 gathered into voxel i:
 //   L(j) = 6 psi(j) - (sum of its 6 neighbours), interior j:       gpsi(i)
 //   J(j)[r][a] = (psi_inv(j + e_a) - psi_inv(j - e_a))[r] / 2:      gpinv(i)
 //   rho1(j) = rho(j) + alpha div(rho, v)(j) (upwind fluxes):        grho(i) = grho1(i) + ..., gv(i)
-__global__ void b_step8(Dims d, const double *__restrict__ gL, const double *__restrict__ gJ,
-                        const double *__restrict__ rho, const double *__restrict__ v,
-                        const double *__restrict__ grho1, double *__restrict__ gpsi, double *__restrict__ gpinv,
-                        double *__restrict__ grho, double *__restrict__ gv, double alpha) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        const int sy = d.X, sz = d.X * d.Y;
+// One thread per (x, y) column of a 64 x 4 tile marching a chunk of zc planes, the z-neighbours
+// from the column's window and the x- and y-neighbours from memory.
+__global__ __launch_bounds__(256) void b_step8(Dims d, int tiles_x, int tiles_y, int zc,
+                                                const double *__restrict__ gL, const double *__restrict__ gJ,
+                                                const double *__restrict__ rho, const double *__restrict__ v,
+                                                const double *__restrict__ grho1, double *__restrict__ gpsi,
+                                                double *__restrict__ gpinv, double *__restrict__ grho,
+                                                double *__restrict__ gv, double alpha) {
+    const int b = blockIdx.x;
+    const int x = (b % tiles_x) * MT_X + (int)(threadIdx.x % MT_X);
+    const int y = ((b / tiles_x) % tiles_y) * MT_Y + (int)(threadIdx.x / MT_X);
+    const int z0 = (b / (tiles_x * tiles_y)) * zc, z1 = min(z0 + zc, d.Z);
+    if (x >= d.X || y >= d.Y) return;
+    const int sy = d.X, sz = d.X * d.Y, c0 = x + d.X * y;
+    B8Win w;
+    for (int k = 0; k < 3; ++k) {  // planes z0-1 .. z0+1
+        const int p = z0 - 1 + k;
+        const bool ok = p >= 0 && p < d.Z;
+        w.gl[k] = ld_or0(gL, c0 + sz * p, ok);
+        w.gj[k] = ld_jz(d, gJ, c0 + sz * p, ok);
+        w.gr[k] = ld1_or0(grho1, c0 + sz * p, ok);
+    }
+    w.r0 = rho[c0 + sz * z0];
+    w.rp = ld1_or0(rho, c0 + sz * (z0 + 1), z0 + 1 < d.Z);
+    w.vzm = ld1_or0(v, 3 * (c0 + sz * (z0 - 1)) + 2, z0 >= 1);
+    for (int z = z0; z < z1; ++z) {
+        const int i = c0 + sz * z;
+        // the entries of the next step (plane z+2), loaded a step ahead
+        const bool okn = z + 2 < d.Z;
+        const int in2 = i + 2 * sz;
+        const D3 n_gl = ld_or0(gL, in2, okn), n_gj = ld_jz(d, gJ, in2, okn);
+        const double n_gr = ld1_or0(grho1, in2, okn), n_r = ld1_or0(rho, in2, okn);
         const int nb[6] = {i + 1, i - 1, i + sy, i - sy, i + sz, i - sz};
         const bool nin[6] = {interior(d, x + 1, y, z), interior(d, x - 1, y, z), interior(d, x, y + 1, z),
                              interior(d, x, y - 1, z), interior(d, x, y, z + 1), interior(d, x, y, z - 1)};
@@ -898,12 +940,12 @@ __global__ void b_step8(Dims d, const double *__restrict__ gL, const double *__r
         // F3
         D3 gp = ld(gpsi, i);
         if (in) {
-            const D3 g = ld(gL, i);
+            const D3 g = w.gl[1];
             gp = d3(gp.x + 6 * g.x, gp.y + 6 * g.y, gp.z + 6 * g.z);
         }
         for (int k = 0; k < 6; ++k)
             if (nin[k]) {
-                const D3 g = ld(gL, nb[k]);
+                const D3 g = k == 4 ? w.gl[2] : k == 5 ? w.gl[0] : ld(gL, nb[k]);
                 gp = d3(gp.x - g.x, gp.y - g.y, gp.z - g.z);
             }
         st(gpsi, i, gp);
@@ -911,49 +953,53 @@ __global__ void b_step8(Dims d, const double *__restrict__ gL, const double *__r
         D3 gq = ld(gpinv, i);
         for (int ax = 0; ax < 3; ++ax) {
             if (nin[2 * ax]) {
-                const double *g = gJ + 9 * nb[2 * ax];
-                gq = d3(gq.x - 0.5 * g[ax], gq.y - 0.5 * g[3 + ax], gq.z - 0.5 * g[6 + ax]);
+                const int j = nb[2 * ax];
+                const D3 c = ax == 2 ? w.gj[2]
+                                     : d3(gJ[gj_at(d, j, ax)], gJ[gj_at(d, j, 3 + ax)], gJ[gj_at(d, j, 6 + ax)]);
+                gq = d3(gq.x - 0.5 * c.x, gq.y - 0.5 * c.y, gq.z - 0.5 * c.z);
             }
             if (nin[2 * ax + 1]) {
-                const double *g = gJ + 9 * nb[2 * ax + 1];
-                gq = d3(gq.x + 0.5 * g[ax], gq.y + 0.5 * g[3 + ax], gq.z + 0.5 * g[6 + ax]);
+                const int j = nb[2 * ax + 1];
+                const D3 c = ax == 2 ? w.gj[0]
+                                     : d3(gJ[gj_at(d, j, ax)], gJ[gj_at(d, j, 3 + ax)], gJ[gj_at(d, j, 6 + ax)]);
+                gq = d3(gq.x + 0.5 * c.x, gq.y + 0.5 * c.y, gq.z + 0.5 * c.z);
             }
         }
         st(gpinv, i, gq);
         // F1: the fluxes t_a1(j) = v(j).a > 0 ? -v(j).a rho(j) : -v(j).a rho(j + e_a) and
         // t_a2(j) = w > 0 ? w rho(j - e_a) : w rho(j), w = v(j - e_a).a
-        double gr = grho1[i];
+        double gr = w.gr[1];
         D3 ga = d3(0, 0, 0);
         const D3 vi = ld(v, i);
-        const double ri = rho[i];
+        const double ri = w.r0;
         const int step[3] = {1, sy, sz};
         for (int ax = 0; ax < 3; ++ax) {
             const double via = comp(vi, ax);
             double gax = 0;
             if (in) {  // j = i
-                const double g = alpha * grho1[i];
+                const double g = alpha * w.gr[1];
                 if (via > 0) {
                     gr += -g * via;
                     gax += -g * ri;
                 } else {
-                    gax += -g * rho[i + step[ax]];
+                    gax += -g * (ax == 2 ? w.rp : rho[i + step[ax]]);
                 }
-                const double w = v[3 * (i - step[ax]) + ax];
-                if (!(w > 0)) gr += g * w;
+                const double wv = ax == 2 ? w.vzm : v[3 * (i - step[ax]) + ax];
+                if (!(wv > 0)) gr += g * wv;
             }
             if (nin[2 * ax + 1]) {  // j = i - e_a: i is j + e_a in t_a1(j)
                 const int j = i - step[ax];
-                const double g = alpha * grho1[j], vj = v[3 * j + ax];
+                const double g = alpha * (ax == 2 ? w.gr[0] : grho1[j]), vj = ax == 2 ? w.vzm : v[3 * j + ax];
                 if (!(vj > 0)) gr += -g * vj;
             }
             if (nin[2 * ax]) {  // j = i + e_a: i is j - e_a in t_a2(j), w = v(i).a
                 const int j = i + step[ax];
-                const double g = alpha * grho1[j];
+                const double g = alpha * (ax == 2 ? w.gr[2] : grho1[j]);
                 if (via > 0) {
                     gr += g * via;
                     gax += g * ri;
                 } else {
-                    gax += g * rho[j];
+                    gax += g * (ax == 2 ? w.rp : rho[j]);
                 }
             }
             if (ax == 0) ga.x = gax; else if (ax == 1) ga.y = gax; else ga.z = gax;
@@ -961,6 +1007,19 @@ __global__ void b_step8(Dims d, const double *__restrict__ gL, const double *__r
         grho[i] = gr;
         const D3 o = ld(gv, i);
         st(gv, i, d3(o.x + ga.x, o.y + ga.y, o.z + ga.z));
+        // slide to plane z+1
+        w.gl[0] = w.gl[1];
+        w.gl[1] = w.gl[2];
+        w.gl[2] = n_gl;
+        w.gj[0] = w.gj[1];
+        w.gj[1] = w.gj[2];
+        w.gj[2] = n_gj;
+        w.gr[0] = w.gr[1];
+        w.gr[1] = w.gr[2];
+        w.gr[2] = n_gr;
+        w.r0 = w.rp;
+        w.rp = n_r;
+        w.vzm = vi.z;
     }
 }
 
@@ -1270,21 +1329,21 @@ int backward_iteration(fr_adjoint *h, const State &in, const State &nx, const In
     hipStream_t s = h->stream;
     const double a = h->alpha;
     double *gr1 = h->grho[*cur], *gr0 = h->grho[1 - *cur];
-    frd::b_step1<<<g, TPB, 0, s>>>(d, in.pinv, nx.v, in.psi, h->gpinv, h->gJ, h->gv, h->gpsi, h->code, h->abc, h->gw,
-                                   a);
+    frd::b_step1<<<g, TPB, 0, s>>>(d, in.pinv, nx.v, in.psi, h->gpinv, h->gv, h->gpsi, h->code, h->abc, h->gw, a);
     frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.psi, h->gw, h->gv);
     frd::b_step3<<<h->gt_blocks, frd::GT_THREADS, 0, s>>>(d, h->gt_tiles_x, h->gt_tiles_y, h->gt_zc, h->code, h->abc,
                                                         h->gw, nx.rho, it.dlv, it.gU, h->gv, h->gdvv, h->gdlv,
                                                         h->ggU, gr1, a, m1t, h->tau, m2, h->part_tau);
     frd::k_sum_accumulate<<<1, 256, 0, s>>>(h->part_tau, h->gt_blocks, h->scal + 2);
     frd::b_step4<<<h->mt_blocks, 256, 0, s>>>(d, h->mt_tiles_x, h->mt_tiles_y, h->mt_zc, in.v, h->gdvv, h->gdlv, h->gv);
-    frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.L, in.pinv, h->ggU, h->gJ, h->gpinv,
-                                   h->gL, h->code, h->abc, h->gw, h->w_reg, h->part_w);
+    frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.L, in.pinv, nx.v, h->ggU, h->gJ, h->gpinv,
+                                   h->gL, h->code, h->abc, h->gw, a, h->w_reg, h->part_w);
     frd::k_sum_accumulate<<<1, 256, 0, s>>>(h->part_w, g, h->scal + 1);
     frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.pinv, h->gw, h->gL);
     frd::b_step7<<<h->gt_blocks, frd::GT_THREADS, 0, s>>>(d, h->gt_tiles_x, h->gt_tiles_y, h->gt_zc, h->code, h->abc,
                                                         h->gw, h->gL);
-    frd::b_step8<<<g, TPB, 0, s>>>(d, h->gL, h->gJ, in.rho, in.v, gr1, h->gpsi, h->gpinv, gr0, h->gv, a);
+    frd::b_step8<<<h->mt_blocks, 256, 0, s>>>(d, h->mt_tiles_x, h->mt_tiles_y, h->mt_zc, h->gL, h->gJ, in.rho, in.v, gr1,
+                                              h->gpsi, h->gpinv, gr0, h->gv, a);
     ADJ_TRY(hipGetLastError());
     *cur = 1 - *cur;
     return FR_OK;
