@@ -627,37 +627,33 @@ __global__ void b_step1(Dims d, const double *__restrict__ pinv, const double *_
 
 // Step 3: gv1 completed by the records' gather, then the adjoint of F7
 //   v1 = v - alpha (m1t v + dvv - tau dlv + q gU), q = m2 / (|rho1| + 1e-5f)   (interior; v1 = v elsewhere):
-//   gv = (1 - alpha m1t) gv1, gdvv = -alpha gv1, gdlv = alpha tau gv1, ggU = -alpha q gv1,
+//   gv = (1 - alpha m1t) gv1, and gdvv = -alpha gv1, gdlv = alpha tau gv1, ggU = -alpha q gv1, which
+//   steps 4 and 5 form from gv1 itself (stored here as gg) where they read them,
 //   grho1 += -alpha (gv1 . gU) dq/drho1, dE/dtau += alpha gv1 . dlv (one partial sum per block into
 //   gtau[blockIdx.x], added up in block order by k_sum_accumulate).
 // gv holds gv1 on entry and gv on exit.
 __global__ __launch_bounds__(GT_THREADS) void b_step3(Dims d, int tiles_x, int tiles_y, int zc, const int *code,
                                                       const double *abc, const double *gw, const double *rho1,
-                                                      const double *dlv, const double *gU, double *gv, double *gdvv,
-                                                      double *gdlv, double *ggU, double *grho1, double alpha,
-                                                      double m1t, double tau, double m2, double *gtau) {
+                                                      const double *dlv, const double *gU, double *gv, double *gg,
+                                                      double *grho1, double alpha, double m1t, double m2,
+                                                      double *gtau) {
     __shared__ double sh[4];
     double acc = 0;
     rec_gather_tile(d, tiles_x, tiles_y, zc, code, abc, gw, [&](int i, int x, int y, int z, D3 sc) {
         const D3 own = ld(gv, i);
         const D3 g = d3(own.x + sc.x, own.y + sc.y, own.z + sc.z);
         if (interior(d, x, y, z)) {
-            const double den = fabs(rho1[i]) + (double)1e-5f, q = m2 / den;
+            const double den = fabs(rho1[i]) + (double)1e-5f;
             const D3 u = ld(gU, i), l = ld(dlv, i);
             st(gv, i, d3((1 - alpha * m1t) * g.x, (1 - alpha * m1t) * g.y, (1 - alpha * m1t) * g.z));
-            st(gdvv, i, d3(-alpha * g.x, -alpha * g.y, -alpha * g.z));
-            st(gdlv, i, d3(alpha * tau * g.x, alpha * tau * g.y, alpha * tau * g.z));
-            st(ggU, i, d3(-alpha * q * g.x, -alpha * q * g.y, -alpha * q * g.z));
             const double gq = -alpha * (g.x * u.x + g.y * u.y + g.z * u.z);
             const double sgn = rho1[i] > 0 ? 1.0 : rho1[i] < 0 ? -1.0 : 0.0;
             grho1[i] += gq * (-m2 * sgn / (den * den));
             acc += alpha * (g.x * l.x + g.y * l.y + g.z * l.z);
         } else {
             st(gv, i, g);
-            st(gdvv, i, d3(0, 0, 0));
-            st(gdlv, i, d3(0, 0, 0));
-            st(ggU, i, d3(0, 0, 0));
         }
+        st(gg, i, g);
     });
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) gtau[blockIdx.x] = s;
@@ -671,23 +667,28 @@ __device__ __forceinline__ bool in_lap(Dims d, int x, int y, int z) {
 // Step 4: transposes of F6 gathered into gv: dvv = (Dv)v (interior) and dlv, the velocity
 // Laplacian (its range; centre 3 times, "far" neighbours (+-2) once, "near" ones (+-1) -2 times,
 // chosen by the sign of the centre's component, with y-1 / z-1 as the near neighbour always).
-// The z-window of one column for step 4: v at z-2..z+2, gdvv at z-1..z+1, gdlv at z-2..z+2
-// (zeros outside the grid; no branch below reads them there)
+// The z-window of one column for step 4: v and gg (gv1 after step 3's gather) at z-2..z+2 (zeros
+// outside the grid; no branch below reads them there)
 struct B4Win {
-    D3 v[5], g[3], l[5];
+    D3 v[5], gg[5];
 };
+// the adjoints of (Dv)v and of the velocity Laplacian at a voxel j, from gv1(j) as step 3's
+// transpose of F7 forms them: gdvv = -alpha gv1, gdlv = alpha tau gv1 (read only at interior j)
+__device__ __forceinline__ D3 dvv_of(D3 g, double alpha) { return d3(-alpha * g.x, -alpha * g.y, -alpha * g.z); }
+__device__ __forceinline__ D3 dlv_of(D3 g, double alpha, double tau) {
+    return d3(alpha * tau * g.x, alpha * tau * g.y, alpha * tau * g.z);
+}
 // the transposes of step 4 gathered for voxel (x, y, z): its z-neighbours from the window, the
 // x- and y-neighbours from memory (rows the block's other lanes and waves load at the same time)
 __device__ __forceinline__ D3 b4_voxel(Dims d, int x, int y, int z, int i, const double *__restrict__ v,
-                                       const double *__restrict__ gdvv, const double *__restrict__ gdlv,
-                                       const B4Win &w) {
+                                       const double *__restrict__ gg, double alpha, double tau, const B4Win &w) {
     const int sy = d.X;
     const D3 vi = w.v[2];
     D3 a = d3(0, 0, 0);
     // (Dv)v, voxel i as the centre
     if (interior(d, x, y, z)) {
         const D3 c = vi, xp = ld(v, i + 1), xm = ld(v, i - 1), yp = ld(v, i + sy), ym = ld(v, i - sy),
-                 zp = w.v[3], zm = w.v[1], G = w.g[1];
+                 zp = w.v[3], zm = w.v[1], G = dvv_of(w.gg[2], alpha);
         a.x += 0.5 * G.x * (2 * mx(c.x) - 2 * mn(c.x));
         if (c.y > 0) { a.y += G.x * (c.x - ym.x); a.x += G.x * c.y; }
         else { a.y += G.x * (yp.x - c.x); a.x -= G.x * c.y; }
@@ -707,39 +708,39 @@ __device__ __forceinline__ D3 b4_voxel(Dims d, int x, int y, int z, int i, const
     // voxel i as the x+1 / x-1 / y+1 / y-1 / z+1 / z-1 neighbour of an interior voxel j
     if (interior(d, x - 1, y, z)) {
         const int j = i - 1;
-        const D3 cj = ld(v, j), G = ld(gdvv, j);
+        const D3 cj = ld(v, j), G = dvv_of(ld(gg, j), alpha);
         a.x += 0.5 * G.x * (2 * mn(vi.x) - 2 * mx(vi.x));
         if (!(cj.x > 0)) { a.y += G.y * cj.x; a.z += G.z * cj.x; }
     }
     if (interior(d, x + 1, y, z)) {
         const int j = i + 1;
-        const D3 cj = ld(v, j), G = ld(gdvv, j);
+        const D3 cj = ld(v, j), G = dvv_of(ld(gg, j), alpha);
         if (cj.x > 0) { a.y -= G.y * cj.x; a.z -= G.z * cj.x; }
     }
     if (interior(d, x, y - 1, z)) {
         const int j = i - sy;
-        const D3 cj = ld(v, j), G = ld(gdvv, j);
+        const D3 cj = ld(v, j), G = dvv_of(ld(gg, j), alpha);
         if (!(cj.y > 0)) { a.x += G.x * cj.y; a.z += G.z * cj.y; }
         a.y += 0.5 * G.y * (2 * mn(vi.y) - 2 * mx(vi.y));
     }
     if (interior(d, x, y + 1, z)) {
         const int j = i + sy;
-        const D3 cj = ld(v, j), G = ld(gdvv, j);
+        const D3 cj = ld(v, j), G = dvv_of(ld(gg, j), alpha);
         if (cj.y > 0) { a.x -= G.x * cj.y; a.z -= G.z * cj.y; }
     }
     if (interior(d, x, y, z - 1)) {
-        const D3 cj = w.v[1], G = w.g[0];
+        const D3 cj = w.v[1], G = dvv_of(w.gg[1], alpha);
         if (!(cj.z > 0)) { a.x += G.x * cj.z; a.y += G.y * cj.z; }
         a.z += 0.5 * G.z * (2 * mn(vi.z) - 2 * mx(vi.z));
     }
     if (interior(d, x, y, z + 1)) {
-        const D3 cj = w.v[3], G = w.g[2];
+        const D3 cj = w.v[3], G = dvv_of(w.gg[3], alpha);
         if (cj.z > 0) { a.x -= G.x * cj.z; a.y -= G.y * cj.z; }
     }
     // the velocity Laplacian: dlv(j) = -(sum over axes of far - 2 near + centre)
     const auto add_v = [&](D3 g, double f) { a = d3(a.x + f * g.x, a.y + f * g.y, a.z + f * g.z); };
-    const auto add = [&](int j, double f) { add_v(ld(gdlv, j), f); };
-    if (in_lap(d, x, y, z)) add_v(w.l[2], -3.0);
+    const auto add = [&](int j, double f) { add_v(dlv_of(ld(gg, j), alpha, tau), f); };
+    if (in_lap(d, x, y, z)) add_v(dlv_of(w.gg[2], alpha, tau), -3.0);
     if (in_lap(d, x - 2, y, z) && v[3 * (i - 2)] < 0) add(i - 2, -1.0);
     if (in_lap(d, x + 2, y, z) && !(v[3 * (i + 2)] < 0)) add(i + 2, -1.0);
     if (in_lap(d, x - 1, y, z) && v[3 * (i - 1)] < 0) add(i - 1, 2.0);
@@ -749,9 +750,9 @@ __device__ __forceinline__ D3 b4_voxel(Dims d, int x, int y, int z, int i, const
         if (v[3 * (i - 2 * sy) + 1] < 0) add(i - 2 * sy, -1.0);
     if (in_lap(d, x, y + 2, z) && !(v[3 * (i + 2 * sy) + 1] < 0)) add(i + 2 * sy, -1.0);
     if (in_lap(d, x, y + 1, z)) add(i + sy, 2.0);
-    if (in_lap(d, x, y, z - 2) && w.v[0].z < 0) add_v(w.l[0], -1.0);
-    if (in_lap(d, x, y, z + 2) && !(w.v[4].z < 0)) add_v(w.l[4], -1.0);
-    if (in_lap(d, x, y, z + 1)) add_v(w.l[3], 2.0);
+    if (in_lap(d, x, y, z - 2) && w.v[0].z < 0) add_v(dlv_of(w.gg[0], alpha, tau), -1.0);
+    if (in_lap(d, x, y, z + 2) && !(w.v[4].z < 0)) add_v(dlv_of(w.gg[4], alpha, tau), -1.0);
+    if (in_lap(d, x, y, z + 1)) add_v(dlv_of(w.gg[3], alpha, tau), 2.0);
     return a;
 }
 __device__ __forceinline__ D3 ld_or0(const double *__restrict__ p, int i, bool ok) {
@@ -762,11 +763,11 @@ __device__ __forceinline__ D3 ld_or0(const double *__restrict__ p, int i, bool o
 // Laplacian (its range; centre 3 times, "far" neighbours (+-2) once, "near" ones (+-1) -2 times,
 // chosen by the sign of the centre's component, with y-1 / z-1 as the near neighbour always).
 // One thread per (x, y) column of a 64 x 4 tile marching a chunk of zc planes, its column's
-// z-neighbours in registers, so that each plane of v, gdvv and gdlv is read from memory once per
-// column (the one-thread-per-voxel form re-read them from beyond the L2 about 3 times).
+// z-neighbours in registers, so that each plane of v and gg is read from memory once per column
+// (the one-thread-per-voxel form re-read them from beyond the L2 about 3 times).
 __global__ __launch_bounds__(256) void b_step4(Dims d, int tiles_x, int tiles_y, int zc,
-                                                const double *__restrict__ v, const double *__restrict__ gdvv,
-                                                const double *__restrict__ gdlv, double *__restrict__ gv) {
+                                                const double *__restrict__ v, const double *__restrict__ gg,
+                                                double alpha, double tau, double *__restrict__ gv) {
     const int b = blockIdx.x;
     const int x = (b % tiles_x) * MT_X + (int)(threadIdx.x % MT_X);
     const int y = ((b / tiles_x) % tiles_y) * MT_Y + (int)(threadIdx.x / MT_X);
@@ -778,29 +779,21 @@ __global__ __launch_bounds__(256) void b_step4(Dims d, int tiles_x, int tiles_y,
         const int p = z0 - 2 + k;
         const bool ok = p >= 0 && p < d.Z;
         w.v[k] = ld_or0(v, c0 + sz * p, ok);
-        w.l[k] = ld_or0(gdlv, c0 + sz * p, ok);
-    }
-    for (int k = 0; k < 3; ++k) {  // planes z0-1 .. z0+1
-        const int p = z0 - 1 + k;
-        w.g[k] = ld_or0(gdvv, c0 + sz * p, p >= 0 && p < d.Z);
+        w.gg[k] = ld_or0(gg, c0 + sz * p, ok);
     }
     for (int z = z0; z < z1; ++z) {
-        // the entries of the next step, loaded a step ahead: plane z+3 of v and gdlv, z+2 of gdvv
-        const D3 nv = ld_or0(v, c0 + sz * (z + 3), z + 3 < d.Z), nl = ld_or0(gdlv, c0 + sz * (z + 3), z + 3 < d.Z),
-                 ng = ld_or0(gdvv, c0 + sz * (z + 2), z + 2 < d.Z);
+        // the entries of the next step, plane z+3, loaded a step ahead
+        const D3 nv = ld_or0(v, c0 + sz * (z + 3), z + 3 < d.Z), ng = ld_or0(gg, c0 + sz * (z + 3), z + 3 < d.Z);
         const int i = c0 + sz * z;
-        const D3 a = b4_voxel(d, x, y, z, i, v, gdvv, gdlv, w);
+        const D3 a = b4_voxel(d, x, y, z, i, v, gg, alpha, tau, w);
         const D3 o = ld(gv, i);
         st(gv, i, d3(o.x + a.x, o.y + a.y, o.z + a.z));
         for (int k = 0; k < 4; ++k) {
             w.v[k] = w.v[k + 1];
-            w.l[k] = w.l[k + 1];
+            w.gg[k] = w.gg[k + 1];
         }
         w.v[4] = nv;
-        w.l[4] = nl;
-        w.g[0] = w.g[1];
-        w.g[1] = w.g[2];
-        w.g[2] = ng;
+        w.gg[4] = ng;
     }
 }
 
@@ -815,10 +808,10 @@ __global__ __launch_bounds__(256) void b_step4(Dims d, int tiles_x, int tiles_y,
 __global__ __launch_bounds__(256) void b_step5(Dims d, const double *__restrict__ S, const double *__restrict__ gS,
                              const double *__restrict__ T, const double *__restrict__ L,
                              const double *__restrict__ pinv, const double *__restrict__ v1,
-                             const double *__restrict__ ggU, double *__restrict__ gJ,
-                             double *__restrict__ gpinv, double *__restrict__ gL, int *__restrict__ code,
-                             double *__restrict__ abc, double *__restrict__ gw, double alpha, double w_reg,
-                             double *__restrict__ gwsum) {
+                             const double *__restrict__ rho1, const double *__restrict__ gg,
+                             double *__restrict__ gJ, double *__restrict__ gpinv, double *__restrict__ gL,
+                             int *__restrict__ code, double *__restrict__ abc, double *__restrict__ gw, double alpha,
+                             double m2, double w_reg, double *__restrict__ gwsum) {
     __shared__ double sh[4];
     double acc = 0;
     FOR_VOXELS(d, i) {
@@ -853,7 +846,10 @@ __global__ __launch_bounds__(256) void b_step5(Dims d, const double *__restrict_
         const double tp = tri(f, k.a, k.b, k.c);
         const D3 dt = tri_dpos(f, k);
         const double det = det_ref(m), diff = S[i] - tp;
-        const D3 g = ld(ggU, i), s = ld(gS, i);
+        // ggU = -alpha q gv1, q = m2 / (|rho1| + 1e-5f), as step 3's transpose of F7 forms it
+        const double q = m2 / (fabs(rho1[i]) + (double)1e-5f);
+        const D3 g1 = ld(gg, i), s = ld(gS, i);
+        const D3 g = d3(-alpha * q * g1.x, -alpha * q * g1.y, -alpha * q * g1.z);
         const double gdet = g.x * (diff * s.x + w_reg * lo.x) + g.y * (diff * s.y + w_reg * lo.y) +
                             g.z * (diff * s.z + w_reg * lo.z);
         const D3 glo = d3(w_reg * det * g.x, w_reg * det * g.y, w_reg * det * g.z);
@@ -1216,7 +1212,7 @@ struct fr_adjoint {
     // adjoint of the state (grho double-buffered: its kernel scatters into the older iteration's)
     double *grho[2] = {nullptr, nullptr}, *gv = nullptr, *gpsi = nullptr, *gpinv = nullptr;
     // adjoints of one iteration's intermediates
-    double *gJ = nullptr, *gL = nullptr, *ggU = nullptr, *gdvv = nullptr, *gdlv = nullptr;
+    double *gJ = nullptr, *gL = nullptr, *gg = nullptr;
     // records of the trilinear transposes: per source voxel a code, the cell fractions, the adjoint
     int *code = nullptr;
     double *abc = nullptr, *gw = nullptr;
@@ -1332,12 +1328,13 @@ int backward_iteration(fr_adjoint *h, const State &in, const State &nx, const In
     frd::b_step1<<<g, TPB, 0, s>>>(d, in.pinv, nx.v, in.psi, h->gpinv, h->gv, h->gpsi, h->code, h->abc, h->gw, a);
     frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.psi, h->gw, h->gv);
     frd::b_step3<<<h->gt_blocks, frd::GT_THREADS, 0, s>>>(d, h->gt_tiles_x, h->gt_tiles_y, h->gt_zc, h->code, h->abc,
-                                                        h->gw, nx.rho, it.dlv, it.gU, h->gv, h->gdvv, h->gdlv,
-                                                        h->ggU, gr1, a, m1t, h->tau, m2, h->part_tau);
+                                                        h->gw, nx.rho, it.dlv, it.gU, h->gv, h->gg, gr1, a, m1t,
+                                                        m2, h->part_tau);
     frd::k_sum_accumulate<<<1, 256, 0, s>>>(h->part_tau, h->gt_blocks, h->scal + 2);
-    frd::b_step4<<<h->mt_blocks, 256, 0, s>>>(d, h->mt_tiles_x, h->mt_tiles_y, h->mt_zc, in.v, h->gdvv, h->gdlv, h->gv);
-    frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.L, in.pinv, nx.v, h->ggU, h->gJ, h->gpinv,
-                                   h->gL, h->code, h->abc, h->gw, a, h->w_reg, h->part_w);
+    frd::b_step4<<<h->mt_blocks, 256, 0, s>>>(d, h->mt_tiles_x, h->mt_tiles_y, h->mt_zc, in.v, h->gg, a, h->tau,
+                                              h->gv);
+    frd::b_step5<<<g, TPB, 0, s>>>(d, h->S, h->gS, h->T, it.L, in.pinv, nx.v, nx.rho, h->gg, h->gJ, h->gpinv,
+                                   h->gL, h->code, h->abc, h->gw, a, m2, h->w_reg, h->part_w);
     frd::k_sum_accumulate<<<1, 256, 0, s>>>(h->part_w, g, h->scal + 1);
     frd::b_scatter_irregular<<<g, TPB, 0, s>>>(d, h->code, in.pinv, h->gw, h->gL);
     frd::b_step7<<<h->gt_blocks, frd::GT_THREADS, 0, s>>>(d, h->gt_tiles_x, h->gt_tiles_y, h->gt_zc, h->code, h->abc,
@@ -1439,7 +1436,7 @@ extern "C" int fr_adjoint_create(const fr_params *p, int K, int ckpt, fr_adjoint
         (rc = h->alloc(&h->gpsi, 3 * n)) || (rc = h->alloc(&h->gpinv, 3 * n)) || (rc = h->alloc(&h->gJ, 9 * n)) ||
         (rc = h->alloc(&h->gL, 3 * n)) || (rc = h->alloc(&h->code, n)) || (rc = h->alloc(&h->abc, 3 * n)) ||
         (rc = h->alloc(&h->gw, 3 * n)) ||
-        (rc = h->alloc(&h->ggU, 3 * n)) || (rc = h->alloc(&h->gdvv, 3 * n)) || (rc = h->alloc(&h->gdlv, 3 * n)) ||
+        (rc = h->alloc(&h->gg, 3 * n)) ||
         (rc = h->alloc(&h->scal, 4)) || (rc = h->alloc(&h->partials, (size_t)h->grid)) ||
         (rc = h->alloc(&h->part_tau, (size_t)h->gt_tiles_x * h->gt_tiles_y * h->d.Z)) || (rc = h->alloc(&h->part_w, (size_t)h->grid)))
         return fail(rc);
