@@ -518,16 +518,18 @@ extern "C" int fr_solver_run(fr_ctx *c) {
     if (!c) return fail(FR_ERR_INVALID, "null context");
     if (!c->initialised) return fail(FR_ERR_STATE, "fr_solver_run before fr_solver_init");
     int rc;
-    // an earlier run not yet collected (it may have stopped early) settles the buffer parity first
-    if (c->have_run && (rc = collect_run(c))) return rc;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(c->stream, &cs));
+    // an earlier run not yet collected (it may have stopped early) settles the buffer parity first;
+    // a caller's capture cannot wait for it (its replays start after fr_solver_init, which writes
+    // both buffers)
+    if (cs == hipStreamCaptureStatusNone && c->have_run && (rc = collect_run(c))) return rc;
     // the control words are cleared by a kernel: eagerly before the context's own captured loop,
     // as a kernel node of a caller's capture
     if ((rc = reset_control(c))) return rc;
     const fr_params &p = c->prm;
     const int par = c->cur;
     c->run_start_buf = par;  // ping-pong index at the start of the run
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    HIP_TRY(hipStreamIsCapturing(c->stream, &cs));
     if (cs != hipStreamCaptureStatusNone) {
         // the caller is capturing this stream: the launches go into the caller's graph
         HIP_TRY(hipEventRecord(c->ev0, c->stream));
