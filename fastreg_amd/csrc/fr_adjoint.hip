@@ -168,45 +168,64 @@ __device__ __forceinline__ void det_ref_grad(const double m[3][3], double g[3][3
     g[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
 }
 
+// tiles of the z-marching one-thread-per-column kernels (64 x 4 columns per block)
+constexpr int MT_X = 64, MT_Y = 4, MT_BLOCKS_PER_CU = 8;
+
 // ---- forward steps ----------------------------------------------------------------------------
 
 // The forward steps of one voxel.  F3 (L) needs psi at the neighbours and F8 (psi1) v1 at the
 // taps, so they are kernels of their own; the other steps of a voxel read only the old state, L
 // and the images, and run in one kernel (f_step) that keeps their results in registers.
 
-// F1: rho1 = rho + alpha * div on the interior; rho elsewhere
-__device__ __forceinline__ double fv_density(Dims d, int x, int y, int z, int i, const double *rho, const double *v,
-                                             double alpha) {
-    const double p = rho[i];
+// The z-window of one column for the forward step: v at z-2..z+2, psi_inv and rho at z-1..z+1
+// (zeros outside the grid; no branch reads them there)
+struct FWin {
+    D3 v[5], p[3];
+    double r[3];
+};
+
+// F1: rho1 = rho + alpha * div on the interior; rho elsewhere (the z-neighbours from the window)
+__device__ __forceinline__ double fv_density(Dims d, int x, int y, int z, const double *rho, const double *v,
+                                             const FWin &w, double alpha) {
+    const double p = w.r[1];
     if (!interior(d, x, y, z)) return p;
-    const D3 vel = ld(v, i);
+    const D3 vel = w.v[2];
     const double t11 = vel.x > 0 ? -vel.x * p : -vel.x * rho[lin(d, x + 1, y, z)];
     const double vxm = v[3 * lin(d, x - 1, y, z)];
     const double t12 = vxm > 0 ? vxm * rho[lin(d, x - 1, y, z)] : vxm * p;
     const double t21 = vel.y > 0 ? -vel.y * p : -vel.y * rho[lin(d, x, y + 1, z)];
     const double vym = v[3 * lin(d, x, y - 1, z) + 1];
     const double t22 = vym > 0 ? vym * rho[lin(d, x, y - 1, z)] : vym * p;
-    const double t31 = vel.z > 0 ? -vel.z * p : -vel.z * rho[lin(d, x, y, z + 1)];
-    const double vzm = v[3 * lin(d, x, y, z - 1) + 2];
-    const double t32 = vzm > 0 ? vzm * rho[lin(d, x, y, z - 1)] : vzm * p;
+    const double t31 = vel.z > 0 ? -vel.z * p : -vel.z * w.r[2];
+    const double vzm = w.v[1].z;
+    const double t32 = vzm > 0 ? vzm * w.r[0] : vzm * p;
     const double div = ((((t11 + t12) + t21) + t22) + t31) + t32;
     return fma(alpha, div, p);
 }
 
 // F2: J = D psi_inv (m[3 r + c]: row r = component, column c = d/dx, d/dy, d/dz), interior; 0
 // elsewhere
-__device__ __forceinline__ void fv_jacobian(Dims d, int x, int y, int z, const double *pinv, double m[9]) {
-    for (int k = 0; k < 9; ++k) m[k] = 0.0;
-    if (!interior(d, x, y, z)) return;
-    const D3 xp = ld(pinv, lin(d, x + 1, y, z)), xm = ld(pinv, lin(d, x - 1, y, z));
-    const D3 yp = ld(pinv, lin(d, x, y + 1, z)), ym = ld(pinv, lin(d, x, y - 1, z));
-    const D3 zp = ld(pinv, lin(d, x, y, z + 1)), zm = ld(pinv, lin(d, x, y, z - 1));
+__device__ __forceinline__ void jac_of(D3 xp, D3 xm, D3 yp, D3 ym, D3 zp, D3 zm, double m[9]) {
     const D3 jx = d3((xp.x - xm.x) * 0.5, (xp.y - xm.y) * 0.5, (xp.z - xm.z) * 0.5);
     const D3 jy = d3((yp.x - ym.x) * 0.5, (yp.y - ym.y) * 0.5, (yp.z - ym.z) * 0.5);
     const D3 jz = d3((zp.x - zm.x) * 0.5, (zp.y - zm.y) * 0.5, (zp.z - zm.z) * 0.5);
     m[0] = jx.x; m[1] = jy.x; m[2] = jz.x;
     m[3] = jx.y; m[4] = jy.y; m[5] = jz.y;
     m[6] = jx.z; m[7] = jy.z; m[8] = jz.z;
+}
+__device__ __forceinline__ void fv_jacobian(Dims d, int x, int y, int z, const double *pinv, double m[9]) {
+    for (int k = 0; k < 9; ++k) m[k] = 0.0;
+    if (!interior(d, x, y, z)) return;
+    jac_of(ld(pinv, lin(d, x + 1, y, z)), ld(pinv, lin(d, x - 1, y, z)), ld(pinv, lin(d, x, y + 1, z)),
+           ld(pinv, lin(d, x, y - 1, z)), ld(pinv, lin(d, x, y, z + 1)), ld(pinv, lin(d, x, y, z - 1)), m);
+}
+// the same with psi_inv at z+-1 from the window
+__device__ __forceinline__ void fv_jacobian_w(Dims d, int x, int y, int z, const double *pinv, const FWin &w,
+                                              double m[9]) {
+    for (int k = 0; k < 9; ++k) m[k] = 0.0;
+    if (!interior(d, x, y, z)) return;
+    jac_of(ld(pinv, lin(d, x + 1, y, z)), ld(pinv, lin(d, x - 1, y, z)), ld(pinv, lin(d, x, y + 1, z)),
+           ld(pinv, lin(d, x, y - 1, z)), w.p[2], w.p[0], m);
 }
 
 // F3: L = -((((((-6 psi + psi(x+1)) + psi(x-1)) + psi(y+1)) + psi(y-1)) + psi(z+1)) + psi(z-1)), interior
@@ -262,12 +281,13 @@ __device__ __forceinline__ double mn(double q) { return fmin(q, 0.0); }
 
 // F6: dvv = (Dv)v on the interior (0 elsewhere), dlv = the reference's upwind Laplacian of v on
 // its range (0 elsewhere)
-__device__ __forceinline__ void fv_vel_terms(Dims d, int x, int y, int z, int i, const double *v, D3 *dvv, D3 *dlv) {
+// (the z-neighbours from the window)
+__device__ __forceinline__ void fv_vel_terms(Dims d, int x, int y, int z, int i, const double *v, const FWin &w,
+                                             D3 *dvv, D3 *dlv) {
     D3 o = d3(0, 0, 0), l = d3(0, 0, 0);
     if (interior(d, x, y, z)) {
-        const D3 c = ld(v, i), xp = ld(v, lin(d, x + 1, y, z)), xm = ld(v, lin(d, x - 1, y, z)),
-                 yp = ld(v, lin(d, x, y + 1, z)), ym = ld(v, lin(d, x, y - 1, z)), zp = ld(v, lin(d, x, y, z + 1)),
-                 zm = ld(v, lin(d, x, y, z - 1));
+        const D3 c = w.v[2], xp = ld(v, lin(d, x + 1, y, z)), xm = ld(v, lin(d, x - 1, y, z)),
+                 yp = ld(v, lin(d, x, y + 1, z)), ym = ld(v, lin(d, x, y - 1, z)), zp = w.v[3], zm = w.v[1];
         const double ax1 = ((mx(c.x) * mx(c.x) - mn(c.x) * mn(c.x)) + mn(xp.x) * mn(xp.x)) - mx(xp.x) * mx(xp.x);
         const double ax2 = c.y > 0 ? c.y * (c.x - ym.x) : c.y * (yp.x - c.x);
         const double ax3 = c.z > 0 ? c.z * (c.x - zm.x) : c.z * (zp.x - c.x);
@@ -280,18 +300,18 @@ __device__ __forceinline__ void fv_vel_terms(Dims d, int x, int y, int z, int i,
         o = d3((0.5 * ax1 + ax2) + ax3, (ay1 + 0.5 * ay2) + ay3, (az1 + az2) + 0.5 * az3);
     }
     if (x >= 2 && x <= d.X - 3 && y >= 2 && y <= d.Y - 2 && z >= 2 && z <= d.Z - 3) {
-        const D3 c = ld(v, i);
-        const int sx = 1, sy = d.X, sz = d.X * d.Y;
+        const D3 c = w.v[2];
+        const int sx = 1, sy = d.X;
         // one-sided second differences with the reference's y-1 / z-1 where +1 is meant
         const int ax = c.x < 0 ? i + 2 * sx : i - 2 * sx, bx = c.x < 0 ? i + sx : i - sx;
         const int ay = c.y < 0 ? i + 2 * sy : i - 2 * sy, by = i - sy;
-        const int az = c.z < 0 ? i + 2 * sz : i - 2 * sz, bz = i - sz;
+        const D3 az = c.z < 0 ? w.v[4] : w.v[0], bz = w.v[1];
         D3 s = d3(0, 0, 0);
         for (int q = 0; q < 3; ++q) {
             const double cv = comp(c, q);
             const double vxx = c.x < 0 ? (v[3 * ax + q] - 2.0 * v[3 * bx + q]) + cv : (cv - 2.0 * v[3 * bx + q]) + v[3 * ax + q];
             const double vyy = c.y < 0 ? (v[3 * ay + q] - 2.0 * v[3 * by + q]) + cv : (cv - 2.0 * v[3 * by + q]) + v[3 * ay + q];
-            const double vzz = c.z < 0 ? (v[3 * az + q] - 2.0 * v[3 * bz + q]) + cv : (cv - 2.0 * v[3 * bz + q]) + v[3 * az + q];
+            const double vzz = c.z < 0 ? (comp(az, q) - 2.0 * comp(bz, q)) + cv : (cv - 2.0 * comp(bz, q)) + comp(az, q);
             const double r = -1.0 * ((vxx + vyy) + vzz);
             if (q == 0) s.x = r; else if (q == 1) s.y = r; else s.z = r;
         }
@@ -322,34 +342,69 @@ __device__ __forceinline__ D3 fv_psi_inv(Dims d, int x, int y, int z, D3 o, cons
     return d3(o.x - u.x, o.y - u.y, o.z - u.z);
 }
 
+__device__ __forceinline__ double ld1_or0(const double *__restrict__ p, int i, bool ok) { return ok ? p[i] : 0.0; }
+__device__ __forceinline__ D3 ld_or0(const double *__restrict__ p, int i, bool ok) {
+    return ok ? ld(p, i) : d3(0, 0, 0);
+}
+
 // F1, F2, F4, F5, F6, F7, F9 of every voxel: the new rho, v and psi_inv from the old state, L and
 // the images; with `record`, also the intermediates the adjoint reads (gU, dlv; the adjoint
-// recomputes J from psi_inv and Lo, Tp from L and T at psi_inv, which it gathers anyway)
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void f_step(Dims d, const double *rho, const double *v, const double *pinv,
-                                              const double *L, const double *T, const double *S, const double *gS,
-                                              double *rho1, double *v1, double *pinv1, double *gU_out,
-                                              double *dlv_out, int record,
+// recomputes J from psi_inv and Lo, Tp from L and T at psi_inv, which it gathers anyway).  One
+// thread per (x, y) column of a 64 x 4 tile marches a chunk of zc planes, holding the column's
+// z-neighbours of v, psi_inv and rho in registers.
+__global__ __launch_bounds__(256) void f_step(Dims d, int tiles_x, int tiles_y, int zc, const double *rho,
+                                              const double *v, const double *pinv, const double *L,
+                                              const double *T, const double *S, const double *gS, double *rho1,
+                                              double *v1, double *pinv1, double *gU_out, double *dlv_out, int record,
                                               double alpha, double w_reg, double m1t, double tau, double m2) {
-    FOR_VOXELS(d, i) {
-        XYZ(d, i);
-        const double r1 = fv_density(d, x, y, z, i, rho, v, alpha);
+    const int b = blockIdx.x;
+    const int x = (b % tiles_x) * MT_X + (int)(threadIdx.x % MT_X);
+    const int y = ((b / tiles_x) % tiles_y) * MT_Y + (int)(threadIdx.x / MT_X);
+    const int z0 = (b / (tiles_x * tiles_y)) * zc, z1 = min(z0 + zc, d.Z);
+    if (x >= d.X || y >= d.Y) return;
+    const int sz = d.X * d.Y, c0 = x + d.X * y;
+    FWin w;
+    for (int k = 0; k < 5; ++k) {  // v at z0-2 .. z0+2
+        const int p = z0 - 2 + k;
+        w.v[k] = ld_or0(v, c0 + sz * p, p >= 0 && p < d.Z);
+    }
+    for (int k = 0; k < 3; ++k) {  // psi_inv and rho at z0-1 .. z0+1
+        const int p = z0 - 1 + k;
+        const bool ok = p >= 0 && p < d.Z;
+        w.p[k] = ld_or0(pinv, c0 + sz * p, ok);
+        w.r[k] = ld1_or0(rho, c0 + sz * p, ok);
+    }
+    for (int z = z0; z < z1; ++z) {
+        // the entries of the next step, loaded a step ahead
+        const D3 nv = ld_or0(v, c0 + sz * (z + 3), z + 3 < d.Z), np = ld_or0(pinv, c0 + sz * (z + 2), z + 2 < d.Z);
+        const double nr = ld1_or0(rho, c0 + sz * (z + 2), z + 2 < d.Z);
+        const int i = c0 + sz * z;
+        const double r1 = fv_density(d, x, y, z, rho, v, w, alpha);
         double J[9];
-        fv_jacobian(d, x, y, z, pinv, J);
+        fv_jacobian_w(d, x, y, z, pinv, w, J);
         D3 lo;
         double tp;
         fv_interp(d, i, L, T, pinv, &lo, &tp);
         const D3 gu = fv_potential(d, x, y, z, i, S, gS, tp, lo, J, w_reg);
         D3 dvv, dlv;
-        fv_vel_terms(d, x, y, z, i, v, &dvv, &dlv);
-        const D3 w = fv_velocity(d, x, y, z, ld(v, i), dvv, dlv, r1, gu, alpha, m1t, tau, m2);
-        const D3 pi = fv_psi_inv(d, x, y, z, ld(pinv, i), J, w, alpha);
+        fv_vel_terms(d, x, y, z, i, v, w, &dvv, &dlv);
+        const D3 wv = fv_velocity(d, x, y, z, w.v[2], dvv, dlv, r1, gu, alpha, m1t, tau, m2);
+        const D3 pi = fv_psi_inv(d, x, y, z, w.p[1], J, wv, alpha);
         rho1[i] = r1;
-        st(v1, i, w);
+        st(v1, i, wv);
         st(pinv1, i, pi);
         if (record) {
             st(gU_out, i, gu);
             st(dlv_out, i, dlv);
         }
+        for (int k = 0; k < 4; ++k) w.v[k] = w.v[k + 1];
+        w.v[4] = nv;
+        w.p[0] = w.p[1];
+        w.p[1] = w.p[2];
+        w.p[2] = np;
+        w.r[0] = w.r[1];
+        w.r[1] = w.r[2];
+        w.r[2] = nr;
     }
 }
 
@@ -468,8 +523,6 @@ __device__ __forceinline__ void put_rec(int *code, double *abc, double *gw, int 
 // sources that are not regular, are stored as code 0 / zeros), so a target reads its 27 sources
 // from LDS and every record is read from memory about 1.6 times instead of 27.
 constexpr int GT_X = 64, GT_Y = 4, GT_THREADS = GT_X * GT_Y;
-// tiles of the z-marching one-thread-per-column kernels (64 x 4 columns per block)
-constexpr int MT_X = 64, MT_Y = 4, MT_BLOCKS_PER_CU = 8;
 constexpr int GS_X = GT_X + 2, GS_Y = GT_Y + 2, GS_N = GS_X * GS_Y;
 constexpr int GS_PER_THREAD = (GS_N + GT_THREADS - 1) / GT_THREADS;
 struct RecStage {
@@ -755,9 +808,6 @@ __device__ __forceinline__ D3 b4_voxel(Dims d, int x, int y, int z, int i, const
     if (in_lap(d, x, y, z + 1)) add_v(dlv_of(w.gg[3], alpha, tau), 2.0);
     return a;
 }
-__device__ __forceinline__ D3 ld_or0(const double *__restrict__ p, int i, bool ok) {
-    return ok ? ld(p, i) : d3(0, 0, 0);
-}
 
 // Step 4: transposes of F6 gathered into gv: dvv = (Dv)v (interior) and dlv, the velocity
 // Laplacian (its range; centre 3 times, "far" neighbours (+-2) once, "near" ones (+-1) -2 times,
@@ -891,7 +941,6 @@ struct B8Win {
 __device__ __forceinline__ D3 ld_jz(Dims d, const double *__restrict__ gJ, int i, bool ok) {
     return ok ? d3(gJ[gj_at(d, i, 2)], gJ[gj_at(d, i, 5)], gJ[gj_at(d, i, 8)]) : d3(0, 0, 0);
 }
-__device__ __forceinline__ double ld1_or0(const double *__restrict__ p, int i, bool ok) { return ok ? p[i] : 0.0; }
 
 // Step 8: transposes of F3, F2 and F1 gathered into voxel i:
 //   L(j) = 6 psi(j) - (sum of its 6 neighbours), interior j:       gpsi(i)
@@ -1223,7 +1272,8 @@ struct fr_adjoint {
     // launch shape of the record gathers: 64 x 4 tiles, zc z-planes per block (gt_zc_auto: the
     // choice made at creation, about two blocks per CU)
     int gt_tiles_x = 0, gt_tiles_y = 0, gt_zc = 0, gt_blocks = 0, gt_zc_auto = 0;
-    // launch shape of the z-marching kernels (b_step4): the same 64 x 4 tiles, mt_zc planes per block
+    // launch shape of the z-marching kernels (f_step, b_step4, b_step8): the same 64 x 4 tiles,
+    // mt_zc planes per block
     int mt_tiles_x = 0, mt_tiles_y = 0, mt_zc = 0, mt_blocks = 0;
     // per-iteration scalars of the last forward (index k-1 for iteration k)
     std::vector<double> m1t, m2;
@@ -1309,8 +1359,9 @@ int forward_iteration(fr_adjoint *h, const State &in, const State &out, const In
     hipStream_t s = h->stream;
     const double a = h->alpha;
     frd::f_laplacian<<<g, TPB, 0, s>>>(d, in.psi, it.L);
-    frd::f_step<<<g, TPB, 0, s>>>(d, in.rho, in.v, in.pinv, it.L, h->T, h->S, h->gS, out.rho, out.v, out.pinv,
-                                  it.gU, it.dlv, record ? 1 : 0, a, h->w_reg, m1t, h->tau, m2);
+    frd::f_step<<<h->mt_blocks, 256, 0, s>>>(d, h->mt_tiles_x, h->mt_tiles_y, h->mt_zc, in.rho, in.v, in.pinv, it.L,
+                                             h->T, h->S, h->gS, out.rho, out.v, out.pinv, it.gU, it.dlv,
+                                             record ? 1 : 0, a, h->w_reg, m1t, h->tau, m2);
     frd::f_psi<<<g, TPB, 0, s>>>(d, in.psi, out.v, out.psi, a);
     ADJ_TRY(hipGetLastError());
     return FR_OK;
