@@ -248,7 +248,7 @@ static int auto_chunking(fr_ctx *c) {
     HIP_TRY(frk::pass_occupancy(2, &occ_b));
     const long long n_cu = cus > 0 ? cus : 1;
     c->zc_auto[0] = 0;
-    c->zc_auto[1] = chunk_for(c->d, frk::XS_A, frk::TY_A, n_cu * (occ_a > 0 ? occ_a : 1));
+    c->zc_auto[1] = chunk_for(c->d, frk::xs_a(c->d), frk::TY_A, n_cu * (occ_a > 0 ? occ_a : 1));
     c->zc_auto[2] = chunk_for_b(c->d, n_cu, occ_b > 0 ? occ_b : 1, c->st);
     for (int k = 0; k < 3; ++k)
         if (!c->zc_user[k]) c->zc[k] = c->zc_auto[k];

@@ -31,6 +31,12 @@ constexpr int WAVE = 64;
 constexpr int TY_A = 8, TY_B = 8;
 constexpr int XS_A = WAVE - 4;
 constexpr int XS_B = WAVE - 2;
+// Pass A's columns per wave for a volume: XS_A, or WAVE - 2 when the interior row has 61 or 62
+// columns (X = 63, 64).  Then one tile spans the row with lane l at x = l: its edge lanes are the
+// boundary columns, and the only lanes next to them, x = 1 and x = X - 2, lie outside the upwind
+// velocity Laplacian's range (x in [2, X - 3]), so no computed lane needs a neighbour two lanes
+// past the wave.  A 64-wide volume thus runs half the x tiles, and its z-chunks are half as long.
+inline int xs_a(Dims d) { return d.X - 2 > XS_A && d.X - 2 <= WAVE - 2 ? WAVE - 2 : XS_A; }
 // Storage of velocity and density: FP32 (3 + 1 floats in two buffers), or one 8-byte record
 // (v.x, v.y, v.z, rho) per voxel in bf16 or fp16.
 constexpr int ST_F32 = 0, ST_BF16 = 1, ST_F16 = 2;

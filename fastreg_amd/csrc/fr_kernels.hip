@@ -42,6 +42,11 @@ __device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}
 __device__ __forceinline__ F3 add3(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
 __device__ __forceinline__ F3 sub3(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ __forceinline__ F3 mul3(float m, F3 a) { return f3(a.x * m, a.y * m, a.z * m); }
+// a - 2b with one rounding: 2b is exact, so this equals the separately rounded a - (2 * b) bit for
+// bit (zero signs included) unless 2b overflows (|b| >= 2^127)
+__device__ __forceinline__ F3 sub_twice(F3 a, F3 b) {
+    return f3(fmaf(-2.f, b.x, a.x), fmaf(-2.f, b.y, a.y), fmaf(-2.f, b.z, a.z));
+}
 
 // lerp(v0, v1, t) = fma(t, v0, fma(-t, v1, v1)) — cuda_utils.hpp:43-46
 __device__ __forceinline__ float lerp(float v0, float v1, float t) { return fmaf(t, v0, fmaf(-t, v1, v1)); }
@@ -554,18 +559,21 @@ __device__ __forceinline__ F3 l_global(const PassAView &w, int x, int y, int z) 
 }
 
 // Whether the 8 taps of corner k (of the voxel at plane z) are all in the L/T stage: columns 1..62
-// (the end lanes lack an x-neighbour), rows y = yb .. yb + 9 of the corner and its +1 row (stage
-// rows 1..10), planes z-1..z+1.  A voxel displaced by a voxel or more fails it.
+// (an end lane lacks an x-neighbour, so its L is valid only where the lane lies on or past the
+// volume's edge column, L being 0 there: lane 0 when it is x = 0, lane 63 when it is x >= X - 1,
+// where a tap's weight is 0 past x = X - 1), rows y = yb .. yb + 9 of the corner and its +1 row
+// (stage rows 1..10), planes z-1..z+1.  A voxel displaced by a voxel or more fails it.
 __device__ __forceinline__ bool taps_in_stage(const PassAView &w, const Corner &k, int z) {
     const int cx = k.gx - w.xb, ry = k.gy - w.yb, dz = k.gz - z;
-    return cx >= 1 && cx <= WAVE - 3 && ry >= 0 && ry <= LT_ROWS - 2 && (dz == -1 || dz == 0);
+    const int lo = w.xb == 0 ? 0 : 1, hi = w.xb + WAVE - 1 >= w.d.X - 1 ? WAVE - 2 : WAVE - 3;
+    return cx >= lo && cx <= hi && ry >= 0 && ry <= LT_ROWS - 2 && (dz == -1 || dz == 0);
 }
 
 // L and T at psi_inv(p) (corner k) of the voxel at plane z from the stage, the indices clamped
 // into it (the values are those of the reference when taps_in_stage holds)
 __device__ __forceinline__ void lt_stage_taps(const PassAView &w, const Corner &k, int z, F3 &Lo, float &Tv) {
     const int dz = k.gz - z;
-    const int cx = min(max(k.gx - w.xb, 1), WAVE - 3);
+    const int cx = min(max(k.gx - w.xb, 0), WAVE - 2);
     const int ry = min(max(k.gy - w.yb, 0), LT_ROWS - 2) + 1;  // the corner's stage row
     const int zg = dz == -1 ? z - 1 : z;
     const int s0 = zg & (LT_PLANES - 1), s1 = (zg + 1) & (LT_PLANES - 1);
@@ -672,9 +680,9 @@ __device__ __forceinline__ void voxel_math(const IterArgs &a, const VoxelIn &q, 
     const F3 Ax = nx ? v_x2 : vc, Bx = nx ? xp : xm, Cx = nx ? vc : v_x2;
     const F3 Ay = ny ? q.v_y2 : vc, Cy = ny ? vc : q.v_y2;
     const F3 Az = nz ? q.v_zpp : vc, Cz = nz ? vc : q.v_zmm;
-    const F3 vxx = add3(sub3(Ax, mul3(2.f, Bx)), Cx);
-    const F3 vyy = add3(sub3(Ay, mul3(2.f, ym)), Cy);
-    const F3 vzz = add3(sub3(Az, mul3(2.f, zm)), Cz);
+    const F3 vxx = add3(sub_twice(Ax, Bx), Cx);
+    const F3 vyy = add3(sub_twice(Ay, ym), Cy);
+    const F3 vzz = add3(sub_twice(Az, zm), Cz);
     const F3 dl = mul3(-1.f, add3(add3(vxx, vyy), vzz));
     const F3 dlt = lap ? dl : f3(0.f, 0.f, 0.f);
 
@@ -1646,7 +1654,7 @@ template <int ST> static void pass_b_kernel(const IterArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_pass_a(const IterArgs &a0, int zc, int st, hipStream_t s) {
-    const IterArgs a = chunked(a0, zc, XS_A, TY_A);
+    const IterArgs a = chunked(a0, zc, xs_a(a0.d), TY_A);
     if (st == ST_BF16) pass_a_kernel<ST_BF16>(a, s);
     else if (st == ST_F16) pass_a_kernel<ST_F16>(a, s);
     else pass_a_kernel<ST_F32>(a, s);
@@ -1662,7 +1670,7 @@ hipError_t launch_pass_b(const IterArgs &a0, int zc, int st, hipStream_t s) {
 }
 
 IterArgs pass_args(const IterArgs &a, int pass, int zc) {
-    return pass == 1 ? chunked(a, zc, XS_A, TY_A) : chunked(a, zc, XS_B, TY_B);
+    return pass == 1 ? chunked(a, zc, xs_a(a.d), TY_A) : chunked(a, zc, XS_B, TY_B);
 }
 
 template <int ST>

@@ -12,7 +12,7 @@ from fastreg_amd import phantoms
 
 pytestmark = pytest.mark.gpu
 
-DIMS = [(24, 20, 18), (33, 17, 21), (16, 16, 16), (40, 12, 30), (70, 21, 12)]
+DIMS = [(24, 20, 18), (33, 17, 21), (16, 16, 16), (40, 12, 30), (70, 21, 12), (64, 15, 14)]  # 64: one x tile per row
 
 
 def say(*a):

@@ -159,6 +159,36 @@ def test_step_with_large_displacements_bit_exact(oracle_mod, dims, seed, amp):
         assert mg == mo and ag == ao, (mg, mo, ag, ao)
 
 
+def _edge_taps(dims, psi_inv):
+    """Interior voxels of columns x = 1 and x = X - 2 whose psi_inv takes its trilinear corner on
+    the boundary column (x = 0) or past the last interior one (x >= X - 2)."""
+    X, Y, Z = dims
+    px = psi_inv[:, 0].reshape(Z, Y, X)[1:-1, 1:-1, :]
+    return int((np.floor(px[..., 1]) == 0).sum()), int((np.floor(px[..., X - 2]) >= X - 2).sum())
+
+
+@pytest.mark.parametrize("dims,seed,amp,smooth", [((64, 21, 13), 41, 0.6, 2.0), ((63, 17, 19), 42, 0.9, 2.0),
+                                                  ((64, 12, 10), 43, 4.0, 3.0)])
+def test_step_one_tile_rows_bit_exact(oracle_mod, dims, seed, amp, smooth):
+    """Volumes 63 and 64 voxels wide run pass A as one x tile per row (lane l at x = l, the edge lanes
+    on the boundary columns).  Teacher-forced steps from states whose psi_inv takes the trilinear
+    taps of voxels next to the boundary onto the edge lanes (and, at the larger amplitude, out of
+    the stage into the recovery) equal the oracle bit for bit."""
+    say("one-tile step", dims, seed, amp)
+    O = oracle_mod
+    g, o, pair, rng = _both(dims, O, seed)
+    st = phantoms.random_state(dims, rng, amp=amp, smooth=smooth)
+    lo, hi = _edge_taps(dims, st["psi_inv"])
+    assert lo > 0 and hi > 0, (lo, hi)
+    g.set_state(**st)
+    o.set_state(**st)
+    for k, t in enumerate([0.004, 0.3]):
+        mg, ag = g.step(t)
+        mo, ao = o.step(t)
+        _assert_state_equal(g.get_state(), o.get_state(), f"step {k}")
+        assert mg == mo and ag == ao, (mg, mo, ag, ao)
+
+
 def test_run_bit_exact_with_log(oracle_mod):
     """A 25-iteration verbose run: state, max-norm history, log lines and Dice equal the oracle."""
     O = oracle_mod

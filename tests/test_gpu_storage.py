@@ -43,7 +43,8 @@ def _same_state(g, o, what):
 
 
 @pytest.mark.parametrize("storage", STORAGES)
-@pytest.mark.parametrize("dims,seed,amp", [((24, 20, 18), 3, 1.5), ((67, 9, 11), 4, 1.5), ((70, 40, 36), 31, 4.0)])
+@pytest.mark.parametrize("dims,seed,amp", [((24, 20, 18), 3, 1.5), ((67, 9, 11), 4, 1.5), ((70, 40, 36), 31, 4.0),
+                                            ((64, 21, 13), 41, 0.6)])
 def test_step_teacher_forced(oracle_mod, storage, dims, seed, amp):
     O = oracle_mod
     say("storage step", storage, dims, amp)
