@@ -81,7 +81,7 @@ struct fr_ctx {
     double *partials = nullptr;
     int nparts = 0;
     // finalisation
-    unsigned *dice_counts = nullptr;  // overlap[no_labels], voxels[no_labels], bad
+    unsigned *dice_counts = nullptr;  // DICE_COPIES x (overlap[no_labels], voxels[no_labels], bad)
     float *det_parts = nullptr;
     int ndet = 0;
     int16_t *lab_tmp = nullptr;
@@ -273,7 +273,7 @@ extern "C" int fr_create(const fr_params *p, fr_ctx **out) {
         (rc = dalloc(&c->v[1], 3 * n)) || (rc = dalloc(&c->pinv[0], 3 * n)) || (rc = dalloc(&c->pinv[1], 3 * n)) ||
         (rc = dalloc(&c->psi, 3 * n)) || (rc = dalloc(&c->ctl, 1)) || (rc = dalloc(&c->maxpack, (size_t)mi * frk::NSLOT)) ||
         (rc = dalloc(&c->energy, 3 * (size_t)mi)) || (rc = dalloc(&c->partials, 2 * (size_t)c->nparts)) ||
-        (rc = dalloc(&c->dice_counts, 2 * (size_t)p->no_labels + 1)) || (rc = dalloc(&c->det_parts, 2 * (size_t)c->ndet)) ||
+        (rc = dalloc(&c->dice_counts, frk::DICE_COPIES * (2 * (size_t)p->no_labels + 1))) || (rc = dalloc(&c->det_parts, 2 * (size_t)c->ndet)) ||
         (rc = dalloc(&c->lab_tmp, n)) || (rc = dalloc(&c->max_slot, 1)) || (rc = dalloc(&c->maxrun, (size_t)mi))) {
         free_ctx(c);
         return rc;
@@ -672,15 +672,18 @@ extern "C" int fr_solver_finalize(fr_ctx *c, float *img1, int16_t *lab1, float *
     int16_t *ls = lab2 ? lab2 : c->lab_tmp;
     HIP_TRY(frk::launch_warp(c->d, c->S, c->LS, c->psi, 3, img2, ls, c->stream));
     if (img1 || lab1) HIP_TRY(frk::launch_warp(c->d, c->T, c->LT, c->pinv[c->cur], 3, img1, lab1, c->stream));
-    HIP_TRY(hipMemsetAsync(c->dice_counts, 0, sizeof(unsigned) * (2 * (size_t)p.no_labels + 1), c->stream));
-    unsigned *ov = c->dice_counts, *vx = c->dice_counts + p.no_labels, *bad = c->dice_counts + 2 * p.no_labels;
-    HIP_TRY(frk::launch_dice(c->d, ls, c->LT, p.no_labels, ov, vx, bad, c->stream));
+    const size_t ncnt = 2 * (size_t)p.no_labels + 1;
+    HIP_TRY(hipMemsetAsync(c->dice_counts, 0, sizeof(unsigned) * frk::DICE_COPIES * ncnt, c->stream));
+    HIP_TRY(frk::launch_dice(c->d, ls, c->LT, p.no_labels, c->dice_counts, c->stream));
     HIP_TRY(frk::launch_min_det(c->d, c->psi, c->det_parts, c->det_parts + c->ndet, c->stream));
-    std::vector<unsigned> cnt(2 * (size_t)p.no_labels + 1);
+    std::vector<unsigned> cnt_copies(frk::DICE_COPIES * ncnt), cnt(ncnt, 0u);
     std::vector<float> dp(2 * (size_t)c->ndet);
-    HIP_TRY(hipMemcpyAsync(cnt.data(), c->dice_counts, sizeof(unsigned) * cnt.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(cnt_copies.data(), c->dice_counts, sizeof(unsigned) * cnt_copies.size(),
+                           hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(dp.data(), c->det_parts, sizeof(float) * dp.size(), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < frk::DICE_COPIES; ++k)
+        for (size_t j = 0; j < ncnt; ++j) cnt[j] += cnt_copies[k * ncnt + j];
     c->h_overlap.assign(p.no_labels, 0);
     c->h_voxels.assign(p.no_labels, 0);
     for (int l = 0; l < p.no_labels; ++l) {

@@ -109,8 +109,10 @@ int energy_partials_count(Dims d);
 // warp: def has `stride` floats per voxel (3 = packed xyz, 4 = float4)
 hipError_t launch_warp(Dims d, const float *img, const int16_t *lab, const float *def, int stride, float *out_img,
                        int16_t *out_lab, hipStream_t s);
-hipError_t launch_dice(Dims d, const int16_t *la, const int16_t *lb, int no_labels, unsigned *overlap, unsigned *voxels,
-                       unsigned *bad, hipStream_t s);
+// Dice counts into DICE_COPIES copies of (overlap[no_labels], voxels[no_labels], bad), each block
+// adding into copy blockIdx % DICE_COPIES (cleared by the caller; the caller sums the copies)
+constexpr int DICE_COPIES = 32;
+hipError_t launch_dice(Dims d, const int16_t *la, const int16_t *lb, int no_labels, unsigned *counts, hipStream_t s);
 int min_det_partials_count(Dims d);
 hipError_t launch_min_det(Dims d, const float *psi, float *part_true, float *part_ref, hipStream_t s);
 hipError_t launch_pack4to3(const float *src4, float *dst3a, float *dst3b, int n, hipStream_t s);

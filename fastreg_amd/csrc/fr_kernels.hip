@@ -1347,31 +1347,57 @@ __global__ void k_warp(Dims d, const float *__restrict__ img, const int16_t *__r
 // go to *bad instead of indexing out of bounds
 constexpr int DICE_LDS_LABELS = 2048;
 
+// Adds to cnt[label] the number of active lanes holding that label: one atomic per distinct label
+// of the wave (neighbouring voxels mostly share a label, so per-lane atomics on one address would
+// serialise).  `act` is the ballot of the lanes taking part; `lab` must be in range on them.
+__device__ __forceinline__ void wave_count(unsigned long long act, int lab, unsigned *cnt) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    while (act) {
+        const int leader = __builtin_ctzll(act);
+        const int l = __builtin_amdgcn_readlane(lab, leader);
+        const unsigned long long m = __ballot(((act >> lane) & 1ull) && lab == l);
+        if (lane == leader) atomicAdd(&cnt[l], (unsigned)__popcll(m));
+        act &= ~m;
+    }
+}
+
+// Per-label counts of the Dice coefficient (solver.cu:9-50) over x < X-1, y < Y-1, z < Z-1: voxels
+// of each label in la and in lb, and voxels where both agree; labels outside [0, no_labels) are
+// counted in `bad`.  Each wave takes whole x-rows (64 voxels per step); the counts go through LDS
+// when the labels fit there.  A block adds into copy blockIdx % DICE_COPIES of the counts, so that
+// the blocks' closing atomics spread over that many addresses per label.
 __global__ void k_dice(Dims d, const int16_t *__restrict__ la, const int16_t *__restrict__ lb, int no_labels,
-                       unsigned *overlap, unsigned *voxels, unsigned *bad) {
+                       unsigned *counts) {
     __shared__ unsigned s_ov[DICE_LDS_LABELS], s_vx[DICE_LDS_LABELS];
+    unsigned *overlap = counts + (size_t)(blockIdx.x % DICE_COPIES) * (2 * (size_t)no_labels + 1);
+    unsigned *voxels = overlap + no_labels, *bad = overlap + 2 * no_labels;
     const bool use_lds = no_labels <= DICE_LDS_LABELS;
     if (use_lds)
         for (int l = threadIdx.x; l < no_labels; l += blockDim.x) s_ov[l] = s_vx[l] = 0u;
     __syncthreads();
-    int n = d.X * d.Y * d.Z;
+    const int lane = threadIdx.x & (WAVE - 1), wpb = blockDim.x / WAVE;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int ny = d.Y - 1, nrows = (d.Y - 1) * (d.Z - 1);
     unsigned nb = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int x = i % d.X, r = i / d.X, y = r % d.Y, z = r / d.Y;
-        if (x >= d.X - 1 || y >= d.Y - 1 || z >= d.Z - 1) continue;
-        int l1 = la[i], l2 = lb[i];
-        if (l1 < 0 || l1 >= no_labels || l2 < 0 || l2 >= no_labels) {
-            nb++;
-            continue;
-        }
-        if (use_lds) {
-            atomicAdd(&s_vx[l1], 1u);
-            atomicAdd(&s_vx[l2], 1u);
-            if (l1 == l2) atomicAdd(&s_ov[l1], 1u);
-        } else {
-            atomicAdd(&voxels[l1], 1u);
-            atomicAdd(&voxels[l2], 1u);
-            if (l1 == l2) atomicAdd(&overlap[l1], 1u);
+    for (int r = blockIdx.x * wpb + wave; r < nrows; r += gridDim.x * wpb) {
+        const int y = r % ny, z = r / ny;
+        const int row = d.X * (y + d.Y * z);
+        for (int x0 = 0; x0 < d.X - 1; x0 += WAVE) {
+            const int x = x0 + lane;
+            const bool in = x < d.X - 1;
+            const int l1 = in ? la[row + x] : 0, l2 = in ? lb[row + x] : 0;
+            const bool ok = in && l1 >= 0 && l1 < no_labels && l2 >= 0 && l2 < no_labels;
+            nb += in && !ok;
+            const unsigned long long both = __ballot(ok), same = __ballot(ok && l1 == l2);
+            if (use_lds) {  // uniform
+                wave_count(both, l1, s_vx);
+                wave_count(both, l2, s_vx);
+                wave_count(same, l1, s_ov);
+            } else {
+                wave_count(both, l1, voxels);
+                wave_count(both, l2, voxels);
+                wave_count(same, l1, overlap);
+            }
         }
     }
     if (nb) atomicAdd(bad, nb);
@@ -1387,19 +1413,25 @@ __global__ void k_dice(Dims d, const int16_t *__restrict__ la, const int16_t *__
 __global__ __launch_bounds__(RED_BLOCK) void k_min_det(Dims d, const float *__restrict__ psi, float *part_true,
                                                        float *part_ref) {
     __shared__ float sh[2][RED_BLOCK / 64];
-    int n = d.X * d.Y * d.Z;
     float mt = INFINITY, mr = INFINITY;
-    for (int i = blockIdx.x * RED_BLOCK + threadIdx.x; i < n; i += gridDim.x * RED_BLOCK) {
-        int x = i % d.X, r = i / d.X, y = r % d.Y, z = r / d.Y;
-        if (x < 1 || x > d.X - 2 || y < 1 || y > d.Y - 2 || z < 1 || z > d.Z - 2) continue;
-        int sy = d.X, sz = d.X * d.Y;
-        F3 jx = mul3(0.5f, sub3(ld3(psi, i + 1), ld3(psi, i - 1)));
-        F3 jy = mul3(0.5f, sub3(ld3(psi, i + sy), ld3(psi, i - sy)));
-        F3 jz = mul3(0.5f, sub3(ld3(psi, i + sz), ld3(psi, i - sz)));
-        float t = det_true(jx.x, jy.x, jz.x, jx.y, jy.y, jz.y, jx.z, jy.z, jz.z);
-        float f = det_ref(jx.x, jy.x, jz.x, jx.y, jy.y, jz.y, jx.z, jy.z, jz.z);
-        mt = fminf(mt, t);
-        mr = fminf(mr, f);
+    // each wave takes whole interior x-rows, 64 voxels per step
+    const int lane = threadIdx.x & (WAVE - 1), wpb = RED_BLOCK / WAVE;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int ny = d.Y - 2, nrows = (d.Y - 2) * (d.Z - 2);
+    const int sy = d.X, sz = d.X * d.Y;
+    for (int r = blockIdx.x * wpb + wave; r < nrows; r += gridDim.x * wpb) {
+        const int y = 1 + r % ny, z = 1 + r / ny;
+        const int row = d.X * (y + d.Y * z);
+        for (int x = 1 + lane; x <= d.X - 2; x += WAVE) {
+            const int i = row + x;
+            F3 jx = mul3(0.5f, sub3(ld3(psi, i + 1), ld3(psi, i - 1)));
+            F3 jy = mul3(0.5f, sub3(ld3(psi, i + sy), ld3(psi, i - sy)));
+            F3 jz = mul3(0.5f, sub3(ld3(psi, i + sz), ld3(psi, i - sz)));
+            float t = det_true(jx.x, jy.x, jz.x, jx.y, jy.y, jz.y, jx.z, jy.z, jz.z);
+            float f = det_ref(jx.x, jy.x, jz.x, jx.y, jy.y, jz.y, jx.z, jy.z, jz.z);
+            mt = fminf(mt, t);
+            mr = fminf(mr, f);
+        }
     }
     for (int m = 32; m >= 1; m >>= 1) {
         mt = fminf(mt, __shfl_xor(mt, m, 64));
@@ -1720,11 +1752,9 @@ hipError_t launch_warp(Dims d, const float *img, const int16_t *lab, const float
     return hipGetLastError();
 }
 
-hipError_t launch_dice(Dims d, const int16_t *la, const int16_t *lb, int no_labels, unsigned *overlap, unsigned *voxels,
-                       unsigned *bad, hipStream_t s) {
-    int n = d.X * d.Y * d.Z;
-    hipLaunchKernelGGL(k_dice, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, d, la, lb, no_labels, overlap, voxels,
-                       bad);
+hipError_t launch_dice(Dims d, const int16_t *la, const int16_t *lb, int no_labels, unsigned *counts, hipStream_t s) {
+    const int rows = (d.Y - 1) * (d.Z - 1);  // at least 2 rows per wave
+    hipLaunchKernelGGL(k_dice, dim3(grid_for(rows, 4 * 2, 2048)), dim3(256), 0, s, d, la, lb, no_labels, counts);
     return hipGetLastError();
 }
 

@@ -238,6 +238,34 @@ def test_degenerate_volume_run_bit_exact(oracle_mod, dims, seed):
     assert np.array_equal(ov, oov) and np.array_equal(vx, ovx)
 
 
+@pytest.mark.parametrize("no_labels,dims", [(9, (70, 21, 12)), (3000, (130, 9, 7))])
+def test_dice_counts_with_labels_out_of_range(oracle_mod, no_labels, dims):
+    """Dice counts (per label: voxels of each volume and agreeing voxels) over labels that include
+    values outside [0, no_labels), with the counts held in LDS (9 labels) and in global memory
+    (3000 labels): equal to the oracle's."""
+    O = oracle_mod
+    X, Y, Z = dims
+    say("dice counts", no_labels, dims)
+    rng = np.random.default_rng(no_labels)
+    base = phantoms.blobs(shape=(Z, Y, X), seed=51)
+    # runs of equal labels (as in real label maps) with scattered out-of-range values
+    lab = np.repeat(rng.integers(-2, no_labels + 2, (Z * Y * X + 6) // 7), 7)[:Z * Y * X]
+    lab2 = lab.copy()
+    flip = rng.random(lab.size) < 0.2
+    lab2[flip] = rng.integers(-1, no_labels + 1, int(flip.sum()))
+    pair = phantoms.Pair(base.target, lab.astype(np.int16).reshape(Z, Y, X), base.source,
+                         lab2.astype(np.int16).reshape(Z, Y, X), no_labels)
+    g, o, _, _ = _both(dims, O, 51, src_pair=pair, max_iter=2, alpha=0.3)
+    g.run()
+    g.finalize()
+    o.run(O.SolverParams(max_iter=2, alpha=0.3, no_labels=no_labels))
+    o.outputs()
+    _, ov, vx = g.dice()
+    oov, ovx, obad = o.dice_counts(no_labels)
+    assert obad > 0
+    assert np.array_equal(ov, oov) and np.array_equal(vx, ovx)
+
+
 def test_register_matches_oracle_c1_shape(oracle_mod):
     """fr_register on the 64^3 blob phantom (40 iterations): in-place outputs, Dice and
     min det J equal the oracle's."""
