@@ -91,6 +91,9 @@ def main():
                     help="oracle iterations per CPU-baseline run at 182x218x182 (3 runs, about 4 s each on "
                          "16 host cores)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--arith", choices=("reference", "fma"), default="reference",
+                    help="plain, c1: arithmetic of pass A; fma times the contracted form and compares its "
+                         "registration with a reference-order one (Dice, min det J, labels, fields)")
     ap.add_argument("--zc", default="0,0",
                     help="interior z-slices per block of pass A, pass B (0 = automatic)")
     ap.add_argument("--workload", choices=("plain", "c1", "batch", "precision", "adjoint"), default="plain",
@@ -165,6 +168,8 @@ def registration_workload(args):
     s = fr.Solver(p)
     s.set_chunking(*zc)
     chunking = s.chunking()
+    if args.arith == "fma":
+        s.set_arith(fr.ARITH_FMA)
     s.init(pair.target, pair.target_labels, pair.source, pair.source_labels)  # inputs now resident
     outs = [fr.DeviceArray(n, np.float32), fr.DeviceArray(n, np.int16), fr.DeviceArray(n, np.float32),
             fr.DeviceArray(n, np.int16)]
@@ -213,6 +218,7 @@ def registration_workload(args):
                 "kernel_ms": {"pass_a": round(ms_a, 4), "pass_b": round(ms_b, 4)},
                 "alg_bytes_per_launch": {"pass_a": bytes_a, "pass_b": bytes_b},
                 "iteration_frac_88B": round(n * 1e3 / (loop_med / iters) * 88 / (HBM_PEAK_GBS * 1e9), 4)}
+    vs_ref = arith_vs_reference(s, registration, outs, st, n) if args.arith == "fma" else None
     for b in outs:
         b.free()
     s.close()
@@ -247,7 +253,48 @@ def registration_workload(args):
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if vs_ref is not None:
+        out["config"]["arith"] = "fma (multiply-adds contracted, reciprocal in the density division)"
+        out["arith_vs_reference"] = vs_ref
     print(json.dumps(out), flush=True)
+
+
+def arith_vs_reference(s, registration, outs, st, n):
+    """The last (contracted-arithmetic) registration of `s` against one in the reference's
+    operation order from the same inputs (the order that equals the oracle bit for bit): Dice, min
+    det J, warped labels that differ, and the relative L2 deviation of the final fields (psi and
+    psi_inv as displacements from the identity)."""
+    import numpy as np
+    import fastreg_amd as fr
+
+    def snapshot():
+        state = s.get_state()
+        labs = (outs[1].get().copy(), outs[3].get().copy())
+        return state, labs, st.dice_all, st.min_det_true, st.iterations
+
+    f_state, f_labs, f_dice, f_md, f_it = snapshot()
+    s.set_arith(fr.ARITH_REFERENCE)
+    registration()
+    r_state, r_labs, r_dice, r_md, r_it = snapshot()
+    X, Y, Z = s.dims
+    zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    ident = np.stack([xx, yy, zz], -1).reshape(-1, 3).astype(np.float64)
+
+    def rel(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        den = np.linalg.norm(b)
+        return float(np.linalg.norm(a - b) / den) if den > 0 else float(np.linalg.norm(a - b))
+
+    fields = {"v": rel(f_state["v"][:, :3], r_state["v"][:, :3]), "rho": rel(f_state["rho"], r_state["rho"]),
+              "psi_minus_id": rel(f_state["psi"][:, :3] - ident, r_state["psi"][:, :3] - ident),
+              "psi_inv_minus_id": rel(f_state["psi_inv"][:, :3] - ident, r_state["psi_inv"][:, :3] - ident),
+              "psi": rel(f_state["psi"][:, :3], r_state["psi"][:, :3])}
+    return {"reference": {"dice_all": r_dice, "min_det_true": r_md, "iterations": r_it},
+            "fma": {"dice_all": f_dice, "min_det_true": f_md, "iterations": f_it},
+            "d_dice": f_dice - r_dice, "d_min_det": f_md - r_md,
+            "labels_differing": {"target_o_psi_inv": int((f_labs[0] != r_labs[0]).sum()),
+                                 "source_o_psi": int((f_labs[1] != r_labs[1]).sum()), "of": n},
+            "final_rel_l2": {k: round(v, 9) for k, v in fields.items()}}
 
 
 def precision_workload(args):

@@ -26,6 +26,7 @@ FR_OK = 0
 FR_ERR_INVALID = 1
 FR_ERR_HIP = 2
 STORAGE_F32, STORAGE_BF16, STORAGE_F16 = 0, 1, 2
+ARITH_REFERENCE, ARITH_FMA = 0, 1
 FR_ERR_ALLOC = 3
 FR_ERR_IO = 4
 FR_ERR_PARSE = 5
@@ -88,6 +89,8 @@ def lib():
     sig("fr_set_chunking", vp, i, i)
     sig("fr_set_storage", vp, i)
     sig("fr_get_storage", vp, P(i))
+    sig("fr_set_arith", vp, i)
+    sig("fr_get_arith", vp, P(i))
     sig("fr_batch_create", P(vp), i, P(vp))
     sig("fr_batch_destroy", vp)
     sig("fr_batch_relayout", vp)
@@ -405,6 +408,16 @@ class Solver:
     def storage(self):
         v = C.c_int()
         check(lib().fr_get_storage(self.h, C.byref(v)))
+        return v.value
+
+    def set_arith(self, arith):
+        """ARITH_REFERENCE (the reference's operation order, the default) or ARITH_FMA (its
+        multiply-adds contracted; FP32 storage only).  The state is kept."""
+        check(lib().fr_set_arith(self.h, int(arith)))
+
+    def arith(self):
+        v = C.c_int()
+        check(lib().fr_get_arith(self.h, C.byref(v)))
         return v.value
 
     def set_chunking(self, zc_a=0, zc_b=0):

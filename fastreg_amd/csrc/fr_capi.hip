@@ -67,7 +67,8 @@ struct fr_ctx {
     float *pinv[2] = {nullptr, nullptr};
     float *psi = nullptr;
     int st = frk::ST_F32;                  // storage of v and rho
-    unsigned storage_gen = 0;              // bumped by every fr_set_storage (batches check it)
+    int arith = frk::ARITH_REF;            // arithmetic of pass A
+    unsigned storage_gen = 0;              // bumped by every fr_set_storage / fr_set_arith (batches check it)
     unsigned *vr[2] = {nullptr, nullptr};  // 16-bit (v, rho) records, 2 unsigned per voxel
     int cur = 0;
     int zc[3] = {0, 8, 2};       // interior z-slices per block of pass A ([1]) and pass B ([2])
@@ -348,6 +349,8 @@ extern "C" int fr_set_storage(fr_ctx *c, int storage) {
     if (!c) return fail(FR_ERR_INVALID, "null context");
     if (storage != FR_STORAGE_F32 && storage != FR_STORAGE_BF16 && storage != FR_STORAGE_F16)
         return fail(FR_ERR_INVALID, "storage must be FR_STORAGE_F32, FR_STORAGE_BF16 or FR_STORAGE_F16");
+    if (storage != FR_STORAGE_F32 && c->arith != frk::ARITH_REF)
+        return fail(FR_ERR_INVALID, "the contracted arithmetic (FR_ARITH_FMA) runs on FP32 storage only");
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (storage != FR_STORAGE_F32) {
         // allocated on first use; a failed second allocation leaves the first for the next call
@@ -361,6 +364,28 @@ extern "C" int fr_set_storage(fr_ctx *c, int storage) {
     if (int rc = auto_chunking(c)) return rc;  // the automatic chunking depends on the storage
     c->initialised = false;  // the state must be initialised in the new storage
     c->have_run = c->have_final = false;
+    return FR_OK;
+}
+
+extern "C" int fr_set_arith(fr_ctx *c, int arith) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (arith != FR_ARITH_REFERENCE && arith != FR_ARITH_FMA)
+        return fail(FR_ERR_INVALID, "arith must be FR_ARITH_REFERENCE or FR_ARITH_FMA");
+    if (arith == FR_ARITH_FMA && c->st != frk::ST_F32)
+        return fail(FR_ERR_INVALID, "the contracted arithmetic (FR_ARITH_FMA) runs on FP32 storage only");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int a = arith == FR_ARITH_FMA ? frk::ARITH_FMA : frk::ARITH_REF;
+    if (a != c->arith) {
+        c->arith = a;
+        ++c->storage_gen;    // a batch laid out before this call must not run this context
+        drop_run_graphs(c);  // the captured loops launch the other kernel
+    }
+    return FR_OK;
+}
+
+extern "C" int fr_get_arith(const fr_ctx *c, int *arith) {
+    if (!c || !arith) return fail(FR_ERR_INVALID, "null argument");
+    *arith = c->arith == frk::ARITH_FMA ? FR_ARITH_FMA : FR_ARITH_REFERENCE;
     return FR_OK;
 }
 
@@ -443,7 +468,7 @@ static int launch_iteration(fr_ctx *c, int iter, float t, bool with_energy) {
                                             e + 2 * mi + (iter - 1), e + (iter - 1), c->stream));
         a.check_energy = 1;
     }
-    HIP_TRY(frk::launch_pass_a(a, c->zc[1], c->st, c->stream));
+    HIP_TRY(frk::launch_pass_a(a, c->zc[1], c->st, c->stream, c->arith));
     HIP_TRY(frk::launch_pass_b(a, c->zc[2], c->st, c->stream));
     c->cur ^= 1;
     return FR_OK;
@@ -895,22 +920,27 @@ static void free_batch(fr_batch *b) {
 }
 
 // every volume must have one storage (the batched kernels are instantiated per storage)
+// and the batched kernels perform the reference's operation order only
 static int batch_same_storage(fr_ctx *const *ctxs, int n) {
-    for (int q = 1; q < n; ++q)
+    for (int q = 0; q < n; ++q) {
         if (ctxs[q]->st != ctxs[0]->st)
             return fail(FR_ERR_INVALID, "context %d: the storage of v and rho must be the same for every volume of a "
                                         "batch", q);
+        if (ctxs[q]->arith != frk::ARITH_REF)
+            return fail(FR_ERR_INVALID, "context %d: batches run the reference-order arithmetic "
+                                        "(fr_set_arith FR_ARITH_REFERENCE)", q);
+    }
     return FR_OK;
 }
 
 // A run uses the buffers and kernels of the storage each volume had when the layout was made: a
-// later fr_set_storage on a volume (new buffers, other kernels) makes the batch unusable until
-// fr_batch_relayout.
+// later fr_set_storage or fr_set_arith on a volume (new buffers, other kernels) makes the batch
+// unusable until fr_batch_relayout.
 static int batch_check_layout(const fr_batch *b) {
     for (size_t q = 0; q < b->ctx.size(); ++q)
         if (b->ctx[q]->storage_gen != b->storage_gen[q] || b->ctx[q]->st != b->st)
-            return fail(FR_ERR_STATE, "batch volume %d: its storage was set after the batch was laid out; call "
-                                      "fr_batch_relayout", (int)q);
+            return fail(FR_ERR_STATE, "batch volume %d: its storage or arithmetic was set after the batch was laid "
+                                      "out; call fr_batch_relayout", (int)q);
     return FR_OK;
 }
 
@@ -1274,13 +1304,13 @@ extern "C" int fr_time_kernels(fr_ctx *c, int reps, float *ms_a, float *ms_b) {
     // slower until then
     for (int r = 0; r < TIME_KERNELS_WARMUP && e == hipSuccess; ++r) {
         a.iter = 1 + r % mi;
-        e = frk::launch_pass_a(a, c->zc[1], c->st, c->stream);
+        e = frk::launch_pass_a(a, c->zc[1], c->st, c->stream, c->arith);
         if (e == hipSuccess) e = frk::launch_pass_b(a, c->zc[2], c->st, c->stream);
     }
     for (int r = 0; r < reps && e == hipSuccess; ++r) {
         a.iter = 1 + (r + 1) % mi;
         e = hipEventRecord(ev[2 * r], c->stream);
-        if (e == hipSuccess) e = frk::launch_pass_a(a, c->zc[1], c->st, c->stream);
+        if (e == hipSuccess) e = frk::launch_pass_a(a, c->zc[1], c->st, c->stream, c->arith);
         if (e == hipSuccess) e = hipEventRecord(ev[2 * r + 1], c->stream);
         if (e == hipSuccess) e = frk::launch_pass_b(a, c->zc[2], c->st, c->stream);
     }

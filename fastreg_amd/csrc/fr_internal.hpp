@@ -40,6 +40,10 @@ inline int xs_a(Dims d) { return d.X - 2 > XS_A && d.X - 2 <= WAVE - 2 ? WAVE - 
 // Storage of velocity and density: FP32 (3 + 1 floats in two buffers), or one 8-byte record
 // (v.x, v.y, v.z, rho) per voxel in bf16 or fp16.
 constexpr int ST_F32 = 0, ST_BF16 = 1, ST_F16 = 2;
+// Arithmetic of pass A: the reference's FP32 operation order (every product rounded before its
+// sum, as the oracle computes it), or with its multiply-adds contracted into fma and the
+// density's division as a reciprocal and a product (FP32 storage, single contexts only).
+constexpr int ARITH_REF = 0, ARITH_FMA = 1;
 
 // Pass B spreads its per-iteration 64-bit atomic maxima over NSLOT addresses (a multiple of the
 // wave size: one wave reads them back).
@@ -79,7 +83,7 @@ hipError_t launch_init(Dims d, const float *tgt, const int16_t *tgt_lab, const f
 // launch sets a's tile-grid fields).
 // st: storage of v and rho (ST_F32, ST_BF16, ST_F16); in the 16-bit forms a.v_o / a.v_n point to
 // the 8-byte (v, rho) records and a.rho_o / a.rho_n are unused
-hipError_t launch_pass_a(const IterArgs &a, int zc, int st, hipStream_t s);
+hipError_t launch_pass_a(const IterArgs &a, int zc, int st, hipStream_t s, int arith = ARITH_REF);
 hipError_t launch_pass_b(const IterArgs &a, int zc, int st, hipStream_t s);
 // resident blocks per CU of pass A (1) or pass B (2)
 hipError_t pass_occupancy(int which, int *blocks_per_cu);

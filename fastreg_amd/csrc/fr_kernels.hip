@@ -702,6 +702,77 @@ __device__ __forceinline__ void voxel_math(const IterArgs &a, const VoxelIn &q, 
     pinv_n = f3(pc.x - ui.x, pc.y - ui.y, pc.z - ui.z);
 }
 
+// The same steps with the reference's multiply-adds contracted into fma (each product feeding a
+// sum is not rounded on its own), the (Dv)v diagonal terms as fma(u, |u|, -w|w|), and
+// m2 / (|rho| + 1e-5) as m2 times the hardware reciprocal (the reference's own division there is
+// not the IEEE one).  The results differ from the reference order by a few ulps per step (the
+// arithmetic mode ARITH_FMA, FP32 storage).
+__device__ __forceinline__ void voxel_math_fma(const IterArgs &a, const VoxelIn &q, bool lap, float &rho_n, F3 &vn,
+                                               F3 &pinv_n) {
+    // 1. density: the six upwind fluxes summed in the reference's order
+    const F3 vc = q.vc;
+    const float rc = q.rc;
+    const float vxm = q.v_xm.x, vym = q.v_ym.y, vzm = q.v_zm.z;
+    float div = -vc.x * ((vc.x > 0.f) ? rc : q.r_xp);
+    div = fmaf(vxm, (vxm > 0.f) ? q.r_xm : rc, div);
+    div = fmaf(-vc.y, (vc.y > 0.f) ? rc : q.r_yp, div);
+    div = fmaf(vym, (vym > 0.f) ? q.r_ym : rc, div);
+    div = fmaf(-vc.z, (vc.z > 0.f) ? rc : q.r_zp, div);
+    div = fmaf(vzm, (vzm > 0.f) ? q.r_zm : rc, div);
+    rho_n = fmaf(a.alpha, div, rc);
+
+    // 2. J_inv and the reference's det()
+    const F3 jx = mul3(0.5f, sub3(q.p_xp, q.p_xm));
+    const F3 jy = mul3(0.5f, sub3(q.p_yp, q.p_ym));
+    const F3 jz = mul3(0.5f, sub3(q.p_zp, q.p_zm));
+    const float m00 = jx.x, m01 = jy.x, m02 = jz.x, m10 = jx.y, m11 = jy.y, m12 = jz.y, m20 = jx.z, m21 = jy.z,
+                m22 = jz.z;
+    const float da = fmaf(m20 * m01, m12, fmaf(m00 * m11, m22, (m10 * m21) * m02));
+    const float db = fmaf(fmaf(m02, m11, m20), (m12 * m21) * m00, (m22 * m01) * m10);
+    const float det = da - db;
+
+    // 3-5. grad S and the potential gradient
+    const F3 gS = f3((q.s_xp - q.s_xm) * 0.5f, (q.s_yp - q.s_ym) * 0.5f, (q.s_zp - q.s_zm) * 0.5f);
+    const float diff = q.sc - q.Tv;
+    const F3 nU = f3(det * fmaf(diff, gS.x, a.w_reg * q.Lo.x), det * fmaf(diff, gS.y, a.w_reg * q.Lo.y),
+                     det * fmaf(diff, gS.z, a.w_reg * q.Lo.z));
+
+    // 6. (Dv)v
+    const F3 xp = q.v_xp, xm = q.v_xm, yp = q.v_yp, ym = q.v_ym, zp = q.v_zp, zm = q.v_zm;
+    const float ax1 = fmaf(vc.x, fabsf(vc.x), -(xp.x * fabsf(xp.x)));
+    const float ax2 = vc.y * ((vc.y > 0.f) ? (vc.x - ym.x) : (yp.x - vc.x));
+    const float ax3 = vc.z * ((vc.z > 0.f) ? (vc.x - zm.x) : (zp.x - vc.x));
+    const float ay1 = vc.x * ((vc.x > 0.f) ? (vc.y - xm.y) : (xp.y - vc.y));
+    const float ay2 = fmaf(vc.y, fabsf(vc.y), -(yp.y * fabsf(yp.y)));
+    const float ay3 = vc.z * ((vc.z > 0.f) ? (vc.y - zm.y) : (zp.y - vc.y));
+    const float az1 = vc.x * ((vc.x > 0.f) ? (vc.z - xm.z) : (xp.z - vc.z));
+    const float az2 = vc.y * ((vc.y > 0.f) ? (vc.z - ym.z) : (yp.z - vc.z));
+    const float az3 = fmaf(vc.z, fabsf(vc.z), -(zp.z * fabsf(zp.z)));
+    const F3 dvv = f3(fmaf(0.5f, ax1, ax2) + ax3, fmaf(0.5f, ay2, ay1) + ay3, fmaf(0.5f, az3, az1 + az2));
+
+    // 7. upwind velocity Laplacian (the reference's indices and ranges, as voxel_math)
+    const bool nx = vc.x < 0.f, ny = vc.y < 0.f, nz = vc.z < 0.f;
+    const F3 v_x2 = nx ? q.v_xpp : q.v_xmm;
+    const F3 Ax = nx ? v_x2 : vc, Bx = nx ? xp : xm, Cx = nx ? vc : v_x2;
+    const F3 Ay = ny ? q.v_y2 : vc, Cy = ny ? vc : q.v_y2;
+    const F3 Az = nz ? q.v_zpp : vc, Cz = nz ? vc : q.v_zmm;
+    const F3 lsum = add3(add3(add3(sub_twice(Ax, Bx), Cx), add3(sub_twice(Ay, ym), Cy)), add3(sub_twice(Az, zm), Cz));
+    const F3 lt = lap ? lsum : f3(0.f, 0.f, 0.f);  // -(the Laplacian term)
+
+    // 8. velocity update: v - alpha ((m1t v + (Dv)v) + tau lsum + qv nU)
+    const float qv = a.m2 * __builtin_amdgcn_rcpf(fabsf(rho_n) + 1e-5f);
+    F3 u = f3(fmaf(a.m1t, vc.x, dvv.x), fmaf(a.m1t, vc.y, dvv.y), fmaf(a.m1t, vc.z, dvv.z));
+    u = f3(fmaf(a.tau, lt.x, u.x), fmaf(a.tau, lt.y, u.y), fmaf(a.tau, lt.z, u.z));
+    u = f3(fmaf(qv, nU.x, u.x), fmaf(qv, nU.y, u.y), fmaf(qv, nU.z, u.z));
+    vn = f3(fmaf(-a.alpha, u.x, vc.x), fmaf(-a.alpha, u.y, vc.y), fmaf(-a.alpha, u.z, vc.z));
+
+    // 10. psi_inv -= alpha * (J_inv v_new)
+    const F3 pc = q.pc;
+    const F3 Jv = f3(fmaf(jz.x, vn.z, fmaf(jy.x, vn.y, jx.x * vn.x)), fmaf(jz.y, vn.z, fmaf(jy.y, vn.y, jx.y * vn.x)),
+                     fmaf(jz.z, vn.z, fmaf(jy.z, vn.y, jx.z * vn.x)));
+    pinv_n = f3(fmaf(-a.alpha, Jv.x, pc.x), fmaf(-a.alpha, Jv.y, pc.y), fmaf(-a.alpha, Jv.z, pc.z));
+}
+
 // One voxel of the z-march: every lane of the wave runs it (the x-neighbours are exchanged between
 // lanes); `store` is false on halo lanes and on the voxel past the chunk.  The taps of L and T come
 // from the stage.  A voxel whose taps are not all there (a displacement of a voxel or more) is
@@ -709,7 +780,7 @@ __device__ __forceinline__ void voxel_math(const IterArgs &a, const VoxelIn &q, 
 // result over what the march stored for it (computed with clamped taps; the same thread writes the
 // same address later in program order).  The march itself thus has no branch and no register
 // pressure from that rare path, and its stores need no per-voxel predicate.
-template <int ST>
+template <int ST, int AR = ARITH_REF>
 __device__ __forceinline__ bool voxel_a(const IterArgs &a, const PassAView &w, const PlaneNb &n,
                                         const ColumnWindow &c, int i, int z, bool lap, bool store) {
     VoxelIn q;
@@ -753,7 +824,8 @@ __device__ __forceinline__ bool voxel_a(const IterArgs &a, const PassAView &w, c
     lt_stage_taps(w, k, z, q.Lo, q.Tv);
     float rho_n;
     F3 vn, pn;
-    voxel_math<ST>(a, q, lap, rho_n, vn, pn);
+    if constexpr (AR == ARITH_FMA) voxel_math_fma(a, q, lap, rho_n, vn, pn);
+    else voxel_math<ST>(a, q, lap, rho_n, vn, pn);
     // a voxel outside the chunk (the odd step of the last trip) stores at an offset past every
     // buffer's range, which the buffer unit drops
     vr_store<ST>(w.wvr, i, store, vn, rho_n);
@@ -763,7 +835,7 @@ __device__ __forceinline__ bool voxel_a(const IterArgs &a, const PassAView &w, c
 
 // The same voxel from memory alone (the recovery after the march): the old state's stencil
 // neighbours, L at the 8 corners of psi_inv from psi, T gathered; the same arithmetic.
-template <int ST>
+template <int ST, int AR = ARITH_REF>
 __device__ __forceinline__ void voxel_a_memory(const IterArgs &a, const PassAView &w, int i, int z, bool lap) {
     const int X = w.X, XY = w.XY;
     VoxelIn q;
@@ -806,7 +878,8 @@ __device__ __forceinline__ void voxel_a_memory(const IterArgs &a, const PassAVie
     lt_memory_taps(w, corner(w.d, q.pc.x, q.pc.y, q.pc.z), q.Lo, q.Tv);
     float rho_n;
     F3 vn, pn;
-    voxel_math<ST>(a, q, lap, rho_n, vn, pn);
+    if constexpr (AR == ARITH_FMA) voxel_math_fma(a, q, lap, rho_n, vn, pn);
+    else voxel_math<ST>(a, q, lap, rho_n, vn, pn);
     vr_store<ST>(w.wvr, i, true, vn, rho_n);
     bst3_nt(w.wp, 12u * i, pn);
 }
@@ -867,7 +940,8 @@ template <int ST> __device__ __forceinline__ void take(ColumnWindow &c, const VR
 #define INTERLEAVE_LOADS() SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); SG_PAIR4(); \
     SG_PAIR4(); SG_PAIR4(); SG_PAIR(); SG_PAIR()
 
-template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int blk) {
+template <int ST, int AR = ARITH_REF> __device__ __forceinline__ void pass_a(const IterArgs &a, int blk) {
+    static_assert(AR == ARITH_REF || ST == ST_F32, "the contracted arithmetic is built for FP32 storage");
     // The block's waves step through z together (a bare s_barrier per half-step, no memory
     // fence), so that the rows one wave loads are still in L1 when the adjacent rows' waves read
     // them as y-neighbours, and adjacent rows gather from overlapping regions at the same time.
@@ -1022,7 +1096,7 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         tn1 = t_row_load(w, x, py1, z + 3);
         if (has_h) hB = yrow_load<ST>(w, ih + XY * (z + 2));
         const WindowIn<ST> in0 = window_in<ST>(w, i, XY);
-        const bool f0 = voxel_a<ST>(a, w, n0, c, i, z, lap_xy && z >= 2 && z <= d.Z - 3, own);
+        const bool f0 = voxel_a<ST, AR>(a, w, n0, c, i, z, lap_xy && z >= 2 && z <= d.Z - 3, own);
         fb_lo = min(fb_lo, f0 ? z : INT_MAX);
         fb_hi = f0 ? z : fb_hi;
         psi_store(w, z + 3, j0, lane, w0.p);
@@ -1045,7 +1119,7 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         tn1 = t_row_load(w, x, py1, z + 4);
         if (has_h) hA = yrow_load<ST>(w, ih + XY * (z + 3));
         const WindowIn<ST> in1 = window_in<ST>(w, i + XY, XY);
-        const bool f1 = voxel_a<ST>(a, w, n1, c, i + XY, z + 1, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3,
+        const bool f1 = voxel_a<ST, AR>(a, w, n1, c, i + XY, z + 1, lap_xy && z + 1 >= 2 && z + 1 <= d.Z - 3,
                                     own && s + 1 < nz);
         fb_lo = min(fb_lo, f1 ? z + 1 : INT_MAX);
         fb_hi = f1 ? z + 1 : fb_hi;
@@ -1068,7 +1142,7 @@ template <int ST> __device__ __forceinline__ void pass_a(const IterArgs &a, int 
         const int ii = x + d.X * y + XY * zz;
         const F3 p = bld3(w.rp, 12u * ii);
         if (!taps_in_stage(w, corner(d, p.x, p.y, p.z), zz))
-            voxel_a_memory<ST>(a, w, ii, zz, lap_xy && zz >= 2 && zz <= d.Z - 3);
+            voxel_a_memory<ST, AR>(a, w, ii, zz, lap_xy && zz >= 2 && zz <= d.Z - 3);
     }
 }
 
@@ -1177,7 +1251,8 @@ template <int ST> __device__ __forceinline__ void pass_b(const IterArgs &a, int 
 // entry points: one volume, or a batch of volumes in one launch per pass
 // --------------------------------------------------------------------------------------------
 
-template <int ST> __global__ __launch_bounds__(WAVE * TY_A) void k_pass_a(IterArgs a) { pass_a<ST>(a, blockIdx.x); }
+template <int ST, int AR = ARITH_REF>
+__global__ __launch_bounds__(WAVE * TY_A) void k_pass_a(IterArgs a) { pass_a<ST, AR>(a, blockIdx.x); }
 template <int ST> __global__ __launch_bounds__(WAVE * TY_B) void k_pass_b(IterArgs a) { pass_b<ST>(a, blockIdx.x); }
 
 // The volume a block of a batched launch works on: the last q with starts[q] <= b (starts has
@@ -1678,16 +1753,19 @@ static IterArgs chunked(IterArgs a, int zc, int xs, int ty) {
 }
 
 
-template <int ST> static void pass_a_kernel(const IterArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_pass_a<ST>, dim3(a.nblocks), dim3(WAVE * TY_A), 0, s, a);
+template <int ST, int AR = ARITH_REF> static void pass_a_kernel(const IterArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL((k_pass_a<ST, AR>), dim3(a.nblocks), dim3(WAVE * TY_A), 0, s, a);
 }
 template <int ST> static void pass_b_kernel(const IterArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_pass_b<ST>, dim3(a.nblocks), dim3(WAVE * TY_B), 0, s, a);
 }
 
-hipError_t launch_pass_a(const IterArgs &a0, int zc, int st, hipStream_t s) {
+hipError_t launch_pass_a(const IterArgs &a0, int zc, int st, hipStream_t s, int arith) {
     const IterArgs a = chunked(a0, zc, xs_a(a0.d), TY_A);
-    if (st == ST_BF16) pass_a_kernel<ST_BF16>(a, s);
+    if (arith == ARITH_FMA) {
+        if (st != ST_F32) return hipErrorInvalidValue;
+        pass_a_kernel<ST_F32, ARITH_FMA>(a, s);
+    } else if (st == ST_BF16) pass_a_kernel<ST_BF16>(a, s);
     else if (st == ST_F16) pass_a_kernel<ST_F16>(a, s);
     else pass_a_kernel<ST_F32>(a, s);
     return hipGetLastError();

@@ -111,6 +111,21 @@ int fr_get_chunking(const fr_ctx *ctx, int *zc_a, int *zc_b);
 int fr_set_storage(fr_ctx *ctx, int storage);
 int fr_get_storage(const fr_ctx *ctx, int *storage);
 
+/* Arithmetic of the iteration: FR_ARITH_REFERENCE (the default) performs the reference's FP32
+ * operations in its order, every product rounded before the sum it feeds, bit for bit the oracle's
+ * results.  FR_ARITH_FMA contracts those multiply-adds into fused ones (as nvcc does by default for
+ * the reference's kernels, src/fastreg/cuda/solver.cu:56-80,101-173,188-211,263-279,
+ * src/fastreg/cuda/vector_fields.cu:352-404,503-535) and divides by |rho| + 1e-5 with the hardware
+ * reciprocal (the reference's division there is not the IEEE one either, solver.cu:205-207): about
+ * 13 % fewer instructions in pass A, results a few ulps per step from the reference order.  FP32
+ * storage only (FR_ERR_INVALID otherwise); batches run the reference order and refuse contexts set
+ * to FR_ARITH_FMA.  The state is kept.  No counterpart in the reference: it is a precision setting
+ * of this implementation. */
+#define FR_ARITH_REFERENCE 0
+#define FR_ARITH_FMA 1
+int fr_set_arith(fr_ctx *ctx, int arith);
+int fr_get_arith(const fr_ctx *ctx, int *arith);
+
 /* FastReg::operator() (src/fastreg/fastreg.cpp:58-75): register the source (img2, lab2) to the
  * target (img1, lab1), all device pointers of no_voxels elements, and overwrite them in place:
  * img1/lab1 <- target o psi_inv, img2/lab2 <- source o psi.  Every call starts from a fresh
@@ -192,8 +207,9 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out);
 int fr_batch_destroy(fr_batch *b);
 /* Re-read each context's z-chunking (fr_set_chunking) and storage into the batch's launch layout.
  * A batch runs the buffers and kernels of the storage its volumes had when the layout was made:
- * after fr_set_storage on any of them, fr_batch_run and fr_batch_time_iterations fail with
- * FR_ERR_STATE until this is called (and it fails while the volumes' storages differ). */
+ * after fr_set_storage or fr_set_arith on any of them, fr_batch_run and fr_batch_time_iterations
+ * fail with FR_ERR_STATE until this is called (and it fails while the volumes' storages differ or
+ * one of them is set to FR_ARITH_FMA). */
 int fr_batch_relayout(fr_batch *b);
 /* MAX_ITER iterations of every volume from its current state (all volumes must have been
  * initialised and be at the same point: fresh, or after the same number of runs); asynchronous */
