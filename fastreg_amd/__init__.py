@@ -89,6 +89,8 @@ def lib():
     sig("fr_set_chunking", vp, i, i)
     sig("fr_set_storage", vp, i)
     sig("fr_get_storage", vp, P(i))
+    sig("fr_set_keep_state", vp, i)
+    sig("fr_get_keep_state", vp, P(i))
     sig("fr_set_arith", vp, i)
     sig("fr_get_arith", vp, P(i))
     sig("fr_batch_create", P(vp), i, P(vp))
@@ -409,6 +411,16 @@ class Solver:
         v = C.c_int()
         check(lib().fr_get_storage(self.h, C.byref(v)))
         return v.value
+
+    def set_keep_state(self, keep=True):
+        """Keep v, psi and psi_inv from one registration to the next (new images, rho reset), as
+        the reference's FastReg does; off by default (every registration starts fresh)."""
+        check(lib().fr_set_keep_state(self.h, 1 if keep else 0))
+
+    def keep_state(self):
+        v = C.c_int()
+        check(lib().fr_get_keep_state(self.h, C.byref(v)))
+        return bool(v.value)
 
     def set_arith(self, arith):
         """ARITH_REFERENCE (the reference's operation order, the default) or ARITH_FMA (its

@@ -75,6 +75,9 @@ struct fr_ctx {
     int zc_auto[3] = {0, 8, 2};  // the automatic choice for the current volume and storage
     bool zc_user[3] = {false, false, false};  // set by fr_set_chunking (else automatic)
     bool initialised = false;
+    // fr_solver_init keeps v, psi and psi_inv of an initialised context (new images, rho reset),
+    // as the reference's FastReg keeps its solver state from one call to the next
+    bool keep_state = false;
     // loop control and history
     frk::Ctl *ctl = nullptr;
     unsigned long long *maxpack = nullptr;  // max_iter x NSLOT slots
@@ -367,6 +370,19 @@ extern "C" int fr_set_storage(fr_ctx *c, int storage) {
     return FR_OK;
 }
 
+extern "C" int fr_set_keep_state(fr_ctx *c, int keep) {
+    if (!c) return fail(FR_ERR_INVALID, "null context");
+    if (keep != 0 && keep != 1) return fail(FR_ERR_INVALID, "keep must be 0 or 1");
+    c->keep_state = keep == 1;
+    return FR_OK;
+}
+
+extern "C" int fr_get_keep_state(const fr_ctx *c, int *keep) {
+    if (!c || !keep) return fail(FR_ERR_INVALID, "null argument");
+    *keep = c->keep_state ? 1 : 0;
+    return FR_OK;
+}
+
 extern "C" int fr_set_arith(fr_ctx *c, int arith) {
     if (!c) return fail(FR_ERR_INVALID, "null context");
     if (arith != FR_ARITH_REFERENCE && arith != FR_ARITH_FMA)
@@ -402,6 +418,17 @@ extern "C" int fr_get_storage(const fr_ctx *c, int *storage) {
 extern "C" int fr_solver_init(fr_ctx *c, const float *tgt, const int16_t *tgt_lab, const float *src,
                               const int16_t *src_lab) {
     if (!c || !tgt || !tgt_lab || !src || !src_lab) return fail(FR_ERR_INVALID, "null argument");
+    if (c->keep_state && c->initialised) {
+        // the state of the last run is the one kept: an eager run not yet collected settles its
+        // buffer parity first (it may have stopped early)
+        if (c->have_run && !c->captured)
+            if (int rc = collect_run(c)) return rc;
+        HIP_TRY(frk::launch_reinit(c->d, tgt, tgt_lab, src, src_lab, c->T, c->LT, c->S, c->LS, c->st, c->rho[0],
+                                   c->rho[1], c->vr[0], c->vr[1], c->prm.rho_0, c->stream));
+        c->have_run = c->captured;
+        c->have_final = false;
+        return FR_OK;
+    }
     HIP_TRY(frk::launch_init(c->d, tgt, tgt_lab, src, src_lab, c->T, c->LT, c->S, c->LS, c->rho[0], c->rho[1], c->v[0],
                              c->v[1], c->pinv[0], c->pinv[1], c->psi, c->prm.rho_0, c->stream));
     if (c->st != frk::ST_F32) HIP_TRY(frk::launch_vr_init(c->st, c->vr[0], c->vr[1], c->n, c->prm.rho_0, c->stream));

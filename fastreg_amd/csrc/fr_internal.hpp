@@ -85,6 +85,11 @@ hipError_t launch_init(Dims d, const float *tgt, const int16_t *tgt_lab, const f
 // the 8-byte (v, rho) records and a.rho_o / a.rho_n are unused
 hipError_t launch_pass_a(const IterArgs &a, int zc, int st, hipStream_t s, int arith = ARITH_REF);
 hipError_t launch_pass_b(const IterArgs &a, int zc, int st, hipStream_t s);
+// new images and rho = rho_0 into a kept state (v, psi, psi_inv unchanged); the 16-bit storages
+// reset rho in the (v, rho) records vr0/vr1, FP32 in rho0/rho1
+hipError_t launch_reinit(Dims d, const float *tgt, const int16_t *tgt_lab, const float *src, const int16_t *src_lab,
+                         float *T, int16_t *LT, float *S, int16_t *LS, int st, float *rho0, float *rho1, unsigned *vr0,
+                         unsigned *vr1, float rho_0, hipStream_t s);
 // resident blocks per CU of pass A (1) or pass B (2)
 hipError_t pass_occupancy(int which, int *blocks_per_cu);
 

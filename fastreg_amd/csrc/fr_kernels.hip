@@ -1331,6 +1331,25 @@ __global__ void k_init(Dims d, const float *tgt, const int16_t *tgt_lab, const f
     }
 }
 
+// New images into a context that keeps its state (the reference's FastReg::operator() re-initialises
+// the four volumes and, in estimate_psi, the density, fastreg.cpp:58-63, solver.cu:317): T, LT, S,
+// LS from the caller's arrays and rho = rho_0 in both buffers (FP32 storage; rho0/rho1 null in the
+// 16-bit forms, whose records are reset by k_vr_reset_rho); v, psi and psi_inv are kept.
+__global__ void k_reinit(Dims d, const float *tgt, const int16_t *tgt_lab, const float *src, const int16_t *src_lab,
+                         float *T, int16_t *LT, float *S, int16_t *LS, float *rho0, float *rho1, float rho_0) {
+    const int n = d.X * d.Y * d.Z;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        T[i] = tgt[i];
+        LT[i] = tgt_lab[i];
+        S[i] = src[i];
+        LS[i] = src_lab[i];
+        if (rho0) {
+            rho0[i] = rho_0;
+            rho1[i] = rho_0;
+        }
+    }
+}
+
 constexpr int RED_BLOCK = 256;
 
 __device__ __forceinline__ double block_sum(double v, double *sh) {
@@ -1694,6 +1713,15 @@ template <int ST> __global__ void k_vr_init(unsigned *r0, unsigned *r1, int n, f
 }
 
 // replace v (float4 per voxel) and/or rho (either may be null) in both buffers of records
+// rho = rho_0 in every (v, rho) record of both buffers, v kept
+template <int ST> __global__ void k_vr_reset_rho(unsigned *r0, unsigned *r1, int n, float rho_0) {
+    const unsigned h = h_of<ST>(rho_0) << 16;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        r0[2 * i + 1] = (r0[2 * i + 1] & 0xFFFFu) | h;
+        r1[2 * i + 1] = (r1[2 * i + 1] & 0xFFFFu) | h;
+    }
+}
+
 template <int ST> __global__ void k_vr_set(unsigned *r0, unsigned *r1, const float *v4, const float *rho, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         unsigned lo = r0[2 * i], hi = r0[2 * i + 1];
@@ -1878,6 +1906,18 @@ hipError_t launch_sum(const float *a, long long n, double *partials, double *out
 hipError_t launch_vr_init(int st, unsigned *r0, unsigned *r1, int n, float rho_0, hipStream_t s) {
     if (st == ST_BF16) hipLaunchKernelGGL(k_vr_init<ST_BF16>, dim3(grid_for(n, 256)), dim3(256), 0, s, r0, r1, n, rho_0);
     else hipLaunchKernelGGL(k_vr_init<ST_F16>, dim3(grid_for(n, 256)), dim3(256), 0, s, r0, r1, n, rho_0);
+    return hipGetLastError();
+}
+
+hipError_t launch_reinit(Dims d, const float *tgt, const int16_t *tgt_lab, const float *src, const int16_t *src_lab,
+                         float *T, int16_t *LT, float *S, int16_t *LS, int st, float *rho0, float *rho1, unsigned *vr0,
+                         unsigned *vr1, float rho_0, hipStream_t s) {
+    const int n = d.X * d.Y * d.Z;
+    const bool f32 = st == ST_F32;
+    hipLaunchKernelGGL(k_reinit, dim3(grid_for(n, 256)), dim3(256), 0, s, d, tgt, tgt_lab, src, src_lab, T, LT, S, LS,
+                       f32 ? rho0 : nullptr, f32 ? rho1 : nullptr, rho_0);
+    if (st == ST_BF16) hipLaunchKernelGGL(k_vr_reset_rho<ST_BF16>, dim3(grid_for(n, 256)), dim3(256), 0, s, vr0, vr1, n, rho_0);
+    else if (st == ST_F16) hipLaunchKernelGGL(k_vr_reset_rho<ST_F16>, dim3(grid_for(n, 256)), dim3(256), 0, s, vr0, vr1, n, rho_0);
     return hipGetLastError();
 }
 

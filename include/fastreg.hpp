@@ -13,11 +13,14 @@
 
 class FastReg {
 public:
+    // As the reference's FastReg, an instance carries its solver state (v, psi, psi_inv) from one
+    // operator() call to the next (fr_set_keep_state); construct a new one to start afresh.
     explicit FastReg(const Params &params) : params_(params) {
         fr_params q = params.to_c();
         fr_ctx *c = nullptr;
         fastreg::check(fr_create(&q, &c));
         ctx_ = c;
+        fastreg::check(fr_set_keep_state(ctx_, 1));
         psi_ = std::make_shared<fastreg::cuda::DeformationField>(params.volume_dims);
     }
     ~FastReg() { fr_destroy(ctx_); }

@@ -126,10 +126,22 @@ int fr_get_storage(const fr_ctx *ctx, int *storage);
 int fr_set_arith(fr_ctx *ctx, int arith);
 int fr_get_arith(const fr_ctx *ctx, int *arith);
 
+/* Whether a context keeps its state from one registration to the next.  The reference's FastReg
+ * does: each operator() call re-initialises the four volumes and the density (rho = rho_0,
+ * src/fastreg/fastreg.cpp:58-63, src/fastreg/cuda/solver.cu:317) but continues from the velocity,
+ * psi and psi_inv the previous call left (include/solver.hpp:64-113 keeps v as a member; FastReg
+ * keeps psi and psi_inv).  With keep = 1, fr_solver_init (and so fr_register) on a context that
+ * holds a state replaces T, S and their labels, resets rho and keeps v, psi and psi_inv; the first
+ * initialisation, and the first after fr_set_storage, starts from the identity as always.  The
+ * default, keep = 0, starts every registration fresh (the same inputs give the same result). */
+int fr_set_keep_state(fr_ctx *ctx, int keep);
+int fr_get_keep_state(const fr_ctx *ctx, int *keep);
+
 /* FastReg::operator() (src/fastreg/fastreg.cpp:58-75): register the source (img2, lab2) to the
  * target (img1, lab1), all device pointers of no_voxels elements, and overwrite them in place:
  * img1/lab1 <- target o psi_inv, img2/lab2 <- source o psi.  Every call starts from a fresh
- * state (psi = psi_inv = id, v = 0, rho = rho_0).  Blocks until the outputs are written. */
+ * state (psi = psi_inv = id, v = 0, rho = rho_0), or with fr_set_keep_state from the state the
+ * previous call left (rho = rho_0).  Blocks until the outputs are written. */
 int fr_register(fr_ctx *ctx, float *d_img1, int16_t *d_lab1, float *d_img2, int16_t *d_lab2);
 
 /* FastReg::getDeformationField (include/fastreg.hpp:22): psi as float4 (x, y, z, 0) per voxel,

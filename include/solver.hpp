@@ -208,27 +208,40 @@ typedef ScalarField Divergence;
 
 class Solver {
 public:
+    // As the reference's Solver, an instance keeps its velocity from one estimate_psi to the next
+    // (until clear()); the deformation it iterates on is the caller's psi and psi_inv.
     explicit Solver(const Params &params) : params_(params) {
         fr_params q = params.to_c();
         fr_ctx *c = nullptr;
         check(fr_create(&q, &c));
         ctx_ = c;
+        check(fr_set_keep_state(ctx_, 1));
     }
     ~Solver() { fr_destroy(ctx_); }
     Solver(const Solver &) = delete;
     Solver &operator=(const Solver &) = delete;
 
-    // The reference clears its derivative fields here (solver.cpp:69-81); every estimate_psi here
-    // starts from a fresh state, so there is nothing to clear.
-    void clear() {}
+    // Solver::clear (solver.cpp:69-83): the next estimate_psi starts from v = 0 (rho = rho_0 and
+    // the derivative fields, which the reference also clears, are set or recomputed by every
+    // estimate_psi here)
+    void clear() { clear_ = true; }
 
-    // estimate_psi (solver.cpp:83-115): register phi_n to phi_global; on return
+    // estimate_psi (solver.cpp:83-115): register phi_n to phi_global, starting from the caller's
+    // psi and psi_inv and the velocity the previous call left; on return
     // phi_global_psi_inv = phi_global o psi_inv, phi_n_psi = phi_n o psi, and psi / psi_inv hold the
-    // deformation and its inverse (either may be null)
+    // deformation and its inverse (either may be null: the context's own is then used)
     void estimate_psi(std::shared_ptr<Volume> phi_global, std::shared_ptr<Volume> phi_global_psi_inv,
                       std::shared_ptr<Volume> phi_n, std::shared_ptr<Volume> phi_n_psi,
                       std::shared_ptr<DeformationField> psi, std::shared_ptr<DeformationField> psi_inv) {
+        if (clear_) check(fr_set_keep_state(ctx_, 0));  // a fresh state: v = 0
         check(fr_solver_init(ctx_, phi_global->image(), phi_global->labels(), phi_n->image(), phi_n->labels()));
+        if (clear_) {
+            check(fr_set_keep_state(ctx_, 1));
+            clear_ = false;
+        }
+        if (psi || psi_inv)
+            check(fr_solver_set_state(ctx_, nullptr, nullptr, psi ? psi->data() : nullptr,
+                                      psi_inv ? psi_inv->data() : nullptr));
         check(fr_solver_run(ctx_));
         check(fr_solver_finalize(ctx_, phi_global_psi_inv->image(), phi_global_psi_inv->labels(), phi_n_psi->image(),
                                  phi_n_psi->labels()));
@@ -256,6 +269,7 @@ public:
 private:
     Params params_;
     fr_ctx *ctx_ = nullptr;
+    bool clear_ = false;
 };
 
 }  // namespace cuda

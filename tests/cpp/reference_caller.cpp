@@ -8,7 +8,13 @@
 //
 // IN_DIR holds tgt.f32, tgt_lab.i16, src.f32, src_lab.i16 (raw, x-fastest); OUT_DIR receives
 //   fastreg_{img1,lab1,img2,lab2}, fastreg_psi   FastReg::operator() in place, getDeformationField()
+//   fastreg2_{img1,lab1,img2,lab2}, fastreg2_psi a second operator() call on the same object and
+//                                                the same inputs: it continues from the state the
+//                                                first left (fastreg.cpp:58-75, solver.cu:317)
 //   solver_{img1,lab1,img2,lab2}, solver_psi, solver_psi_inv   fastreg::cuda::Solver::estimate_psi
+//   solver2_{img,lab,n_img,n_lab}, solver2_psi   a second estimate_psi on the same solver, from the
+//                                                psi and psi_inv the first returned
+//   solver3_{img,lab,n_img,n_lab}                estimate_psi after Solver::clear(), from the identity
 //   apply_{img,lab}                              DeformationField::apply(phi_n, .) with the solver's psi
 //   identity_{img,lab}, identity_psi             the same after DeformationField::clear()
 //   cleared_{img,lab}                            a Volume after Volume::clear()
@@ -85,15 +91,21 @@ int main(int argc, char **argv) {
         float *img1 = device_copy(tgt), *img2 = device_copy(src);
         short *lab1 = device_copy(tgt_lab), *lab2 = device_copy(src_lab);
         FastReg fastreg(params);
-        if (!fastreg(img1, lab1, img2, lab2)) return 1;
-        write_raw(out + "/fastreg_img1", host_copy(img1, n));
-        write_raw(out + "/fastreg_lab1", host_copy(lab1, n));
-        write_raw(out + "/fastreg_img2", host_copy(img2, n));
-        write_raw(out + "/fastreg_lab2", host_copy(lab2, n));
-        std::shared_ptr<fastreg::cuda::DeformationField> psi = fastreg.getDeformationField();
-        std::vector<float> h(4 * n);
-        psi->download(h.data());
-        write_raw(out + "/fastreg_psi", h);
+        for (const std::string stem : {"fastreg", "fastreg2"}) {
+            fastreg::check(fr_memcpy_htod(img1, tgt.data(), n * sizeof(float)));
+            fastreg::check(fr_memcpy_htod(lab1, tgt_lab.data(), n * sizeof(short)));
+            fastreg::check(fr_memcpy_htod(img2, src.data(), n * sizeof(float)));
+            fastreg::check(fr_memcpy_htod(lab2, src_lab.data(), n * sizeof(short)));
+            if (!fastreg(img1, lab1, img2, lab2)) return 1;
+            write_raw(out + "/" + stem + "_img1", host_copy(img1, n));
+            write_raw(out + "/" + stem + "_lab1", host_copy(lab1, n));
+            write_raw(out + "/" + stem + "_img2", host_copy(img2, n));
+            write_raw(out + "/" + stem + "_lab2", host_copy(lab2, n));
+            std::shared_ptr<fastreg::cuda::DeformationField> psi = fastreg.getDeformationField();
+            std::vector<float> h(4 * n);
+            psi->download(h.data());
+            write_raw(out + "/" + stem + "_psi", h);
+        }
         if (fastreg.getParams().max_iter != params.max_iter) return 1;
         fr_dev_free(img1), fr_dev_free(img2), fr_dev_free(lab1), fr_dev_free(lab2);
     }
@@ -121,6 +133,26 @@ int main(int argc, char **argv) {
     write_raw(out + "/solver_psi", h);
     psi_inv->download(h.data());
     write_raw(out + "/solver_psi_inv", h);
+
+    // a second estimate_psi continues from the velocity the first left and the psi / psi_inv it
+    // returned (here copies of them); after clear() the velocity starts from 0 again
+    {
+        auto psi2 = std::make_shared<fastreg::cuda::DeformationField>(params.volume_dims);
+        auto psi_inv2 = std::make_shared<fastreg::cuda::DeformationField>(params.volume_dims);
+        fastreg::check(fr_memcpy_dtod(psi2->data(), psi->data(), 4 * n * sizeof(float)));
+        fastreg::check(fr_memcpy_dtod(psi_inv2->data(), psi_inv->data(), 4 * n * sizeof(float)));
+        solver->estimate_psi(phi_global, phi_global_psi_inv, phi_n, phi_n_psi, psi2, psi_inv2);
+        save_volume(out + "/solver2", *phi_global_psi_inv, n);
+        save_volume(out + "/solver2_n", *phi_n_psi, n);
+        psi2->download(h.data());
+        write_raw(out + "/solver2_psi", h);
+        auto psi3 = std::make_shared<fastreg::cuda::DeformationField>(params.volume_dims);
+        auto psi_inv3 = std::make_shared<fastreg::cuda::DeformationField>(params.volume_dims);
+        solver->clear();
+        solver->estimate_psi(phi_global, phi_global_psi_inv, phi_n, phi_n_psi, psi3, psi_inv3);
+        save_volume(out + "/solver3", *phi_global_psi_inv, n);
+        save_volume(out + "/solver3_n", *phi_n_psi, n);
+    }
 
     // DeformationField::apply with the estimated psi (vector_fields.cpp:103-113)
     auto warped = std::make_shared<fastreg::cuda::Volume>(phi_n->get_dims());

@@ -1,6 +1,7 @@
 """The reference's C++ interface on the GPU: tests/cpp/reference_caller (built by build(), written
 against the reference's FastReg, Params, fastreg::cuda::Solver, Volume and DeformationField) runs a
-registration through FastReg::operator() and through Solver::estimate_psi, then exercises
+registration through FastReg::operator() and through Solver::estimate_psi, twice each (the second
+continuing from the state the first left, as the reference's objects do), then exercises
 DeformationField::apply and clear and Volume::clear.  Every output is compared with the oracle."""
 import os
 import subprocess
@@ -62,6 +63,24 @@ def test_reference_cpp_interface_matches_oracle(oracle_mod, tmp_path):
         psi = f32(f"{stem}_psi", 4)
         assert np.array_equal(psi.view(np.uint32), st["psi"].view(np.uint32)), stem
     assert np.array_equal(f32("solver_psi_inv", 4).view(np.uint32), st["psi_inv"].view(np.uint32))
+    # the second FastReg call and the second estimate_psi continue from the first's v, psi and
+    # psi_inv with rho reset (the reference's state carried between calls); after clear() the
+    # solver starts afresh
+    o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+    o.set_state(v=st["v"], psi=st["psi"], psi_inv=st["psi_inv"])
+    o.run(O.SolverParams(max_iter=iters, no_labels=pair.no_labels))
+    img1b, lab1b, img2b, lab2b = o.outputs()
+    st2 = o.get_state()
+    assert not np.array_equal(st2["psi"], st["psi"])
+    for a, b, nm in ((f32("fastreg2_img1"), img1b, "img1"), (i16("fastreg2_lab1"), lab1b, "lab1"),
+                     (f32("fastreg2_img2"), img2b, "img2"), (i16("fastreg2_lab2"), lab2b, "lab2"),
+                     (f32("solver2_img"), img1b, "solver2 img"), (i16("solver2_lab"), lab1b, "solver2 lab"),
+                     (f32("solver2_n_img"), img2b, "solver2 n img"), (i16("solver2_n_lab"), lab2b, "solver2 n lab"),
+                     (f32("solver3_img"), img1, "solver3 img"), (i16("solver3_lab"), lab1, "solver3 lab"),
+                     (f32("solver3_n_img"), img2, "solver3 n img"), (i16("solver3_n_lab"), lab2, "solver3 n lab")):
+        assert np.array_equal(a, b), nm
+    for name in ("fastreg2_psi", "solver2_psi"):
+        assert np.array_equal(f32(name, 4).view(np.uint32), st2["psi"].view(np.uint32)), name
     # DeformationField::apply with the estimated psi is the source warp of the finalisation
     assert np.array_equal(f32("apply_img"), img2) and np.array_equal(i16("apply_lab"), lab2)
     # clear() makes psi the identity, whose warp is an exact copy

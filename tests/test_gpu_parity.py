@@ -266,6 +266,38 @@ def test_dice_counts_with_labels_out_of_range(oracle_mod, no_labels, dims):
     assert np.array_equal(ov, oov) and np.array_equal(vx, ovx)
 
 
+@pytest.mark.parametrize("keep", [False, True])
+def test_second_registration_on_one_context(oracle_mod, keep):
+    """Two registrations in a row on one context, the second of another pair.  With
+    fr_set_keep_state the second continues from the v, psi and psi_inv the first left (new images,
+    rho = rho_0), as the reference's FastReg does from one call to the next; without it the second
+    starts afresh.  Outputs and state bit-exact against the oracle doing the same."""
+    O = oracle_mod
+    dims = (30, 26, 22)
+    X, Y, Z = dims
+    say("second registration", keep)
+    a, b = phantoms.blobs(shape=(Z, Y, X), seed=61), phantoms.blobs(shape=(Z, Y, X), seed=62)
+    nl = max(a.no_labels, b.no_labels)
+    p = _params(dims, no_labels=nl, max_iter=15, alpha=0.3)
+    g = fr.Solver(p)
+    g.set_keep_state(keep)
+    assert g.keep_state() == keep
+    g.register(a.target, a.target_labels, a.source, a.source_labels)
+    outs = g.register(b.target, b.target_labels, b.source, b.source_labels)
+    o = O.Oracle(dims, "f32")
+    prm = O.SolverParams(max_iter=15, alpha=0.3, no_labels=nl)
+    o.init(a.target, a.target_labels, a.source, a.source_labels)
+    o.run(prm)
+    st = o.get_state()
+    o.init(b.target, b.target_labels, b.source, b.source_labels)
+    if keep:
+        o.set_state(v=st["v"], psi=st["psi"], psi_inv=st["psi_inv"])
+    o.run(prm)
+    for x, y, nm in zip(outs, o.outputs(), ("img1", "lab1", "img2", "lab2")):
+        assert np.array_equal(np.asarray(x).reshape(-1), y), nm
+    _assert_state_equal(g.get_state(), o.get_state(), "second registration")
+
+
 def test_register_matches_oracle_c1_shape(oracle_mod):
     """fr_register on the 64^3 blob phantom (40 iterations): in-place outputs, Dice and
     min det J equal the oracle's."""

@@ -79,6 +79,26 @@ def test_run_with_log(oracle_mod, storage):
         assert np.array_equal(a, b), nm
 
 
+@pytest.mark.parametrize("storage", STORAGES)
+def test_kept_state_in_16_bit_storage(oracle_mod, storage):
+    """A context that keeps its state: the second registration resets rho inside the (v, rho)
+    records and keeps v; equal to the oracle with the same storage doing the same."""
+    O = oracle_mod
+    dims = (28, 22, 20)
+    say("kept state", storage)
+    g, o, pair = _both(O, dims, 71, storage, max_iter=12, alpha=0.3)
+    g.set_keep_state(True)
+    prm = O.SolverParams(max_iter=12, alpha=0.3, no_labels=pair.no_labels)
+    for k in range(2):
+        g.register(pair.target, pair.target_labels, pair.source, pair.source_labels)
+        st = o.get_state()
+        if k:
+            o.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+            o.set_state(v=st["v"], psi=st["psi"], psi_inv=st["psi_inv"])
+        o.run(prm)
+        _same_state(g, o, f"registration {k}")
+
+
 def test_batch_in_bf16(oracle_mod):
     O = oracle_mod
     vols = []
