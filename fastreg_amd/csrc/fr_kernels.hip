@@ -480,12 +480,18 @@ template <int ST> __device__ __forceinline__ void ys_store_row(const PassAView &
     if constexpr (ST == ST_F32) ys_store(w, zz, row, lane, h.v, h.r, h.p, h.s);
     else ys_store(w, zz, row, lane, vr_v<ST>(h.vr), vr_r<ST>(h.vr), h.p, h.s);
 }
+// (x, y, z) of a 16-byte stage entry as one 12-byte LDS read (ds_read_b96): the post-barrier burst of
+// stage reads is the critical path of a step, and the fourth component is not needed here
+__device__ __forceinline__ F3 xyz_of(const float4 &e) {
+    const float *p = &e.x;
+    return f3(p[0], p[1], p[2]);
+}
 // the y-neighbours of plane zz for the wave whose row is j, the upwind side chosen by vneg (v.y < 0)
 __device__ __forceinline__ PlaneNb ys_read(const PassAView &w, int zz, int j, int lane, bool vneg) {
     const int sl = zz & (YS_PLANES - 1);
-    const float4 a = w.ys->vr[sl][j + 1][lane], b = w.ys->vr[sl][j - 1][lane],
-                 c = w.ys->vr[sl][vneg ? j + 2 : j - 2][lane], e = w.ys->ps[sl][j + 1][lane],
+    const float4 a = w.ys->vr[sl][j + 1][lane], b = w.ys->vr[sl][j - 1][lane], e = w.ys->ps[sl][j + 1][lane],
                  f = w.ys->ps[sl][j - 1][lane];
+    const F3 c = xyz_of(w.ys->vr[sl][vneg ? j + 2 : j - 2][lane]);  // rho unused: a 12-byte read
     PlaneNb n;
     n.v_yp = f3(a.x, a.y, a.z);
     n.r_yp = a.w;
@@ -513,8 +519,7 @@ __device__ __forceinline__ void psi_store(const PassAView &w, int zz, int row, i
 }
 __device__ __forceinline__ PsiWin psi_win(F3 m, F3 c, F3 p) { return PsiWin{m, c, p}; }
 __device__ __forceinline__ F3 ps_at(const PassAView &w, int zz, int row, int lane) {
-    const float4 q = w.ps->v[zz & (PS_PLANES - 1)][row][lane];
-    return f3(q.x, q.y, q.z);
+    return xyz_of(w.ps->v[zz & (PS_PLANES - 1)][row][lane]);
 }
 
 // L of stage row j (y = yb - 1 + j) of plane zz, with T of that row, into the L/T stage: psi of the
@@ -1205,9 +1210,10 @@ template <int ST> __device__ __forceinline__ void pass_b(const IterArgs &a, int 
         ry = min(max(ry, 0), VS_ROWS - 2);
         const int zg = dz == -1 ? z - 1 : z;
         const int s0 = zg & (VS_PLANES - 1), s1 = (zg + 1) & (VS_PLANES - 1);
-        const float4 f000 = vs.v[s0][ry][cx], f001 = vs.v[s1][ry][cx], f010 = vs.v[s0][ry + 1][cx],
-                     f011 = vs.v[s1][ry + 1][cx], f100 = vs.v[s0][ry][cx + 1], f101 = vs.v[s1][ry][cx + 1],
-                     f110 = vs.v[s0][ry + 1][cx + 1], f111 = vs.v[s1][ry + 1][cx + 1];
+        const F3 f000 = xyz_of(vs.v[s0][ry][cx]), f001 = xyz_of(vs.v[s1][ry][cx]), f010 = xyz_of(vs.v[s0][ry + 1][cx]),
+                 f011 = xyz_of(vs.v[s1][ry + 1][cx]), f100 = xyz_of(vs.v[s0][ry][cx + 1]),
+                 f101 = xyz_of(vs.v[s1][ry][cx + 1]), f110 = xyz_of(vs.v[s0][ry + 1][cx + 1]),
+                 f111 = xyz_of(vs.v[s1][ry + 1][cx + 1]);
         F3 w = f3(tri(f111.x, f110.x, f101.x, f100.x, f011.x, f010.x, f001.x, f000.x, k.a, k.b, k.c),
                   tri(f111.y, f110.y, f101.y, f100.y, f011.y, f010.y, f001.y, f000.y, k.a, k.b, k.c),
                   tri(f111.z, f110.z, f101.z, f100.z, f011.z, f010.z, f001.z, f000.z, k.a, k.b, k.c));
