@@ -420,9 +420,15 @@ extern "C" int fr_solver_init(fr_ctx *c, const float *tgt, const int16_t *tgt_la
     if (!c || !tgt || !tgt_lab || !src || !src_lab) return fail(FR_ERR_INVALID, "null argument");
     if (c->keep_state && c->initialised) {
         // the state of the last run is the one kept: an eager run not yet collected settles its
-        // buffer parity first (it may have stopped early)
-        if (c->have_run && !c->captured)
+        // buffer parity first (it may have stopped early), which a stream capture cannot wait for
+        if (c->have_run && !c->captured) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            HIP_TRY(hipStreamIsCapturing(c->stream, &cs));
+            if (cs != hipStreamCaptureStatusNone)
+                return fail(FR_ERR_STATE, "fr_solver_init keeping the state of a run not yet collected, inside a "
+                                          "stream capture: read the run's results (fr_get_stats) before capturing");
             if (int rc = collect_run(c)) return rc;
+        }
         HIP_TRY(frk::launch_reinit(c->d, tgt, tgt_lab, src, src_lab, c->T, c->LT, c->S, c->LS, c->st, c->rho[0],
                                    c->rho[1], c->vr[0], c->vr[1], c->prm.rho_0, c->stream));
         c->have_run = c->captured;

@@ -133,7 +133,9 @@ int fr_get_arith(const fr_ctx *ctx, int *arith);
  * keeps psi and psi_inv).  With keep = 1, fr_solver_init (and so fr_register) on a context that
  * holds a state replaces T, S and their labels, resets rho and keeps v, psi and psi_inv; the first
  * initialisation, and the first after fr_set_storage, starts from the identity as always.  The
- * default, keep = 0, starts every registration fresh (the same inputs give the same result). */
+ * kept state is that of the last run, read back first if it has not been; inside a stream capture
+ * that read is not possible and fr_solver_init returns FR_ERR_STATE.  The default, keep = 0,
+ * starts every registration fresh (the same inputs give the same result). */
 int fr_set_keep_state(fr_ctx *ctx, int keep);
 int fr_get_keep_state(const fr_ctx *ctx, int *keep);
 
