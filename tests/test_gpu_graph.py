@@ -142,3 +142,37 @@ def test_caller_captured_loop_matches_oracle_on_every_replay(oracle_mod):
         fr.lib().fr_set_stream(g.h, None)
         g.close()
         hip.hipStreamDestroy(stream)
+
+
+def test_kept_state_init_inside_a_capture_needs_the_run_collected():
+    """A context that keeps its state must read back an earlier run before its next fr_solver_init;
+    inside a stream capture that read cannot run, so the call returns FR_ERR_STATE (and captures
+    nothing).  Outside the capture, after the run is collected, it succeeds."""
+    hip = _hip()
+    pair = phantoms.blobs(shape=(16, 18, 20), seed=23)
+    g = fr.Solver(fr.default_params(vol_dims=pair.dims, no_labels=pair.no_labels, max_iter=4))
+    stream = C.c_void_p()
+    _ok(hip.hipStreamCreate(C.byref(stream)), "hipStreamCreate")
+    graph = C.c_void_p()
+    try:
+        fr.check(fr.lib().fr_set_stream(g.h, stream))
+        g.set_keep_state(True)
+        g.init(pair.target, pair.target_labels, pair.source, pair.source_labels)
+        fr.check(fr.lib().fr_solver_run(g.h))  # not collected
+        _ok(hip.hipStreamSynchronize(stream), "sync")
+        inputs = [b.ptr for b in g._inputs]
+        _ok(hip.hipStreamBeginCapture(stream, HIP_CAPTURE_THREAD_LOCAL), "hipStreamBeginCapture")
+        rc = fr.lib().fr_solver_init(g.h, *inputs)
+        rc_end = hip.hipStreamEndCapture(stream, C.byref(graph))
+        assert rc == fr.FR_ERR_STATE, rc
+        _ok(rc_end, "hipStreamEndCapture")
+        assert g.stats().iterations == 4  # collects the run
+        fr.check(fr.lib().fr_solver_init(g.h, *inputs))
+        g.run()
+        assert g.stats().iterations == 4
+    finally:
+        if graph.value:
+            hip.hipGraphDestroy(graph)
+        fr.lib().fr_set_stream(g.h, None)
+        g.close()
+        hip.hipStreamDestroy(stream)
